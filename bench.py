@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the SPH2S solver step on one MI355X.
+
+Workload: the 1,000,000-particle single-scale dam break (100^3 lattice,
+domain [0, 3.125]^3, h = 0.0457, FP32 SoA float4 state), stepped with the
+captured hipGraph.  One "step" = hash -> radix sort -> reorder + cell ranges
+-> density -> force + symplectic Euler + wall clamp over all particles.
+
+Prints ONE JSON line with:
+  value       particle-updates/s = N * K / t over K timed graph launches
+              (state resident in HBM; hipEvents on the solver's stream)
+  roofline    dominant kernel (force+integrate): algorithmic bytes per launch
+              (52 N + 8 C) / its mean duration, measured with hipEvents around
+              each pass in a second, eager run of the same steps
+  cpu_baseline the FP32 OpenMP CPU step (a port of the same algorithm) on the
+              same scene, timed for a bounded number of steps
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+METRIC = "particle-updates/s at 1M particles, fwd and fwd+adjoint, % HBM roofline"
+
+
+def algorithmic_bytes(n: int, cells: int) -> dict:
+    """Compulsory traffic per pass (each array read once / written once)."""
+    return {
+        "hash_reset": 20 * n,
+        "radix_sort": 16 * n,
+        "reorder_cells": 56 * n + 8 * cells,
+        "density": 16 * n + 8 * cells,
+        "force_integrate": 52 * n + 8 * cells,
+        "step": 160 * n + 24 * cells,
+    }
+
+
+def load_traffic(workload: str):
+    """Measured HBM bytes per force launch from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_force_integrate_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(scene, budget_s: float, max_steps: int):
+    import oracle as O  # the CPU port of the step; test/bench infrastructure only
+    cs = O.CpuSolver(O.params_from_config(scene.config), scene.positions, scene.velocities)
+    cs.step(scene.dt, 1)  # warm-up
+    done, t0 = 0, time.perf_counter()
+    while done < max_steps:
+        cs.step(scene.dt, 1)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": scene.n * done / dt,
+        "unit": "particle-updates/s",
+        "cores": O.CpuSolver.threads(),
+        "kind": "port",
+        "sample": f"{scene.name}: {done} FP32 OpenMP steps of {scene.n} particles after 1 warm-up "
+                  f"step ({dt:.1f} s)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
+    ap.add_argument("--cpu-max-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not supported "
+              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    from sph2s_final_amd import Solver, ParticleData, scenes
+    from sph2s_final_amd import _capi as C
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+
+    scene = scenes.dam_break_1m()
+    n = scene.n
+    solver = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
+    views = solver.views()
+    cells = int(views.num_cells)
+    stream = torch.cuda.ExternalStream(views.stream)
+
+    solver.Advance(scene.dt, args.warmup)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    solver.Advance(scene.dt, args.steps)
+    stop.record(stream)
+    solver.synchronize()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    ms_per_step = ms / args.steps
+    value = n * args.steps / (ms / 1e3)
+
+    # Per-pass durations (eager launches bracketed by hipEvents on the solver stream).
+    per_pass = (C.ctypes.c_float * len(C.PASS_NAMES))()
+    C.check(C.lib().sph_profile_passes(solver._h, scene.dt, args.profile_steps, per_pass))
+    pass_ms = {name: per_pass[i] / args.profile_steps for i, name in enumerate(C.PASS_NAMES)}
+    solver.synchronize()
+    pos = solver.positions()
+    finite = bool(__import__("numpy").isfinite(pos).all())
+    solver.close()
+
+    alg = algorithmic_bytes(n, cells)
+    force_s = pass_ms["force_integrate"] / 1e3
+    achieved = alg["force_integrate"] / force_s / 1e9
+    traffic = load_traffic(scene.name)
+    step_gbs = alg["step"] / (ms_per_step / 1e3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": scene.name, "particles": n, "cells": cells,
+                   "domain": [scene.config.domain_hi[d] for d in range(3)],
+                   "h": round(float(scene.config.h), 6), "dt": scene.dt, "graph": True,
+                   "path": "forward"},
+        "roofline": {"bound": "hbm", "kernel": "force_integrate", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "bytes_per_launch": alg["force_integrate"],
+                     "launch_ms": pass_ms["force_integrate"]},
+        "step_hbm": {"alg_bytes": alg["step"], "achieved_gbs": step_gbs,
+                     "frac": step_gbs / HBM_PEAK_GBS},
+        "pass_ms": pass_ms,
+        "state_finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,202 @@
+/*
+ * sph2s.h -- C ABI of the MI355X-native SPH2S solver step (libsph2s.so).
+ *
+ * The library replaces the device half of the reference solver
+ * (SPH2S_FINAL_SINGLE_SCALE/Solver.cu, SPH2S_FINAL_TWO_SCALE/Solver.cu) and the
+ * Thrust sort it calls.  Every entry point returns an int status (SPH_OK on
+ * success); no C++ exception crosses this boundary.  sph_last_error() returns
+ * the message of the last failure on the calling thread.  The C++ facade in
+ * include/sph2s/Solver.hpp (ParticleData / Solver / SolverConfiguration /
+ * Grid, the reference's class names) turns a non-zero status into a
+ * std::runtime_error, as the reference's CUDA::SafeCall does
+ * (SPH2S_FINAL_TWO_SCALE/cuda.h:115-129).
+ *
+ * Device pointers handed out by sph_get_views() are borrowed: they stay valid
+ * until the next sph_advance, sph_set_state or sph_destroy on the same handle.
+ * A handle may be driven by one host thread at a time; there is no
+ * process-global state (the reference keeps one __constant__ configuration per
+ * process, SPH2S_FINAL_SINGLE_SCALE/Solver.cu:12).
+ */
+#ifndef SPH2S_H_
+#define SPH2S_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define SPH_OK                    0
+#define SPH_ERR_INVALID_ARGUMENT  1
+#define SPH_ERR_HIP               2
+#define SPH_ERR_CAPACITY          3   /* a split would exceed max particles   */
+#define SPH_ERR_NONFINITE         4   /* NaN/Inf found in the particle state   */
+#define SPH_ERR_UNSUPPORTED       5
+#define SPH_ERR_NO_DEVICE         6
+
+/* Empty cell marker of cell_start/cell_end
+ * (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:862,1118-1127). */
+#define SPH_EMPTY_CELL 0xFFFFFFFFu
+
+/* ---- Grid: replaces struct Grid (SPH2S_FINAL_SINGLE_SCALE/Grid.h:9-59) ---- */
+typedef struct sph_grid {
+    float   origin[3];
+    int32_t dims[3];     /* number of cells per axis */
+    float   spacing;     /* cell edge length         */
+} sph_grid;
+
+/* Grid::MakeGrid(origin, end, spacing): dims = ceil((end-origin)/spacing) in
+ * FP32 per axis (SPH2S_FINAL_SINGLE_SCALE/Grid.h:16-35). */
+int sph_make_grid(const float origin[3], const float end[3], float spacing,
+                  sph_grid* out);
+/* Grid::ComputeNumGridCells (SPH2S_FINAL_SINGLE_SCALE/Grid.h:11-14). */
+uint32_t sph_grid_num_cells(const sph_grid* grid);
+
+/* ---- configuration: replaces SolverConfiguration + __constant__ upload ---- */
+#define SPH_FLAG_WALL_CLAMP  1u  /* clamp x to [domain_lo, domain_hi], v *= -e */
+#define SPH_FLAG_KAHAN       2u  /* compensated density/force accumulation   */
+
+typedef struct sph_config {
+    float    h;              /* EffectiveRadius (Solver.h:51)            */
+    float    mass;           /* FluidParticleMass                        */
+    float    rest_density;   /* RestDensity                              */
+    float    stiffness;      /* BulkModulus: p = k (rho - rho0)          */
+    float    viscosity;      /* Viscosity (mu)                           */
+    float    tension;        /* TensionCoefficient (sigma)               */
+    float    gravity[3];     /* the reference hard-codes (0,-9.81,0)     */
+    float    restitution;    /* wall clamp restitution e                 */
+    float    domain_lo[3];   /* wall clamp box                           */
+    float    domain_hi[3];
+    sph_grid grid;           /* hashing grid (Solver.h:60)               */
+    uint32_t flags;          /* SPH_FLAG_*                               */
+} sph_config;
+
+/* Fill a configuration from explicit physical constants; the grid is
+ * MakeGrid(domain_lo, domain_hi, h); gravity (0,-9.81,0), tension 0,
+ * restitution 0.5, flags WALL_CLAMP|KAHAN. */
+int sph_config_init(sph_config* out, float h, float mass, float rest_density,
+                    float stiffness, float viscosity, const float domain_lo[3],
+                    const float domain_hi[3]);
+
+/* SolverConfiguration::MakeConfiguration (SPH2S_FINAL_SINGLE_SCALE/Solver.h:15-48):
+ * h = (0.75 V avg / (pi N))^(1/3), mass = rho0 V / N, grid over [origin,end]
+ * at spacing h.  speed_sound only matters for boundary particles and is
+ * ignored here. */
+int sph_make_configuration(const float origin[3], const float end[3],
+                           float volume, uint32_t num_particles,
+                           uint32_t avg_particles, float rest_density,
+                           float bulk_modulus, float viscosity,
+                           float speed_sound, float tension, sph_config* out);
+
+/* ---- scene generation (replaces ParticleData::CreateParticleBox,
+ *      SPH2S_FINAL_SINGLE_SCALE/ParticleData.cpp:141-176) ------------------- */
+/* Lattice of n[0]*n[1]*n[2] particles at lo + (i+1/2)*spacing, x fastest,
+ * with SplitMix64(seed) jitter U[-jitter_frac, jitter_frac)*spacing per axis.
+ * pos_xyz receives 3*n0*n1*n2 floats (AoS). */
+int sph_scene_lattice(const int32_t n[3], float spacing, const float lo[3],
+                      float jitter_frac, uint64_t seed, float* pos_xyz);
+/* n velocities with components N(0, sigma^2) (SplitMix64 + Box-Muller). */
+int sph_scene_gaussian_velocity(uint32_t n, float sigma, uint64_t seed,
+                                float* vel_xyz);
+
+/* ---- the single-scale solver --------------------------------------------- */
+typedef struct sph_solver sph_solver;
+
+/* Solver(fluid, boundary, cfg) (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1003-1061)
+ * constructed from host particle arrays: pos_xyz/vel_xyz are AoS 3*n floats
+ * (vel_xyz may be NULL = at rest).  Particle i gets persistent id i. */
+int sph_create(const sph_config* cfg, const float* pos_xyz,
+               const float* vel_xyz, uint32_t n, uint32_t max_n,
+               sph_solver** out);
+int sph_destroy(sph_solver* s);
+
+/* Solver::Advance(dt) (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1082-1096), run
+ * n_steps times: hash -> radix sort -> reorder + cell ranges -> density ->
+ * force + symplectic Euler + wall clamp.  The steps are replayed from a
+ * captured hipGraph on the solver's own stream; the call returns without
+ * waiting for the GPU. */
+int sph_advance(sph_solver* s, float dt, uint32_t n_steps);
+/* Same, launching each kernel directly instead of replaying the graph. */
+int sph_advance_eager(sph_solver* s, float dt, uint32_t n_steps);
+int sph_synchronize(sph_solver* s);
+
+/* Replace the particle state from host arrays (ids restart at 0..n-1). */
+int sph_set_state(sph_solver* s, const float* pos_xyz, const float* vel_xyz,
+                  uint32_t n);
+/* ParticleData::NumParticles. */
+int sph_num_particles(const sph_solver* s, uint32_t* n);
+int sph_get_config(const sph_solver* s, sph_config* out);
+
+/* Borrowed device views of the state, in the sorted order of the last step
+ * (the reference rewrites dPositions in sorted order each step,
+ * SPH2S_FINAL_SINGLE_SCALE/Solver.cu:748-754). */
+typedef struct sph_views {
+    void*     pos4;          /* float4[n]: x, y, z, w = persistent id bits */
+    void*     vel4;          /* float4[n]: vx, vy, vz, w = 0              */
+    float*    density;       /* float[n] rho of the last step             */
+    uint32_t* sorted_hash;   /* uint[n] cell hash, ascending              */
+    uint32_t* sorted_index;  /* uint[n] stable permutation of the last sort */
+    uint32_t* cell_start;    /* uint[num_cells], SPH_EMPTY_CELL if empty  */
+    uint32_t* cell_end;      /* uint[num_cells], SPH_EMPTY_CELL if empty  */
+    void*     accel4;        /* float4[n] accelerations (SPH_DEBUG_ACCEL) or NULL */
+    uint32_t  n;
+    uint32_t  num_cells;
+    void*     stream;        /* hipStream_t the solver launches on        */
+} sph_views;
+int sph_get_views(sph_solver* s, sph_views* out);
+
+/* Host copies of one field (synchronises the solver's stream).  bytes must be
+ * at least the field's size: POSITION/VELOCITY/ACCEL 12*n (AoS xyz),
+ * DENSITY/ID/SORTED_* 4*n, CELL_* 4*num_cells. */
+#define SPH_FIELD_POSITION      0
+#define SPH_FIELD_VELOCITY      1
+#define SPH_FIELD_DENSITY       2
+#define SPH_FIELD_ID            3
+#define SPH_FIELD_SORTED_HASH   4
+#define SPH_FIELD_SORTED_INDEX  5
+#define SPH_FIELD_CELL_START    6
+#define SPH_FIELD_CELL_END      7
+#define SPH_FIELD_ACCEL         8
+int sph_download(sph_solver* s, int field, void* host_dst, size_t bytes);
+
+/* Renderer contract: positions as 3 floats per particle
+ * (SPH2S_FINAL_TWO_SCALE/Renderer.cpp:40-42).  dst is a host pointer, or a
+ * device pointer when dst_is_device != 0 (then the copy is stream-ordered). */
+int sph_export_positions_aos(sph_solver* s, float* dst, int dst_is_device);
+
+#define SPH_DEBUG_ACCEL 1u   /* keep per-particle accelerations in accel4 */
+int sph_set_debug(sph_solver* s, uint32_t flags);
+
+/* Per-pass timing: runs n_steps steps eagerly with hipEvents recorded on the
+ * solver's stream between passes and adds each pass's milliseconds into
+ * ms_out[0..SPH_NUM_PASSES-1] (hash+cell reset, radix sort, reorder+cell
+ * ranges, density, force+integrate).  Synchronises after every step. */
+#define SPH_NUM_PASSES 5
+int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out);
+
+/* ---- standalone stable LSD radix sort (replaces thrust::sort_by_key,
+ *      SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115) ----------------------- */
+/* Temporary device bytes needed to sort n pairs. */
+size_t sph_radix_sort_temp_bytes(uint32_t n);
+/* Stable sort of (key, value) pairs by the low key_bits bits of the key
+ * (1..32).  vals_in == NULL means values 0..n-1.  All pointers are device
+ * pointers; keys_out/vals_out must not alias the inputs.  stream is a
+ * hipStream_t (NULL = default stream).  Asynchronous. */
+int sph_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
+                         uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                         uint32_t key_bits, void* temp, size_t temp_bytes,
+                         void* stream);
+
+/* ---- diagnostics ----------------------------------------------------------- */
+const char* sph_last_error(void);
+/* Number of visible HIP devices (0 on a host without a GPU; never fails). */
+int sph_device_count(void);
+const char* sph_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH2S_H_ */

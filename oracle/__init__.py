@@ -1,0 +1,206 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes wrapper of the host oracle (liboracle.so).
+
+The oracle is a CPU restatement of the reference SPH2S step (FP32 hashing with
+the reference's IEEE operation order, FP64 physics; see sph_oracle.h).  Only
+tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use it, and
+only as the checker / the timed CPU baseline -- never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, Structure, c_double, c_float, c_int, c_int32, c_uint32, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OracleGrid(Structure):
+    _fields_ = [("origin", c_float * 3), ("dims", c_int32 * 3), ("spacing", c_float)]
+
+
+class OracleParams(Structure):
+    _fields_ = [
+        ("h", c_double), ("mass", c_double), ("rest_density", c_double),
+        ("stiffness", c_double), ("viscosity", c_double), ("tension", c_double),
+        ("gravity", c_double * 3), ("restitution", c_double),
+        ("lo", c_double * 3), ("hi", c_double * 3),
+        ("wall_clamp", c_int32), ("h_f32", c_float), ("grid", OracleGrid),
+    ]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = POINTER(OracleParams)
+        vp = c_void_p
+        L.oracle_hash.argtypes = [POINTER(OracleGrid), vp, c_uint32, vp]
+        L.oracle_stable_sort.argtypes = [vp, c_uint32, vp, vp]
+        L.oracle_cell_ranges.argtypes = [vp, c_uint32, c_uint32, vp, vp]
+        L.oracle_step.argtypes = [P, vp, vp, c_uint32, c_double, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_rollout.argtypes = [P, vp, vp, c_uint32, c_double, c_uint32, vp, vp, vp]
+        L.oracle_density_bruteforce.argtypes = [P, vp, c_uint32, vp]
+        L.oracle_accel_bruteforce.argtypes = [P, vp, vp, vp, c_uint32, vp]
+        for f in ("oracle_poly6", "oracle_spiky_grad_mag", "oracle_visc_laplacian"):
+            getattr(L, f).argtypes = [c_double, c_double]
+            getattr(L, f).restype = c_double
+        L.cpu_solver_create.argtypes = [P, vp, vp, c_uint32]
+        L.cpu_solver_create.restype = vp
+        L.cpu_solver_step.argtypes = [vp, c_float, c_uint32]
+        L.cpu_solver_get.argtypes = [vp, vp, vp, vp, vp]
+        L.cpu_solver_threads.restype = c_int
+        L.cpu_solver_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(c_void_p)
+
+
+def params_from_config(cfg) -> OracleParams:
+    """Oracle parameters from a library configuration (sph_config)."""
+    p = OracleParams()
+    p.h = float(cfg.h)
+    p.mass = float(cfg.mass)
+    p.rest_density = float(cfg.rest_density)
+    p.stiffness = float(cfg.stiffness)
+    p.viscosity = float(cfg.viscosity)
+    p.tension = float(cfg.tension)
+    for d in range(3):
+        p.gravity[d] = float(cfg.gravity[d])
+        p.lo[d] = float(cfg.domain_lo[d])
+        p.hi[d] = float(cfg.domain_hi[d])
+        p.grid.origin[d] = cfg.grid.origin[d]
+        p.grid.dims[d] = cfg.grid.dims[d]
+    p.grid.spacing = cfg.grid.spacing
+    p.restitution = float(cfg.restitution)
+    p.wall_clamp = 1 if (cfg.flags & 1) else 0
+    p.h_f32 = cfg.h
+    return p
+
+
+def num_cells(p: OracleParams) -> int:
+    return int(p.grid.dims[0]) * int(p.grid.dims[1]) * int(p.grid.dims[2])
+
+
+def hash_positions(p: OracleParams, pos: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32)
+    out = np.empty(pos.shape[0], np.uint32)
+    lib().oracle_hash(ctypes.byref(p.grid), _p(pos), pos.shape[0], _p(out))
+    return out
+
+
+def stable_sort(keys: np.ndarray):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    ks = np.empty_like(keys)
+    perm = np.empty_like(keys)
+    lib().oracle_stable_sort(_p(keys), keys.size, _p(ks), _p(perm))
+    return ks, perm
+
+
+def cell_ranges(sorted_keys: np.ndarray, cells: int):
+    sk = np.ascontiguousarray(sorted_keys, np.uint32)
+    st = np.empty(cells, np.uint32)
+    en = np.empty(cells, np.uint32)
+    lib().oracle_cell_ranges(_p(sk), sk.size, cells, _p(st), _p(en))
+    return st, en
+
+
+def step(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float) -> dict:
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    n = pos.shape[0]
+    C = num_cells(p)
+    out = dict(hash=np.empty(n, np.uint32), perm=np.empty(n, np.uint32),
+               cell_start=np.empty(C, np.uint32), cell_end=np.empty(C, np.uint32),
+               rho=np.empty(n), acc=np.empty((n, 3)), pos=np.empty((n, 3)), vel=np.empty((n, 3)))
+    lib().oracle_step(ctypes.byref(p), _p(pos), _p(vel), n, float(dt), _p(out["hash"]),
+                      _p(out["perm"]), _p(out["cell_start"]), _p(out["cell_end"]), _p(out["rho"]),
+                      _p(out["acc"]), _p(out["pos"]), _p(out["vel"]))
+    return out
+
+
+def rollout(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, steps: int):
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    n = pos.shape[0]
+    x = np.empty((n, 3))
+    v = np.empty((n, 3))
+    ids = np.empty(n, np.uint32)
+    lib().oracle_rollout(ctypes.byref(p), _p(pos), _p(vel), n, float(dt), int(steps), _p(x),
+                         _p(v), _p(ids))
+    return x, v, ids
+
+
+def density_bruteforce(p: OracleParams, pos: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float64)
+    out = np.empty(pos.shape[0])
+    lib().oracle_density_bruteforce(ctypes.byref(p), _p(pos), pos.shape[0], _p(out))
+    return out
+
+
+def accel_bruteforce(p: OracleParams, pos, vel, rho) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float64)
+    vel = np.ascontiguousarray(vel, np.float64)
+    rho = np.ascontiguousarray(rho, np.float64)
+    out = np.empty((pos.shape[0], 3))
+    lib().oracle_accel_bruteforce(ctypes.byref(p), _p(pos), _p(vel), _p(rho), pos.shape[0],
+                                  _p(out))
+    return out
+
+
+def poly6(r: float, h: float) -> float:
+    return lib().oracle_poly6(r, h)
+
+
+def spiky_grad_mag(r: float, h: float) -> float:
+    return lib().oracle_spiky_grad_mag(r, h)
+
+
+def visc_laplacian(r: float, h: float) -> float:
+    return lib().oracle_visc_laplacian(r, h)
+
+
+class CpuSolver:
+    """FP32 OpenMP CPU step (the bench's CPU baseline)."""
+
+    def __init__(self, p: OracleParams, pos: np.ndarray, vel: np.ndarray):
+        self._pos = np.ascontiguousarray(pos, np.float32)
+        self._vel = np.ascontiguousarray(vel, np.float32)
+        self.n = self._pos.shape[0]
+        self._h = lib().cpu_solver_create(ctypes.byref(p), _p(self._pos), _p(self._vel), self.n)
+
+    def step(self, dt: float, steps: int = 1) -> None:
+        lib().cpu_solver_step(self._h, float(dt), int(steps))
+
+    def state(self):
+        x = np.empty((self.n, 3), np.float32)
+        v = np.empty((self.n, 3), np.float32)
+        r = np.empty(self.n, np.float32)
+        ids = np.empty(self.n, np.uint32)
+        lib().cpu_solver_get(self._h, _p(x), _p(v), _p(r), _p(ids))
+        return x, v, r, ids
+
+    @staticmethod
+    def threads() -> int:
+        return int(lib().cpu_solver_threads())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().cpu_solver_destroy(self._h)
+            self._h = None

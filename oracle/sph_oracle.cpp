@@ -1,0 +1,538 @@
+// sph_oracle.cpp -- TEST INFRASTRUCTURE ONLY: host restatement of the
+// reference SPH2S step (see sph_oracle.h for the precision contract).
+// Compiled with -ffp-contract=off so that the FP32 cell-coordinate expressions
+// round exactly like the reference's separate IEEE operations.
+
+#include "sph_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr double kPi = 3.14159265358979323846;  // Solver.h:11 (M_PI)
+
+// computeCoordinatesOff (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:20-38):
+// c = clamp((int)((x + off - o)/spacing), 0, D-1), all in FP32.
+inline int coord_off(float x, float off, float o, float sp, int dim) {
+    float t = x + off;
+    t = t - o;
+    t = t / sp;
+    int c = static_cast<int>(t);
+    return std::max(0, std::min(c, dim - 1));
+}
+
+// computeCoordinates (Solver.cu:40-56): (int)((x - o)/spacing), clamped.
+inline int coord(float x, float o, float sp, int dim) {
+    float t = x - o;
+    t = t / sp;
+    int c = static_cast<int>(t);
+    return std::max(0, std::min(c, dim - 1));
+}
+
+// computeHash (Solver.cu:58-68): cx + Dx*(cy + Dy*cz).
+inline uint32_t hash_of(const oracle_grid& g, int cx, int cy, int cz) {
+    return static_cast<uint32_t>(cx + g.dims[0] * (cy + g.dims[1] * cz));
+}
+
+inline uint32_t hash_pos(const oracle_grid& g, float x, float y, float z) {
+    return hash_of(g, coord(x, g.origin[0], g.spacing, g.dims[0]),
+                   coord(y, g.origin[1], g.spacing, g.dims[1]),
+                   coord(z, g.origin[2], g.spacing, g.dims[2]));
+}
+
+uint32_t num_cells(const oracle_grid& g) {
+    return static_cast<uint32_t>(g.dims[0]) * static_cast<uint32_t>(g.dims[1]) *
+           static_cast<uint32_t>(g.dims[2]);
+}
+
+// Kernel coefficients (Solver.cu:103-167): Poly6 315/(64 pi h^9),
+// spiky gradient -45/(pi h^6)(h-r)^2 r_hat, viscosity Laplacian 45/(pi h^6)(h-r).
+struct Coeffs {
+    double c6, c45;
+    explicit Coeffs(double h) {
+        const double h3 = h * h * h;
+        c6 = 315.0 / (64.0 * kPi * h3 * h3 * h3);
+        c45 = 45.0 / (kPi * h3 * h3);
+    }
+};
+
+// Particle state in sorted slot order, FP64.
+struct State {
+    std::vector<double> x, v;     // AoS 3n
+    std::vector<uint32_t> id;     // persistent ids
+};
+
+struct Neighbourhood {
+    std::vector<uint32_t> hash_sorted, perm, start, end;
+};
+
+// computeNeighborhoods (Solver.cu:1098-1140): hash -> stable sort ->
+// cell start/end -> reorder.  Hashes are taken from the FP32 value of the
+// position, as the reference state is FP32.
+void build_neighbourhood(const oracle_params& p, State& s, Neighbourhood& nb) {
+    const uint32_t n = static_cast<uint32_t>(s.id.size());
+    std::vector<uint32_t> hash(n);
+    for (uint32_t i = 0; i < n; ++i)
+        hash[i] = hash_pos(p.grid, static_cast<float>(s.x[3 * i]),
+                           static_cast<float>(s.x[3 * i + 1]),
+                           static_cast<float>(s.x[3 * i + 2]));
+    nb.hash_sorted.resize(n);
+    nb.perm.resize(n);
+    oracle_stable_sort(hash.data(), n, nb.hash_sorted.data(), nb.perm.data());
+    const uint32_t C = num_cells(p.grid);
+    nb.start.assign(C, kEmpty);
+    nb.end.assign(C, kEmpty);
+    oracle_cell_ranges(nb.hash_sorted.data(), n, C, nb.start.data(), nb.end.data());
+    // reorderAndComputeCellStartEndD (Solver.cu:478-534): gather x, v by perm.
+    State t;
+    t.x.resize(3 * static_cast<size_t>(n));
+    t.v.resize(3 * static_cast<size_t>(n));
+    t.id.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t j = nb.perm[i];
+        for (int d = 0; d < 3; ++d) {
+            t.x[3 * i + d] = s.x[3 * j + d];
+            t.v[3 * i + d] = s.v[3 * j + d];
+        }
+        t.id[i] = s.id[j];
+    }
+    s = std::move(t);
+}
+
+// The 27-cell (cs..ce inclusive) stencil of Solver.cu:559-593: bounds from the
+// FP32 position offset by -h/+h.
+template <class F>
+void for_each_stencil_cell(const oracle_params& p, const double* xi, F&& f) {
+    const oracle_grid& g = p.grid;
+    const float x = static_cast<float>(xi[0]), y = static_cast<float>(xi[1]),
+                z = static_cast<float>(xi[2]);
+    const float h = p.h_f32;
+    const int csx = coord_off(x, -h, g.origin[0], g.spacing, g.dims[0]);
+    const int csy = coord_off(y, -h, g.origin[1], g.spacing, g.dims[1]);
+    const int csz = coord_off(z, -h, g.origin[2], g.spacing, g.dims[2]);
+    const int cex = coord_off(x, h, g.origin[0], g.spacing, g.dims[0]);
+    const int cey = coord_off(y, h, g.origin[1], g.spacing, g.dims[1]);
+    const int cez = coord_off(z, h, g.origin[2], g.spacing, g.dims[2]);
+    for (int cz = csz; cz <= cez; ++cz)
+        for (int cy = csy; cy <= cey; ++cy)
+            for (int cx = csx; cx <= cex; ++cx) f(hash_of(g, cx, cy, cz));
+}
+
+// computeDensitiesPressuresD + computeDensityCell (Solver.cu:198-229,536-599):
+// rho_i = m sum_{j: r < h} W_poly6(r), self included.
+void densities(const oracle_params& p, const State& s, const Neighbourhood& nb,
+               std::vector<double>& rho) {
+    const uint32_t n = static_cast<uint32_t>(s.id.size());
+    const Coeffs k(p.h);
+    const double h2 = p.h * p.h;
+    rho.assign(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* xi = &s.x[3 * i];
+        double sum = 0.0;
+        for_each_stencil_cell(p, xi, [&](uint32_t c) {
+            const uint32_t b = nb.start[c], e = nb.end[c];
+            if (b == kEmpty) return;
+            for (uint32_t j = b; j < e; ++j) {
+                const double dx = xi[0] - s.x[3 * j], dy = xi[1] - s.x[3 * j + 1],
+                             dz = xi[2] - s.x[3 * j + 2];
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (r < p.h) {
+                    const double a = h2 - r * r;
+                    sum += k.c6 * a * a * a;
+                }
+            }
+        });
+        rho[i] = p.mass * sum;
+    }
+}
+
+// Pairwise acceleration term of computeAccelerationCell (Solver.cu:231-309):
+// pressure (spiky gradient), viscosity (Laplacian) and tension (Poly6), for
+// 0 < r < h.  Accumulates the force fi that is later divided by rho_i.
+inline void pair_force(const oracle_params& p, const Coeffs& k, const double* xi,
+                       const double* vi, double rhoi, double pi,
+                       const double* xj, const double* vj, double rhoj,
+                       double* fi) {
+    const double xij[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+    const double r = std::sqrt(xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2]);
+    if (!(r != 0.0 && r < p.h)) return;
+    const double pj = p.stiffness * (rhoj - p.rest_density);
+    const double coeffP = -rhoi * p.mass * (pi / (rhoi * rhoi) + pj / (rhoj * rhoj));
+    const double gradA = -k.c45 * (p.h - r) * (p.h - r);   // spiky gradient magnitude
+    const double coeffV = p.viscosity * p.mass / rhoj;
+    const double lapl = k.c45 * (p.h - r);
+    const double a = p.h * p.h - r * r;
+    const double coeffT = -(k.c6 * a * a * a) * p.mass * p.tension;
+    for (int d = 0; d < 3; ++d) {
+        fi[d] += coeffP * gradA * xij[d] / r;
+        fi[d] += coeffV * (vj[d] - vi[d]) * lapl;
+        fi[d] += coeffT * xij[d];
+    }
+}
+
+// computeAccelerationsD (Solver.cu:601-711) without boundary particles:
+// a_i = f_i / rho_i + g.
+void accelerations(const oracle_params& p, const State& s, const Neighbourhood& nb,
+                   const std::vector<double>& rho, std::vector<double>& acc) {
+    const uint32_t n = static_cast<uint32_t>(s.id.size());
+    const Coeffs k(p.h);
+    acc.assign(3 * static_cast<size_t>(n), 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* xi = &s.x[3 * i];
+        const double* vi = &s.v[3 * i];
+        const double rhoi = rho[i];
+        const double pi = p.stiffness * (rhoi - p.rest_density);
+        double fi[3] = {0.0, 0.0, 0.0};
+        for_each_stencil_cell(p, xi, [&](uint32_t c) {
+            const uint32_t b = nb.start[c], e = nb.end[c];
+            if (b == kEmpty) return;
+            for (uint32_t j = b; j < e; ++j)
+                pair_force(p, k, xi, vi, rhoi, pi, &s.x[3 * j], &s.v[3 * j], rho[j], fi);
+        });
+        for (int d = 0; d < 3; ++d) acc[3 * i + d] = fi[d] / rhoi + p.gravity[d];
+    }
+}
+
+// integrateD (Solver.cu:713-755): v += dt a; x += dt v; plus the wall clamp
+// of the benchmark scenes: outside [lo, hi] on an axis, x is put on the wall
+// and that velocity component is reflected with restitution e.
+void integrate(const oracle_params& p, State& s, const std::vector<double>& acc,
+               double dt) {
+    const size_t n = s.id.size();
+    for (size_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            double v = s.v[3 * i + d] + dt * acc[3 * i + d];
+            double x = s.x[3 * i + d] + dt * v;
+            if (p.wall_clamp) {
+                if (x < p.lo[d]) { x = p.lo[d]; v = -p.restitution * v; }
+                if (x > p.hi[d]) { x = p.hi[d]; v = -p.restitution * v; }
+            }
+            s.v[3 * i + d] = v;
+            s.x[3 * i + d] = x;
+        }
+    }
+}
+
+State make_state(const float* pos, const float* vel, uint32_t n) {
+    State s;
+    s.x.resize(3 * static_cast<size_t>(n));
+    s.v.assign(3 * static_cast<size_t>(n), 0.0);
+    s.id.resize(n);
+    for (size_t i = 0; i < 3 * static_cast<size_t>(n); ++i) {
+        s.x[i] = pos[i];
+        if (vel) s.v[i] = vel[i];
+    }
+    std::iota(s.id.begin(), s.id.end(), 0u);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_hash(const oracle_grid* g, const float* pos, uint32_t n, uint32_t* out) {
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = hash_pos(*g, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+}
+
+void oracle_stable_sort(const uint32_t* keys, uint32_t n, uint32_t* keys_out,
+                        uint32_t* perm_out) {
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    for (uint32_t i = 0; i < n; ++i) {
+        perm_out[i] = idx[i];
+        keys_out[i] = keys[idx[i]];
+    }
+}
+
+void oracle_cell_ranges(const uint32_t* k, uint32_t n, uint32_t C, uint32_t* start,
+                        uint32_t* end) {
+    for (uint32_t c = 0; c < C; ++c) start[c] = end[c] = kEmpty;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (i == 0 || k[i] != k[i - 1]) {
+            start[k[i]] = i;
+            if (i > 0) end[k[i - 1]] = i;
+        }
+        if (i == n - 1) end[k[i]] = n;
+    }
+}
+
+void oracle_step(const oracle_params* p, const float* pos, const float* vel, uint32_t n,
+                 double dt, uint32_t* hash_sorted, uint32_t* perm, uint32_t* cell_start,
+                 uint32_t* cell_end, double* rho, double* acc, double* pos_out,
+                 double* vel_out) {
+    State s = make_state(pos, vel, n);
+    Neighbourhood nb;
+    build_neighbourhood(*p, s, nb);
+    std::vector<double> r, a;
+    densities(*p, s, nb, r);
+    accelerations(*p, s, nb, r, a);
+    integrate(*p, s, a, dt);
+    const uint32_t C = num_cells(p->grid);
+    if (hash_sorted) std::copy(nb.hash_sorted.begin(), nb.hash_sorted.end(), hash_sorted);
+    if (perm) std::copy(nb.perm.begin(), nb.perm.end(), perm);
+    if (cell_start) std::copy(nb.start.begin(), nb.start.begin() + C, cell_start);
+    if (cell_end) std::copy(nb.end.begin(), nb.end.begin() + C, cell_end);
+    if (rho) std::copy(r.begin(), r.end(), rho);
+    if (acc) std::copy(a.begin(), a.end(), acc);
+    if (pos_out) std::copy(s.x.begin(), s.x.end(), pos_out);
+    if (vel_out) std::copy(s.v.begin(), s.v.end(), vel_out);
+}
+
+void oracle_rollout(const oracle_params* p, const float* pos, const float* vel,
+                    uint32_t n, double dt, uint32_t n_steps, double* pos_out,
+                    double* vel_out, uint32_t* ids_out) {
+    State s = make_state(pos, vel, n);
+    Neighbourhood nb;
+    std::vector<double> r, a;
+    for (uint32_t t = 0; t < n_steps; ++t) {
+        build_neighbourhood(*p, s, nb);
+        densities(*p, s, nb, r);
+        accelerations(*p, s, nb, r, a);
+        integrate(*p, s, a, dt);
+    }
+    if (pos_out) std::copy(s.x.begin(), s.x.end(), pos_out);
+    if (vel_out) std::copy(s.v.begin(), s.v.end(), vel_out);
+    if (ids_out) std::copy(s.id.begin(), s.id.end(), ids_out);
+}
+
+void oracle_density_bruteforce(const oracle_params* p, const double* x, uint32_t n,
+                               double* rho) {
+    const Coeffs k(p->h);
+    for (uint32_t i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (uint32_t j = 0; j < n; ++j) {
+            const double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1],
+                         dz = x[3 * i + 2] - x[3 * j + 2];
+            const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (r < p->h) {
+                const double a = p->h * p->h - r * r;
+                sum += k.c6 * a * a * a;
+            }
+        }
+        rho[i] = p->mass * sum;
+    }
+}
+
+void oracle_accel_bruteforce(const oracle_params* p, const double* x, const double* v,
+                             const double* rho, uint32_t n, double* acc) {
+    const Coeffs k(p->h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double rhoi = rho[i];
+        const double pi = p->stiffness * (rhoi - p->rest_density);
+        double fi[3] = {0.0, 0.0, 0.0};
+        for (uint32_t j = 0; j < n; ++j)
+            pair_force(*p, k, &x[3 * i], &v[3 * i], rhoi, pi, &x[3 * j], &v[3 * j], rho[j], fi);
+        for (int d = 0; d < 3; ++d) acc[3 * i + d] = fi[d] / rhoi + p->gravity[d];
+    }
+}
+
+double oracle_poly6(double r, double h) {
+    if (!(r < h)) return 0.0;
+    const Coeffs k(h);
+    const double a = h * h - r * r;
+    return k.c6 * a * a * a;
+}
+
+double oracle_spiky_grad_mag(double r, double h) {
+    if (r == 0.0 || r > h) return 0.0;
+    const Coeffs k(h);
+    return -k.c45 * (h - r) * (h - r);
+}
+
+double oracle_visc_laplacian(double r, double h) {
+    if (!(r < h)) return 0.0;
+    const Coeffs k(h);
+    return k.c45 * (h - r);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FP32 OpenMP CPU step (the CPU baseline).  Same algorithm as the GPU path:
+// FP32 hash, a stable counting sort by cell (which yields the cell ranges),
+// reorder, density, force fused with symplectic Euler and the wall clamp.
+// ---------------------------------------------------------------------------
+struct cpu_solver {
+    oracle_params p;
+    uint32_t n = 0, C = 0;
+    std::vector<float> x, v, tx, tv, rho;
+    std::vector<uint32_t> id, tid, hash, count, start, end, perm;
+};
+
+extern "C" {
+
+cpu_solver* cpu_solver_create(const oracle_params* p, const float* pos, const float* vel,
+                              uint32_t n) {
+    auto* s = new cpu_solver;
+    s->p = *p;
+    s->n = n;
+    s->C = num_cells(p->grid);
+    s->x.assign(pos, pos + 3 * static_cast<size_t>(n));
+    s->v.assign(3 * static_cast<size_t>(n), 0.0f);
+    if (vel) s->v.assign(vel, vel + 3 * static_cast<size_t>(n));
+    s->id.resize(n);
+    std::iota(s->id.begin(), s->id.end(), 0u);
+    s->tx.resize(3 * static_cast<size_t>(n));
+    s->tv.resize(3 * static_cast<size_t>(n));
+    s->tid.resize(n);
+    s->rho.resize(n);
+    s->hash.resize(n);
+    s->perm.resize(n);
+    s->count.resize(s->C + 1);
+    s->start.resize(s->C);
+    s->end.resize(s->C);
+    return s;
+}
+
+void cpu_solver_step(cpu_solver* s, float dt, uint32_t n_steps) {
+    const oracle_params& p = s->p;
+    const uint32_t n = s->n, C = s->C;
+    const float h = p.h_f32, h2 = h * h;
+    const Coeffs kd(static_cast<double>(h));
+    const float c6m = static_cast<float>(kd.c6 * p.mass);
+    const float c45 = static_cast<float>(kd.c45);
+    const float m = static_cast<float>(p.mass), k = static_cast<float>(p.stiffness),
+                rho0 = static_cast<float>(p.rest_density), mu = static_cast<float>(p.viscosity),
+                sig = static_cast<float>(p.tension), e = static_cast<float>(p.restitution);
+    const float c6 = static_cast<float>(kd.c6);
+    const float g[3] = {static_cast<float>(p.gravity[0]), static_cast<float>(p.gravity[1]),
+                        static_cast<float>(p.gravity[2])};
+    const float lo[3] = {static_cast<float>(p.lo[0]), static_cast<float>(p.lo[1]),
+                         static_cast<float>(p.lo[2])};
+    const float hi[3] = {static_cast<float>(p.hi[0]), static_cast<float>(p.hi[1]),
+                         static_cast<float>(p.hi[2])};
+    for (uint32_t step = 0; step < n_steps; ++step) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < static_cast<int64_t>(n); ++i)
+            s->hash[i] = hash_pos(p.grid, s->x[3 * i], s->x[3 * i + 1], s->x[3 * i + 2]);
+        // stable counting sort by cell
+        std::fill(s->count.begin(), s->count.end(), 0u);
+        for (uint32_t i = 0; i < n; ++i) ++s->count[s->hash[i] + 1];
+        for (uint32_t c = 0; c < C; ++c) s->count[c + 1] += s->count[c];
+#pragma omp parallel for schedule(static)
+        for (int64_t c = 0; c < static_cast<int64_t>(C); ++c) {
+            const bool empty = s->count[c] == s->count[c + 1];
+            s->start[c] = empty ? kEmpty : s->count[c];
+            s->end[c] = empty ? kEmpty : s->count[c + 1];
+        }
+        for (uint32_t i = 0; i < n; ++i) s->perm[s->count[s->hash[i]]++] = i;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) {
+            const uint32_t j = s->perm[i];
+            for (int d = 0; d < 3; ++d) {
+                s->tx[3 * i + d] = s->x[3 * j + d];
+                s->tv[3 * i + d] = s->v[3 * j + d];
+            }
+            s->tid[i] = s->id[j];
+        }
+        auto stencil = [&](const float* xi, auto&& f) {
+            const oracle_grid& gr = p.grid;
+            const int csx = coord_off(xi[0], -h, gr.origin[0], gr.spacing, gr.dims[0]);
+            const int csy = coord_off(xi[1], -h, gr.origin[1], gr.spacing, gr.dims[1]);
+            const int csz = coord_off(xi[2], -h, gr.origin[2], gr.spacing, gr.dims[2]);
+            const int cex = coord_off(xi[0], h, gr.origin[0], gr.spacing, gr.dims[0]);
+            const int cey = coord_off(xi[1], h, gr.origin[1], gr.spacing, gr.dims[1]);
+            const int cez = coord_off(xi[2], h, gr.origin[2], gr.spacing, gr.dims[2]);
+            for (int cz = csz; cz <= cez; ++cz)
+                for (int cy = csy; cy <= cey; ++cy) {
+                    uint32_t b = kEmpty, en = 0;
+                    for (int cx = csx; cx <= cex; ++cx) {
+                        const uint32_t c = hash_of(gr, cx, cy, cz);
+                        if (s->start[c] != kEmpty) {
+                            b = std::min(b, s->start[c]);
+                            en = std::max(en, s->end[c]);
+                        }
+                    }
+                    for (uint32_t j = b; j < en; ++j) f(j);
+                }
+        };
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) {
+            const float* xi = &s->tx[3 * i];
+            float sum = 0.0f;
+            stencil(xi, [&](uint32_t j) {
+                const float dx = xi[0] - s->tx[3 * j], dy = xi[1] - s->tx[3 * j + 1],
+                            dz = xi[2] - s->tx[3 * j + 2];
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < h2) {
+                    const float a = h2 - r2;
+                    sum += a * a * a;
+                }
+            });
+            s->rho[i] = c6m * sum;
+        }
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) {
+            const float* xi = &s->tx[3 * i];
+            const float* vi = &s->tv[3 * i];
+            const float rhoi = s->rho[i];
+            const float Ai = k * (rhoi - rho0) / (rhoi * rhoi);
+            float P[3] = {0, 0, 0}, V[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+            stencil(xi, [&](uint32_t j) {
+                const float d[3] = {xi[0] - s->tx[3 * j], xi[1] - s->tx[3 * j + 1],
+                                    xi[2] - s->tx[3 * j + 2]};
+                const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (r2 < h2 && r2 > 0.0f) {
+                    const float r = std::sqrt(r2);
+                    const float hr = h - r;
+                    const float rhoj = s->rho[j];
+                    const float Aj = k * (rhoj - rho0) / (rhoj * rhoj);
+                    const float tp = (Ai + Aj) * hr * hr / r;
+                    const float tv = hr / rhoj;
+                    const float a = h2 - r2;
+                    const float a3 = a * a * a;
+                    for (int q = 0; q < 3; ++q) {
+                        P[q] += tp * d[q];
+                        V[q] += tv * (s->tv[3 * j + q] - vi[q]);
+                        T[q] += a3 * d[q];
+                    }
+                }
+            });
+            for (int q = 0; q < 3; ++q) {
+                const float acc = m * c45 * P[q] + mu * m * c45 / rhoi * V[q] -
+                                  sig * m * c6 / rhoi * T[q] + g[q];
+                float vq = vi[q] + dt * acc;
+                float xq = xi[q] + dt * vq;
+                if (p.wall_clamp) {
+                    if (xq < lo[q]) { xq = lo[q]; vq = -e * vq; }
+                    if (xq > hi[q]) { xq = hi[q]; vq = -e * vq; }
+                }
+                s->x[3 * i + q] = xq;
+                s->v[3 * i + q] = vq;
+            }
+        }
+        std::swap(s->id, s->tid);
+    }
+}
+
+void cpu_solver_get(const cpu_solver* s, float* pos_out, float* vel_out, float* rho_out,
+                    uint32_t* ids_out) {
+    if (pos_out) std::copy(s->x.begin(), s->x.end(), pos_out);
+    if (vel_out) std::copy(s->v.begin(), s->v.end(), vel_out);
+    if (rho_out) std::copy(s->rho.begin(), s->rho.end(), rho_out);
+    if (ids_out) std::copy(s->id.begin(), s->id.end(), ids_out);
+}
+
+int cpu_solver_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void cpu_solver_destroy(cpu_solver* s) { delete s; }
+
+}  // extern "C"

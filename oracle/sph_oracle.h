@@ -1,0 +1,103 @@
+/*
+ * sph_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Host (CPU) restatement of the reference SPH2S solver step, used as the
+ * parity checker of the HIP library and as the CPU baseline of the bench.
+ * Nothing in the product (sph2s_final_amd/, include/) links or calls it.
+ *
+ * Precision: hashing and cell coordinates are computed in FP32 with the exact
+ * IEEE operation order of the reference (subtract, add, divide, truncate,
+ * clamp; no FMA contraction -- this file is compiled with -ffp-contract=off),
+ * because the reference computes them in FP32
+ * (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:20-83).  Everything else (density,
+ * pressure, forces, integration, wall clamp) is FP64.
+ *
+ * Every function names the reference lines it restates.  How each one is
+ * pinned independently of this file (closed forms, brute force, invariants)
+ * is documented in tests/test_oracle.py.
+ */
+#ifndef SPH_ORACLE_H_
+#define SPH_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_grid {
+    float   origin[3];
+    int32_t dims[3];
+    float   spacing;
+} oracle_grid;
+
+typedef struct oracle_params {
+    double  h, mass, rest_density, stiffness, viscosity, tension;
+    double  gravity[3];
+    double  restitution;
+    double  lo[3], hi[3];
+    int32_t wall_clamp;
+    float   h_f32;          /* the FP32 h used for stencil offsets          */
+    oracle_grid grid;
+} oracle_params;
+
+/* Cell hash of each particle (FP32 positions, AoS 3n).
+ * SPH2S_FINAL_SINGLE_SCALE/Solver.cu:40-83 (computeCoordinates, computeHash). */
+void oracle_hash(const oracle_grid* g, const float* pos, uint32_t n,
+                 uint32_t* hash_out);
+
+/* Stable sort of indices 0..n-1 by key (ties keep input order); the
+ * semantics of thrust::sort_by_key(hash, ids) with ids = 0..n-1
+ * (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115, ids from :416). */
+void oracle_stable_sort(const uint32_t* keys, uint32_t n, uint32_t* keys_out,
+                        uint32_t* perm_out);
+
+/* Cell start/end from sorted keys; SPH_EMPTY (0xFFFFFFFF) for empty cells.
+ * SPH2S_FINAL_SINGLE_SCALE/Solver.cu:478-534, 1118-1127. */
+void oracle_cell_ranges(const uint32_t* sorted_keys, uint32_t n,
+                        uint32_t num_cells, uint32_t* start, uint32_t* end);
+
+/* One full FP64 step from an FP32 input state (AoS 3n) -- Solver::Advance,
+ * SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1082-1198 plus the wall clamp.
+ * Outputs are in sorted order: hash_sorted[n], perm[n], cell_start/end[C],
+ * rho[n], acc[3n] (FP64), pos_out/vel_out[3n] (FP64, after integration). */
+void oracle_step(const oracle_params* p, const float* pos, const float* vel,
+                 uint32_t n, double dt, uint32_t* hash_sorted, uint32_t* perm,
+                 uint32_t* cell_start, uint32_t* cell_end, double* rho,
+                 double* acc, double* pos_out, double* vel_out);
+
+/* n_steps FP64 steps.  State is kept in FP64 and in sorted order, as the
+ * reference rewrites its buffers in sorted order each step; ids_out[i] is the
+ * input index of the particle now at slot i. */
+void oracle_rollout(const oracle_params* p, const float* pos, const float* vel,
+                    uint32_t n, double dt, uint32_t n_steps, double* pos_out,
+                    double* vel_out, uint32_t* ids_out);
+
+/* O(n^2) density and acceleration over all pairs (no grid).  Pins neighbour
+ * completeness of the grid path. */
+void oracle_density_bruteforce(const oracle_params* p, const double* pos,
+                               uint32_t n, double* rho);
+void oracle_accel_bruteforce(const oracle_params* p, const double* pos,
+                             const double* vel, const double* rho, uint32_t n,
+                             double* acc);
+
+/* Kernel functions (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:103-167) in FP64. */
+double oracle_poly6(double r, double h);
+double oracle_spiky_grad_mag(double r, double h);   /* d/dr of spiky W */
+double oracle_visc_laplacian(double r, double h);
+
+/* ---- FP32 OpenMP CPU step (the CPU baseline the bench times) ---------- */
+typedef struct cpu_solver cpu_solver;
+cpu_solver* cpu_solver_create(const oracle_params* p, const float* pos,
+                              const float* vel, uint32_t n);
+void cpu_solver_step(cpu_solver* s, float dt, uint32_t n_steps);
+void cpu_solver_get(const cpu_solver* s, float* pos_out, float* vel_out,
+                    float* rho_out, uint32_t* ids_out);
+int  cpu_solver_threads(void);
+void cpu_solver_destroy(cpu_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

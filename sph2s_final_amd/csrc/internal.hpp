@@ -1,0 +1,90 @@
+// internal.hpp -- declarations shared by the translation units of libsph2s.so.
+// Not installed; the public surface is include/sph2s.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/sph2s.h"
+
+namespace sph {
+
+constexpr uint32_t kEmptyCell = SPH_EMPTY_CELL;
+
+// Records msg as the calling thread's last error and returns code.
+int set_error(int code, const std::string& msg);
+int hip_error(hipError_t e, const char* what, const char* file, int line);
+
+#define SPH_HIP(call)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) return ::sph::hip_error(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// Everything the step kernels need, passed by value as a kernel argument (the
+// reference uploads an equivalent struct to __constant__ memory,
+// SPH2S_FINAL_SINGLE_SCALE/Solver.cu:12,1068-1080).  Kernel coefficients are
+// folded in double on the host and rounded to float once
+// (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:103-167 computes them with the double
+// literal M_PI).
+struct StepParams {
+    float ox, oy, oz;       // grid origin
+    int   dx, dy, dz;       // grid dims
+    float spacing;          // cell edge
+    float h, h2;            // effective radius and its square
+    float rest_density, stiffness;
+    float c6m;              // mass * 315/(64 pi h^9): rho = c6m * sum (h^2-r^2)^3
+    float c45m;             // mass * 45/(pi h^6): pressure acceleration factor
+    float visc;             // mu * mass * 45/(pi h^6)
+    float tens;             // sigma * mass * 315/(64 pi h^9)
+    float gx, gy, gz;
+    float restitution;
+    float lox, loy, loz, hix, hiy, hiz;
+    uint32_t flags;         // SPH_FLAG_*
+    uint32_t n;             // particle count
+    uint32_t num_cells;
+};
+
+StepParams make_step_params(const sph_config& cfg, uint32_t n);
+uint32_t key_bits_for_cells(uint32_t num_cells);
+
+// ---- radix sort (radix_sort.hip) ----
+size_t radix_sort_temp_bytes(uint32_t n);
+// Stable LSD sort of (key, value) pairs by the low key_bits of key.
+// vals_in == nullptr means values 0..n-1.  Launches on stream.
+int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
+                     uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                     uint32_t key_bits, void* temp, size_t temp_bytes,
+                     hipStream_t stream);
+
+// ---- single-scale step kernels (step.hip) ----
+struct StepBuffers {
+    float4* pos;        // state, sorted order of the previous step (w = id bits)
+    float4* vel;
+    float4* tpos;       // sorted copy used by the neighbour passes
+    float4* tvel;
+    float*  rho;
+    float4* accel;      // optional (debug)
+    uint32_t* keys;     // unsorted cell hashes
+    uint32_t* skeys;    // sorted cell hashes
+    uint32_t* perm;     // sorted slot -> previous slot
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    void*  sort_temp;
+    size_t sort_temp_bytes;
+};
+
+// Passes of one step, in launch order.
+enum StepPass { kPassHash = 0, kPassSort, kPassReorder, kPassDensity, kPassForce, kNumPasses };
+int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt, uint32_t key_bits,
+                bool write_accel, hipStream_t stream);
+int launch_step(const StepParams& p, const StepBuffers& b, float dt, uint32_t key_bits,
+                bool write_accel, hipStream_t stream);
+int launch_pack_state(const float* pos_xyz_dev, const float* vel_xyz_dev, uint32_t n,
+                      float4* pos, float4* vel, hipStream_t stream);
+int launch_export_aos(const float4* src, uint32_t n, float* dst_xyz, hipStream_t stream);
+
+}  // namespace sph

@@ -1,0 +1,87 @@
+"""Benchmark scenes: seeded dam-break lattices with fixed physical constants.
+
+Shared constants: h = 0.0457, lattice spacing s = h/2, rho0 = 1000,
+m = rho0 s^3, stiffness k = 3, viscosity mu = 3.5, tension 0,
+g = (0, -9.81, 0), dt = 1e-3, wall clamp with restitution 0.5, grid cell = h.
+A lattice of n = (nx, ny, nz) particles sits at (i + 1/2) s from the domain
+minimum with SplitMix64 jitter U[-0.01 s, 0.01 s) per axis; the domain is
+[0, n_d / 32] on each axis, so the 32^3 lattice fills [0, 0.731]^3 of the unit
+box and larger lattices keep that block-to-domain ratio.
+
+Named workloads:
+  dam_break_32k  32^3 = 32,768 particles, domain [0, 1]^3, seed 1
+  dam_break_1m   100^3 = 1,000,000 particles, domain [0, 3.125]^3, seed 2
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from ctypes import c_int32, c_void_p
+
+import numpy as np
+
+from . import _capi as C
+from .solver import SolverConfiguration
+
+H = 0.0457
+SPACING = H / 2.0
+REST_DENSITY = 1000.0
+STIFFNESS = 3.0
+VISCOSITY = 3.5
+DT = 1e-3
+JITTER = 0.01
+REFERENCE_LATTICE = 32  # lattice edge that fills the unit domain
+
+
+@dataclass
+class Scene:
+    name: str
+    config: C.SphConfig
+    positions: np.ndarray   # (N, 3) float32
+    velocities: np.ndarray  # (N, 3) float32
+    dt: float = DT
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+
+def particle_mass(spacing: float = SPACING, rest_density: float = REST_DENSITY) -> float:
+    return float(np.float32(rest_density * spacing ** 3))
+
+
+def lattice(n, spacing=SPACING, lo=(0.0, 0.0, 0.0), jitter=JITTER, seed=1) -> np.ndarray:
+    n = [int(v) for v in n]
+    out = np.empty((n[0] * n[1] * n[2], 3), np.float32)
+    C.check(C.lib().sph_scene_lattice((c_int32 * 3)(*n), float(spacing), C.f3(lo), float(jitter),
+                                      int(seed), out.ctypes.data_as(c_void_p)))
+    return out
+
+
+def gaussian_velocities(count: int, sigma: float, seed: int) -> np.ndarray:
+    out = np.empty((int(count), 3), np.float32)
+    C.check(C.lib().sph_scene_gaussian_velocity(int(count), float(sigma), int(seed),
+                                                out.ctypes.data_as(c_void_p)))
+    return out
+
+
+def dam_break(n=(32, 32, 32), seed=1, *, stiffness=STIFFNESS, viscosity=VISCOSITY,
+              velocity_sigma=0.0, velocity_seed=None, name=None, kahan=True) -> Scene:
+    n = tuple(int(v) for v in n)
+    hi = tuple(v / REFERENCE_LATTICE for v in n)
+    cfg = SolverConfiguration.FromConstants(H, particle_mass(), REST_DENSITY, stiffness,
+                                            viscosity, (0.0, 0.0, 0.0), hi, kahan=kahan)
+    pos = lattice(n, seed=seed)
+    if velocity_sigma > 0.0:
+        vseed = (seed * 0x9E3779B1 + 7) if velocity_seed is None else velocity_seed
+        vel = gaussian_velocities(pos.shape[0], velocity_sigma, vseed)
+    else:
+        vel = np.zeros_like(pos)
+    return Scene(name or f"dam_break_{n[0]}x{n[1]}x{n[2]}", cfg, pos, vel)
+
+
+def dam_break_32k() -> Scene:
+    return dam_break((32, 32, 32), seed=1, name="dam_break_32k")
+
+
+def dam_break_1m() -> Scene:
+    return dam_break((100, 100, 100), seed=2, name="dam_break_1m")

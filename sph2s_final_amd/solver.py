@@ -1,0 +1,240 @@
+"""Host-side mirror of the reference's simulation API over the C ABI.
+
+Names follow the reference (SPH2S_FINAL_SINGLE_SCALE/Solver.h, ParticleData.h,
+Grid.h): ``Grid.MakeGrid``, ``SolverConfiguration.MakeConfiguration``,
+``ParticleData`` with ``NumParticles``/``MaxParticles``/``dPositions`` and
+``Solver(fluid, boundary, configuration).Advance(dt)``.  Errors raise
+``SPHError`` the way ``CUDA::SafeCall`` throws std::runtime_error
+(SPH2S_FINAL_TWO_SCALE/cuda.h:115-129).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_uint32, c_void_p
+
+import numpy as np
+
+from . import _capi as C
+
+
+def _f32(a, shape_last=3) -> np.ndarray:
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if arr.ndim == 1:
+        arr = arr.reshape(-1, shape_last)
+    return arr
+
+
+class Grid:
+    """struct Grid (SPH2S_FINAL_SINGLE_SCALE/Grid.h:9-59)."""
+
+    @staticmethod
+    def MakeGrid(origin, end, spacing: float) -> C.SphGrid:
+        g = C.SphGrid()
+        C.check(C.lib().sph_make_grid(C.f3(origin), C.f3(end), float(spacing), byref(g)))
+        return g
+
+    @staticmethod
+    def ComputeNumGridCells(grid: C.SphGrid) -> int:
+        return int(C.lib().sph_grid_num_cells(byref(grid)))
+
+
+class SolverConfiguration:
+    """SolverConfiguration (SPH2S_FINAL_SINGLE_SCALE/Solver.h:13-61)."""
+
+    @staticmethod
+    def MakeConfiguration(origin, end, volume, numParticles, avgParticles, restDensity,
+                          bulkModulus, viscosity, speedSound, tensionCoefficient,
+                          epsilon: float = 0.0) -> C.SphConfig:
+        """Solver.h:15-48: h from the average neighbour count, m = rho0 V / N."""
+        del epsilon  # XSPH coefficient: the XSPH integrator is disabled in the reference
+        cfg = C.SphConfig()
+        C.check(C.lib().sph_make_configuration(
+            C.f3(origin), C.f3(end), float(volume), int(numParticles), int(avgParticles),
+            float(restDensity), float(bulkModulus), float(viscosity), float(speedSound),
+            float(tensionCoefficient), byref(cfg)))
+        return cfg
+
+    @staticmethod
+    def FromConstants(h, mass, rest_density, stiffness, viscosity, domain_lo, domain_hi, *,
+                      tension=0.0, gravity=(0.0, -9.81, 0.0), restitution=0.5,
+                      wall_clamp=True, kahan=True) -> C.SphConfig:
+        """Explicit constants (the benchmark scenes give h and m directly)."""
+        cfg = C.SphConfig()
+        C.check(C.lib().sph_config_init(byref(cfg), float(h), float(mass), float(rest_density),
+                                        float(stiffness), float(viscosity), C.f3(domain_lo),
+                                        C.f3(domain_hi)))
+        cfg.tension = float(tension)
+        for d in range(3):
+            cfg.gravity[d] = float(gravity[d])
+        cfg.restitution = float(restitution)
+        cfg.flags = (C.SPH_FLAG_WALL_CLAMP if wall_clamp else 0) | (C.SPH_FLAG_KAHAN if kahan else 0)
+        return cfg
+
+
+class ParticleData:
+    """ParticleData (SPH2S_FINAL_SINGLE_SCALE/ParticleData.h:9-53).
+
+    Holds host arrays until a Solver takes it; from then on ``dPositions`` /
+    ``dDensities`` are the solver's device buffers (float4 x,y,z,id and float).
+    """
+
+    def __init__(self, maxParticles: int, positions=None, velocities=None):
+        self.MaxParticles = int(maxParticles)
+        self.positions = None if positions is None else _f32(positions)
+        self.velocities = None if velocities is None else _f32(velocities)
+        self.NumParticles = 0 if self.positions is None else int(self.positions.shape[0])
+        if self.NumParticles > self.MaxParticles:
+            raise ValueError("NumParticles exceeds MaxParticles")
+        self._solver: "Solver | None" = None
+
+    @staticmethod
+    def CreateParticleBox(grid: C.SphGrid) -> "ParticleData":
+        """A lattice at origin + (i, j, k) * spacing, x fastest (ParticleData.cpp:141-176)."""
+        nx, ny, nz = (int(grid.dims[d]) for d in range(3))
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        o = np.array([grid.origin[d] for d in range(3)], dtype=np.float32)
+        sp = np.float32(grid.spacing)
+        pos = np.stack([o[0] + i.ravel().astype(np.float32) * sp,
+                        o[1] + j.ravel().astype(np.float32) * sp,
+                        o[2] + k.ravel().astype(np.float32) * sp], axis=1).astype(np.float32)
+        return ParticleData(pos.shape[0], pos)
+
+    @staticmethod
+    def Union(a: "ParticleData", b: "ParticleData") -> "ParticleData":
+        """Concatenation of two particle sets (ParticleData.cpp:68-139)."""
+        if a is None or b is None:
+            return None
+        pos = np.concatenate([a.host_positions(), b.host_positions()])
+        vel = np.concatenate([a.host_velocities(), b.host_velocities()])
+        return ParticleData(a.MaxParticles + b.MaxParticles, pos, vel)
+
+    def host_positions(self) -> np.ndarray:
+        if self._solver is not None:
+            return self._solver.positions()
+        return np.zeros((0, 3), np.float32) if self.positions is None else self.positions
+
+    def host_velocities(self) -> np.ndarray:
+        if self._solver is not None:
+            return self._solver.velocities()
+        if self.velocities is None:
+            return np.zeros((self.NumParticles, 3), np.float32)
+        return self.velocities
+
+    @property
+    def dPositions(self) -> int:
+        if self._solver is None:
+            raise RuntimeError("ParticleData is not bound to a Solver")
+        return self._solver.views().pos4
+
+    @property
+    def dDensities(self) -> int:
+        if self._solver is None:
+            raise RuntimeError("ParticleData is not bound to a Solver")
+        return self._solver.views().density
+
+
+class Solver:
+    """Solver (SPH2S_FINAL_SINGLE_SCALE/Solver.h:109-132) on one MI355X.
+
+    ``boundaryData`` is accepted for signature parity; boundary-particle walls
+    are not part of this path (walls are the configuration's clamp box).
+    """
+
+    def __init__(self, fluidData: ParticleData, boundaryData=None,
+                 configuration: C.SphConfig | None = None):
+        if configuration is None:
+            raise ValueError("configuration is required")
+        if boundaryData is not None and getattr(boundaryData, "NumParticles", 0):
+            raise NotImplementedError("boundary particles are not supported; use the wall clamp")
+        self._h = c_void_p()
+        pos = fluidData.positions if fluidData.positions is not None else np.zeros((0, 3), np.float32)
+        vel = fluidData.velocities
+        n = int(pos.shape[0])
+        C.check(C.lib().sph_create(byref(configuration),
+                                   pos.ctypes.data_as(c_void_p) if n else None,
+                                   vel.ctypes.data_as(c_void_p) if vel is not None else None,
+                                   n, int(fluidData.MaxParticles), byref(self._h)))
+        self.configuration = configuration
+        self.fluid = fluidData
+        fluidData._solver = self
+
+    # -- reference API --
+    def Bind(self) -> None:
+        """No-op: the configuration travels as a kernel argument, not a global."""
+
+    def Advance(self, dt: float, steps: int = 1) -> None:
+        C.check(C.lib().sph_advance(self._h, float(dt), int(steps)))
+
+    # -- extras --
+    def advance_eager(self, dt: float, steps: int = 1) -> None:
+        C.check(C.lib().sph_advance_eager(self._h, float(dt), int(steps)))
+
+    def synchronize(self) -> None:
+        C.check(C.lib().sph_synchronize(self._h))
+
+    def set_debug(self, flags: int) -> None:
+        C.check(C.lib().sph_set_debug(self._h, int(flags)))
+
+    def set_state(self, positions, velocities=None) -> None:
+        pos = _f32(positions)
+        vel = None if velocities is None else _f32(velocities)
+        C.check(C.lib().sph_set_state(self._h, pos.ctypes.data_as(c_void_p),
+                                      None if vel is None else vel.ctypes.data_as(c_void_p),
+                                      int(pos.shape[0])))
+        self.fluid.NumParticles = int(pos.shape[0])
+
+    @property
+    def NumParticles(self) -> int:
+        n = c_uint32()
+        C.check(C.lib().sph_num_particles(self._h, byref(n)))
+        return int(n.value)
+
+    def views(self) -> C.SphViews:
+        v = C.SphViews()
+        C.check(C.lib().sph_get_views(self._h, byref(v)))
+        return v
+
+    def download(self, field: int) -> np.ndarray:
+        n = self.NumParticles
+        cells = int(self.views().num_cells)
+        if field in (C.FIELD_POSITION, C.FIELD_VELOCITY, C.FIELD_ACCEL):
+            out = np.empty((n, 3), np.float32)
+        elif field == C.FIELD_DENSITY:
+            out = np.empty(n, np.float32)
+        elif field in (C.FIELD_ID, C.FIELD_SORTED_HASH, C.FIELD_SORTED_INDEX):
+            out = np.empty(n, np.uint32)
+        elif field in (C.FIELD_CELL_START, C.FIELD_CELL_END):
+            out = np.empty(cells, np.uint32)
+        else:
+            raise ValueError(f"unknown field {field}")
+        C.check(C.lib().sph_download(self._h, int(field), out.ctypes.data_as(c_void_p), out.nbytes))
+        return out
+
+    def positions(self) -> np.ndarray:
+        return self.download(C.FIELD_POSITION)
+
+    def velocities(self) -> np.ndarray:
+        return self.download(C.FIELD_VELOCITY)
+
+    def densities(self) -> np.ndarray:
+        return self.download(C.FIELD_DENSITY)
+
+    def ids(self) -> np.ndarray:
+        return self.download(C.FIELD_ID)
+
+    def close(self) -> None:
+        if self._h:
+            C.lib().sph_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,150 @@
+"""GPU parity of the single-scale step against the host oracle (through the C ABI).
+
+Bars (FP32 GPU vs FP64 oracle on the same FP32 input state):
+  * cell hashes, the stable sorted permutation and cell start/end: bit-exact;
+  * density: max_i |rho_i - rho*_i| / rho*_i <= 1e-5;
+  * acceleration: max_i |a_i - a*_i| / max_i |a*_i| <= 1e-5;
+  * one integrated step: positions within 1e-6 absolute (a dt^2-scaled
+    acceleration error), velocities within 1e-5 of max |v*|;
+  * 10-step rollout (compared per persistent id): positions within 1e-5
+    absolute, velocities within 1e-3 of max |v*|, densities 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from sph2s_final_amd import (FIELD_ACCEL, FIELD_CELL_END, FIELD_CELL_START, FIELD_SORTED_HASH,
+                             FIELD_SORTED_INDEX, SPH_DEBUG_ACCEL, ParticleData, Solver, scenes)
+
+pytestmark = pytest.mark.gpu
+
+RHO_TOL = 1e-5
+ACC_TOL = 1e-5
+
+
+def _solver(sc, debug=True):
+    s = Solver(ParticleData(sc.n, sc.positions, sc.velocities), None, sc.config)
+    if debug:
+        s.set_debug(SPH_DEBUG_ACCEL)
+    return s
+
+
+def _one_step(sc, eager=False):
+    with _solver(sc) as s:
+        if eager:
+            s.advance_eager(sc.dt)
+        else:
+            s.Advance(sc.dt)
+        s.synchronize()
+        return dict(hash=s.download(FIELD_SORTED_HASH), perm=s.download(FIELD_SORTED_INDEX),
+                    cell_start=s.download(FIELD_CELL_START), cell_end=s.download(FIELD_CELL_END),
+                    rho=s.densities(), acc=s.download(FIELD_ACCEL), pos=s.positions(),
+                    vel=s.velocities(), ids=s.ids())
+
+
+def _check_step(sc, got, ref):
+    print(f"case {sc.name}: n={sc.n}", flush=True)
+    assert np.array_equal(got["hash"], ref["hash"])
+    assert np.array_equal(got["perm"], ref["perm"])
+    assert np.array_equal(got["cell_start"], ref["cell_start"])
+    assert np.array_equal(got["cell_end"], ref["cell_end"])
+    assert np.array_equal(got["ids"], ref["perm"])  # ids follow the sorted order
+    rho_err = np.max(np.abs(got["rho"] - ref["rho"]) / ref["rho"])
+    acc_err = np.max(np.abs(got["acc"] - ref["acc"])) / np.max(np.abs(ref["acc"]))
+    pos_err = np.max(np.abs(got["pos"] - ref["pos"]))
+    vel_err = np.max(np.abs(got["vel"] - ref["vel"])) / np.max(np.abs(ref["vel"]))
+    print(f"  rho {rho_err:.3e}  acc {acc_err:.3e}  pos {pos_err:.3e}  vel {vel_err:.3e}",
+          flush=True)
+    assert rho_err <= RHO_TOL
+    assert acc_err <= ACC_TOL
+    assert pos_err <= 1e-6
+    assert vel_err <= 1e-5
+
+
+@pytest.mark.parametrize("shape,seed", [((8, 8, 8), 3), ((16, 16, 16), 1), ((32, 32, 32), 1),
+                                        ((24, 12, 40), 5)])
+def test_single_step_parity(shape, seed):
+    sc = scenes.dam_break(shape, seed=seed)
+    ref = O.step(O.params_from_config(sc.config), sc.positions, sc.velocities, sc.dt)
+    _check_step(sc, _one_step(sc), ref)
+
+
+def test_single_step_parity_moving_state():
+    # Non-zero initial velocities exercise the viscosity term from step one.
+    sc = scenes.dam_break((20, 20, 20), seed=9, velocity_sigma=0.5)
+    ref = O.step(O.params_from_config(sc.config), sc.positions, sc.velocities, sc.dt)
+    _check_step(sc, _one_step(sc), ref)
+
+
+def test_single_step_parity_with_tension():
+    sc = scenes.dam_break((16, 16, 16), seed=4, velocity_sigma=0.2)
+    sc.config.tension = 0.8
+    ref = O.step(O.params_from_config(sc.config), sc.positions, sc.velocities, sc.dt)
+    _check_step(sc, _one_step(sc), ref)
+
+
+def test_graph_equals_eager_bitwise():
+    sc = scenes.dam_break((16, 16, 16), seed=2, velocity_sigma=0.1)
+    a = _one_step(sc, eager=False)
+    b = _one_step(sc, eager=True)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_rollout_10_steps():
+    sc = scenes.dam_break((16, 16, 16), seed=1)
+    steps = 10
+    x_ref, v_ref, ids_ref = O.rollout(O.params_from_config(sc.config), sc.positions,
+                                      sc.velocities, sc.dt, steps)
+    with _solver(sc, debug=False) as s:
+        s.Advance(sc.dt, steps)
+        s.synchronize()
+        x, v, ids = s.positions(), s.velocities(), s.ids()
+    order_ref = np.argsort(ids_ref)
+    order = np.argsort(ids)
+    dx = np.max(np.abs(x[order] - x_ref[order_ref]))
+    dv = np.max(np.abs(v[order] - v_ref[order_ref])) / np.max(np.abs(v_ref))
+    print(f"rollout {steps} steps: pos {dx:.3e}  vel {dv:.3e}", flush=True)
+    assert dx <= 1e-5
+    assert dv <= 1e-3
+
+
+def test_one_million_particle_step_integer_parity():
+    sc = scenes.dam_break_1m()
+    ref = O.step(O.params_from_config(sc.config), sc.positions, sc.velocities, sc.dt)
+    got = _one_step(sc)
+    _check_step(sc, got, ref)
+
+
+def test_repeated_advance_keeps_sorted_invariants():
+    sc = scenes.dam_break((32, 32, 32), seed=1)
+    with _solver(sc, debug=False) as s:
+        s.Advance(sc.dt, 50)
+        s.synchronize()
+        h = s.download(FIELD_SORTED_HASH)
+        st = s.download(FIELD_CELL_START)
+        en = s.download(FIELD_CELL_END)
+        ids = s.ids()
+        pos = s.positions()
+    assert np.all(np.diff(h.astype(np.int64)) >= 0)
+    assert np.array_equal(np.sort(ids), np.arange(sc.n, dtype=np.uint32))
+    nonempty = st != 0xFFFFFFFF
+    assert np.array_equal(nonempty, en != 0xFFFFFFFF)
+    assert int(np.sum(en[nonempty] - st[nonempty])) == sc.n
+    assert np.all(np.isfinite(pos))
+    lo = np.array([sc.config.domain_lo[d] for d in range(3)], np.float32)
+    hi = np.array([sc.config.domain_hi[d] for d in range(3)], np.float32)
+    assert np.all(pos >= lo) and np.all(pos <= hi)
+
+
+def test_export_positions_aos_matches_download():
+    import torch
+    sc = scenes.dam_break((12, 12, 12), seed=6)
+    with _solver(sc, debug=False) as s:
+        s.Advance(sc.dt, 3)
+        s.synchronize()
+        dst = torch.empty(sc.n * 3, dtype=torch.float32, device="cuda")
+        from sph2s_final_amd import _capi as C
+        C.check(C.lib().sph_export_positions_aos(s._h, dst.data_ptr(), 1))
+        s.synchronize()
+        assert np.array_equal(dst.cpu().numpy().reshape(-1, 3), s.positions())
