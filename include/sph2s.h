@@ -176,6 +176,13 @@ int sph_set_debug(sph_solver* s, uint32_t flags);
 #define SPH_NUM_PASSES 5
 int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out);
 
+/* Device-side error bits raised since the solver was created or last
+ * queried (then cleared); synchronises the solver's stream.
+ * SPH_DEVERR_SORT_WAIT: a radix-sort look-back wait exceeded its bound and
+ * gave up, so the step that raised it has an invalid ordering. */
+#define SPH_DEVERR_SORT_WAIT 1u
+int sph_device_errors(sph_solver* s, uint32_t* bits);
+
 /* ---- standalone stable LSD radix sort (replaces thrust::sort_by_key,
  *      SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115) ----------------------- */
 /* Temporary device bytes needed to sort n pairs. */

@@ -99,6 +99,7 @@ SIGNATURES = [
     ("sph_download", c_int, [c_void_p, c_int, c_void_p, c_size_t]),
     ("sph_export_positions_aos", c_int, [c_void_p, c_void_p, c_int]),
     ("sph_set_debug", c_int, [c_void_p, c_uint32]),
+    ("sph_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_profile_passes", c_int, [c_void_p, c_float, c_uint32, POINTER(c_float)]),
     ("sph_radix_sort_temp_bytes", c_size_t, [c_uint32]),
     ("sph_radix_sort_pairs", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32,

@@ -334,7 +334,8 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
         align_up(16 * N), align_up(16 * N), align_up(16 * N), align_up(16 * N),  // pos vel tpos tvel
         align_up(4 * N),  align_up(16 * N),                                      // rho accel
         align_up(4 * N),  align_up(4 * N),  align_up(4 * N),                     // keys skeys perm
-        align_up(4 * C),  align_up(4 * C),  align_up(sort_temp)};                // cells, sort temp
+        align_up(4 * C),  align_up(4 * C),  align_up(sort_temp),                 // cells, sort temp
+        align_up(8 * N)};                                                        // aux
     size_t total = 0;
     for (size_t b : sizes) total += b;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -367,6 +368,7 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
     s->buf.cell_start = static_cast<uint32_t*>(take(9));
     s->buf.cell_end = static_cast<uint32_t*>(take(10));
     s->buf.sort_temp = take(11);
+    s->buf.aux = static_cast<float2*>(take(12));
     s->buf.sort_temp_bytes = sort_temp;
 
     e = hipMemsetAsync(s->arena, 0, total, s->stream);
@@ -490,6 +492,20 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+int sph_device_errors(sph_solver* s, uint32_t* bits) {
+    if (!s || !bits) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t* word = radix_sort_error_word(s->buf.sort_temp);
+    uint32_t v = 0;
+    SPH_HIP(hipMemcpyAsync(&v, word, sizeof(v), hipMemcpyDeviceToHost, s->stream));
+    SPH_HIP(hipStreamSynchronize(s->stream));
+    if (v) {
+        SPH_HIP(hipMemsetAsync(word, 0, sizeof(v), s->stream));
+        SPH_HIP(hipStreamSynchronize(s->stream));
+    }
+    *bits = v ? SPH_DEVERR_SORT_WAIT : 0u;
+    return SPH_OK;
 }
 
 int sph_set_debug(sph_solver* s, uint32_t flags) {

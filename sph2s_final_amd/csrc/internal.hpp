@@ -51,14 +51,53 @@ struct StepParams {
 StepParams make_step_params(const sph_config& cfg, uint32_t n);
 uint32_t key_bits_for_cells(uint32_t num_cells);
 
+// computeCoordinates / computeHash (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:20-83)
+// in FP32: clamp((int)((x + off - o) / spacing), 0, D-1) and
+// cx + Dx (cy + Dy cz).  The library is built with
+// -fhip-fp32-correctly-rounded-divide-sqrt, so the division is IEEE and hashes
+// and stencil bounds are bit-identical to the FP32 reference arithmetic.
+__device__ __forceinline__ int cell_coord_off(float x, float off, float o, float sp, int dim) {
+    float t = __fadd_rn(x, off);
+    t = __fsub_rn(t, o);
+    t = __fdiv_rn(t, sp);
+    const int c = static_cast<int>(t);
+    return max(0, min(c, dim - 1));
+}
+
+__device__ __forceinline__ int cell_coord(float x, float o, float sp, int dim) {
+    float t = __fsub_rn(x, o);
+    t = __fdiv_rn(t, sp);
+    const int c = static_cast<int>(t);
+    return max(0, min(c, dim - 1));
+}
+
+__device__ __forceinline__ uint32_t cell_hash(float x, float y, float z, float ox, float oy,
+                                              float oz, float sp, int dx, int dy, int dz) {
+    const int cx = cell_coord(x, ox, sp, dx);
+    const int cy = cell_coord(y, oy, sp, dy);
+    const int cz = cell_coord(z, oz, sp, dz);
+    return static_cast<uint32_t>(cx + dx * (cy + dy * cz));
+}
+
 // ---- radix sort (radix_sort.hip) ----
 size_t radix_sort_temp_bytes(uint32_t n);
+// Device word set to non-zero if a look-back wait of the sort gave up.
+uint32_t* radix_sort_error_word(void* temp);
 // Stable LSD sort of (key, value) pairs by the low key_bits of key.
 // vals_in == nullptr means values 0..n-1.  Launches on stream.
 int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
                      uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
                      uint32_t key_bits, void* temp, size_t temp_bytes,
                      hipStream_t stream);
+
+// Zeroes the sort state, resets cell_start/cell_end to SPH_EMPTY_CELL and
+// hashes pos into keys while building the digit histograms.
+int hash_hist_cells(const StepParams& p, const float4* pos, uint32_t* keys, uint32_t key_bits,
+                    void* temp, size_t temp_bytes, uint32_t* cell_start, uint32_t* cell_end,
+                    hipStream_t stream);
+// The digit passes: (keys, slot) -> (skeys, perm).
+int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint32_t* perm,
+               uint32_t key_bits, void* temp, hipStream_t stream);
 
 // ---- single-scale step kernels (step.hip) ----
 struct StepBuffers {
@@ -67,6 +106,7 @@ struct StepBuffers {
     float4* tpos;       // sorted copy used by the neighbour passes
     float4* tvel;
     float*  rho;
+    float2* aux;        // per particle (k (rho - rho0) / rho^2, 1 / rho)
     float4* accel;      // optional (debug)
     uint32_t* keys;     // unsorted cell hashes
     uint32_t* skeys;    // sorted cell hashes

@@ -1,0 +1,98 @@
+// neighbors.hpp -- device-side neighbour enumeration shared by the density,
+// force and adjoint kernels.
+//
+// The stencil bounds are the reference's: cells clamp((int)((x -/+ h - o)/D))
+// per axis (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:559-593).  Within one grid row
+// (cy, cz) the cells cx are consecutive hashes, so the selected cells of a row
+// form one contiguous run [b, e) of sorted particles.  Rows and cells that
+// cannot hold a particle closer than h are skipped: a row whose y/z distance
+// to the particle is at least h, and, inside a row, the cells outside
+// x +/- sqrt(h^2 - d_yz^2).  The test uses h (1 + 1e-3), far above the
+// rounding of positions and cell bounds, so no particle with r < h is ever
+// skipped and the neighbour set {j : r_ij^2 < h^2} is unchanged; only the
+// number of candidates tested drops.
+#pragma once
+
+#include "internal.hpp"
+
+namespace sph {
+
+struct GridView {
+    float ox, oy, oz, spacing, inv_spacing;
+    int dx, dy, dz;
+    float h;            // search radius
+    float prune_h2;     // (h (1 + 1e-3))^2
+};
+
+__host__ __device__ inline GridView make_grid_view(float ox, float oy, float oz, float spacing,
+                                                   int dx, int dy, int dz, float h) {
+    GridView g;
+    g.ox = ox;
+    g.oy = oy;
+    g.oz = oz;
+    g.spacing = spacing;
+    g.inv_spacing = 1.0f / spacing;
+    g.dx = dx;
+    g.dy = dy;
+    g.dz = dz;
+    g.h = h;
+    const float ph = h * 1.001f;
+    g.prune_h2 = ph * ph;
+    return g;
+}
+
+// Distance from f to the interval [c*sp, (c+1)*sp].
+__device__ __forceinline__ float slab_gap(float f, int c, float sp) {
+    const float lo = static_cast<float>(c) * sp;
+    const float hi = lo + sp;
+    return fmaxf(0.0f, fmaxf(lo - f, f - hi));
+}
+
+// Contiguous particle run [b, e) covering cells lo..hi of a row starting at
+// hash `row`.  Cells with a start always have an end <= n; min() keeps the
+// loop bound in range regardless.
+__device__ __forceinline__ void row_run(const uint32_t* __restrict__ cs,
+                                        const uint32_t* __restrict__ ce, uint32_t row, int lo,
+                                        int hi, uint32_t n, uint32_t& b, uint32_t& e) {
+    b = kEmptyCell;
+    e = 0u;
+    for (int cx = lo; cx <= hi; ++cx) {
+        const uint32_t st = cs[row + cx];
+        if (st != kEmptyCell) {
+            b = min(b, st);
+            e = max(e, ce[row + cx]);
+        }
+    }
+    e = min(e, n);
+}
+
+// Calls f(b, e) for every non-empty candidate run around (x, y, z).
+template <class F>
+__device__ __forceinline__ void for_each_run(const GridView& g, const uint32_t* __restrict__ cs,
+                                             const uint32_t* __restrict__ ce, uint32_t n, float x,
+                                             float y, float z, F&& f) {
+    const int sx = cell_coord_off(x, -g.h, g.ox, g.spacing, g.dx);
+    const int sy = cell_coord_off(y, -g.h, g.oy, g.spacing, g.dy);
+    const int sz = cell_coord_off(z, -g.h, g.oz, g.spacing, g.dz);
+    const int ex = cell_coord_off(x, g.h, g.ox, g.spacing, g.dx);
+    const int ey = cell_coord_off(y, g.h, g.oy, g.spacing, g.dy);
+    const int ez = cell_coord_off(z, g.h, g.oz, g.spacing, g.dz);
+    const float fx = x - g.ox, fy = y - g.oy, fz = z - g.oz;
+    for (int cz = sz; cz <= ez; ++cz) {
+        const float gz = slab_gap(fz, cz, g.spacing);
+        for (int cy = sy; cy <= ey; ++cy) {
+            const float gy = slab_gap(fy, cy, g.spacing);
+            const float d2 = gy * gy + gz * gz;
+            if (d2 >= g.prune_h2) continue;
+            const float w = __builtin_sqrtf(g.prune_h2 - d2);
+            const int lo = max(sx, static_cast<int>(floorf((fx - w) * g.inv_spacing)));
+            const int hi = min(ex, static_cast<int>(floorf((fx + w) * g.inv_spacing)));
+            if (lo > hi) continue;
+            uint32_t b, e;
+            row_run(cs, ce, static_cast<uint32_t>(g.dx) * (cy + g.dy * cz), lo, hi, n, b, e);
+            if (b < e) f(b, e);
+        }
+    }
+}
+
+}  // namespace sph

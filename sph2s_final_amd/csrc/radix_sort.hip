@@ -2,16 +2,22 @@
 // for gfx950.  Replaces thrust::sort_by_key(hash, ids)
 // (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115).
 //
-// One pass per 8-bit digit (fewer bits in the last pass), reduce-then-scan:
-//   upsweep   per-tile digit histogram in LDS -> hist[digit][tile]
-//   scan      exclusive scan of hist (digit-major, so tiles of one digit are
-//             contiguous and earlier tiles come first: this is what makes the
-//             sort stable across tiles)
-//   downsweep re-reads the tile, ranks each key among equal digits with
-//             wave64 ballots (lanes in key order), scatters to
-//             hist[digit][tile] + rank.
-// A tile is 4096 keys: 4 waves x 16 rounds x 64 lanes; wave w owns a
-// contiguous run of 1024 keys so the in-tile order is (wave, round, lane).
+// Single-sweep scheme, 8-bit digits (fewer bits in the last pass):
+//   k_prepare   zeroes the digit histograms, tile tickets and look-back
+//               status words (and, for the solver, resets the cell ranges)
+//   k_hist      one read of the keys -> LDS digit histograms of every pass,
+//               added into the global per-pass histograms
+//   k_pass      one kernel per digit: a block takes a tile ticket (so tiles
+//               start in order), ranks its keys among equal digits with wave64
+//               ballots, publishes its per-digit counts, looks back over the
+//               status words of earlier tiles for its exclusive prefix
+//               (decoupled look-back), and scatters.
+// A tile is kBlock*kItems keys: wave w owns a contiguous run of 64*kItems
+// keys, so the in-tile order is (wave, round, lane) and the sort is stable.
+// Every look-back wait is bounded: a wait that exceeds kSpinLimit polls sets
+// an error bit and gives up instead of hanging.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace sph {
@@ -19,14 +25,22 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kItems = 16;
+constexpr int kItems = 8;
 constexpr int kWaveKeys = 64 * kItems;
 constexpr int kTile = kBlock * kItems;
-constexpr int kMaxRadix = 256;
-constexpr int kScanThreads = 1024;
+constexpr int kRadix = 256;
+constexpr int kMaxPasses = 4;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kCountMask = (1u << 30) - 1u;
+constexpr uint32_t kSpinLimit = 1u << 22;
 
-// Lanes (among valid ones) whose digit equals this lane's, via one ballot per
-// digit bit.
+// Header of the temporary buffer, in 32-bit words.
+constexpr uint32_t kHistWords = kMaxPasses * kRadix;        // global histograms
+constexpr uint32_t kCounterWords = 64;                      // tickets + error word
+constexpr uint32_t kHeaderWords = kHistWords + kCounterWords;
+constexpr uint32_t kErrorWord = kHeaderWords - 1;           // never zeroed by k_prepare
+
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, int bits) {
     uint64_t m = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
@@ -41,69 +55,103 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
     return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
-__global__ __launch_bounds__(kBlock) void k_upsweep(const uint32_t* __restrict__ keys,
-                                                    uint32_t n, int shift, int bits,
-                                                    uint32_t* __restrict__ hist,
-                                                    uint32_t num_tiles) {
-    __shared__ uint32_t cnt[kMaxRadix];
-    const int radix = 1 << bits;
-    for (int i = threadIdx.x; i < radix; i += kBlock) cnt[i] = 0;
+__device__ __forceinline__ uint32_t load_status(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_status(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kBlock) void k_prepare(uint32_t* __restrict__ zero, uint32_t zero_words,
+                                                    uint32_t* __restrict__ fill_a,
+                                                    uint32_t* __restrict__ fill_b,
+                                                    uint32_t fill_words) {
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < zero_words; i += stride)
+        if (i != kErrorWord) zero[i] = 0u;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < fill_words; i += stride) {
+        fill_a[i] = kEmptyCell;
+        fill_b[i] = kEmptyCell;
+    }
+}
+
+// Keys read from an array.
+struct ArrayKeys {
+    const uint32_t* keys;
+    __device__ uint32_t operator()(uint32_t i) const { return keys[i]; }
+};
+
+// Keys computed as cell hashes of float4 positions (computeHashs,
+// SPH2S_FINAL_SINGLE_SCALE/Solver.cu:401-424), written out for the first pass.
+struct HashKeys {
+    const float4* pos;
+    uint32_t* keys_out;
+    float ox, oy, oz, spacing;
+    int dx, dy, dz;
+    __device__ uint32_t operator()(uint32_t i) const {
+        const float4 x = pos[i];
+        const uint32_t k = cell_hash(x.x, x.y, x.z, ox, oy, oz, spacing, dx, dy, dz);
+        keys_out[i] = k;
+        return k;
+    }
+};
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_hist(Src src, uint32_t n, int passes, uint32_t key_bits,
+                                                 uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t cnt[kMaxPasses][kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t base = tile * kTile + wave * kWaveKeys;
-    const uint32_t mask = static_cast<uint32_t>(radix - 1);
+    const uint32_t base = blockIdx.x * kTile + wave * kWaveKeys;
     for (int r = 0; r < kItems; ++r) {
         const uint32_t idx = base + r * 64 + lane;
         const bool valid = idx < n;
-        const uint32_t d = valid ? (keys[idx] >> shift) & mask : 0u;
-        const uint64_t peers = match_digit(d, valid, bits);
-        if (valid && lane == __builtin_ctzll(peers))
-            atomicAdd(&cnt[d], static_cast<uint32_t>(__popcll(peers)));
+        const uint32_t key = valid ? src(idx) : 0u;
+        for (int p = 0; p < passes; ++p) {
+            const int shift = 8 * p;
+            const int bits = min(8, static_cast<int>(key_bits) - shift);
+            const uint32_t d = (key >> shift) & ((1u << bits) - 1u);
+            const uint64_t peers = match_digit(d, valid, bits);
+            if (valid && lane == __builtin_ctzll(peers))
+                atomicAdd(&cnt[p][d], static_cast<uint32_t>(__popcll(peers)));
+        }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < radix; i += kBlock) hist[i * num_tiles + tile] = cnt[i];
-}
-
-// Exclusive scan of len entries by one workgroup: each thread owns a
-// contiguous chunk, chunk sums are scanned in LDS.
-__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ data,
-                                                       uint32_t len) {
-    __shared__ uint32_t sums[kScanThreads];
-    const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (len + kScanThreads - 1) / kScanThreads;
-    const uint32_t b = min(t * chunk, len), e = min(b + chunk, len);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    sums[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
-        const uint32_t v = (t >= off) ? sums[t - off] : 0u;
-        __syncthreads();
-        sums[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = (t > 0) ? sums[t - 1] : 0u;
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
+    for (int p = 0; p < passes; ++p) {
+        const int bits = min(8, static_cast<int>(key_bits) - 8 * p);
+        for (int d = threadIdx.x; d < (1 << bits); d += kBlock)
+            if (cnt[p][d]) atomicAdd(&ghist[p * kRadix + d], cnt[p][d]);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_downsweep(
+__global__ __launch_bounds__(kBlock) void k_pass(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n,
-    int shift, int bits, const uint32_t* __restrict__ hist, uint32_t num_tiles) {
-    __shared__ uint32_t wcnt[kWaves][kMaxRadix];
-    __shared__ uint32_t gbase[kMaxRadix];
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+    int bits, const uint32_t* __restrict__ ghist, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ ticket, uint32_t* __restrict__ error_word) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t wcnt[kWaves][kRadix];
+    __shared__ uint32_t dbase[kRadix];
     const int radix = 1 << bits;
     const uint32_t mask = static_cast<uint32_t>(radix - 1);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t tile = blockIdx.x;
-    for (int i = threadIdx.x; i < kWaves * kMaxRadix; i += kBlock) (&wcnt[0][0])[i] = 0u;
-    for (int i = threadIdx.x; i < radix; i += kBlock) gbase[i] = hist[i * num_tiles + tile];
+    const int t = threadIdx.x;
+
+    if (t == 0) s_tile = atomicAdd(ticket, 1u);
+    for (int i = t; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0u;
+    // Exclusive scan of this pass's global digit histogram.
+    dbase[t] = (t < radix) ? ghist[t] : 0u;
     __syncthreads();
+    for (int off = 1; off < kRadix; off <<= 1) {
+        const uint32_t v = (t >= off) ? dbase[t - off] : 0u;
+        __syncthreads();
+        dbase[t] += v;
+        __syncthreads();
+    }
+    const uint32_t digit_excl = dbase[t] - ((t < radix) ? ghist[t] : 0u);
+    const uint32_t tile = s_tile;
 
     const uint32_t base = tile * kTile + wave * kWaveKeys;
     const uint64_t lt = lanemask_lt(lane);
@@ -123,22 +171,48 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(
         const uint64_t peers = match_digit(d, valid, bits);
         const uint32_t before = wcnt[wave][d];
         rank[r] = before + static_cast<uint32_t>(__popcll(peers & lt));
-        // The read above is one wave instruction issued before this write, and
-        // the written value depends on it, so every lane sees the pre-round count.
+        // One wave instruction reads the count before the leader's write, and
+        // the written value depends on the read, so all lanes see the old count.
         if (valid && lane == __builtin_ctzll(peers))
             wcnt[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // Per digit: exclusive prefix over waves (in-tile order is wave-major).
-    for (int d = threadIdx.x; d < radix; d += kBlock) {
+    if (t < radix) {
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
-            const uint32_t c = wcnt[w][d];
-            wcnt[w][d] = run;
+            const uint32_t c = wcnt[w][t];
+            wcnt[w][t] = run;
             run += c;
         }
+        // Decoupled look-back over earlier tiles for digit t.
+        uint32_t* my = status + static_cast<size_t>(tile) * kRadix + t;
+        uint32_t excl = 0;
+        if (tile == 0) {
+            store_status(my, kFlagPrefix | run);
+        } else {
+            store_status(my, kFlagAggregate | run);
+            int32_t pred = static_cast<int32_t>(tile) - 1;
+            uint32_t spins = 0;
+            while (pred >= 0) {
+                const uint32_t v = load_status(status + static_cast<size_t>(pred) * kRadix + t);
+                const uint32_t flag = v & ~kCountMask;
+                if (flag == 0u) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(error_word, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & kCountMask;
+                if (flag == kFlagPrefix) break;
+                --pred;
+            }
+            store_status(my, kFlagPrefix | ((excl + run) & kCountMask));
+        }
+        dbase[t] = digit_excl + excl;
     }
     __syncthreads();
 #pragma unroll
@@ -146,8 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(
         const uint32_t idx = base + r * 64 + lane;
         if (idx < n) {
             const uint32_t d = (key[r] >> shift) & mask;
-            const uint32_t dst = gbase[d] + wcnt[wave][d] + rank[r];
-            if (dst < n) {  // always true for a consistent histogram; guards the scatter
+            const uint32_t dst = dbase[d] + wcnt[wave][d] + rank[r];
+            if (dst < n) {  // always true for consistent counts; guards the scatter
                 keys_out[dst] = key[r];
                 vals_out[dst] = val[r];
             }
@@ -156,58 +230,129 @@ __global__ __launch_bounds__(kBlock) void k_downsweep(
 }
 
 inline uint32_t tiles_for(uint32_t n) { return (n + kTile - 1) / kTile; }
+inline size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-}  // namespace
+struct TempLayout {
+    uint32_t* header;   // ghist[kMaxPasses][256], tickets, error word
+    uint32_t* status;   // [passes][tiles][256]
+    uint32_t* ktmp;
+    uint32_t* vtmp;
+    size_t status_words;
+};
 
-size_t radix_sort_temp_bytes(uint32_t n) {
+TempLayout layout(void* temp, uint32_t n) {
+    TempLayout L;
     const size_t tiles = tiles_for(n > 0 ? n : 1);
-    const size_t hist = static_cast<size_t>(kMaxRadix) * tiles * sizeof(uint32_t);
-    const size_t hist_aligned = (hist + 255) & ~static_cast<size_t>(255);
-    const size_t buf = (static_cast<size_t>(n) * sizeof(uint32_t) + 255) & ~static_cast<size_t>(255);
-    return hist_aligned + 2 * buf;
+    char* p = static_cast<char*>(temp);
+    L.header = reinterpret_cast<uint32_t*>(p);
+    p += align_up(kHeaderWords * sizeof(uint32_t));
+    L.status = reinterpret_cast<uint32_t*>(p);
+    L.status_words = static_cast<size_t>(kMaxPasses) * tiles * kRadix;
+    p += align_up(L.status_words * sizeof(uint32_t));
+    L.ktmp = reinterpret_cast<uint32_t*>(p);
+    p += align_up(static_cast<size_t>(n) * sizeof(uint32_t));
+    L.vtmp = reinterpret_cast<uint32_t*>(p);
+    return L;
 }
 
-int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                     uint32_t* vals_out, uint32_t n, uint32_t key_bits, void* temp,
-                     size_t temp_bytes, hipStream_t stream) {
+int check_sort_args(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                    uint32_t* vals_out, uint32_t n, uint32_t key_bits, void* temp,
+                    size_t temp_bytes) {
     if (key_bits < 1 || key_bits > 32)
         return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: key_bits must be in [1, 32]");
+    if (n >= (1u << 30)) return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: n >= 2^30");
     if (n == 0) return SPH_OK;
-    if (!keys_in || !keys_out || !vals_out)
+    if (!keys_out || !vals_out)
         return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: null buffer");
-    if (keys_in == keys_out || vals_in == vals_out)
+    if (keys_in == keys_out || (vals_in && vals_in == vals_out))
         return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: outputs alias inputs");
     if (!temp || temp_bytes < radix_sort_temp_bytes(n))
         return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: temp buffer too small");
-    const uint32_t tiles = tiles_for(n);
-    const size_t hist_bytes = static_cast<size_t>(kMaxRadix) * tiles * sizeof(uint32_t);
-    const size_t hist_aligned = (hist_bytes + 255) & ~static_cast<size_t>(255);
-    const size_t buf = (static_cast<size_t>(n) * sizeof(uint32_t) + 255) & ~static_cast<size_t>(255);
-    auto* hist = static_cast<uint32_t*>(temp);
-    auto* ktmp = reinterpret_cast<uint32_t*>(static_cast<char*>(temp) + hist_aligned);
-    auto* vtmp = reinterpret_cast<uint32_t*>(static_cast<char*>(temp) + hist_aligned + buf);
+    return SPH_OK;
+}
 
+// k_prepare + k_hist: zero the sort state (and reset the cell ranges), then
+// build the digit histograms of every pass in one read of the keys.
+template <class Src>
+int prepare_hist(Src src, uint32_t n, uint32_t key_bits, void* temp, uint32_t* fill_a,
+                 uint32_t* fill_b, uint32_t fill_words, hipStream_t stream) {
+    const uint32_t tiles = tiles_for(n);
+    const TempLayout L = layout(temp, n);
+    const int passes = static_cast<int>((key_bits + 7) / 8);
+    const uint32_t zero_words =
+        static_cast<uint32_t>(L.status - L.header) + static_cast<uint32_t>(passes) * tiles * kRadix;
+    const uint32_t prep_words = zero_words > fill_words ? zero_words : fill_words;
+    const uint32_t prep_blocks = std::min<uint32_t>((prep_words + kBlock - 1) / kBlock, 2048u);
+    hipLaunchKernelGGL(k_prepare, dim3(prep_blocks), dim3(kBlock), 0, stream, L.header, zero_words,
+                       fill_a, fill_b, fill_words);
+    hipLaunchKernelGGL(k_hist<Src>, dim3(tiles), dim3(kBlock), 0, stream, src, n, passes, key_bits,
+                       L.header);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+// One k_pass launch per digit; the last pass writes keys_out/vals_out.
+int run_passes(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+               uint32_t* vals_out, uint32_t n, uint32_t key_bits, void* temp,
+               hipStream_t stream) {
+    const uint32_t tiles = tiles_for(n);
+    const TempLayout L = layout(temp, n);
     const int passes = static_cast<int>((key_bits + 7) / 8);
     const uint32_t* kin = keys_in;
     const uint32_t* vin = vals_in;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        const int bits = static_cast<int>(key_bits) - shift < 8 ? static_cast<int>(key_bits) - shift : 8;
-        // The last pass must land in the caller's output buffers.
+        const int bits = std::min(8, static_cast<int>(key_bits) - shift);
         const bool to_out = ((passes - 1 - p) % 2) == 0;
-        uint32_t* kout = to_out ? keys_out : ktmp;
-        uint32_t* vout = to_out ? vals_out : vtmp;
-        hipLaunchKernelGGL(k_upsweep, dim3(tiles), dim3(kBlock), 0, stream, kin, n, shift, bits,
-                           hist, tiles);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, stream, hist,
-                           static_cast<uint32_t>(1u << bits) * tiles);
-        hipLaunchKernelGGL(k_downsweep, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, kout,
-                           vout, n, shift, bits, hist, tiles);
+        uint32_t* kout = to_out ? keys_out : L.ktmp;
+        uint32_t* vout = to_out ? vals_out : L.vtmp;
+        hipLaunchKernelGGL(k_pass, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, kout, vout, n,
+                           shift, bits, L.header + p * kRadix,
+                           L.status + static_cast<size_t>(p) * tiles * kRadix,
+                           L.header + kHistWords + p, L.header + kErrorWord);
         kin = kout;
         vin = vout;
     }
     SPH_HIP(hipGetLastError());
     return SPH_OK;
+}
+
+}  // namespace
+
+size_t radix_sort_temp_bytes(uint32_t n) {
+    const size_t tiles = tiles_for(n > 0 ? n : 1);
+    return align_up(kHeaderWords * sizeof(uint32_t)) +
+           align_up(static_cast<size_t>(kMaxPasses) * tiles * kRadix * sizeof(uint32_t)) +
+           2 * align_up(static_cast<size_t>(n) * sizeof(uint32_t));
+}
+
+uint32_t* radix_sort_error_word(void* temp) {
+    return static_cast<uint32_t*>(temp) + kErrorWord;
+}
+
+int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                     uint32_t* vals_out, uint32_t n, uint32_t key_bits, void* temp,
+                     size_t temp_bytes, hipStream_t stream) {
+    int rc = check_sort_args(keys_in, vals_in, keys_out, vals_out, n, key_bits, temp, temp_bytes);
+    if (rc != SPH_OK || n == 0) return rc;
+    if (!keys_in) return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: null keys");
+    rc = prepare_hist(ArrayKeys{keys_in}, n, key_bits, temp, nullptr, nullptr, 0u, stream);
+    if (rc != SPH_OK) return rc;
+    return run_passes(keys_in, vals_in, keys_out, vals_out, n, key_bits, temp, stream);
+}
+
+int hash_hist_cells(const StepParams& p, const float4* pos, uint32_t* keys, uint32_t key_bits,
+                    void* temp, size_t temp_bytes, uint32_t* cell_start, uint32_t* cell_end,
+                    hipStream_t stream) {
+    if (!temp || temp_bytes < radix_sort_temp_bytes(p.n))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "hash: sort temp buffer too small");
+    HashKeys src{pos, keys, p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz};
+    return prepare_hist(src, p.n, key_bits, temp, cell_start, cell_end, p.num_cells, stream);
+}
+
+int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint32_t* perm,
+               uint32_t key_bits, void* temp, hipStream_t stream) {
+    return run_passes(keys, nullptr, skeys, perm, p.n, key_bits, temp, stream);
 }
 
 }  // namespace sph

@@ -1,9 +1,10 @@
 // step.hip -- the single-scale SPH step on gfx950.
 //
 // Pass order of Solver::Advance (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1082-1198):
-//   k_hash_reset   cell hash per particle + empty-cell reset of the ranges
+//   hash+hist      cell hash per particle fused with the radix-sort digit
+//                  histograms, plus the empty-cell reset of the ranges
 //                  (computeHashs :401-424 and the two cudaMemsets :1118-1127)
-//   radix sort     stable (hash, slot) sort          (thrust, :1110-1115)
+//   radix passes   stable (hash, slot) sort          (thrust, :1110-1115)
 //   k_reorder      gather of x, v into sorted slots and cell start/end
 //                  (reorderAndComputeCellStartEndD :478-534)
 //   k_density      Poly6 density                    (:536-599, :198-229)
@@ -14,50 +15,12 @@
 // (vx, vy, vz, 0), one 16-byte load per particle.  Within one row (cy, cz)
 // of the grid the cells cx are consecutive hashes, so a stencil row of up to
 // three cells is one contiguous run of sorted particles.
-#include "internal.hpp"
+#include "neighbors.hpp"
 
 namespace sph {
 namespace {
 
 constexpr int kBlock = 256;
-
-// computeCoordinatesOff (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:20-38), FP32:
-// clamp((int)((x + off - o) / spacing), 0, D-1).  Division is IEEE (the
-// library is built with -fhip-fp32-correctly-rounded-divide-sqrt) so hashes
-// and stencil bounds are bit-identical to the FP32 reference arithmetic.
-__device__ __forceinline__ int cell_coord_off(float x, float off, float o, float sp, int dim) {
-    float t = __fadd_rn(x, off);
-    t = __fsub_rn(t, o);
-    t = __fdiv_rn(t, sp);
-    const int c = static_cast<int>(t);
-    return max(0, min(c, dim - 1));
-}
-
-__device__ __forceinline__ int cell_coord(float x, float o, float sp, int dim) {
-    float t = __fsub_rn(x, o);
-    t = __fdiv_rn(t, sp);
-    const int c = static_cast<int>(t);
-    return max(0, min(c, dim - 1));
-}
-
-__global__ __launch_bounds__(kBlock) void k_hash_reset(const float4* __restrict__ pos,
-                                                       StepParams p,
-                                                       uint32_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ cell_start,
-                                                       uint32_t* __restrict__ cell_end) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < p.n) {
-        const float4 x = pos[i];
-        const int cx = cell_coord(x.x, p.ox, p.spacing, p.dx);
-        const int cy = cell_coord(x.y, p.oy, p.spacing, p.dy);
-        const int cz = cell_coord(x.z, p.oz, p.spacing, p.dz);
-        keys[i] = static_cast<uint32_t>(cx + p.dx * (cy + p.dy * cz));
-    }
-    if (i < p.num_cells) {
-        cell_start[i] = kEmptyCell;
-        cell_end[i] = kEmptyCell;
-    }
-}
 
 // Boundary detection compares each sorted key with its predecessor read
 // straight from global memory (no block barrier, so no divergent
@@ -91,148 +54,136 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
     tvel[s] = vel[j];
 }
 
-struct Stencil {
-    int sx, sy, sz, ex, ey, ez;
-};
+constexpr int kUnroll = 4;
 
-__device__ __forceinline__ Stencil stencil_of(const StepParams& p, float x, float y, float z) {
-    Stencil st;
-    st.sx = cell_coord_off(x, -p.h, p.ox, p.spacing, p.dx);
-    st.sy = cell_coord_off(y, -p.h, p.oy, p.spacing, p.dy);
-    st.sz = cell_coord_off(z, -p.h, p.oz, p.spacing, p.dz);
-    st.ex = cell_coord_off(x, p.h, p.ox, p.spacing, p.dx);
-    st.ey = cell_coord_off(y, p.h, p.oy, p.spacing, p.dy);
-    st.ez = cell_coord_off(z, p.h, p.oz, p.spacing, p.dz);
-    return st;
-}
-
-// Contiguous particle run [b, e) covering cells sx..ex of row (cy, cz).
-// Cells with a start always have an end <= n; the final min() keeps the loop
-// in bounds regardless.
-__device__ __forceinline__ void row_range(const uint32_t* __restrict__ cs,
-                                          const uint32_t* __restrict__ ce, uint32_t row,
-                                          int sx, int ex, uint32_t n, uint32_t& b, uint32_t& e) {
-    b = kEmptyCell;
-    e = 0u;
-    for (int cx = sx; cx <= ex; ++cx) {
-        const uint32_t st = cs[row + cx];
-        if (st != kEmptyCell) {
-            b = min(b, st);
-            e = max(e, ce[row + cx]);
-        }
-    }
-    e = min(e, n);
-}
-
+// computeDensitiesPressuresD + computeDensityCell (Solver.cu:198-229,536-599):
+// rho_i = m c6 sum_{r_ij < h} (h^2 - r_ij^2)^3, self included, accumulated
+// with Kahan compensation.  Also emits the per-particle factors the force
+// pass needs for every neighbour j: A_j = k (rho_j - rho0) / rho_j^2 and
+// B_j = 1 / rho_j.  Candidates are loaded kUnroll at a time so their loads
+// are in flight together.
+template <bool kKahan>
 __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
-                                                    StepParams p, float* __restrict__ rho) {
+                                                    StepParams p, GridView g,
+                                                    float* __restrict__ rho,
+                                                    float2* __restrict__ aux) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= p.n) return;
     const float4 xi = tpos[s];
-    const Stencil st = stencil_of(p, xi.x, xi.y, xi.z);
-    const bool kahan = (p.flags & SPH_FLAG_KAHAN) != 0u;
     float sum = 0.0f, comp = 0.0f;
-    for (int cz = st.sz; cz <= st.ez; ++cz) {
-        for (int cy = st.sy; cy <= st.ey; ++cy) {
-            const uint32_t row = static_cast<uint32_t>(p.dx) * (cy + p.dy * cz);
-            uint32_t b, e;
-            row_range(cs, ce, row, st.sx, st.ex, p.n, b, e);
-            for (uint32_t j = b; j < e; ++j) {
-                const float4 xj = tpos[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    for_each_run(g, cs, ce, p.n, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
+        for (uint32_t j0 = b; j0 < e; j0 += kUnroll) {
+            float4 q[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) q[u] = tpos[min(j0 + u, e - 1)];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < p.h2) {
-                    const float a = p.h2 - r2;
-                    const float w = a * a * a;
-                    if (kahan) {
-                        const float y = w - comp;
-                        const float t = sum + y;
-                        comp = (t - sum) - y;
-                        sum = t;
-                    } else {
-                        sum += w;
-                    }
+                const bool in = (j0 + u < e) && (r2 < p.h2);
+                const float a = p.h2 - r2;
+                const float w = in ? a * a * a : 0.0f;
+                if (kKahan) {
+                    const float y = w - comp;
+                    const float t = sum + y;
+                    const float c = (t - sum) - y;
+                    sum = in ? t : sum;
+                    comp = in ? c : comp;
+                } else {
+                    sum += w;
                 }
             }
         }
-    }
-    rho[s] = p.c6m * sum;
+    });
+    const float r = p.c6m * sum;
+    rho[s] = r;
+    const float inv = __builtin_amdgcn_rcpf(r);
+    aux[s] = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
 }
 
-// a_i = m c45 sum (A_i + A_j)(h-r)^2/r x_ij            (pressure, A = p/rho^2)
-//     + (mu m c45 / rho_i) sum (v_j - v_i)(h-r)/rho_j  (viscosity)
-//     - (sigma m c6 / rho_i) sum (h^2-r^2)^3 x_ij      (tension)
-//     + g
-// which is f_i/rho_i of Solver.cu:231-309 with rho_i factored out.
+// a_i = m c45 sum_j (A_i + A_j)(h-r)^2/r x_ij            (pressure)
+//     + (mu m c45 / rho_i) sum_j B_j (h-r)(v_j - v_i)     (viscosity)
+//     - (sigma m c6 / rho_i) sum_j (h^2-r^2)^3 x_ij       (tension)
+//     + g,                     0 < r_ij < h
+// which is f_i / rho_i of Solver.cu:231-309,601-711 with rho_i factored out,
+// fused with integrateD (Solver.cu:713-755) and the wall clamp.
 template <bool kTension>
 __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
-                                                  const float* __restrict__ rho,
+                                                  const float2* __restrict__ aux,
                                                   const uint32_t* __restrict__ cs,
                                                   const uint32_t* __restrict__ ce, StepParams p,
-                                                  float dt, float4* __restrict__ pos,
+                                                  GridView g, float dt,
+                                                  float4* __restrict__ pos,
                                                   float4* __restrict__ vel,
                                                   float4* __restrict__ accel) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= p.n) return;
     const float4 xi = tpos[s];
     const float4 vi = tvel[s];
-    const float rhoi = rho[s];
-    const float inv_rhoi = __builtin_amdgcn_rcpf(rhoi);
-    const float Ai = p.stiffness * (rhoi - p.rest_density) * inv_rhoi * inv_rhoi;
-    const Stencil st = stencil_of(p, xi.x, xi.y, xi.z);
+    const float2 ai = aux[s];
     float Px = 0.f, Py = 0.f, Pz = 0.f;
     float Vx = 0.f, Vy = 0.f, Vz = 0.f;
     float Tx = 0.f, Ty = 0.f, Tz = 0.f;
-    for (int cz = st.sz; cz <= st.ez; ++cz) {
-        for (int cy = st.sy; cy <= st.ey; ++cy) {
-            const uint32_t row = static_cast<uint32_t>(p.dx) * (cy + p.dy * cz);
-            uint32_t b, e;
-            row_range(cs, ce, row, st.sx, st.ex, p.n, b, e);
-            for (uint32_t j = b; j < e; ++j) {
-                const float4 xj = tpos[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < p.h2 && r2 > 0.0f) {
-                    const float4 vj = tvel[j];
-                    const float rhoj = rho[j];
-                    const float inv_rhoj = __builtin_amdgcn_rcpf(rhoj);
-                    const float Aj = p.stiffness * (rhoj - p.rest_density) * inv_rhoj * inv_rhoj;
-                    const float inv_r = __builtin_amdgcn_rsqf(r2);
-                    const float r = r2 * inv_r;
+    for_each_run(g, cs, ce, p.n, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
+        for (uint32_t j0 = b; j0 < e; j0 += kUnroll) {
+            float4 q[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) q[u] = tpos[min(j0 + u, e - 1)];
+            float dx[kUnroll], dy[kUnroll], dz[kUnroll], r2[kUnroll];
+            bool in[kUnroll];
+            float4 vj[kUnroll];
+            float2 aj[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                dx[u] = xi.x - q[u].x;
+                dy[u] = xi.y - q[u].y;
+                dz[u] = xi.z - q[u].z;
+                r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+                in[u] = (j0 + u < e) && (r2[u] < p.h2) && (r2[u] > 0.0f);
+                if (in[u]) {
+                    vj[u] = tvel[j0 + u];
+                    aj[u] = aux[j0 + u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                if (in[u]) {
+                    const float inv_r = __builtin_amdgcn_rsqf(r2[u]);
+                    const float r = r2[u] * inv_r;
                     const float hr = p.h - r;
-                    const float tp = (Ai + Aj) * hr * hr * inv_r;
-                    Px += tp * dx;
-                    Py += tp * dy;
-                    Pz += tp * dz;
-                    const float tv = hr * inv_rhoj;
-                    Vx += tv * (vj.x - vi.x);
-                    Vy += tv * (vj.y - vi.y);
-                    Vz += tv * (vj.z - vi.z);
+                    const float tp = (ai.x + aj[u].x) * hr * hr * inv_r;
+                    Px += tp * dx[u];
+                    Py += tp * dy[u];
+                    Pz += tp * dz[u];
+                    const float tv = hr * aj[u].y;
+                    Vx += tv * (vj[u].x - vi.x);
+                    Vy += tv * (vj[u].y - vi.y);
+                    Vz += tv * (vj[u].z - vi.z);
                     if (kTension) {
-                        const float a = p.h2 - r2;
+                        const float a = p.h2 - r2[u];
                         const float w = a * a * a;
-                        Tx += w * dx;
-                        Ty += w * dy;
-                        Tz += w * dz;
+                        Tx += w * dx[u];
+                        Ty += w * dy[u];
+                        Tz += w * dz[u];
                     }
                 }
             }
         }
-    }
-    const float vc = p.visc * inv_rhoi;
+    });
+    const float vc = p.visc * ai.y;
     float ax = p.c45m * Px + vc * Vx + p.gx;
     float ay = p.c45m * Py + vc * Vy + p.gy;
     float az = p.c45m * Pz + vc * Vz + p.gz;
     if (kTension) {
-        const float tc = p.tens * inv_rhoi;
+        const float tc = p.tens * ai.y;
         ax -= tc * Tx;
         ay -= tc * Ty;
         az -= tc * Tz;
     }
-    // integrateD (Solver.cu:713-755): v += dt a; x += dt v.
+    // integrateD (Solver.cu:713-755): v += dt a; x += dt v; then the wall clamp.
     float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
     float x = xi.x + dt * vx, y = xi.y + dt * vy, z = xi.z + dt * vz;
     if (p.flags & SPH_FLAG_WALL_CLAMP) {
@@ -276,16 +227,16 @@ inline dim3 blocks_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock); }
 int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
                 uint32_t key_bits, bool write_accel, hipStream_t stream) {
     if (p.n == 0) return SPH_OK;
+    const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
     switch (pass) {
         case kPassHash: {
-            const uint32_t reset_n = p.n > p.num_cells ? p.n : p.num_cells;
-            hipLaunchKernelGGL(k_hash_reset, blocks_for(reset_n), dim3(kBlock), 0, stream, b.pos,
-                               p, b.keys, b.cell_start, b.cell_end);
+            const int rc = hash_hist_cells(p, b.pos, b.keys, key_bits, b.sort_temp,
+                                           b.sort_temp_bytes, b.cell_start, b.cell_end, stream);
+            if (rc != SPH_OK) return rc;
             break;
         }
         case kPassSort: {
-            const int rc = radix_sort_pairs(b.keys, nullptr, b.skeys, b.perm, p.n, key_bits,
-                                            b.sort_temp, b.sort_temp_bytes, stream);
+            const int rc = sort_cells(p, b.keys, b.skeys, b.perm, key_bits, b.sort_temp, stream);
             if (rc != SPH_OK) return rc;
             break;
         }
@@ -295,19 +246,23 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
                                p.n, p.num_cells);
             break;
         case kPassDensity:
-            hipLaunchKernelGGL(k_density, blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
-                               b.cell_start, b.cell_end, p, b.rho);
+            if (p.flags & SPH_FLAG_KAHAN)
+                hipLaunchKernelGGL(k_density<true>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.cell_start, b.cell_end, p, g, b.rho, b.aux);
+            else
+                hipLaunchKernelGGL(k_density<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.cell_start, b.cell_end, p, g, b.rho, b.aux);
             break;
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (p.tens != 0.0f)
                 hipLaunchKernelGGL(k_force<true>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.tvel, b.rho, b.cell_start, b.cell_end, p, dt, b.pos,
-                                   b.vel, acc);
+                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g, dt,
+                                   b.pos, b.vel, acc);
             else
                 hipLaunchKernelGGL(k_force<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.tvel, b.rho, b.cell_start, b.cell_end, p, dt, b.pos,
-                                   b.vel, acc);
+                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g, dt,
+                                   b.pos, b.vel, acc);
             break;
         }
         default:

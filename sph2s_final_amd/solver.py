@@ -175,6 +175,12 @@ class Solver:
     def set_debug(self, flags: int) -> None:
         C.check(C.lib().sph_set_debug(self._h, int(flags)))
 
+    def device_errors(self) -> int:
+        """Device-side error bits since the last query (SPH_DEVERR_*); synchronises."""
+        bits = c_uint32()
+        C.check(C.lib().sph_device_errors(self._h, byref(bits)))
+        return int(bits.value)
+
     def set_state(self, positions, velocities=None) -> None:
         pos = _f32(positions)
         vel = None if velocities is None else _f32(velocities)

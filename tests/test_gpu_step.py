@@ -36,6 +36,7 @@ def _one_step(sc, eager=False):
         else:
             s.Advance(sc.dt)
         s.synchronize()
+        assert s.device_errors() == 0
         return dict(hash=s.download(FIELD_SORTED_HASH), perm=s.download(FIELD_SORTED_INDEX),
                     cell_start=s.download(FIELD_CELL_START), cell_end=s.download(FIELD_CELL_END),
                     rho=s.densities(), acc=s.download(FIELD_ACCEL), pos=s.positions(),
@@ -121,6 +122,7 @@ def test_repeated_advance_keeps_sorted_invariants():
     with _solver(sc, debug=False) as s:
         s.Advance(sc.dt, 50)
         s.synchronize()
+        assert s.device_errors() == 0
         h = s.download(FIELD_SORTED_HASH)
         st = s.download(FIELD_CELL_START)
         en = s.download(FIELD_CELL_END)
@@ -148,3 +150,22 @@ def test_export_positions_aos_matches_download():
         C.check(C.lib().sph_export_positions_aos(s._h, dst.data_ptr(), 1))
         s.synchronize()
         assert np.array_equal(dst.cpu().numpy().reshape(-1, 3), s.positions())
+
+
+def test_compressed_state_parity_after_300_steps():
+    # After a few hundred steps the soft (k = 3) column has compressed to
+    # several times rest density, so cells hold many more particles: the GPU
+    # step from that state must still match the oracle step from the same state.
+    sc = scenes.dam_break((16, 16, 16), seed=8)
+    with _solver(sc, debug=False) as s:
+        s.Advance(sc.dt, 300)
+        s.synchronize()
+        assert s.device_errors() == 0
+        pos, vel = s.positions(), s.velocities()
+    rho_max = float(np.max(s_rho := O.step(O.params_from_config(sc.config), pos, vel,
+                                              sc.dt)["rho"]))
+    assert rho_max > 1.5 * sc.config.rest_density, "state is not compressed"
+    sc2 = scenes.Scene("dam_break_16_compressed", sc.config, pos, vel)
+    ref = O.step(O.params_from_config(sc.config), pos, vel, sc.dt)
+    _check_step(sc2, _one_step(sc2), ref)
+    del s_rho
