@@ -11,7 +11,9 @@ from ctypes import (POINTER, Structure, c_char_p, c_float, c_int, c_int32, c_siz
                     c_uint32, c_uint64, c_void_p)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsph2s.so")
+# SPH2S_LIBRARY may name another build of the same library (e.g. a tuning
+# variant compiled with different -D options); default is the in-tree build.
+LIB_PATH = os.environ.get("SPH2S_LIBRARY") or os.path.join(HERE, "libsph2s.so")
 
 SPH_OK = 0
 SPH_ERR_INVALID_ARGUMENT = 1
@@ -120,6 +122,16 @@ def lib() -> ctypes.CDLL:
             raise LibraryNotBuilt(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
                 "g.build()'` (there is no CPU fallback)")
+        # torch ships its own libamdhip64 with the same soname as the system
+        # one.  Loading torch first makes libsph2s.so bind to that already
+        # loaded runtime, so the process has a single HIP runtime and device
+        # pointers and streams can be shared with torch tensors.  Loading
+        # libsph2s.so first would leave torch with a second runtime that sees
+        # no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

@@ -96,3 +96,82 @@ __device__ __forceinline__ void for_each_run(const GridView& g, const uint32_t* 
 }
 
 }  // namespace sph
+
+namespace sph {
+
+// Per-particle view of the reference stencil: own cell, stencil bounds and
+// position relative to the grid origin.
+struct Probe {
+    float fx, fy, fz;
+    int cx, cy, cz;
+    int sx, sy, sz, ex, ey, ez;
+};
+
+__device__ __forceinline__ Probe make_probe(const GridView& g, float x, float y, float z) {
+    Probe pr;
+    pr.sx = cell_coord_off(x, -g.h, g.ox, g.spacing, g.dx);
+    pr.sy = cell_coord_off(y, -g.h, g.oy, g.spacing, g.dy);
+    pr.sz = cell_coord_off(z, -g.h, g.oz, g.spacing, g.dz);
+    pr.ex = cell_coord_off(x, g.h, g.ox, g.spacing, g.dx);
+    pr.ey = cell_coord_off(y, g.h, g.oy, g.spacing, g.dy);
+    pr.ez = cell_coord_off(z, g.h, g.oz, g.spacing, g.dz);
+    pr.cx = cell_coord(x, g.ox, g.spacing, g.dx);
+    pr.cy = cell_coord(y, g.oy, g.spacing, g.dy);
+    pr.cz = cell_coord(z, g.oz, g.spacing, g.dz);
+    pr.fx = x - g.ox;
+    pr.fy = y - g.oy;
+    pr.fz = z - g.oz;
+    return pr;
+}
+
+// Candidate run [b, e) of grid row (ry, rz) for the probe: empty when the row
+// is outside the reference stencil or farther than h; otherwise the cells of
+// the row that can hold a particle closer than h (see the file comment).
+__device__ __forceinline__ void probe_row_run(const GridView& g, const Probe& pr,
+                                              const uint32_t* __restrict__ cs,
+                                              const uint32_t* __restrict__ ce, uint32_t n,
+                                              int ry, int rz, uint32_t& b, uint32_t& e) {
+    b = kEmptyCell;
+    e = 0u;
+    if (ry < pr.sy || ry > pr.ey || rz < pr.sz || rz > pr.ez) return;
+    const float gy = slab_gap(pr.fy, ry, g.spacing);
+    const float gz = slab_gap(pr.fz, rz, g.spacing);
+    const float d2 = gy * gy + gz * gz;
+    if (d2 >= g.prune_h2) return;
+    const float w = __builtin_amdgcn_sqrtf(g.prune_h2 - d2);
+    const int lo = max(pr.sx, static_cast<int>(floorf((pr.fx - w) * g.inv_spacing)));
+    const int hi = min(pr.ex, static_cast<int>(floorf((pr.fx + w) * g.inv_spacing)));
+    if (lo > hi) return;
+    row_run(cs, ce, static_cast<uint32_t>(g.dx) * (ry + g.dy * rz), lo, hi, n, b, e);
+    if (b >= e) {
+        b = kEmptyCell;
+        e = 0u;
+    }
+}
+
+// Block-wide [min b, max e) over the lanes' runs (empty runs contribute
+// b = SPH_EMPTY_CELL, e = 0).  red must hold 2 * (blockDim / 64) words.
+template <int kBlockThreads>
+__device__ __forceinline__ void block_run_union(uint32_t b, uint32_t e, uint32_t* red,
+                                                uint32_t& ub, uint32_t& ue) {
+    for (int off = 32; off > 0; off >>= 1) {
+        b = min(b, static_cast<uint32_t>(__shfl_xor(static_cast<int>(b), off)));
+        e = max(e, static_cast<uint32_t>(__shfl_xor(static_cast<int>(e), off)));
+    }
+    constexpr int kW = kBlockThreads / 64;
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave] = b;
+        red[kW + wave] = e;
+    }
+    __syncthreads();
+    ub = red[0];
+    ue = red[kW];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) {
+        ub = min(ub, red[w]);
+        ue = max(ue, red[kW + w]);
+    }
+}
+
+}  // namespace sph

@@ -54,14 +54,55 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
     tvel[s] = vel[j];
 }
 
-constexpr int kUnroll = 4;
+#ifndef SPH_DENSITY_TILE
+#define SPH_DENSITY_TILE 1024
+#endif
+#ifndef SPH_FORCE_TILE
+#define SPH_FORCE_TILE 768
+#endif
+#ifndef SPH_LIST_CAP
+#define SPH_LIST_CAP 8
+#endif
+constexpr int kDensityTile = SPH_DENSITY_TILE;
+constexpr int kForceTile = SPH_FORCE_TILE;
+constexpr int kListCap = SPH_LIST_CAP;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+// The neighbour passes walk the nine grid rows (cy + dy, cz + dz),
+// dy, dz in {-1, 0, 1}, as block-wide phases.  In each phase every lane finds
+// its candidate run in that row (neighbors.hpp), the block takes the union of
+// the runs of its 256 consecutive sorted targets -- one contiguous index range
+// when the targets share a grid row -- and loads it into LDS with coalesced
+// reads; each lane then scans its own run out of LDS.  A run that lies outside
+// the loaded range (targets spanning distant rows, or a range larger than the
+// tile) is read from global memory instead, and so are the rare extra rows a
+// reference stencil reaches when rounding puts a bound two cells away.
+
+__device__ __forceinline__ float poly6_term(const float4& xi, const float4& xj, float h2) {
+    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float a = h2 - r2;
+    return (r2 < h2) ? a * a * a : 0.0f;
+}
+
+template <class Load>
+__device__ __forceinline__ float poly6_run(const float4& xi, float h2, uint32_t m, Load&& load) {
+    float rs = 0.0f;
+    uint32_t k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const float4 q0 = load(k), q1 = load(k + 1), q2 = load(k + 2), q3 = load(k + 3);
+        rs += poly6_term(xi, q0, h2) + poly6_term(xi, q1, h2) + poly6_term(xi, q2, h2) +
+              poly6_term(xi, q3, h2);
+    }
+    for (; k < m; ++k) rs += poly6_term(xi, load(k), h2);
+    return rs;
+}
 
 // computeDensitiesPressuresD + computeDensityCell (Solver.cu:198-229,536-599):
-// rho_i = m c6 sum_{r_ij < h} (h^2 - r_ij^2)^3, self included, accumulated
-// with Kahan compensation.  Also emits the per-particle factors the force
-// pass needs for every neighbour j: A_j = k (rho_j - rho0) / rho_j^2 and
-// B_j = 1 / rho_j.  Candidates are loaded kUnroll at a time so their loads
-// are in flight together.
+// rho_i = m c6 sum_{r_ij < h} (h^2 - r_ij^2)^3, self included.  Each run is
+// summed in plain FP32 and the run sums are combined with Kahan compensation
+// (with kKahan).  Also emits the per-particle factors the force pass needs for
+// every neighbour j: A_j = k (rho_j - rho0) / rho_j^2 and B_j = 1 / rho_j.
 template <bool kKahan>
 __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
@@ -69,34 +110,57 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
                                                     StepParams p, GridView g,
                                                     float* __restrict__ rho,
                                                     float2* __restrict__ aux) {
+    __shared__ float4 tile[kDensityTile];
+    __shared__ uint32_t red[2 * kWavesPerBlock];
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= p.n) return;
-    const float4 xi = tpos[s];
+    const bool active = s < p.n;
+    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sum = 0.0f, comp = 0.0f;
-    for_each_run(g, cs, ce, p.n, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-        for (uint32_t j0 = b; j0 < e; j0 += kUnroll) {
-            float4 q[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) q[u] = tpos[min(j0 + u, e - 1)];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const bool in = (j0 + u < e) && (r2 < p.h2);
-                const float a = p.h2 - r2;
-                const float w = in ? a * a * a : 0.0f;
-                if (kKahan) {
-                    const float y = w - comp;
-                    const float t = sum + y;
-                    const float c = (t - sum) - y;
-                    sum = in ? t : sum;
-                    comp = in ? c : comp;
+    auto accumulate = [&](float rs) {
+        if (kKahan) {
+            const float y = rs - comp;
+            const float t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        } else {
+            sum += rs;
+        }
+    };
+    for (int dz = -1; dz <= 1; ++dz) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            uint32_t b = kEmptyCell, e = 0u;
+            if (active) probe_row_run(g, pr, cs, ce, p.n, pr.cy + dy, pr.cz + dz, b, e);
+            uint32_t ub, ue;
+            block_run_union<kBlock>(b, e, red, ub, ue);
+            // Loaded range [ub, seg_e); empty when every lane's run is empty.
+            const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kDensityTile)) : 0u;
+            const uint32_t seg_e = ub + seg_n;
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) tile[k] = tpos[ub + k];
+            __syncthreads();
+            if (b < e) {
+                if (b >= ub && e <= seg_e) {
+                    const float4* t = tile + (b - ub);
+                    accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
                 } else {
-                    sum += w;
+                    const float4* t = tpos + b;
+                    accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
                 }
             }
         }
-    });
+    }
+    if (!active) return;
+    for (int rz = pr.sz; rz <= pr.ez; ++rz)
+        for (int ry = pr.sy; ry <= pr.ey; ++ry) {
+            if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
+            uint32_t b, e;
+            probe_row_run(g, pr, cs, ce, p.n, ry, rz, b, e);
+            if (b < e) {
+                const float4* t = tpos + b;
+                accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
+            }
+        }
     const float r = p.c6m * sum;
     rho[s] = r;
     const float inv = __builtin_amdgcn_rcpf(r);
@@ -109,6 +173,12 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
 //     + g,                     0 < r_ij < h
 // which is f_i / rho_i of Solver.cu:231-309,601-711 with rho_i factored out,
 // fused with integrateD (Solver.cu:713-755) and the wall clamp.
+//
+// Candidates are tested as they are scanned; in-range ones are appended to
+// the lane's column of an LDS list (list[k][lane], conflict-free for any k)
+// and their pair terms are evaluated when some lane's column is full and at
+// the end of every phase, so the pair body runs for neighbours only.  The
+// tile holds (x, y, z, A_j) and (vx, vy, vz, B_j) per candidate.
 template <bool kTension>
 __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
@@ -119,60 +189,112 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
                                                   float4* __restrict__ pos,
                                                   float4* __restrict__ vel,
                                                   float4* __restrict__ accel) {
+    __shared__ float4 tp[kForceTile];
+    __shared__ float4 tv[kForceTile];
+    __shared__ uint32_t list[kListCap][kBlock];
+    __shared__ uint32_t red[2 * kWavesPerBlock];
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= p.n) return;
-    const float4 xi = tpos[s];
-    const float4 vi = tvel[s];
-    const float2 ai = aux[s];
+    const bool active = s < p.n;
+    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float2 ai = active ? aux[s] : make_float2(0.f, 1.f);
+    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float Px = 0.f, Py = 0.f, Pz = 0.f;
     float Vx = 0.f, Vy = 0.f, Vz = 0.f;
     float Tx = 0.f, Ty = 0.f, Tz = 0.f;
-    for_each_run(g, cs, ce, p.n, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-        for (uint32_t j0 = b; j0 < e; j0 += kUnroll) {
-            float4 q[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) q[u] = tpos[min(j0 + u, e - 1)];
-            float dx[kUnroll], dy[kUnroll], dz[kUnroll], r2[kUnroll];
-            bool in[kUnroll];
-            float4 vj[kUnroll];
-            float2 aj[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                dx[u] = xi.x - q[u].x;
-                dy[u] = xi.y - q[u].y;
-                dz[u] = xi.z - q[u].z;
-                r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-                in[u] = (j0 + u < e) && (r2[u] < p.h2) && (r2[u] > 0.0f);
-                if (in[u]) {
-                    vj[u] = tvel[j0 + u];
-                    aj[u] = aux[j0 + u];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                if (in[u]) {
-                    const float inv_r = __builtin_amdgcn_rsqf(r2[u]);
-                    const float r = r2[u] * inv_r;
-                    const float hr = p.h - r;
-                    const float tp = (ai.x + aj[u].x) * hr * hr * inv_r;
-                    Px += tp * dx[u];
-                    Py += tp * dy[u];
-                    Pz += tp * dz[u];
-                    const float tv = hr * aj[u].y;
-                    Vx += tv * (vj[u].x - vi.x);
-                    Vy += tv * (vj[u].y - vi.y);
-                    Vz += tv * (vj[u].z - vi.z);
-                    if (kTension) {
-                        const float a = p.h2 - r2[u];
-                        const float w = a * a * a;
-                        Tx += w * dx[u];
-                        Ty += w * dy[u];
-                        Tz += w * dz[u];
-                    }
-                }
+    uint32_t cnt = 0;
+
+    // xa = (x_j, A_j), vb = (v_j, B_j); requires 0 < r_ij < h.
+    auto pair = [&](const float4& xa, const float4& vb) {
+        const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float hr = p.h - r2 * inv_r;
+        const float tpr = (ai.x + xa.w) * hr * hr * inv_r;
+        Px += tpr * dx;
+        Py += tpr * dy;
+        Pz += tpr * dz;
+        const float tvs = hr * vb.w;
+        Vx += tvs * (vb.x - vi.x);
+        Vy += tvs * (vb.y - vi.y);
+        Vz += tvs * (vb.z - vi.z);
+        if (kTension) {
+            const float a = p.h2 - r2;
+            const float w = a * a * a;
+            Tx += w * dx;
+            Ty += w * dy;
+            Tz += w * dz;
+        }
+    };
+    auto in_range = [&](const float4& xj) {
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        return (r2 < p.h2) && (r2 > 0.0f);
+    };
+    auto flush = [&]() {
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t t = list[k][threadIdx.x];
+            pair(tp[t], tv[t]);
+        }
+        cnt = 0;
+    };
+    auto global_run = [&](uint32_t b, uint32_t e) {
+        for (uint32_t j = b; j < e; ++j) {
+            const float4 xj = tpos[j];
+            if (in_range(xj)) {
+                const float4 vj = tvel[j];
+                const float2 aj = aux[j];
+                pair(make_float4(xj.x, xj.y, xj.z, aj.x), make_float4(vj.x, vj.y, vj.z, aj.y));
             }
         }
-    });
+    };
+
+    for (int dz = -1; dz <= 1; ++dz) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            uint32_t b = kEmptyCell, e = 0u;
+            if (active) probe_row_run(g, pr, cs, ce, p.n, pr.cy + dy, pr.cz + dz, b, e);
+            uint32_t ub, ue;
+            block_run_union<kBlock>(b, e, red, ub, ue);
+            const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kForceTile)) : 0u;
+            const uint32_t seg_e = ub + seg_n;
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) {
+                const float4 x = tpos[ub + k];
+                const float4 v = tvel[ub + k];
+                const float2 a = aux[ub + k];
+                tp[k] = make_float4(x.x, x.y, x.z, a.x);
+                tv[k] = make_float4(v.x, v.y, v.z, a.y);
+            }
+            __syncthreads();
+            if (b < e) {
+                if (b >= ub && e <= seg_e) {
+                    const uint32_t t0 = b - ub, t1 = e - ub;
+                    for (uint32_t t = t0; t < t1; t += 4) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (t + u < t1 && in_range(tp[t + u])) {
+                                list[cnt][threadIdx.x] = t + u;
+                                ++cnt;
+                            }
+                        }
+                        if (__ballot(cnt > kListCap - 4) != 0ull) flush();
+                    }
+                } else {
+                    global_run(b, e);
+                }
+            }
+            flush();
+        }
+    }
+    if (!active) return;
+    for (int rz = pr.sz; rz <= pr.ez; ++rz)
+        for (int ry = pr.sy; ry <= pr.ey; ++ry) {
+            if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
+            uint32_t b, e;
+            probe_row_run(g, pr, cs, ce, p.n, ry, rz, b, e);
+            global_run(b, e);
+        }
+
     const float vc = p.visc * ai.y;
     float ax = p.c45m * Px + vc * Vx + p.gx;
     float ay = p.c45m * Py + vc * Vy + p.gy;
