@@ -10,8 +10,10 @@ Prints ONE JSON line with:
   value       particle-updates/s = N * K / t over K timed graph launches
               (state resident in HBM; hipEvents on the solver's stream)
   roofline    dominant kernel (force+integrate): algorithmic bytes per launch
-              (52 N + 8 C) / its mean duration, measured with hipEvents around
-              each pass in a second, eager run of the same steps
+              (52 N + 8 C) / its mean duration over the timed steps, measured
+              with hipEvents around each pass on the solver's stream while a
+              second solver replays the same W + K steps eagerly (graph and
+              eager launches give bitwise-identical states)
   cpu_baseline the FP32 OpenMP CPU step (a port of the same algorithm) on the
               same scene, timed for a bounded number of steps
 """
@@ -78,11 +80,24 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="timed steps replayed for the per-pass profile (0 = all)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
-    ap.add_argument("--cpu-max-steps", type=int, default=20)
+    ap.add_argument("--cpu-max-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run plain 256 MiB device copies (scales rocprofv3 byte counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda").uniform_()
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 5}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not supported "
@@ -118,14 +133,22 @@ def main() -> int:
     ms_per_step = ms / args.steps
     value = n * args.steps / (ms / 1e3)
 
-    # Per-pass durations (eager launches bracketed by hipEvents on the solver stream).
-    per_pass = (C.ctypes.c_float * len(C.PASS_NAMES))()
-    C.check(C.lib().sph_profile_passes(solver._h, scene.dt, args.profile_steps, per_pass))
-    pass_ms = {name: per_pass[i] / args.profile_steps for i, name in enumerate(C.PASS_NAMES)}
-    solver.synchronize()
     pos = solver.positions()
     finite = bool(__import__("numpy").isfinite(pos).all())
+    dev_errors = solver.device_errors()
     solver.close()
+
+    # Per-pass durations over the same steps: a second solver replays the
+    # warm-up with the graph, then the timed steps eagerly with hipEvents
+    # bracketing every pass on its stream.
+    replay = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
+    replay.Advance(scene.dt, args.warmup)
+    replay.synchronize()
+    prof_steps = args.steps if args.profile_steps <= 0 else min(args.profile_steps, args.steps)
+    per_pass = (C.ctypes.c_float * len(C.PASS_NAMES))()
+    C.check(C.lib().sph_profile_passes(replay._h, scene.dt, prof_steps, per_pass))
+    pass_ms = {name: per_pass[i] / prof_steps for i, name in enumerate(C.PASS_NAMES)}
+    replay.close()
 
     alg = algorithmic_bytes(n, cells)
     force_s = pass_ms["force_integrate"] / 1e3
@@ -159,6 +182,8 @@ def main() -> int:
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": pass_ms,
         "state_finite": finite,
+        "device_errors": dev_errors,
+        "profiled_steps": prof_steps,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)

@@ -128,3 +128,30 @@ int launch_pack_state(const float* pos_xyz_dev, const float* vel_xyz_dev, uint32
 int launch_export_aos(const float4* src, uint32_t n, float* dst_xyz, hipStream_t stream);
 
 }  // namespace sph
+
+namespace sph {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Neighbour passes process particles in sorted (spatial)
+// order, so consecutive blocks share most of their neighbour data.  This maps
+// hardware block b to a logical block so that each XCD receives chunks of
+// kXcdChunk consecutive logical blocks (dealt to the XCDs in turn, which
+// keeps the per-XCD work balanced when cost varies along the sorted order)
+// and its L2 sees spatially local data.  Blocks past the last whole round of
+// chunks keep their own index, so the map is a bijection for any grid size.
+constexpr uint32_t kNumXcds = 8;
+#ifndef SPH_XCD_CHUNK
+#define SPH_XCD_CHUNK 4
+#endif
+constexpr uint32_t kXcdChunk = SPH_XCD_CHUNK;
+
+__device__ __forceinline__ uint32_t xcd_local_block(uint32_t b, uint32_t nblocks) {
+    if (kXcdChunk == 0) return b;
+    const uint32_t round = kNumXcds * kXcdChunk;
+    const uint32_t full = (nblocks / round) * round;
+    if (b >= full) return b;
+    const uint32_t x = b % kNumXcds, k = b / kNumXcds;
+    return ((k / kXcdChunk) * kNumXcds + x) * kXcdChunk + (k % kXcdChunk);
+}
+
+}  // namespace sph

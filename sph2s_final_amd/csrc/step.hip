@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ cell_start,
                                                     uint32_t* __restrict__ cell_end, uint32_t n,
                                                     uint32_t num_cells) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= n) return;
     // Keys are clamped cell hashes (< num_cells) and perm is a permutation of
     // [0, n); the min() guards keep every access in bounds regardless.
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
                                                     float2* __restrict__ aux) {
     __shared__ float4 tile[kDensityTile];
     __shared__ uint32_t red[2 * kWavesPerBlock];
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool active = s < p.n;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
     __shared__ float4 tv[kForceTile];
     __shared__ uint32_t list[kListCap][kBlock];
     __shared__ uint32_t red[2 * kWavesPerBlock];
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool active = s < p.n;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
