@@ -159,6 +159,7 @@ int sph_get_views(sph_solver* s, sph_views* out);
 #define SPH_FIELD_CELL_START    6
 #define SPH_FIELD_CELL_END      7
 #define SPH_FIELD_ACCEL         8
+#define SPH_FIELD_TAG           9   /* two-scale high set: (slot << 29) | group */
 int sph_download(sph_solver* s, int field, void* host_dst, size_t bytes);
 
 /* Renderer contract: positions as 3 floats per particle
@@ -181,7 +182,49 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
  * SPH_DEVERR_SORT_WAIT: a radix-sort look-back wait exceeded its bound and
  * gave up, so the step that raised it has an invalid ordering. */
 #define SPH_DEVERR_SORT_WAIT 1u
+#define SPH_DEVERR_CAPACITY  2u   /* two-scale: a split/merge exceeded a capacity */
 int sph_device_errors(sph_solver* s, uint32_t* bits);
+
+/* ---- the two-scale solver (SPH2S_FINAL_TWO_SCALE/Solver.h:109-144) ---------
+ * A low-resolution set (h0, m0 from cfg) and a high-resolution set (h1 = h0/2,
+ * m1 = m0/8, TWO_SCALE/Solver.h:36-43), each with its own grid over the
+ * domain, coupled through the complementary averaged-kernel density and force
+ * sums of TWO_SCALE/Solver.cu:252-290,394-499 (blend weight 1).  Every step,
+ * low-resolution particles inside [region_lo, region_hi] split into 8 children
+ * (mass m1, x +/- h1/4 per axis, velocity copied; TWO_SCALE/Solver.cu:541-586
+ * with the benchmark's offsets), and groups of 8 siblings that are all outside
+ * the region expanded by h0/2 merge back into one particle (centroid, mean
+ * velocity).  Counts stay on the device; the whole step replays as a hipGraph.
+ * Ids: low-resolution particles keep ids < max_low (a merged particle takes its
+ * group's id); child k of particle p has id max_low + 8 p + k, group p, slot k. */
+typedef struct sph_two_scale sph_two_scale;
+int sph2_create(const sph_config* cfg, const float region_lo[3],
+                const float region_hi[3], const float* pos_xyz,
+                const float* vel_xyz, uint32_t n_low, uint32_t max_low,
+                uint32_t max_high, sph_two_scale** out);
+int sph2_destroy(sph_two_scale* t);
+/* Replace both sets' states.  High-resolution particles need ids and tags
+ * ((slot << 29) | group); ids_l may be NULL (0..n_l-1). */
+int sph2_set_state(sph_two_scale* t, const float* pos_l, const float* vel_l,
+                   const uint32_t* ids_l, uint32_t n_l, const float* pos_h,
+                   const float* vel_h, const uint32_t* ids_h,
+                   const uint32_t* tags_h, uint32_t n_h);
+/* Solver::Advance(dt) of TWO_SCALE/Solver.cu:1568-1614, n_steps times. */
+int sph2_advance(sph_two_scale* t, float dt, uint32_t n_steps);
+int sph2_advance_eager(sph_two_scale* t, float dt, uint32_t n_steps);
+/* Runs only the split/merge/compaction stage on the current state. */
+int sph2_apply_split_merge(sph_two_scale* t);
+int sph2_synchronize(sph_two_scale* t);
+int sph2_set_debug(sph_two_scale* t, uint32_t flags);
+/* counts[0..1] = current (low, high) counts; counts[2..3] = the counts the
+ * last step ran with.  Synchronises. */
+int sph2_counts(sph_two_scale* t, uint32_t counts[4]);
+int sph2_device_errors(sph_two_scale* t, uint32_t* bits);
+/* Host copy of one field of resolution 0 (low) or 1 (high).  POSITION,
+ * VELOCITY, ID, TAG: current (post split/merge) state; DENSITY, ACCEL,
+ * SORTED_*: the last step's sorted order; CELL_*: the last step's grid. */
+int sph2_download(sph_two_scale* t, int resolution, int field, void* host_dst,
+                  size_t bytes);
 
 /* ---- standalone stable LSD radix sort (replaces thrust::sort_by_key,
  *      SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115) ----------------------- */

@@ -204,3 +204,82 @@ class CpuSolver:
         if getattr(self, "_h", None):
             lib().cpu_solver_destroy(self._h)
             self._h = None
+
+
+class OracleTwoScale:
+    """FP64 two-scale step (test infrastructure): same coupling and split/merge
+    rules as the HIP two-scale solver."""
+
+    FIELDS = {"pos": (0, np.float64, 3), "vel": (1, np.float64, 3), "id": (2, np.uint32, 1),
+              "tag": (3, np.uint32, 1), "rho": (4, np.float64, 1), "acc": (5, np.float64, 3),
+              "hash": (6, np.uint32, 1), "perm": (7, np.uint32, 1)}
+
+    def __init__(self, p_low: OracleParams, grid_high, region_lo, region_hi, max_low: int):
+        L = lib()
+        if not hasattr(L, "_ts_bound"):
+            P = POINTER(OracleParams)
+            vp = c_void_p
+            L.oracle_ts_create.argtypes = [P, POINTER(OracleGrid), vp, vp, c_uint32]
+            L.oracle_ts_create.restype = vp
+            L.oracle_ts_destroy.argtypes = [vp]
+            L.oracle_ts_set_state.argtypes = [vp, vp, vp, vp, c_uint32, vp, vp, vp, vp, c_uint32]
+            L.oracle_ts_step.argtypes = [vp, c_double]
+            L.oracle_ts_split_merge.argtypes = [vp]
+            L.oracle_ts_counts.argtypes = [vp, vp]
+            L.oracle_ts_get.argtypes = [vp, c_int, c_int, vp]
+            L._ts_bound = True
+        g = OracleGrid()
+        for d in range(3):
+            g.origin[d] = grid_high.origin[d]
+            g.dims[d] = grid_high.dims[d]
+        g.spacing = grid_high.spacing
+        self._grid_high = g
+        self._lo = np.ascontiguousarray(region_lo, np.float32)
+        self._hi = np.ascontiguousarray(region_hi, np.float32)
+        self._h = L.oracle_ts_create(ctypes.byref(p_low), ctypes.byref(g), _p(self._lo),
+                                     _p(self._hi), int(max_low))
+        self.cells = [num_cells(p_low), int(g.dims[0]) * int(g.dims[1]) * int(g.dims[2])]
+
+    def set_state(self, pos_l, vel_l, ids_l=None, pos_h=None, vel_h=None, ids_h=None,
+                  tags_h=None):
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return _p(a)
+        nh = 0 if pos_h is None else len(pos_h)
+        lib().oracle_ts_set_state(self._h, arr(pos_l, np.float32), arr(vel_l, np.float32),
+                                  arr(ids_l, np.uint32), len(pos_l), arr(pos_h, np.float32),
+                                  arr(vel_h, np.float32), arr(ids_h, np.uint32),
+                                  arr(tags_h, np.uint32), nh)
+
+    def step(self, dt: float) -> None:
+        lib().oracle_ts_step(self._h, float(dt))
+
+    def split_merge(self) -> None:
+        lib().oracle_ts_split_merge(self._h)
+
+    def counts(self):
+        c = np.empty(4, np.uint32)
+        lib().oracle_ts_counts(self._h, _p(c))
+        return tuple(int(v) for v in c)
+
+    def get(self, res: int, name: str) -> np.ndarray:
+        c = self.counts()
+        if name in ("start", "end"):
+            out = np.empty(self.cells[res], np.uint32)
+            lib().oracle_ts_get(self._h, res, 8 if name == "start" else 9, _p(out))
+            return out
+        fid, dt, k = self.FIELDS[name]
+        n = c[res] if name in ("pos", "vel", "id", "tag") else c[2 + res]
+        out = np.empty((n, k) if k > 1 else n, dt)
+        lib().oracle_ts_get(self._h, res, fid, _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_ts_destroy(self._h)
+            self._h = None

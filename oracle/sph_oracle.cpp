@@ -536,3 +536,332 @@ int cpu_solver_threads(void) {
 void cpu_solver_destroy(cpu_solver* s) { delete s; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Two-scale step (FP64).  Restates TWO_SCALE/Solver.cu:1568-1614 with the
+// coupling and split/merge rules of the HIP implementation: complementary
+// averaged-kernel sums (TWO_SCALE/Solver.cu:252-290, 394-499) with blend weight
+// 1, split of low particles inside the region into 8 children at +/- h1/4,
+// merge of sibling groups that are all outside the region expanded by h0/2.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TsSet {
+    oracle_params p;
+    std::vector<double> x, v;            // state, AoS
+    std::vector<uint32_t> id, tag;
+    // last step, sorted order
+    std::vector<double> rho, acc;
+    Neighbourhood nb;
+    uint32_t n_step = 0;
+};
+
+template <class F>
+void for_each_cell_in(const oracle_grid& g, float hs, const double* xi, F&& f) {
+    const float x = static_cast<float>(xi[0]), y = static_cast<float>(xi[1]),
+                z = static_cast<float>(xi[2]);
+    const int csx = coord_off(x, -hs, g.origin[0], g.spacing, g.dims[0]);
+    const int csy = coord_off(y, -hs, g.origin[1], g.spacing, g.dims[1]);
+    const int csz = coord_off(z, -hs, g.origin[2], g.spacing, g.dims[2]);
+    const int cex = coord_off(x, hs, g.origin[0], g.spacing, g.dims[0]);
+    const int cey = coord_off(y, hs, g.origin[1], g.spacing, g.dims[1]);
+    const int cez = coord_off(z, hs, g.origin[2], g.spacing, g.dims[2]);
+    for (int cz = csz; cz <= cez; ++cz)
+        for (int cy = csy; cy <= cey; ++cy)
+            for (int cx = csx; cx <= cex; ++cx) f(hash_of(g, cx, cy, cz));
+}
+
+void sort_set(TsSet& S) {
+    State st;
+    st.x = S.x;
+    st.v = S.v;
+    st.id = S.id;
+    const uint32_t n = static_cast<uint32_t>(S.id.size());
+    build_neighbourhood(S.p, st, S.nb);
+    std::vector<uint32_t> tag(n);
+    for (uint32_t i = 0; i < n; ++i) tag[i] = S.tag[S.nb.perm[i]];
+    S.x = std::move(st.x);
+    S.v = std::move(st.v);
+    S.id = std::move(st.id);
+    S.tag = std::move(tag);
+    S.n_step = n;
+}
+
+}  // namespace
+
+struct oracle_ts {
+    TsSet set[2];
+    float rlo[3], rhi[3], margin_f, child_d_f;
+    uint32_t max_low = 0;
+};
+
+extern "C" {
+
+oracle_ts* oracle_ts_create(const oracle_params* p_low, const oracle_grid* grid_high,
+                            const float region_lo[3], const float region_hi[3],
+                            uint32_t max_low) {
+    auto* t = new oracle_ts;
+    t->set[0].p = *p_low;
+    t->set[1].p = *p_low;
+    oracle_params& ph = t->set[1].p;
+    ph.h_f32 = 0.5f * p_low->h_f32;
+    ph.h = static_cast<double>(ph.h_f32);
+    ph.mass = static_cast<double>(static_cast<float>(p_low->mass) / 8.0f);
+    ph.grid = *grid_high;
+    for (int d = 0; d < 3; ++d) {
+        t->rlo[d] = region_lo[d];
+        t->rhi[d] = region_hi[d];
+    }
+    t->margin_f = 0.5f * p_low->h_f32;
+    t->child_d_f = 0.25f * ph.h_f32;
+    t->max_low = max_low;
+    return t;
+}
+
+void oracle_ts_destroy(oracle_ts* t) { delete t; }
+
+void oracle_ts_set_state(oracle_ts* t, const float* pos_l, const float* vel_l,
+                         const uint32_t* ids_l, uint32_t n_l, const float* pos_h,
+                         const float* vel_h, const uint32_t* ids_h, const uint32_t* tags_h,
+                         uint32_t n_h) {
+    const float* P[2] = {pos_l, pos_h};
+    const float* V[2] = {vel_l, vel_h};
+    const uint32_t* I[2] = {ids_l, ids_h};
+    const uint32_t N[2] = {n_l, n_h};
+    for (int r = 0; r < 2; ++r) {
+        TsSet& S = t->set[r];
+        const uint32_t n = N[r];
+        S.x.assign(3 * static_cast<size_t>(n), 0.0);
+        S.v.assign(3 * static_cast<size_t>(n), 0.0);
+        S.id.resize(n);
+        S.tag.assign(n, 0u);
+        for (size_t i = 0; i < 3 * static_cast<size_t>(n); ++i) {
+            S.x[i] = P[r][i];
+            if (V[r]) S.v[i] = V[r][i];
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            S.id[i] = I[r] ? I[r][i] : i;
+            if (r == 1) S.tag[i] = tags_h[i];
+        }
+    }
+}
+
+void oracle_ts_step(oracle_ts* t, double dt) {
+    TsSet& L = t->set[0];
+    TsSet& H = t->set[1];
+    sort_set(H);
+    sort_set(L);
+    const double h0 = L.p.h, h1 = H.p.h;
+    const Coeffs k0(h0), k1(h1);
+    // own-set densities
+    for (int r = 0; r < 2; ++r) {
+        TsSet& S = t->set[r];
+        State st;
+        st.x = S.x;
+        st.v = S.v;
+        st.id = S.id;
+        densities(S.p, st, S.nb, S.rho);
+    }
+    // complementary densities: + m_B sum_{r < h0} 1/2 (W_h0 + W_h1)
+    std::vector<double> rho_new[2];
+    for (int r = 0; r < 2; ++r) {
+        TsSet& A = t->set[r];
+        TsSet& B = t->set[1 - r];
+        const uint32_t na = static_cast<uint32_t>(A.id.size());
+        rho_new[r] = A.rho;
+        for (uint32_t i = 0; i < na; ++i) {
+            const double* xi = &A.x[3 * i];
+            double sum = 0.0;
+            if (!B.id.empty())
+                for_each_cell_in(B.p.grid, L.p.h_f32, xi, [&](uint32_t c) {
+                    const uint32_t b = B.nb.start[c], e = B.nb.end[c];
+                    if (b == kEmpty) return;
+                    for (uint32_t j = b; j < e; ++j) {
+                        const double dx = xi[0] - B.x[3 * j], dy = xi[1] - B.x[3 * j + 1],
+                                     dz = xi[2] - B.x[3 * j + 2];
+                        const double rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+                        if (rr < h0) {
+                            const double a0 = h0 * h0 - rr * rr, a1 = h1 * h1 - rr * rr;
+                            const double w0 = k0.c6 * a0 * a0 * a0;
+                            const double w1 = (rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0;
+                            sum += 0.5 * (w0 + w1);
+                        }
+                    }
+                });
+            rho_new[r][i] += B.p.mass * sum;
+        }
+    }
+    L.rho = rho_new[0];
+    H.rho = rho_new[1];
+    // forces: own set (with gravity) + complementary, then integrate
+    for (int r = 0; r < 2; ++r) {
+        TsSet& A = t->set[r];
+        TsSet& B = t->set[1 - r];
+        State st;
+        st.x = A.x;
+        st.v = A.v;
+        st.id = A.id;
+        accelerations(A.p, st, A.nb, A.rho, A.acc);
+        const uint32_t na = static_cast<uint32_t>(A.id.size());
+        for (uint32_t i = 0; i < na; ++i) {
+            const double* xi = &A.x[3 * i];
+            const double* vi = &A.v[3 * i];
+            const double rhoi = A.rho[i];
+            const double Ai = A.p.stiffness * (rhoi - A.p.rest_density) / (rhoi * rhoi);
+            double f[3] = {0.0, 0.0, 0.0};
+            if (!B.id.empty())
+                for_each_cell_in(B.p.grid, L.p.h_f32, xi, [&](uint32_t c) {
+                    const uint32_t b = B.nb.start[c], e = B.nb.end[c];
+                    if (b == kEmpty) return;
+                    for (uint32_t j = b; j < e; ++j) {
+                        const double xij[3] = {xi[0] - B.x[3 * j], xi[1] - B.x[3 * j + 1],
+                                               xi[2] - B.x[3 * j + 2]};
+                        const double rr = std::sqrt(xij[0] * xij[0] + xij[1] * xij[1] +
+                                                    xij[2] * xij[2]);
+                        if (!(rr != 0.0 && rr < h0)) continue;
+                        const double rhoj = B.rho[j];
+                        const double Aj = A.p.stiffness * (rhoj - A.p.rest_density) / (rhoj * rhoj);
+                        const double hr0 = h0 - rr, hr1 = std::max(h1 - rr, 0.0);
+                        const double g = 0.5 * (k0.c45 * hr0 * hr0 + k1.c45 * hr1 * hr1) / rr;
+                        const double l = 0.5 * (k0.c45 * hr0 + k1.c45 * hr1);
+                        const double a0 = h0 * h0 - rr * rr, a1 = h1 * h1 - rr * rr;
+                        const double w = 0.5 * (k0.c6 * a0 * a0 * a0 +
+                                                ((rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0));
+                        for (int d = 0; d < 3; ++d) {
+                            f[d] += B.p.mass * (Ai + Aj) * g * xij[d];
+                            f[d] += A.p.viscosity * B.p.mass / (rhoi * rhoj) * l *
+                                    (B.v[3 * j + d] - vi[d]);
+                            f[d] -= A.p.tension * B.p.mass / rhoi * w * xij[d];
+                        }
+                    }
+                });
+            for (int d = 0; d < 3; ++d) A.acc[3 * i + d] += f[d];
+        }
+    }
+    for (int r = 0; r < 2; ++r) {
+        TsSet& A = t->set[r];
+        State st;
+        st.x = A.x;
+        st.v = A.v;
+        st.id = A.id;
+        integrate(L.p, st, A.acc, dt);  // clamp box and restitution of the low configuration
+        A.x = std::move(st.x);
+        A.v = std::move(st.v);
+    }
+    oracle_ts_split_merge(t);
+}
+
+void oracle_ts_split_merge(oracle_ts* t) {
+    TsSet& L = t->set[0];
+    TsSet& H = t->set[1];
+    const uint32_t nl = static_cast<uint32_t>(L.id.size()), nh = static_cast<uint32_t>(H.id.size());
+    const float m = t->margin_f;
+    const float olo[3] = {t->rlo[0] - m, t->rlo[1] - m, t->rlo[2] - m};
+    const float ohi[3] = {t->rhi[0] + m, t->rhi[1] + m, t->rhi[2] + m};
+    auto inside = [&](const double* x) {
+        for (int d = 0; d < 3; ++d)
+            if (!(x[d] >= t->rlo[d] && x[d] <= t->rhi[d])) return false;
+        return true;
+    };
+    auto outside_margin = [&](const double* x) {
+        for (int d = 0; d < 3; ++d)
+            if (x[d] < olo[d] || x[d] > ohi[d]) return true;
+        return false;
+    };
+    constexpr uint32_t kMask = (1u << 29) - 1u;
+    // groups whose 8 children are all outside the expanded region
+    std::vector<uint32_t> cnt(t->max_low, 0u);
+    std::vector<double> gx(static_cast<size_t>(t->max_low) * 24, 0.0),
+        gv(static_cast<size_t>(t->max_low) * 24, 0.0);
+    for (uint32_t i = 0; i < nh; ++i) {
+        const uint32_t g = H.tag[i] & kMask, k = H.tag[i] >> 29;
+        if (g >= t->max_low || !outside_margin(&H.x[3 * i])) continue;
+        ++cnt[g];
+        for (int d = 0; d < 3; ++d) {
+            gx[(static_cast<size_t>(g) * 8 + k) * 3 + d] = H.x[3 * i + d];
+            gv[(static_cast<size_t>(g) * 8 + k) * 3 + d] = H.v[3 * i + d];
+        }
+    }
+    TsSet nL, nH;
+    auto push = [](TsSet& S, const double* x, const double* v, uint32_t id, uint32_t tag) {
+        for (int d = 0; d < 3; ++d) {
+            S.x.push_back(x[d]);
+            S.v.push_back(v[d]);
+        }
+        S.id.push_back(id);
+        S.tag.push_back(tag);
+    };
+    std::vector<uint32_t> split;
+    for (uint32_t i = 0; i < nl; ++i) {
+        if (inside(&L.x[3 * i]))
+            split.push_back(i);
+        else
+            push(nL, &L.x[3 * i], &L.v[3 * i], L.id[i], 0u);
+    }
+    std::vector<uint32_t> leaders;
+    for (uint32_t i = 0; i < nh; ++i) {
+        const uint32_t g = H.tag[i] & kMask, k = H.tag[i] >> 29;
+        const bool merged = g < t->max_low && cnt[g] == 8u;
+        if (!merged)
+            push(nH, &H.x[3 * i], &H.v[3 * i], H.id[i], H.tag[i]);
+        else if (k == 0u)
+            leaders.push_back(g);
+    }
+    for (uint32_t g : leaders) {
+        double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < 3; ++d) {
+                x[d] += gx[(static_cast<size_t>(g) * 8 + k) * 3 + d];
+                v[d] += gv[(static_cast<size_t>(g) * 8 + k) * 3 + d];
+            }
+        for (int d = 0; d < 3; ++d) {
+            x[d] /= 8.0;
+            v[d] /= 8.0;
+        }
+        push(nL, x, v, g, 0u);
+    }
+    const double cd = static_cast<double>(t->child_d_f);
+    for (uint32_t i : split) {
+        const uint32_t pid = L.id[i];
+        for (uint32_t k = 0; k < 8; ++k) {
+            double x[3];
+            for (int d = 0; d < 3; ++d)
+                x[d] = L.x[3 * i + d] + (((k >> d) & 1u) ? cd : -cd);
+            push(nH, x, &L.v[3 * i], t->max_low + 8u * pid + k, (k << 29) | (pid & kMask));
+        }
+    }
+    L.x = std::move(nL.x);
+    L.v = std::move(nL.v);
+    L.id = std::move(nL.id);
+    L.tag = std::move(nL.tag);
+    H.x = std::move(nH.x);
+    H.v = std::move(nH.v);
+    H.id = std::move(nH.id);
+    H.tag = std::move(nH.tag);
+}
+
+void oracle_ts_counts(const oracle_ts* t, uint32_t counts[4]) {
+    counts[0] = static_cast<uint32_t>(t->set[0].id.size());
+    counts[1] = static_cast<uint32_t>(t->set[1].id.size());
+    counts[2] = t->set[0].n_step;
+    counts[3] = t->set[1].n_step;
+}
+
+void oracle_ts_get(const oracle_ts* t, int res, int field, void* out) {
+    const TsSet& S = t->set[res];
+    switch (field) {
+        case 0: std::copy(S.x.begin(), S.x.end(), static_cast<double*>(out)); break;
+        case 1: std::copy(S.v.begin(), S.v.end(), static_cast<double*>(out)); break;
+        case 2: std::copy(S.id.begin(), S.id.end(), static_cast<uint32_t*>(out)); break;
+        case 3: std::copy(S.tag.begin(), S.tag.end(), static_cast<uint32_t*>(out)); break;
+        case 4: std::copy(S.rho.begin(), S.rho.end(), static_cast<double*>(out)); break;
+        case 5: std::copy(S.acc.begin(), S.acc.end(), static_cast<double*>(out)); break;
+        case 6: std::copy(S.nb.hash_sorted.begin(), S.nb.hash_sorted.end(), static_cast<uint32_t*>(out)); break;
+        case 7: std::copy(S.nb.perm.begin(), S.nb.perm.end(), static_cast<uint32_t*>(out)); break;
+        case 8: std::copy(S.nb.start.begin(), S.nb.start.end(), static_cast<uint32_t*>(out)); break;
+        case 9: std::copy(S.nb.end.begin(), S.nb.end.end(), static_cast<uint32_t*>(out)); break;
+        default: break;
+    }
+}
+
+}  // extern "C"

@@ -101,3 +101,34 @@ void cpu_solver_destroy(cpu_solver* s);
 #endif
 
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- two-scale step (FP64), SPH2S_FINAL_TWO_SCALE/Solver.cu:1568-1614 with
+ * the split/merge and coupling rules documented in two_scale.hip ---------- */
+typedef struct oracle_ts oracle_ts;
+/* p_low: the low-resolution parameters; the high set uses h/2, m/8 and a grid
+ * at spacing h/2 over the same domain.  max_low bounds low ids (child ids
+ * start at max_low). */
+oracle_ts* oracle_ts_create(const oracle_params* p_low, const oracle_grid* grid_high,
+                            const float region_lo[3], const float region_hi[3],
+                            uint32_t max_low);
+void oracle_ts_destroy(oracle_ts* t);
+void oracle_ts_set_state(oracle_ts* t, const float* pos_l, const float* vel_l,
+                         const uint32_t* ids_l, uint32_t n_l, const float* pos_h,
+                         const float* vel_h, const uint32_t* ids_h, const uint32_t* tags_h,
+                         uint32_t n_h);
+void oracle_ts_step(oracle_ts* t, double dt);
+void oracle_ts_split_merge(oracle_ts* t);
+/* counts[0..1] current (low, high); counts[2..3] the counts the last step ran with. */
+void oracle_ts_counts(const oracle_ts* t, uint32_t counts[4]);
+/* field: 0 pos (3n f64), 1 vel (3n f64), 2 id (n u32), 3 tag (n u32),
+ * 4 rho (n f64, last step sorted order), 5 acc (3n f64), 6 sorted hash (n u32),
+ * 7 perm (n u32), 8 cell start, 9 cell end (u32 per cell). */
+void oracle_ts_get(const oracle_ts* t, int res, int field, void* out);
+
+#ifdef __cplusplus
+}
+#endif

@@ -7,11 +7,11 @@ There is no CPU fallback: without the built library every call raises.
 """
 from ._capi import (LibraryNotBuilt, SPHError, lib, FIELD_POSITION, FIELD_VELOCITY,  # noqa: F401
                     FIELD_DENSITY, FIELD_ID, FIELD_SORTED_HASH, FIELD_SORTED_INDEX,
-                    FIELD_CELL_START, FIELD_CELL_END, FIELD_ACCEL, SPH_DEBUG_ACCEL,
+                    FIELD_CELL_START, FIELD_CELL_END, FIELD_ACCEL, FIELD_TAG, SPH_DEBUG_ACCEL,
                     SPH_EMPTY_CELL, SPH_FLAG_KAHAN, SPH_FLAG_WALL_CLAMP)
-from .solver import Grid, ParticleData, Solver, SolverConfiguration  # noqa: F401
+from .solver import Grid, ParticleData, Solver, SolverConfiguration, TwoScaleSolver  # noqa: F401
 
-__all__ = ["Grid", "ParticleData", "Solver", "SolverConfiguration", "SPHError",
+__all__ = ["Grid", "ParticleData", "Solver", "SolverConfiguration", "TwoScaleSolver", "SPHError",
            "LibraryNotBuilt", "lib"]
 
 

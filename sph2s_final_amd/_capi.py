@@ -37,6 +37,10 @@ FIELD_SORTED_INDEX = 5
 FIELD_CELL_START = 6
 FIELD_CELL_END = 7
 FIELD_ACCEL = 8
+FIELD_TAG = 9
+
+SPH_DEVERR_SORT_WAIT = 1
+SPH_DEVERR_CAPACITY = 2
 
 PASS_NAMES = ["hash_reset", "radix_sort", "reorder_cells", "density", "force_integrate"]
 
@@ -103,6 +107,19 @@ SIGNATURES = [
     ("sph_set_debug", c_int, [c_void_p, c_uint32]),
     ("sph_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_profile_passes", c_int, [c_void_p, c_float, c_uint32, POINTER(c_float)]),
+    ("sph2_create", c_int, [POINTER(SphConfig), POINTER(c_float), POINTER(c_float), c_void_p,
+                            c_void_p, c_uint32, c_uint32, c_uint32, POINTER(c_void_p)]),
+    ("sph2_destroy", c_int, [c_void_p]),
+    ("sph2_set_state", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_uint32]),
+    ("sph2_advance", c_int, [c_void_p, c_float, c_uint32]),
+    ("sph2_advance_eager", c_int, [c_void_p, c_float, c_uint32]),
+    ("sph2_apply_split_merge", c_int, [c_void_p]),
+    ("sph2_synchronize", c_int, [c_void_p]),
+    ("sph2_set_debug", c_int, [c_void_p, c_uint32]),
+    ("sph2_counts", c_int, [c_void_p, POINTER(c_uint32)]),
+    ("sph2_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
+    ("sph2_download", c_int, [c_void_p, c_int, c_int, c_void_p, c_size_t]),
     ("sph_radix_sort_temp_bytes", c_size_t, [c_uint32]),
     ("sph_radix_sort_pairs", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32,
                                      c_void_p, c_size_t, c_void_p]),

@@ -20,8 +20,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libsph2s.so"
-SOURCES = ["api.hip", "radix_sort.hip", "step.hip"]
-HEADERS = ["internal.hpp"]
+SOURCES = ["api.hip", "radix_sort.hip", "step.hip", "two_scale.hip"]
+HEADERS = ["internal.hpp", "neighbors.hpp"]
 
 
 def hipcc_path() -> str:

@@ -142,6 +142,8 @@ int upload_state(sph_solver* s, const float* pos, const float* vel, uint32_t n) 
     if (n > s->max_n) return set_error(SPH_ERR_CAPACITY, "n exceeds max_n");
     if (n > 0 && !pos) return set_error(SPH_ERR_INVALID_ARGUMENT, "positions are null");
     s->n = n;
+    SPH_HIP(hipMemcpyAsync(s->buf.n_dev, &s->n, sizeof(uint32_t), hipMemcpyHostToDevice, s->stream));
+    SPH_HIP(hipStreamSynchronize(s->stream));
     if (n == 0) return SPH_OK;
     // Stage AoS host arrays through the (not yet used) sorted-copy buffers and
     // pack them into float4 on the device.
@@ -335,7 +337,8 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
         align_up(4 * N),  align_up(16 * N),                                      // rho accel
         align_up(4 * N),  align_up(4 * N),  align_up(4 * N),                     // keys skeys perm
         align_up(4 * C),  align_up(4 * C),  align_up(sort_temp),                 // cells, sort temp
-        align_up(8 * N)};                                                        // aux
+        align_up(8 * N),                                                         // aux
+        align_up(4)};                                                            // live count
     size_t total = 0;
     for (size_t b : sizes) total += b;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -369,6 +372,7 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
     s->buf.cell_end = static_cast<uint32_t*>(take(10));
     s->buf.sort_temp = take(11);
     s->buf.aux = static_cast<float2*>(take(12));
+    s->buf.n_dev = static_cast<uint32_t*>(take(13));
     s->buf.sort_temp_bytes = sort_temp;
 
     e = hipMemsetAsync(s->arena, 0, total, s->stream);

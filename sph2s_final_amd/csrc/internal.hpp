@@ -44,7 +44,7 @@ struct StepParams {
     float restitution;
     float lox, loy, loz, hix, hiy, hiz;
     uint32_t flags;         // SPH_FLAG_*
-    uint32_t n;             // particle count
+    uint32_t n;             // launch capacity (particles); the live count is in device memory
     uint32_t num_cells;
 };
 
@@ -92,9 +92,10 @@ int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
 
 // Zeroes the sort state, resets cell_start/cell_end to SPH_EMPTY_CELL and
 // hashes pos into keys while building the digit histograms.
-int hash_hist_cells(const StepParams& p, const float4* pos, uint32_t* keys, uint32_t key_bits,
-                    void* temp, size_t temp_bytes, uint32_t* cell_start, uint32_t* cell_end,
-                    hipStream_t stream);
+// p.n is the launch capacity; the live count is *n_dev (device memory).
+int hash_hist_cells(const StepParams& p, const uint32_t* n_dev, const float4* pos, uint32_t* keys,
+                    uint32_t key_bits, void* temp, size_t temp_bytes, uint32_t* cell_start,
+                    uint32_t* cell_end, hipStream_t stream);
 // The digit passes: (keys, slot) -> (skeys, perm).
 int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint32_t* perm,
                uint32_t key_bits, void* temp, hipStream_t stream);
@@ -115,6 +116,7 @@ struct StepBuffers {
     uint32_t* cell_end;
     void*  sort_temp;
     size_t sort_temp_bytes;
+    uint32_t* n_dev;    // live particle count (device word)
 };
 
 // Passes of one step, in launch order.
@@ -123,6 +125,12 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt, u
                 bool write_accel, hipStream_t stream);
 int launch_step(const StepParams& p, const StepBuffers& b, float dt, uint32_t key_bits,
                 bool write_accel, hipStream_t stream);
+// Two-scale building blocks: own-set density without the A/B finalisation and
+// own-set acceleration (gravity included) without integration.
+struct GridView;
+int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t stream);
+int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& b,
+                     float4* accel_own, hipStream_t stream);
 int launch_pack_state(const float* pos_xyz_dev, const float* vel_xyz_dev, uint32_t n,
                       float4* pos, float4* vel, hipStream_t stream);
 int launch_export_aos(const float4* src, uint32_t n, float* dst_xyz, hipStream_t stream);

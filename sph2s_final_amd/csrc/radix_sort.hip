@@ -40,6 +40,7 @@ constexpr uint32_t kHistWords = kMaxPasses * kRadix;        // global histograms
 constexpr uint32_t kCounterWords = 64;                      // tickets + error word
 constexpr uint32_t kHeaderWords = kHistWords + kCounterWords;
 constexpr uint32_t kErrorWord = kHeaderWords - 1;           // never zeroed by k_prepare
+constexpr uint32_t kNWord = kHeaderWords - 2;               // live key count, set by k_prepare
 
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, int bits) {
     uint64_t m = __ballot(valid);
@@ -63,13 +64,21 @@ __device__ __forceinline__ void store_status(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Also publishes the live key count (from n_src, or n_host when n_src is
+// null) in the header word kNWord, which every later sort kernel reads.
 __global__ __launch_bounds__(kBlock) void k_prepare(uint32_t* __restrict__ zero, uint32_t zero_words,
                                                     uint32_t* __restrict__ fill_a,
                                                     uint32_t* __restrict__ fill_b,
-                                                    uint32_t fill_words) {
+                                                    uint32_t fill_words,
+                                                    const uint32_t* __restrict__ n_src,
+                                                    uint32_t n_host) {
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < zero_words; i += stride)
-        if (i != kErrorWord) zero[i] = 0u;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < zero_words; i += stride) {
+        if (i == kNWord)
+            zero[i] = n_src ? *n_src : n_host;
+        else if (i != kErrorWord)
+            zero[i] = 0u;
+    }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < fill_words; i += stride) {
         fill_a[i] = kEmptyCell;
         fill_b[i] = kEmptyCell;
@@ -98,8 +107,11 @@ struct HashKeys {
 };
 
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_hist(Src src, uint32_t n, int passes, uint32_t key_bits,
+__global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __restrict__ n_dev,
+                                                 int passes, uint32_t key_bits,
                                                  uint32_t* __restrict__ ghist) {
+    const uint32_t n = *n_dev;
+    if (blockIdx.x * kTile >= n) return;  // block-uniform: the grid covers the capacity
     __shared__ uint32_t cnt[kMaxPasses][kRadix];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
     __syncthreads();
@@ -128,9 +140,10 @@ __global__ __launch_bounds__(kBlock) void k_hist(Src src, uint32_t n, int passes
 
 __global__ __launch_bounds__(kBlock) void k_pass(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
-    int bits, const uint32_t* __restrict__ ghist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ ticket, uint32_t* __restrict__ error_word) {
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    const uint32_t* __restrict__ n_dev, int shift, int bits, const uint32_t* __restrict__ ghist,
+    uint32_t* __restrict__ status, uint32_t* __restrict__ ticket,
+    uint32_t* __restrict__ error_word) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t wcnt[kWaves][kRadix];
     __shared__ uint32_t dbase[kRadix];
@@ -152,6 +165,10 @@ __global__ __launch_bounds__(kBlock) void k_pass(
     }
     const uint32_t digit_excl = dbase[t] - ((t < radix) ? ghist[t] : 0u);
     const uint32_t tile = s_tile;
+    // The grid covers the capacity; tickets past the live tiles have no keys.
+    // Tickets are taken in dispatch order, so live tiles never wait on them.
+    const uint32_t n = *n_dev;
+    if (tile * kTile >= n) return;
 
     const uint32_t base = tile * kTile + wave * kWaveKeys;
     const uint64_t lt = lanemask_lt(lane);
@@ -272,31 +289,33 @@ int check_sort_args(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* 
 }
 
 // k_prepare + k_hist: zero the sort state (and reset the cell ranges), then
-// build the digit histograms of every pass in one read of the keys.
+// build the digit histograms of every pass in one read of the keys.  cap is
+// the launch capacity; the live count comes from n_src (device) or n_host.
 template <class Src>
-int prepare_hist(Src src, uint32_t n, uint32_t key_bits, void* temp, uint32_t* fill_a,
-                 uint32_t* fill_b, uint32_t fill_words, hipStream_t stream) {
-    const uint32_t tiles = tiles_for(n);
-    const TempLayout L = layout(temp, n);
+int prepare_hist(Src src, uint32_t cap, const uint32_t* n_src, uint32_t n_host, uint32_t key_bits,
+                 void* temp, uint32_t* fill_a, uint32_t* fill_b, uint32_t fill_words,
+                 hipStream_t stream) {
+    const uint32_t tiles = tiles_for(cap);
+    const TempLayout L = layout(temp, cap);
     const int passes = static_cast<int>((key_bits + 7) / 8);
     const uint32_t zero_words =
         static_cast<uint32_t>(L.status - L.header) + static_cast<uint32_t>(passes) * tiles * kRadix;
     const uint32_t prep_words = zero_words > fill_words ? zero_words : fill_words;
     const uint32_t prep_blocks = std::min<uint32_t>((prep_words + kBlock - 1) / kBlock, 2048u);
     hipLaunchKernelGGL(k_prepare, dim3(prep_blocks), dim3(kBlock), 0, stream, L.header, zero_words,
-                       fill_a, fill_b, fill_words);
-    hipLaunchKernelGGL(k_hist<Src>, dim3(tiles), dim3(kBlock), 0, stream, src, n, passes, key_bits,
-                       L.header);
+                       fill_a, fill_b, fill_words, n_src, n_host);
+    hipLaunchKernelGGL(k_hist<Src>, dim3(tiles), dim3(kBlock), 0, stream, src,
+                       L.header + kNWord, passes, key_bits, L.header);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
 
 // One k_pass launch per digit; the last pass writes keys_out/vals_out.
 int run_passes(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-               uint32_t* vals_out, uint32_t n, uint32_t key_bits, void* temp,
+               uint32_t* vals_out, uint32_t cap, uint32_t key_bits, void* temp,
                hipStream_t stream) {
-    const uint32_t tiles = tiles_for(n);
-    const TempLayout L = layout(temp, n);
+    const uint32_t tiles = tiles_for(cap);
+    const TempLayout L = layout(temp, cap);
     const int passes = static_cast<int>((key_bits + 7) / 8);
     const uint32_t* kin = keys_in;
     const uint32_t* vin = vals_in;
@@ -306,8 +325,8 @@ int run_passes(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t* kout = to_out ? keys_out : L.ktmp;
         uint32_t* vout = to_out ? vals_out : L.vtmp;
-        hipLaunchKernelGGL(k_pass, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, kout, vout, n,
-                           shift, bits, L.header + p * kRadix,
+        hipLaunchKernelGGL(k_pass, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, kout, vout,
+                           L.header + kNWord, shift, bits, L.header + p * kRadix,
                            L.status + static_cast<size_t>(p) * tiles * kRadix,
                            L.header + kHistWords + p, L.header + kErrorWord);
         kin = kout;
@@ -336,18 +355,20 @@ int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t*
     int rc = check_sort_args(keys_in, vals_in, keys_out, vals_out, n, key_bits, temp, temp_bytes);
     if (rc != SPH_OK || n == 0) return rc;
     if (!keys_in) return set_error(SPH_ERR_INVALID_ARGUMENT, "radix sort: null keys");
-    rc = prepare_hist(ArrayKeys{keys_in}, n, key_bits, temp, nullptr, nullptr, 0u, stream);
+    rc = prepare_hist(ArrayKeys{keys_in}, n, nullptr, n, key_bits, temp, nullptr, nullptr, 0u,
+                      stream);
     if (rc != SPH_OK) return rc;
     return run_passes(keys_in, vals_in, keys_out, vals_out, n, key_bits, temp, stream);
 }
 
-int hash_hist_cells(const StepParams& p, const float4* pos, uint32_t* keys, uint32_t key_bits,
-                    void* temp, size_t temp_bytes, uint32_t* cell_start, uint32_t* cell_end,
-                    hipStream_t stream) {
+int hash_hist_cells(const StepParams& p, const uint32_t* n_dev, const float4* pos, uint32_t* keys,
+                    uint32_t key_bits, void* temp, size_t temp_bytes, uint32_t* cell_start,
+                    uint32_t* cell_end, hipStream_t stream) {
     if (!temp || temp_bytes < radix_sort_temp_bytes(p.n))
         return set_error(SPH_ERR_INVALID_ARGUMENT, "hash: sort temp buffer too small");
     HashKeys src{pos, keys, p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz};
-    return prepare_hist(src, p.n, key_bits, temp, cell_start, cell_end, p.num_cells, stream);
+    return prepare_hist(src, p.n, n_dev, p.n, key_bits, temp, cell_start, cell_end, p.num_cells,
+                        stream);
 }
 
 int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint32_t* perm,

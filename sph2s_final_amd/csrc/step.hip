@@ -32,8 +32,10 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
                                                     float4* __restrict__ tpos,
                                                     float4* __restrict__ tvel,
                                                     uint32_t* __restrict__ cell_start,
-                                                    uint32_t* __restrict__ cell_end, uint32_t n,
+                                                    uint32_t* __restrict__ cell_end,
+                                                    const uint32_t* __restrict__ n_dev,
                                                     uint32_t num_cells) {
+    const uint32_t n = *n_dev;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= n) return;
     // Keys are clamped cell hashes (< num_cells) and perm is a permutation of
@@ -103,17 +105,21 @@ __device__ __forceinline__ float poly6_run(const float4& xi, float h2, uint32_t 
 // summed in plain FP32 and the run sums are combined with Kahan compensation
 // (with kKahan).  Also emits the per-particle factors the force pass needs for
 // every neighbour j: A_j = k (rho_j - rho0) / rho_j^2 and B_j = 1 / rho_j.
-template <bool kKahan>
+template <bool kKahan, bool kFinalize>
 __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
                                                     StepParams p, GridView g,
+                                                    const uint32_t* __restrict__ n_dev,
                                                     float* __restrict__ rho,
                                                     float2* __restrict__ aux) {
     __shared__ float4 tile[kDensityTile];
     __shared__ uint32_t red[2 * kWavesPerBlock];
-    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    const bool active = s < p.n;
+    const uint32_t n = *n_dev;
+    const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
+    if (block0 >= n) return;  // block-uniform: the grid covers the capacity
+    const uint32_t s = block0 + threadIdx.x;
+    const bool active = s < n;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sum = 0.0f, comp = 0.0f;
@@ -130,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row_run(g, pr, cs, ce, p.n, pr.cy + dy, pr.cz + dz, b, e);
+            if (active) probe_row_run(g, pr, cs, ce, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
             block_run_union<kBlock>(b, e, red, ub, ue);
             // Loaded range [ub, seg_e); empty when every lane's run is empty.
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
         for (int ry = pr.sy; ry <= pr.ey; ++ry) {
             if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
             uint32_t b, e;
-            probe_row_run(g, pr, cs, ce, p.n, ry, rz, b, e);
+            probe_row_run(g, pr, cs, ce, n, ry, rz, b, e);
             if (b < e) {
                 const float4* t = tpos + b;
                 accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
@@ -163,8 +169,10 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
         }
     const float r = p.c6m * sum;
     rho[s] = r;
-    const float inv = __builtin_amdgcn_rcpf(r);
-    aux[s] = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
+    if (kFinalize) {
+        const float inv = __builtin_amdgcn_rcpf(r);
+        aux[s] = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
+    }
 }
 
 // a_i = m c45 sum_j (A_i + A_j)(h-r)^2/r x_ij            (pressure)
@@ -179,22 +187,27 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
 // and their pair terms are evaluated when some lane's column is full and at
 // the end of every phase, so the pair body runs for neighbours only.  The
 // tile holds (x, y, z, A_j) and (vx, vy, vz, B_j) per candidate.
-template <bool kTension>
+// With kIntegrate = false (two-scale own-set pass) the kernel only writes the
+// acceleration, gravity included, to accel.
+template <bool kTension, bool kIntegrate>
 __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
                                                   const float2* __restrict__ aux,
                                                   const uint32_t* __restrict__ cs,
                                                   const uint32_t* __restrict__ ce, StepParams p,
-                                                  GridView g, float dt,
-                                                  float4* __restrict__ pos,
+                                                  GridView g, const uint32_t* __restrict__ n_dev,
+                                                  float dt, float4* __restrict__ pos,
                                                   float4* __restrict__ vel,
                                                   float4* __restrict__ accel) {
     __shared__ float4 tp[kForceTile];
     __shared__ float4 tv[kForceTile];
     __shared__ uint32_t list[kListCap][kBlock];
     __shared__ uint32_t red[2 * kWavesPerBlock];
-    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    const bool active = s < p.n;
+    const uint32_t n = *n_dev;
+    const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
+    if (block0 >= n) return;  // block-uniform: the grid covers the capacity
+    const uint32_t s = block0 + threadIdx.x;
+    const bool active = s < n;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float2 ai = active ? aux[s] : make_float2(0.f, 1.f);
@@ -252,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row_run(g, pr, cs, ce, p.n, pr.cy + dy, pr.cz + dz, b, e);
+            if (active) probe_row_run(g, pr, cs, ce, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
             block_run_union<kBlock>(b, e, red, ub, ue);
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kForceTile)) : 0u;
@@ -291,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
         for (int ry = pr.sy; ry <= pr.ey; ++ry) {
             if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
             uint32_t b, e;
-            probe_row_run(g, pr, cs, ce, p.n, ry, rz, b, e);
+            probe_row_run(g, pr, cs, ce, n, ry, rz, b, e);
             global_run(b, e);
         }
 
@@ -304,6 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
         ax -= tc * Tx;
         ay -= tc * Ty;
         az -= tc * Tz;
+    }
+    if (!kIntegrate) {
+        accel[s] = make_float4(ax, ay, az, 0.0f);
+        return;
     }
     // integrateD (Solver.cu:713-755): v += dt a; x += dt v; then the wall clamp.
     float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
@@ -318,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
         if (z > p.hiz) { z = p.hiz; vz = -e * vz; }
     }
     pos[s] = make_float4(x, y, z, xi.w);
-    vel[s] = make_float4(vx, vy, vz, 0.0f);
+    vel[s] = make_float4(vx, vy, vz, vi.w);
     if (accel) accel[s] = make_float4(ax, ay, az, 0.0f);
 }
 
@@ -352,7 +369,7 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
     const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
     switch (pass) {
         case kPassHash: {
-            const int rc = hash_hist_cells(p, b.pos, b.keys, key_bits, b.sort_temp,
+            const int rc = hash_hist_cells(p, b.n_dev, b.pos, b.keys, key_bits, b.sort_temp,
                                            b.sort_temp_bytes, b.cell_start, b.cell_end, stream);
             if (rc != SPH_OK) return rc;
             break;
@@ -365,26 +382,26 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
         case kPassReorder:
             hipLaunchKernelGGL(k_reorder, blocks_for(p.n), dim3(kBlock), 0, stream, b.skeys,
                                b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start, b.cell_end,
-                               p.n, p.num_cells);
+                               b.n_dev, p.num_cells);
             break;
         case kPassDensity:
             if (p.flags & SPH_FLAG_KAHAN)
-                hipLaunchKernelGGL(k_density<true>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.cell_start, b.cell_end, p, g, b.rho, b.aux);
+                hipLaunchKernelGGL((k_density<true, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
             else
-                hipLaunchKernelGGL(k_density<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.cell_start, b.cell_end, p, g, b.rho, b.aux);
+                hipLaunchKernelGGL((k_density<false, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
             break;
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (p.tens != 0.0f)
-                hipLaunchKernelGGL(k_force<true>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g, dt,
-                                   b.pos, b.vel, acc);
+                hipLaunchKernelGGL((k_force<true, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g,
+                                   b.n_dev, dt, b.pos, b.vel, acc);
             else
-                hipLaunchKernelGGL(k_force<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g, dt,
-                                   b.pos, b.vel, acc);
+                hipLaunchKernelGGL((k_force<false, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g,
+                                   b.n_dev, dt, b.pos, b.vel, acc);
             break;
         }
         default:
@@ -400,6 +417,34 @@ int launch_step(const StepParams& p, const StepBuffers& b, float dt, uint32_t ke
         const int rc = launch_pass(pass, p, b, dt, key_bits, write_accel, stream);
         if (rc != SPH_OK) return rc;
     }
+    return SPH_OK;
+}
+
+int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t stream) {
+    if (p.n == 0) return SPH_OK;
+    const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
+    if (p.flags & SPH_FLAG_KAHAN)
+        hipLaunchKernelGGL((k_density<true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
+    else
+        hipLaunchKernelGGL((k_density<false, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& b,
+                     float4* accel_own, hipStream_t stream) {
+    if (p.n == 0) return SPH_OK;
+    if (p.tens != 0.0f)
+        hipLaunchKernelGGL((k_force<true, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
+                           b.tvel, b.aux, b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f, nullptr,
+                           nullptr, accel_own);
+    else
+        hipLaunchKernelGGL((k_force<false, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
+                           b.tvel, b.aux, b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f, nullptr,
+                           nullptr, accel_own);
+    SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
 

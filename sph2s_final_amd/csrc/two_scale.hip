@@ -1,0 +1,979 @@
+// two_scale.hip -- the two-scale step: a low-resolution set (h0, m0) and a
+// high-resolution set (h1 = h0/2, m1 = m0/8) advanced together, with the 1->8
+// split of low-resolution particles inside an axis-aligned region and the
+// 8->1 merge of sibling groups that have all left it.
+//
+// Reference: SPH2S_FINAL_TWO_SCALE/Solver.cu (Advance :1568-1614,
+// computeNeighborhoods :1616-1680, the complementary density/force sums
+// :252-290, :394-499, :732-845, :847-1048, integrateD + insertHighResParticles
+// :1163-1289, :541-586) and Solver.h:16-50 (h1 = h0/2, m1 = m0/8, one grid per
+// resolution).  Departures, all deliberate:
+//   * coupling: each set sees the other through the reference's averaged
+//     kernels 1/2 (W_h0 + W_h1) (and their gradients / Laplacians) with cutoff
+//     h0 and the other set's mass, with blend weight lambda = 1 throughout (no
+//     fade-in/fade-out timeline);
+//   * split: children at x +/- h1/4 on each axis (a uniform child lattice of
+//     spacing h1/2, the low-resolution lattice's h-relative spacing), mass m1,
+//     velocity copied; the reference places them at r/sqrt(3) with
+//     r = 1/2 (3 m0 / (4 pi rho))^(1/3) and fades them in;
+//   * merge (absent in the reference): a group of 8 siblings merges when all 8
+//     are outside the region expanded by h0/2, into one low-resolution particle
+//     at their centroid with their mean velocity (mass and momentum conserved);
+//   * bookkeeping: split/merge flags and counts come from prefix sums in
+//     particle order (deterministic), counts stay on the device, and
+//     emissions beyond capacity are dropped and flagged instead of written out
+//     of bounds (the reference's atomicAdd has no bound, :566, and its
+//     atomic compaction order is nondeterministic, :1287).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "neighbors.hpp"
+
+namespace sph {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kPi = 3.14159265358979323846;
+
+// Cross-resolution constants (the complementary sums of TWO_SCALE/Solver.cu).
+struct CrossParams {
+    float h0, h1, h02, h12;
+    float c6_0, c6_1;      // 315 / (64 pi h^9)
+    float c45_0, c45_1;    // 45 / (pi h^6)
+    float m_other;         // mass of the other set's particles
+    float stiffness, rest_density;
+    float visc;            // mu
+    float tension;         // sigma
+    float gx, gy, gz, restitution;
+    float lox, loy, loz, hix, hiy, hiz;
+    uint32_t flags;
+};
+
+// x + offset of child k: bit 0 -> x, bit 1 -> y, bit 2 -> z; set = +d.
+__device__ __forceinline__ float child_offset(int k, int axis, float d) {
+    return ((k >> axis) & 1) ? d : -d;
+}
+
+// ---------------------------------------------------------------------------
+// complementary density: rho_i += m_B sum_{j in B, r < h0} 1/2 (W_h0 + W_h1),
+// then A_i, B_i for the force passes.
+__global__ __launch_bounds__(kBlock) void k_cross_density(
+    const float4* __restrict__ tpos_a, const uint32_t* __restrict__ n_a,
+    const float4* __restrict__ tpos_b, const uint32_t* __restrict__ cs_b,
+    const uint32_t* __restrict__ ce_b, const uint32_t* __restrict__ n_b, GridView gb,
+    CrossParams c, float* __restrict__ rho_a, float2* __restrict__ aux_a) {
+    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= na) return;
+    const float4 xi = tpos_a[s];
+    float sum = 0.0f;
+    if (nb > 0) {
+        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = tpos_b[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < c.h02) {
+                    const float a0 = c.h02 - r2;
+                    const float a1 = c.h12 - r2;
+                    const float w0 = c.c6_0 * a0 * a0 * a0;
+                    const float w1 = (r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f;
+                    sum += 0.5f * (w0 + w1);
+                }
+            }
+        });
+    }
+    const float r = rho_a[s] + c.m_other * sum;
+    rho_a[s] = r;
+    const float inv = __builtin_amdgcn_rcpf(r);
+    aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+}
+
+// complementary force (TWO_SCALE/Solver.cu:394-499 with lambda = 1) added to
+// the own-set acceleration, then symplectic Euler and the wall clamp.
+template <bool kTension>
+__global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
+    const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
+    const float2* __restrict__ aux_a, const float4* __restrict__ acc_a,
+    const uint32_t* __restrict__ n_a, const float4* __restrict__ tpos_b,
+    const float4* __restrict__ tvel_b, const float2* __restrict__ aux_b,
+    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
+    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, float dt,
+    float4* __restrict__ pos_a, float4* __restrict__ vel_a, float4* __restrict__ acc_out) {
+    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= na) return;
+    const float4 xi = tpos_a[s];
+    const float4 vi = tvel_a[s];
+    const float2 ai = aux_a[s];
+    float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f;
+    if (nb > 0) {
+        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = tpos_b[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < c.h02 && r2 > 0.0f) {
+                    const float4 vj = tvel_b[j];
+                    const float2 aj = aux_b[j];
+                    const float inv_r = __builtin_amdgcn_rsqf(r2);
+                    const float r = r2 * inv_r;
+                    const float hr0 = c.h0 - r;
+                    const float hr1 = fmaxf(c.h1 - r, 0.0f);
+                    // 1/2 (|grad W_h0| + |grad W_h1|) / r and 1/2 (lap W_h0 + lap W_h1)
+                    const float g = 0.5f * (c.c45_0 * hr0 * hr0 + c.c45_1 * hr1 * hr1) * inv_r;
+                    const float l = 0.5f * (c.c45_0 * hr0 + c.c45_1 * hr1);
+                    const float tp = (ai.x + aj.x) * g;
+                    Px += tp * dx;
+                    Py += tp * dy;
+                    Pz += tp * dz;
+                    const float tv = l * aj.y;
+                    Vx += tv * (vj.x - vi.x);
+                    Vy += tv * (vj.y - vi.y);
+                    Vz += tv * (vj.z - vi.z);
+                    if (kTension) {
+                        const float a0 = c.h02 - r2, a1 = c.h12 - r2;
+                        const float w = 0.5f * (c.c6_0 * a0 * a0 * a0 +
+                                                ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
+                        Tx += w * dx;
+                        Ty += w * dy;
+                        Tz += w * dz;
+                    }
+                }
+            }
+        });
+    }
+    const float4 a_own = acc_a[s];
+    const float mv = c.m_other * c.visc * ai.y;
+    float ax = a_own.x + c.m_other * Px + mv * Vx;
+    float ay = a_own.y + c.m_other * Py + mv * Vy;
+    float az = a_own.z + c.m_other * Pz + mv * Vz;
+    if (kTension) {
+        const float tc = c.tension * c.m_other * ai.y;
+        ax -= tc * Tx;
+        ay -= tc * Ty;
+        az -= tc * Tz;
+    }
+    float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
+    float x = xi.x + dt * vx, y = xi.y + dt * vy, z = xi.z + dt * vz;
+    if (c.flags & SPH_FLAG_WALL_CLAMP) {
+        const float e = c.restitution;
+        if (x < c.lox) { x = c.lox; vx = -e * vx; }
+        if (x > c.hix) { x = c.hix; vx = -e * vx; }
+        if (y < c.loy) { y = c.loy; vy = -e * vy; }
+        if (y > c.hiy) { y = c.hiy; vy = -e * vy; }
+        if (z < c.loz) { z = c.loz; vz = -e * vz; }
+        if (z > c.hiz) { z = c.hiz; vz = -e * vz; }
+    }
+    pos_a[s] = make_float4(x, y, z, xi.w);
+    vel_a[s] = make_float4(vx, vy, vz, vi.w);
+    if (acc_out) acc_out[s] = make_float4(ax, ay, az, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// split / merge bookkeeping
+
+struct Region {
+    float lox, loy, loz, hix, hiy, hiz;
+    float margin;      // merge hysteresis
+    float child_d;     // child offset per axis
+};
+
+__device__ __forceinline__ bool inside(const Region& r, float x, float y, float z) {
+    return x >= r.lox && x <= r.hix && y >= r.loy && y <= r.hiy && z >= r.loz && z <= r.hiz;
+}
+
+__device__ __forceinline__ bool outside_margin(const Region& r, float x, float y, float z) {
+    return x < r.lox - r.margin || x > r.hix + r.margin || y < r.loy - r.margin ||
+           y > r.hiy + r.margin || z < r.loz - r.margin || z > r.hiz + r.margin;
+}
+
+// High-resolution particles carry (slot << 29) | group in vel.w.
+constexpr uint32_t kGroupMask = (1u << 29) - 1u;
+
+// Low set: split flag.  High set: children outside the expanded region record
+// their state in their group's slots and count themselves.
+__global__ __launch_bounds__(kBlock) void k_split_mark(const float4* __restrict__ pos_l,
+                                                       const uint32_t* __restrict__ n_l, Region rg,
+                                                       uint32_t* __restrict__ flag_l) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= *n_l) return;
+    const float4 x = pos_l[i];
+    flag_l[i] = inside(rg, x.x, x.y, x.z) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_merge_mark(const float4* __restrict__ pos_h,
+                                                       const float4* __restrict__ vel_h,
+                                                       const uint32_t* __restrict__ n_h, Region rg,
+                                                       float4* __restrict__ grp_pos,
+                                                       float4* __restrict__ grp_vel,
+                                                       uint32_t* __restrict__ grp_cnt,
+                                                       uint32_t num_groups) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= *n_h) return;
+    const float4 x = pos_h[i];
+    const float4 v = vel_h[i];
+    const uint32_t tag = __float_as_uint(v.w);
+    const uint32_t g = tag & kGroupMask, k = tag >> 29;
+    if (g >= num_groups) return;
+    if (outside_margin(rg, x.x, x.y, x.z)) {
+        grp_pos[static_cast<size_t>(g) * 8 + k] = x;
+        grp_vel[static_cast<size_t>(g) * 8 + k] = v;
+        atomicAdd(&grp_cnt[g], 1u);
+    }
+}
+
+// flag_h: bit 0 = merged away, bit 1 = emits the merged particle (slot 0).
+__global__ __launch_bounds__(kBlock) void k_merge_flags(const float4* __restrict__ vel_h,
+                                                        const uint32_t* __restrict__ n_h,
+                                                        const uint32_t* __restrict__ grp_cnt,
+                                                        uint32_t num_groups,
+                                                        uint32_t* __restrict__ flag_h) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= *n_h) return;
+    const uint32_t tag = __float_as_uint(vel_h[i].w);
+    const uint32_t g = tag & kGroupMask, k = tag >> 29;
+    const bool merge = g < num_groups && grp_cnt[g] == 8u;
+    flag_h[i] = merge ? (1u | ((k == 0u) ? 2u : 0u)) : 0u;
+}
+
+// Per-block counts of two predicates over the flags of one set.
+// set L: c0 = kept (flag == 0), c1 = split (flag == 1)
+// set H: c0 = kept (bit 0 clear), c1 = merge leaders (bit 1 set)
+__global__ __launch_bounds__(kBlock) void k_block_counts(const uint32_t* __restrict__ flag,
+                                                         const uint32_t* __restrict__ n_dev,
+                                                         int high, uint32_t* __restrict__ counts,
+                                                         uint32_t nblocks) {
+    __shared__ uint32_t c0s, c1s;
+    if (threadIdx.x == 0) c0s = c1s = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = *n_dev;
+    bool c0 = false, c1 = false;
+    if (i < n) {
+        const uint32_t f = flag[i];
+        c0 = high ? !(f & 1u) : (f == 0u);
+        c1 = high ? ((f & 2u) != 0u) : (f == 1u);
+    }
+    const uint64_t b0 = __ballot(c0), b1 = __ballot(c1);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&c0s, static_cast<uint32_t>(__popcll(b0)));
+        atomicAdd(&c1s, static_cast<uint32_t>(__popcll(b1)));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        counts[blockIdx.x] = c0s;
+        counts[nblocks + blockIdx.x] = c1s;
+    }
+}
+
+// Exclusive scans of the four block-count sequences (one workgroup), new
+// totals with the capacity check.  tot[] = {keepL, splitL, keepH, leadH,
+// newL, newH}.
+__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cl, uint32_t nbl,
+                                                      uint32_t* __restrict__ ch, uint32_t nbh,
+                                                      uint32_t cap_l, uint32_t cap_h,
+                                                      uint32_t* __restrict__ tot,
+                                                      uint32_t* __restrict__ error_word) {
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t totals[4];
+    uint32_t* seqs[4] = {cl, cl + nbl, ch, ch + nbh};
+    const uint32_t lens[4] = {nbl, nbl, nbh, nbh};
+    const uint32_t t = threadIdx.x;
+    for (int q = 0; q < 4; ++q) {
+        uint32_t* d = seqs[q];
+        const uint32_t len = lens[q];
+        const uint32_t chunk = (len + 1023) / 1024;
+        const uint32_t b = min(t * chunk, len), e = min(b + chunk, len);
+        uint32_t sum = 0;
+        for (uint32_t i = b; i < e; ++i) sum += d[i];
+        part[t] = sum;
+        __syncthreads();
+        for (uint32_t off = 1; off < 1024; off <<= 1) {
+            const uint32_t v = (t >= off) ? part[t - off] : 0u;
+            __syncthreads();
+            part[t] += v;
+            __syncthreads();
+        }
+        uint32_t run = (t > 0) ? part[t - 1] : 0u;
+        for (uint32_t i = b; i < e; ++i) {
+            const uint32_t v = d[i];
+            d[i] = run;
+            run += v;
+        }
+        if (t == 1023) totals[q] = part[1023];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const uint32_t keep_l = totals[0], split_l = totals[1], keep_h = totals[2],
+                       lead_h = totals[3];
+        const uint64_t new_l = static_cast<uint64_t>(keep_l) + lead_h;
+        const uint64_t new_h = static_cast<uint64_t>(keep_h) + 8ull * split_l;
+        if (new_l > cap_l || new_h > cap_h) atomicOr(error_word, 2u);
+        tot[0] = keep_l;
+        tot[1] = split_l;
+        tot[2] = keep_h;
+        tot[3] = lead_h;
+        tot[4] = static_cast<uint32_t>(new_l < cap_l ? new_l : cap_l);
+        tot[5] = static_cast<uint32_t>(new_h < cap_h ? new_h : cap_h);
+    }
+}
+
+// In-block exclusive rank of a predicate (block of 256, in index order).
+__device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t* wsum) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(pred);
+    const uint32_t below = static_cast<uint32_t>(
+        __popcll(m & (lane == 0 ? 0ull : (~0ull >> (64 - lane)))));
+    if (lane == 0) wsum[wave] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    uint32_t off = 0;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    __syncthreads();
+    return off + below;
+}
+
+// Low set: survivors to new_l[keep rank]; split particles emit 8 children to
+// new_h[keepH + 8 * split rank + k].  ids: child id = id_base + 8 * pid + k,
+// group = pid, slot = k.
+__global__ __launch_bounds__(kBlock) void k_scatter_low(
+    const float4* __restrict__ pos_l, const float4* __restrict__ vel_l,
+    const uint32_t* __restrict__ flag_l, const uint32_t* __restrict__ n_l,
+    const uint32_t* __restrict__ cl, uint32_t nbl, const uint32_t* __restrict__ tot,
+    Region rg, uint32_t id_base, uint32_t cap_l, uint32_t cap_h, float4* __restrict__ new_pos_l,
+    float4* __restrict__ new_vel_l, float4* __restrict__ new_pos_h,
+    float4* __restrict__ new_vel_h) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = *n_l;
+    const bool live = i < n;
+    const uint32_t f = live ? flag_l[i] : 2u;
+    const uint32_t rk = block_rank(live && f == 0u, wsum);
+    const uint32_t rs = block_rank(live && f == 1u, wsum);
+    if (!live) return;
+    const float4 x = pos_l[i];
+    const float4 v = vel_l[i];
+    if (f == 0u) {
+        const uint32_t dst = cl[blockIdx.x] + rk;
+        if (dst < cap_l) {
+            new_pos_l[dst] = x;
+            new_vel_l[dst] = make_float4(v.x, v.y, v.z, 0.0f);
+        }
+    } else {
+        const uint32_t base = tot[2] + 8u * (cl[nbl + blockIdx.x] + rs);
+        const uint32_t pid = __float_as_uint(x.w);
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t dst = base + k;
+            if (dst >= cap_h) break;
+            new_pos_h[dst] = make_float4(x.x + child_offset(k, 0, rg.child_d),
+                                         x.y + child_offset(k, 1, rg.child_d),
+                                         x.z + child_offset(k, 2, rg.child_d),
+                                         __uint_as_float(id_base + 8u * pid + k));
+            new_vel_h[dst] = make_float4(v.x, v.y, v.z,
+                                         __uint_as_float((static_cast<uint32_t>(k) << 29) |
+                                                         (pid & kGroupMask)));
+        }
+    }
+}
+
+// High set: survivors to new_h[keep rank]; merge leaders emit the merged
+// particle (centroid, mean velocity, id = group) to new_l[keepL + lead rank].
+// Every high particle clears its group's counter for the next step.
+__global__ __launch_bounds__(kBlock) void k_scatter_high(
+    const float4* __restrict__ pos_h, const float4* __restrict__ vel_h,
+    const uint32_t* __restrict__ flag_h, const uint32_t* __restrict__ n_h,
+    const uint32_t* __restrict__ ch, uint32_t nbh, const uint32_t* __restrict__ tot,
+    const float4* __restrict__ grp_pos, const float4* __restrict__ grp_vel,
+    uint32_t* __restrict__ grp_cnt, uint32_t num_groups, uint32_t cap_l, uint32_t cap_h,
+    float4* __restrict__ new_pos_l, float4* __restrict__ new_vel_l,
+    float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = *n_h;
+    const bool live = i < n;
+    const uint32_t f = live ? flag_h[i] : 1u;
+    const uint32_t rk = block_rank(live && !(f & 1u), wsum);
+    const uint32_t rl = block_rank(live && (f & 2u), wsum);
+    if (!live) return;
+    const float4 v = vel_h[i];
+    const uint32_t g = __float_as_uint(v.w) & kGroupMask;
+    if (g < num_groups) grp_cnt[g] = 0u;
+    if (!(f & 1u)) {
+        const uint32_t dst = ch[blockIdx.x] + rk;
+        if (dst < cap_h) {
+            new_pos_h[dst] = pos_h[i];
+            new_vel_h[dst] = v;
+        }
+    } else if (f & 2u) {
+        const uint32_t dst = tot[0] + ch[nbh + blockIdx.x] + rl;
+        if (dst < cap_l) {
+            float sx = 0.f, sy = 0.f, sz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
+            for (int k = 0; k < 8; ++k) {
+                const float4 p = grp_pos[static_cast<size_t>(g) * 8 + k];
+                const float4 q = grp_vel[static_cast<size_t>(g) * 8 + k];
+                sx += p.x;
+                sy += p.y;
+                sz += p.z;
+                ux += q.x;
+                uy += q.y;
+                uz += q.z;
+            }
+            new_pos_l[dst] = make_float4(0.125f * sx, 0.125f * sy, 0.125f * sz, __uint_as_float(g));
+            new_vel_l[dst] = make_float4(0.125f * ux, 0.125f * uy, 0.125f * uz, 0.0f);
+        }
+    }
+}
+
+// tot[6], tot[7] keep the counts the step ran with (for inspection).
+__global__ void k_commit_counts(uint32_t* __restrict__ tot, uint32_t* __restrict__ n_l,
+                                uint32_t* __restrict__ n_h) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        tot[6] = *n_l;
+        tot[7] = *n_h;
+        *n_l = tot[4];
+        *n_h = tot[5];
+    }
+}
+
+__global__ void k_pack_two_scale(const float* __restrict__ pxyz, const float* __restrict__ vxyz,
+                                 const uint32_t* __restrict__ ids,
+                                 const uint32_t* __restrict__ tags, uint32_t n,
+                                 float4* __restrict__ pos, float4* __restrict__ vel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos[i] = make_float4(pxyz[3 * i], pxyz[3 * i + 1], pxyz[3 * i + 2],
+                         __uint_as_float(ids ? ids[i] : i));
+    const float w = __uint_as_float(tags ? tags[i] : 0u);
+    vel[i] = vxyz ? make_float4(vxyz[3 * i], vxyz[3 * i + 1], vxyz[3 * i + 2], w)
+                  : make_float4(0.f, 0.f, 0.f, w);
+}
+
+__global__ void k_copy_state(const float4* __restrict__ sp, const float4* __restrict__ sv,
+                             const uint32_t* __restrict__ n_dev, float4* __restrict__ dp,
+                             float4* __restrict__ dv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    dp[i] = sp[i];
+    dv[i] = sv[i];
+}
+
+inline dim3 blocks_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1); }
+
+}  // namespace
+}  // namespace sph
+
+using namespace sph;
+
+// One resolution's buffers; `state_*` is the compacted state the next step
+// reads, `pos/vel` (in StepBuffers) the integrated state before split/merge.
+struct TwoScaleSet {
+    sph_config cfg{};        // h, mass and grid of this resolution
+    uint32_t cap = 0, num_cells = 0, key_bits = 1;
+    StepBuffers buf{};
+    float4* state_pos = nullptr;
+    float4* state_vel = nullptr;
+    float4* accel = nullptr;
+    uint32_t* flag = nullptr;
+    uint32_t* counts = nullptr;   // 2 * nblocks
+};
+
+struct sph_two_scale {
+    sph_config cfg{};            // low-resolution configuration
+    TwoScaleSet set[2];          // 0 = low (h0, m0), 1 = high (h1, m1)
+    Region region{};
+    uint32_t num_groups = 0;     // low-resolution ids ever issued (group ids)
+    uint32_t id_base = 0;        // first high-resolution id
+    float4* grp_pos = nullptr;
+    float4* grp_vel = nullptr;
+    uint32_t* grp_cnt = nullptr;
+    uint32_t* tot = nullptr;     // 6 words
+    uint32_t* error_word = nullptr;
+    hipStream_t stream = nullptr;
+    void* arena = nullptr;
+    hipGraphExec_t graph = nullptr;
+    float graph_dt = 0.0f;
+    bool graph_accel = false;
+    uint32_t debug = 0;
+};
+
+namespace {
+
+void destroy_graph2(sph_two_scale* t) {
+    if (t->graph) {
+        (void)hipGraphExecDestroy(t->graph);
+        t->graph = nullptr;
+    }
+}
+
+CrossParams cross_params(const sph_two_scale* t, int target) {
+    const sph_config& lo = t->set[0].cfg;
+    const sph_config& hi = t->set[1].cfg;
+    CrossParams c{};
+    c.h0 = lo.h;
+    c.h1 = hi.h;
+    c.h02 = lo.h * lo.h;
+    c.h12 = hi.h * hi.h;
+    const double h0 = lo.h, h1 = hi.h;
+    c.c6_0 = static_cast<float>(315.0 / (64.0 * kPi * std::pow(h0, 9)));
+    c.c6_1 = static_cast<float>(315.0 / (64.0 * kPi * std::pow(h1, 9)));
+    c.c45_0 = static_cast<float>(45.0 / (kPi * std::pow(h0, 6)));
+    c.c45_1 = static_cast<float>(45.0 / (kPi * std::pow(h1, 6)));
+    c.m_other = t->set[1 - target].cfg.mass;
+    c.stiffness = lo.stiffness;
+    c.rest_density = lo.rest_density;
+    c.visc = lo.viscosity;
+    c.tension = lo.tension;
+    c.gx = lo.gravity[0];
+    c.gy = lo.gravity[1];
+    c.gz = lo.gravity[2];
+    c.restitution = lo.restitution;
+    c.lox = lo.domain_lo[0];
+    c.loy = lo.domain_lo[1];
+    c.loz = lo.domain_lo[2];
+    c.hix = lo.domain_hi[0];
+    c.hiy = lo.domain_hi[1];
+    c.hiz = lo.domain_hi[2];
+    c.flags = lo.flags;
+    return c;
+}
+
+// The integrated states -> split/merge -> compacted states and new counts.
+int launch_split_merge(sph_two_scale* t) {
+    TwoScaleSet& L = t->set[0];
+    TwoScaleSet& H = t->set[1];
+    hipStream_t st = t->stream;
+    const uint32_t nbl = blocks_for(L.cap).x, nbh = blocks_for(H.cap).x;
+    hipLaunchKernelGGL(k_split_mark, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.n_dev,
+                       t->region, L.flag);
+    hipLaunchKernelGGL(k_merge_mark, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel,
+                       H.buf.n_dev, t->region, t->grp_pos, t->grp_vel, t->grp_cnt, t->num_groups);
+    hipLaunchKernelGGL(k_merge_flags, dim3(nbh), dim3(kBlock), 0, st, H.buf.vel, H.buf.n_dev,
+                       t->grp_cnt, t->num_groups, H.flag);
+    hipLaunchKernelGGL(k_block_counts, dim3(nbl), dim3(kBlock), 0, st, L.flag, L.buf.n_dev, 0,
+                       L.counts, nbl);
+    hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, st, H.flag, H.buf.n_dev, 1,
+                       H.counts, nbh);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
+                       L.cap, H.cap, t->tot, t->error_word);
+    hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel, L.flag,
+                       L.buf.n_dev, L.counts, nbl, t->tot, t->region, t->id_base, L.cap, H.cap,
+                       L.state_pos, L.state_vel, H.state_pos, H.state_vel);
+    hipLaunchKernelGGL(k_scatter_high, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel,
+                       H.flag, H.buf.n_dev, H.counts, nbh, t->tot, t->grp_pos, t->grp_vel,
+                       t->grp_cnt, t->num_groups, L.cap, H.cap, L.state_pos, L.state_vel,
+                       H.state_pos, H.state_vel);
+    hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(64), 0, st, t->tot, L.buf.n_dev,
+                       H.buf.n_dev);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
+    hipStream_t st = t->stream;
+    StepParams P[2];
+    GridView G[2], X[2];
+    for (int r = 0; r < 2; ++r) {
+        TwoScaleSet& S = t->set[r];
+        P[r] = make_step_params(S.cfg, S.cap);
+        G[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
+                              P[r].h);
+        // Cross search: the other set's grid, searched to h0 (the coupling cutoff).
+        X[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
+                              t->set[0].cfg.h);
+    }
+    // Neighbourhoods: the reference order is high then low (Solver.cu:1576-1577).
+    // The state the step reads is the compacted state; the reorder gathers from it.
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& S = t->set[r];
+        StepBuffers b = S.buf;
+        b.pos = S.state_pos;
+        b.vel = S.state_vel;
+        int rc = launch_pass(kPassHash, P[r], b, dt, S.key_bits, false, st);
+        if (rc == SPH_OK) rc = launch_pass(kPassSort, P[r], b, dt, S.key_bits, false, st);
+        if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, dt, S.key_bits, false, st);
+        if (rc != SPH_OK) return rc;
+    }
+    // Own-set densities, then the complementary sums (which also finalise A, B).
+    for (int r = 1; r >= 0; --r) {
+        int rc = launch_density_own(P[r], t->set[r].buf, st);
+        if (rc != SPH_OK) return rc;
+    }
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& A = t->set[r];
+        TwoScaleSet& B = t->set[1 - r];
+        hipLaunchKernelGGL(k_cross_density, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+                           A.buf.n_dev, B.buf.tpos, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
+                           X[1 - r], cross_params(t, r), A.buf.rho, A.buf.aux);
+    }
+    // Own-set forces (no integration), then complementary forces + integration.
+    for (int r = 1; r >= 0; --r) {
+        int rc = launch_force_own(P[r], G[r], t->set[r].buf, t->set[r].accel, st);
+        if (rc != SPH_OK) return rc;
+    }
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& A = t->set[r];
+        TwoScaleSet& B = t->set[1 - r];
+        const CrossParams c = cross_params(t, r);
+        float4* acc = accel_out ? A.buf.accel : nullptr;
+        if (c.tension != 0.0f)
+            hipLaunchKernelGGL(k_cross_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0, st,
+                               A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev, B.buf.tpos,
+                               B.buf.tvel, B.buf.aux, B.buf.cell_start, B.buf.cell_end,
+                               B.buf.n_dev, X[1 - r], c, dt, A.buf.pos, A.buf.vel, acc);
+        else
+            hipLaunchKernelGGL(k_cross_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
+                               st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev,
+                               B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
+                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, dt, A.buf.pos, A.buf.vel,
+                               acc);
+    }
+    SPH_HIP(hipGetLastError());
+    return launch_split_merge(t);
+}
+
+int capture_two_scale(sph_two_scale* t, float dt, bool accel) {
+    destroy_graph2(t);
+    hipGraph_t g = nullptr;
+    SPH_HIP(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_two_scale_step(t, dt, accel);
+    hipError_t e = hipStreamEndCapture(t->stream, &g);
+    if (rc != SPH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    e = hipGraphInstantiate(&t->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        t->graph = nullptr;
+        return hip_error(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    t->graph_dt = dt;
+    t->graph_accel = accel;
+    return SPH_OK;
+}
+
+size_t al(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+}  // namespace
+
+extern "C" {
+
+int sph2_create(const sph_config* cfg, const float region_lo[3], const float region_hi[3],
+                const float* pos, const float* vel, uint32_t n_low, uint32_t max_low,
+                uint32_t max_high, sph_two_scale** out) {
+    if (!out || !cfg || !region_lo || !region_hi)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (n_low > 0 && !pos) return set_error(SPH_ERR_INVALID_ARGUMENT, "positions are null");
+    if (max_low < n_low) max_low = n_low;
+    if (max_low == 0 || max_high == 0)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "capacities must be > 0");
+    if (n_low >= (1u << 29) || max_low >= (1u << 29))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "two-scale ids need n < 2^29");
+    if (!(cfg->h > 0.0f) || !(cfg->mass > 0.0f) || !(cfg->rest_density > 0.0f))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "config: h, mass, rest_density must be > 0");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return set_error(SPH_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    auto* t = new (std::nothrow) sph_two_scale;
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "out of host memory");
+    t->cfg = *cfg;
+    // Solver.h:36-43: h1 = h0/2, m1 = m0/8, one grid per resolution over the domain.
+    t->set[0].cfg = *cfg;
+    t->set[1].cfg = *cfg;
+    t->set[1].cfg.h = 0.5f * cfg->h;
+    t->set[1].cfg.mass = cfg->mass / 8.0f;
+    int rc = sph_make_grid(cfg->grid.origin, cfg->domain_hi, t->set[1].cfg.h, &t->set[1].cfg.grid);
+    if (rc != SPH_OK) {
+        delete t;
+        return rc;
+    }
+    t->set[0].cap = max_low;
+    t->set[1].cap = max_high;
+    t->num_groups = max_low;  // low-resolution ids stay below max_low
+    t->id_base = max_low;
+    for (int d = 0; d < 3; ++d) {
+        (&t->region.lox)[d] = region_lo[d];
+        (&t->region.hix)[d] = region_hi[d];
+    }
+    t->region.margin = 0.5f * cfg->h;
+    t->region.child_d = 0.25f * t->set[1].cfg.h;
+
+    size_t total = 0;
+    std::vector<size_t> sz;
+    auto add = [&](size_t b) {
+        sz.push_back(al(b));
+        total += al(b);
+    };
+    for (int r = 0; r < 2; ++r) {
+        TwoScaleSet& S = t->set[r];
+        S.num_cells = sph_grid_num_cells(&S.cfg.grid);
+        S.key_bits = key_bits_for_cells(S.num_cells);
+        const size_t N = S.cap, C = S.num_cells, nb = blocks_for(S.cap).x;
+        for (int k = 0; k < 4; ++k) add(16 * N);  // pos vel tpos tvel
+        add(4 * N);                                // rho
+        add(16 * N);                               // accel (debug)
+        for (int k = 0; k < 3; ++k) add(4 * N);    // keys skeys perm
+        add(4 * C);
+        add(4 * C);
+        add(radix_sort_temp_bytes(S.cap));
+        add(8 * N);                                // aux
+        add(4);                                    // n_dev
+        add(16 * N);                               // state_pos
+        add(16 * N);                               // state_vel
+        add(16 * N);                               // own-set accel
+        add(4 * N);                                // flags
+        add(8 * nb);                               // block counts
+    }
+    add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
+    add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
+    add(4 * static_cast<size_t>(t->num_groups));    // grp_cnt
+    add(64);                                        // tot[8] + error word
+
+    hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete t;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    e = hipMalloc(&t->arena, total);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(t->stream);
+        delete t;
+        return hip_error(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    char* a = static_cast<char*>(t->arena);
+    size_t idx = 0;
+    auto take = [&]() {
+        void* p = a;
+        a += sz[idx++];
+        return p;
+    };
+    for (int r = 0; r < 2; ++r) {
+        TwoScaleSet& S = t->set[r];
+        S.buf.pos = static_cast<float4*>(take());
+        S.buf.vel = static_cast<float4*>(take());
+        S.buf.tpos = static_cast<float4*>(take());
+        S.buf.tvel = static_cast<float4*>(take());
+        S.buf.rho = static_cast<float*>(take());
+        S.buf.accel = static_cast<float4*>(take());
+        S.buf.keys = static_cast<uint32_t*>(take());
+        S.buf.skeys = static_cast<uint32_t*>(take());
+        S.buf.perm = static_cast<uint32_t*>(take());
+        S.buf.cell_start = static_cast<uint32_t*>(take());
+        S.buf.cell_end = static_cast<uint32_t*>(take());
+        S.buf.sort_temp = take();
+        S.buf.sort_temp_bytes = radix_sort_temp_bytes(S.cap);
+        S.buf.aux = static_cast<float2*>(take());
+        S.buf.n_dev = static_cast<uint32_t*>(take());
+        S.state_pos = static_cast<float4*>(take());
+        S.state_vel = static_cast<float4*>(take());
+        S.accel = static_cast<float4*>(take());
+        S.flag = static_cast<uint32_t*>(take());
+        S.counts = static_cast<uint32_t*>(take());
+    }
+    t->grp_pos = static_cast<float4*>(take());
+    t->grp_vel = static_cast<float4*>(take());
+    t->grp_cnt = static_cast<uint32_t*>(take());
+    t->tot = static_cast<uint32_t*>(take());
+    t->error_word = t->tot + 8;
+
+    e = hipMemsetAsync(t->arena, 0, total, t->stream);
+    if (e != hipSuccess) {
+        sph2_destroy(t);
+        return hip_error(e, "hipMemsetAsync", __FILE__, __LINE__);
+    }
+    rc = sph2_set_state(t, pos, vel, nullptr, n_low, nullptr, nullptr, nullptr, nullptr, 0);
+    if (rc != SPH_OK) {
+        sph2_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return SPH_OK;
+}
+
+namespace {
+
+// Uploads one resolution's state (AoS host arrays) into its compacted-state
+// buffers, staging through the sorted-copy buffers.
+int upload_set(sph_two_scale* t, int r, const float* pos, const float* vel, const uint32_t* ids,
+               const uint32_t* tags, uint32_t n) {
+    TwoScaleSet& S = t->set[r];
+    if (n > S.cap) return set_error(SPH_ERR_CAPACITY, "two-scale: n exceeds capacity");
+    if (n > 0 && !pos) return set_error(SPH_ERR_INVALID_ARGUMENT, "positions are null");
+    SPH_HIP(hipMemcpyAsync(S.buf.n_dev, &n, sizeof(uint32_t), hipMemcpyHostToDevice, t->stream));
+    if (n == 0) return SPH_OK;
+    float* dp = reinterpret_cast<float*>(S.buf.tpos);
+    float* dv = reinterpret_cast<float*>(S.buf.tvel);
+    uint32_t* di = S.buf.keys;
+    uint32_t* dt = S.buf.skeys;
+    SPH_HIP(hipMemcpyAsync(dp, pos, 12ull * n, hipMemcpyHostToDevice, t->stream));
+    if (vel) SPH_HIP(hipMemcpyAsync(dv, vel, 12ull * n, hipMemcpyHostToDevice, t->stream));
+    if (ids) SPH_HIP(hipMemcpyAsync(di, ids, 4ull * n, hipMemcpyHostToDevice, t->stream));
+    if (tags) SPH_HIP(hipMemcpyAsync(dt, tags, 4ull * n, hipMemcpyHostToDevice, t->stream));
+    hipLaunchKernelGGL(k_pack_two_scale, blocks_for(n), dim3(kBlock), 0, t->stream, dp,
+                       vel ? dv : nullptr, ids ? di : nullptr, tags ? dt : nullptr, n, S.state_pos,
+                       S.state_vel);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+}  // namespace
+
+int sph2_set_state(sph_two_scale* t, const float* pos_l, const float* vel_l,
+                   const uint32_t* ids_l, uint32_t n_l, const float* pos_h, const float* vel_h,
+                   const uint32_t* ids_h, const uint32_t* tags_h, uint32_t n_h) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    if (n_h > 0 && (!ids_h || !tags_h))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "high-resolution particles need ids and tags");
+    int rc = upload_set(t, 0, pos_l, vel_l, ids_l, nullptr, n_l);
+    if (rc == SPH_OK) rc = upload_set(t, 1, pos_h, vel_h, ids_h, tags_h, n_h);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    destroy_graph2(t);
+    return SPH_OK;
+}
+
+int sph2_advance(sph_two_scale* t, float dt, uint32_t n_steps) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    if (n_steps == 0) return SPH_OK;
+    const bool accel = (t->debug & SPH_DEBUG_ACCEL) != 0u;
+    if (!t->graph || t->graph_dt != dt || t->graph_accel != accel) {
+        const int rc = capture_two_scale(t, dt, accel);
+        if (rc != SPH_OK) return rc;
+    }
+    for (uint32_t i = 0; i < n_steps; ++i) SPH_HIP(hipGraphLaunch(t->graph, t->stream));
+    return SPH_OK;
+}
+
+int sph2_advance_eager(sph_two_scale* t, float dt, uint32_t n_steps) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    if (!(dt > 0.0f)) return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    const bool accel = (t->debug & SPH_DEBUG_ACCEL) != 0u;
+    for (uint32_t i = 0; i < n_steps; ++i) {
+        const int rc = launch_two_scale_step(t, dt, accel);
+        if (rc != SPH_OK) return rc;
+    }
+    return SPH_OK;
+}
+
+int sph2_apply_split_merge(sph_two_scale* t) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    for (int r = 0; r < 2; ++r) {
+        TwoScaleSet& S = t->set[r];
+        hipLaunchKernelGGL(k_copy_state, blocks_for(S.cap), dim3(kBlock), 0, t->stream, S.state_pos,
+                           S.state_vel, S.buf.n_dev, S.buf.pos, S.buf.vel);
+    }
+    SPH_HIP(hipGetLastError());
+    return launch_split_merge(t);
+}
+
+int sph2_synchronize(sph_two_scale* t) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
+}
+
+int sph2_set_debug(sph_two_scale* t, uint32_t flags) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    t->debug = flags;
+    return SPH_OK;
+}
+
+int sph2_counts(sph_two_scale* t, uint32_t counts[4]) {
+    if (!t || !counts) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t tot[8] = {};
+    SPH_HIP(hipMemcpyAsync(&counts[0], t->set[0].buf.n_dev, 4, hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipMemcpyAsync(&counts[1], t->set[1].buf.n_dev, 4, hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipMemcpyAsync(tot, t->tot, sizeof(tot), hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    counts[2] = tot[6];
+    counts[3] = tot[7];
+    return SPH_OK;
+}
+
+int sph2_device_errors(sph_two_scale* t, uint32_t* bits) {
+    if (!t || !bits) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t v[3] = {};
+    SPH_HIP(hipMemcpyAsync(&v[0], t->error_word, 4, hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipMemcpyAsync(&v[1], radix_sort_error_word(t->set[0].buf.sort_temp), 4,
+                           hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipMemcpyAsync(&v[2], radix_sort_error_word(t->set[1].buf.sort_temp), 4,
+                           hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    *bits = ((v[1] | v[2]) ? SPH_DEVERR_SORT_WAIT : 0u) | ((v[0] & 2u) ? SPH_DEVERR_CAPACITY : 0u);
+    if (*bits) {
+        SPH_HIP(hipMemsetAsync(t->error_word, 0, 4, t->stream));
+        SPH_HIP(hipMemsetAsync(radix_sort_error_word(t->set[0].buf.sort_temp), 0, 4, t->stream));
+        SPH_HIP(hipMemsetAsync(radix_sort_error_word(t->set[1].buf.sort_temp), 0, 4, t->stream));
+        SPH_HIP(hipStreamSynchronize(t->stream));
+    }
+    return SPH_OK;
+}
+
+int sph2_download(sph_two_scale* t, int resolution, int field, void* dst, size_t bytes) {
+    if (!t || !dst || resolution < 0 || resolution > 1)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "bad argument");
+    uint32_t c[4];
+    int rc = sph2_counts(t, c);
+    if (rc != SPH_OK) return rc;
+    TwoScaleSet& S = t->set[resolution];
+    const size_t n_now = c[resolution], n_step = c[2 + resolution], C = S.num_cells;
+    const float4* vec = nullptr;
+    const void* src = nullptr;
+    size_t n = 0, need = 0;
+    int mode = 0;  // 0 raw copy, 1 xyz of float4, 2 w bits of float4
+    switch (field) {
+        case SPH_FIELD_POSITION: vec = S.state_pos; n = n_now; need = 12 * n; mode = 1; break;
+        case SPH_FIELD_VELOCITY: vec = S.state_vel; n = n_now; need = 12 * n; mode = 1; break;
+        case SPH_FIELD_ID: vec = S.state_pos; n = n_now; need = 4 * n; mode = 2; break;
+        case SPH_FIELD_TAG: vec = S.state_vel; n = n_now; need = 4 * n; mode = 2; break;
+        case SPH_FIELD_DENSITY: src = S.buf.rho; n = n_step; need = 4 * n; break;
+        case SPH_FIELD_SORTED_HASH: src = S.buf.skeys; n = n_step; need = 4 * n; break;
+        case SPH_FIELD_SORTED_INDEX: src = S.buf.perm; n = n_step; need = 4 * n; break;
+        case SPH_FIELD_CELL_START: src = S.buf.cell_start; need = 4 * C; break;
+        case SPH_FIELD_CELL_END: src = S.buf.cell_end; need = 4 * C; break;
+        case SPH_FIELD_ACCEL:
+            if (!(t->debug & SPH_DEBUG_ACCEL))
+                return set_error(SPH_ERR_INVALID_ARGUMENT, "accelerations need SPH_DEBUG_ACCEL");
+            vec = S.buf.accel; n = n_step; need = 12 * n; mode = 1; break;
+        default: return set_error(SPH_ERR_INVALID_ARGUMENT, "unknown field");
+    }
+    if (bytes < need) return set_error(SPH_ERR_INVALID_ARGUMENT, "destination too small");
+    if (need == 0) return SPH_OK;
+    if (mode == 0) {
+        SPH_HIP(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, t->stream));
+        SPH_HIP(hipStreamSynchronize(t->stream));
+        return SPH_OK;
+    }
+    std::vector<float> tmp(4 * n);
+    SPH_HIP(hipMemcpyAsync(tmp.data(), vec, 16 * n, hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    if (mode == 2) {
+        auto* o = static_cast<uint32_t*>(dst);
+        for (size_t i = 0; i < n; ++i) std::memcpy(&o[i], &tmp[4 * i + 3], 4);
+    } else {
+        auto* o = static_cast<float*>(dst);
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) o[3 * i + d] = tmp[4 * i + d];
+    }
+    return SPH_OK;
+}
+
+int sph2_destroy(sph_two_scale* t) {
+    if (!t) return SPH_OK;
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    destroy_graph2(t);
+    if (t->arena) (void)hipFree(t->arena);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+    return SPH_OK;
+}
+
+}  // extern "C"

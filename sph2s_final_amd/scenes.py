@@ -11,6 +11,9 @@ box and larger lattices keep that block-to-domain ratio.
 Named workloads:
   dam_break_32k  32^3 = 32,768 particles, domain [0, 1]^3, seed 1
   dam_break_1m   100^3 = 1,000,000 particles, domain [0, 3.125]^3, seed 2
+  two_scale_250k 63^3 = 250,047 low-resolution particles, domain [0, 1.969]^3,
+                 seed 3, high-resolution region x in [0.6 L, L], y in [0, 0.5 L],
+                 z in [0, L] (20% of the domain volume, the far-wall impact zone)
 """
 from __future__ import annotations
 
@@ -85,3 +88,36 @@ def dam_break_32k() -> Scene:
 
 def dam_break_1m() -> Scene:
     return dam_break((100, 100, 100), seed=2, name="dam_break_1m")
+
+
+@dataclass
+class TwoScaleScene:
+    name: str
+    config: C.SphConfig
+    positions: np.ndarray
+    velocities: np.ndarray
+    region_lo: tuple
+    region_hi: tuple
+    dt: float = DT
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+
+def impact_region(extent):
+    """Axis-aligned box over the far-wall impact zone: 20% of the domain volume."""
+    L = [float(v) for v in extent]
+    return (0.6 * L[0], 0.0, 0.0), (L[0], 0.5 * L[1], L[2])
+
+
+def two_scale_dam_break(n=63, seed=3, name=None) -> TwoScaleScene:
+    sc = dam_break((n, n, n), seed=seed)
+    hi = [sc.config.domain_hi[d] for d in range(3)]
+    lo_r, hi_r = impact_region(hi)
+    return TwoScaleScene(name or f"two_scale_{n}", sc.config, sc.positions, sc.velocities, lo_r,
+                         hi_r)
+
+
+def two_scale_250k() -> TwoScaleScene:
+    return two_scale_dam_break(63, seed=3, name="two_scale_250k")

@@ -244,3 +244,109 @@ class Solver:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class TwoScaleSolver:
+    """Two-scale Solver (SPH2S_FINAL_TWO_SCALE/Solver.h:109-144): a
+    low-resolution set (h0, m0) and a high-resolution set (h0/2, m0/8) coupled
+    through the complementary averaged-kernel sums, with 1->8 split inside
+    ``region`` and 8->1 merge of sibling groups that leave it.  Resolution 0 is
+    the low set, 1 the high set."""
+
+    def __init__(self, configuration: C.SphConfig, region_lo, region_hi, positions,
+                 velocities=None, max_low: int | None = None, max_high: int | None = None):
+        pos = _f32(positions)
+        vel = None if velocities is None else _f32(velocities)
+        n = int(pos.shape[0])
+        self.max_low = int(max_low if max_low is not None else n)
+        self.max_high = int(max_high if max_high is not None else 8 * self.max_low)
+        self._h = c_void_p()
+        C.check(C.lib().sph2_create(byref(configuration), C.f3(region_lo), C.f3(region_hi),
+                                    pos.ctypes.data_as(c_void_p) if n else None,
+                                    None if vel is None else vel.ctypes.data_as(c_void_p),
+                                    n, self.max_low, self.max_high, byref(self._h)))
+        self.configuration = configuration
+
+    def Advance(self, dt: float, steps: int = 1) -> None:
+        C.check(C.lib().sph2_advance(self._h, float(dt), int(steps)))
+
+    def advance_eager(self, dt: float, steps: int = 1) -> None:
+        C.check(C.lib().sph2_advance_eager(self._h, float(dt), int(steps)))
+
+    def apply_split_merge(self) -> None:
+        C.check(C.lib().sph2_apply_split_merge(self._h))
+
+    def synchronize(self) -> None:
+        C.check(C.lib().sph2_synchronize(self._h))
+
+    def set_debug(self, flags: int) -> None:
+        C.check(C.lib().sph2_set_debug(self._h, int(flags)))
+
+    def set_state(self, pos_l, vel_l=None, ids_l=None, pos_h=None, vel_h=None, ids_h=None,
+                  tags_h=None) -> None:
+        pl = _f32(pos_l)
+        vl = None if vel_l is None else _f32(vel_l)
+        il = None if ids_l is None else np.ascontiguousarray(ids_l, np.uint32)
+        nh = 0 if pos_h is None else len(pos_h)
+        ph = None if pos_h is None else _f32(pos_h)
+        vh = None if vel_h is None else _f32(vel_h)
+        ih = None if ids_h is None else np.ascontiguousarray(ids_h, np.uint32)
+        th = None if tags_h is None else np.ascontiguousarray(tags_h, np.uint32)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(c_void_p)  # noqa: E731
+        C.check(C.lib().sph2_set_state(self._h, ptr(pl), ptr(vl), ptr(il), len(pl), ptr(ph),
+                                       ptr(vh), ptr(ih), ptr(th), nh))
+
+    def counts(self) -> tuple[int, int, int, int]:
+        """(low, high) now and (low, high) the last step ran with."""
+        c = (c_uint32 * 4)()
+        C.check(C.lib().sph2_counts(self._h, c))
+        return tuple(int(v) for v in c)
+
+    def device_errors(self) -> int:
+        bits = c_uint32()
+        C.check(C.lib().sph2_device_errors(self._h, byref(bits)))
+        return int(bits.value)
+
+    def download(self, resolution: int, field: int) -> np.ndarray:
+        cnt = self.counts()
+        n_now, n_step = cnt[resolution], cnt[2 + resolution]
+        if field in (C.FIELD_POSITION, C.FIELD_VELOCITY):
+            out = np.empty((n_now, 3), np.float32)
+        elif field == C.FIELD_ACCEL:
+            out = np.empty((n_step, 3), np.float32)
+        elif field in (C.FIELD_ID, C.FIELD_TAG):
+            out = np.empty(n_now, np.uint32)
+        elif field == C.FIELD_DENSITY:
+            out = np.empty(n_step, np.float32)
+        elif field in (C.FIELD_SORTED_HASH, C.FIELD_SORTED_INDEX):
+            out = np.empty(n_step, np.uint32)
+        elif field in (C.FIELD_CELL_START, C.FIELD_CELL_END):
+            g = self.configuration.grid
+            dims = [int(g.dims[d]) for d in range(3)]
+            if resolution == 1:
+                ext = [self.configuration.domain_hi[d] - g.origin[d] for d in range(3)]
+                hh = np.float32(0.5) * np.float32(self.configuration.h)
+                dims = [int(np.ceil(np.float32(e) / hh)) for e in ext]
+            out = np.empty(dims[0] * dims[1] * dims[2], np.uint32)
+        else:
+            raise ValueError(f"unknown field {field}")
+        C.check(C.lib().sph2_download(self._h, int(resolution), int(field),
+                                      out.ctypes.data_as(c_void_p), out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            C.lib().sph2_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
