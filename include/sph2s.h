@@ -226,6 +226,35 @@ int sph2_device_errors(sph_two_scale* t, uint32_t* bits);
 int sph2_download(sph_two_scale* t, int resolution, int field, void* host_dst,
                   size_t bytes);
 
+/* ---- batched independent scenes --------------------------------------------
+ * The reference runs one scene per process through a global __constant__
+ * configuration (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:12,1068-1080).  A batch
+ * advances n_scenes independent single-scale scenes, each with its own
+ * sph_config (grid, stiffness, viscosity, domain), in one segmented hash/sort
+ * and one launch per pass; the step replays as a hipGraph.  Scene s owns the
+ * particle range [part_offsets[s], part_offsets[s+1]) and the global cell
+ * range [cell_offsets[s], cell_offsets[s+1]) of the batch's arrays; every
+ * scene's results equal those of an sph_solver running that scene alone.
+ * pos/vel: host AoS 3 * sum(counts), scene-major; vel may be NULL. */
+typedef struct sph_batch sph_batch;
+int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes,
+                     const uint32_t* counts, const float* pos_xyz,
+                     const float* vel_xyz, sph_batch** out);
+int sph_batch_destroy(sph_batch* b);
+int sph_batch_advance(sph_batch* b, float dt, uint32_t n_steps);
+int sph_batch_advance_eager(sph_batch* b, float dt, uint32_t n_steps);
+int sph_batch_synchronize(sph_batch* b);
+int sph_batch_set_debug(sph_batch* b, uint32_t flags);
+/* part_offsets and cell_offsets (n_scenes + 1 entries each) may be NULL. */
+int sph_batch_info(const sph_batch* b, uint32_t* n_scenes, uint32_t* n_particles,
+                   uint32_t* n_cells, uint32_t* part_offsets, uint32_t* cell_offsets);
+int sph_batch_device_errors(sph_batch* b, uint32_t* bits);
+/* Adds the milliseconds of each pass over n_steps eager steps to ms[5]. */
+int sph_batch_profile_passes(sph_batch* b, float dt, uint32_t n_steps, float* ms);
+/* Host copy of one field over the whole batch (fields as sph_download; TAG
+ * returns each particle's scene index; CELL_* cover all scenes' cells). */
+int sph_batch_download(sph_batch* b, int field, void* host_dst, size_t bytes);
+
 /* ---- standalone stable LSD radix sort (replaces thrust::sort_by_key,
  *      SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115) ----------------------- */
 /* Temporary device bytes needed to sort n pairs. */

@@ -22,6 +22,7 @@ struct GridView {
     int dx, dy, dz;
     float h;            // search radius
     float prune_h2;     // (h (1 + 1e-3))^2
+    uint32_t cell_offset;  // first global cell of this grid (batched scenes share one cell array)
 };
 
 __host__ __device__ inline GridView make_grid_view(float ox, float oy, float oz, float spacing,
@@ -38,6 +39,7 @@ __host__ __device__ inline GridView make_grid_view(float ox, float oy, float oz,
     g.h = h;
     const float ph = h * 1.001f;
     g.prune_h2 = ph * ph;
+    g.cell_offset = 0u;
     return g;
 }
 
@@ -89,7 +91,8 @@ __device__ __forceinline__ void for_each_run(const GridView& g, const uint32_t* 
             const int hi = min(ex, static_cast<int>(floorf((fx + w) * g.inv_spacing)));
             if (lo > hi) continue;
             uint32_t b, e;
-            row_run(cs, ce, static_cast<uint32_t>(g.dx) * (cy + g.dy * cz), lo, hi, n, b, e);
+            row_run(cs, ce, g.cell_offset + static_cast<uint32_t>(g.dx) * (cy + g.dy * cz), lo, hi,
+                    n, b, e);
             if (b < e) f(b, e);
         }
     }
@@ -98,6 +101,24 @@ __device__ __forceinline__ void for_each_run(const GridView& g, const uint32_t* 
 }  // namespace sph
 
 namespace sph {
+
+// Per-scene constants of a batch: the scene's step parameters and grid, whose
+// cells start at g.cell_offset in the batch's global cell array.
+struct SceneParams {
+    StepParams p;
+    GridView g;
+};
+
+// Batched-scene passes (step.hip / radix_sort.hip): cap.n is the launch
+// capacity, cap.num_cells the batch's total cell count, cap.h/spacing the grid
+// used when a pass needs no per-scene constants.
+int hash_hist_cells_batched(const StepParams& cap, const uint32_t* n_dev, const float4* pos,
+                            const float4* vel, const SceneParams* scenes, uint32_t* keys,
+                            uint32_t key_bits, void* temp, size_t temp_bytes,
+                            uint32_t* cell_start, uint32_t* cell_end, hipStream_t stream);
+int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
+                      const SceneParams* scenes, bool kahan, bool tension, float dt,
+                      uint32_t key_bits, bool write_accel, hipStream_t stream);
 
 // Per-particle view of the reference stencil: own cell, stencil bounds and
 // position relative to the grid origin.
@@ -142,7 +163,8 @@ __device__ __forceinline__ void probe_row_run(const GridView& g, const Probe& pr
     const int lo = max(pr.sx, static_cast<int>(floorf((pr.fx - w) * g.inv_spacing)));
     const int hi = min(pr.ex, static_cast<int>(floorf((pr.fx + w) * g.inv_spacing)));
     if (lo > hi) return;
-    row_run(cs, ce, static_cast<uint32_t>(g.dx) * (ry + g.dy * rz), lo, hi, n, b, e);
+    row_run(cs, ce, g.cell_offset + static_cast<uint32_t>(g.dx) * (ry + g.dy * rz), lo, hi, n, b,
+            e);
     if (b >= e) {
         b = kEmptyCell;
         e = 0u;

@@ -18,7 +18,7 @@
 // an error bit and gives up instead of hanging.
 #include <algorithm>
 
-#include "internal.hpp"
+#include "neighbors.hpp"
 
 namespace sph {
 namespace {
@@ -101,6 +101,24 @@ struct HashKeys {
     __device__ uint32_t operator()(uint32_t i) const {
         const float4 x = pos[i];
         const uint32_t k = cell_hash(x.x, x.y, x.z, ox, oy, oz, spacing, dx, dy, dz);
+        keys_out[i] = k;
+        return k;
+    }
+};
+
+// Batched scenes: the scene id is in vel.w; the key is the scene's first global
+// cell plus the reference hash of the position in the scene's own grid, so a
+// stable sort keeps every scene contiguous and in its own hash order.
+struct BatchHashKeys {
+    const float4* pos;
+    const float4* vel;
+    const SceneParams* scenes;
+    uint32_t* keys_out;
+    __device__ uint32_t operator()(uint32_t i) const {
+        const float4 x = pos[i];
+        const GridView& g = scenes[__float_as_uint(vel[i].w)].g;
+        const uint32_t k =
+            g.cell_offset + cell_hash(x.x, x.y, x.z, g.ox, g.oy, g.oz, g.spacing, g.dx, g.dy, g.dz);
         keys_out[i] = k;
         return k;
     }
@@ -369,6 +387,17 @@ int hash_hist_cells(const StepParams& p, const uint32_t* n_dev, const float4* po
     HashKeys src{pos, keys, p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz};
     return prepare_hist(src, p.n, n_dev, p.n, key_bits, temp, cell_start, cell_end, p.num_cells,
                         stream);
+}
+
+int hash_hist_cells_batched(const StepParams& cap, const uint32_t* n_dev, const float4* pos,
+                            const float4* vel, const SceneParams* scenes, uint32_t* keys,
+                            uint32_t key_bits, void* temp, size_t temp_bytes,
+                            uint32_t* cell_start, uint32_t* cell_end, hipStream_t stream) {
+    if (!temp || temp_bytes < radix_sort_temp_bytes(cap.n))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "hash: sort temp buffer too small");
+    BatchHashKeys src{pos, vel, scenes, keys};
+    return prepare_hist(src, cap.n, n_dev, cap.n, key_bits, temp, cell_start, cell_end,
+                        cap.num_cells, stream);
 }
 
 int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint32_t* perm,

@@ -105,14 +105,30 @@ __device__ __forceinline__ float poly6_run(const float4& xi, float h2, uint32_t 
 // summed in plain FP32 and the run sums are combined with Kahan compensation
 // (with kKahan).  Also emits the per-particle factors the force pass needs for
 // every neighbour j: A_j = k (rho_j - rho0) / rho_j^2 and B_j = 1 / rho_j.
-template <bool kKahan, bool kFinalize>
+// Batched scenes (kBatched): each particle's scene id is in tvel[s].w and its
+// constants come from scenes[id]; runs never leave a scene because each
+// scene's cells occupy their own range of the global cell array.
+template <bool kBatched>
+__device__ __forceinline__ void scene_constants(const SceneParams* __restrict__ scenes,
+                                                const float4* __restrict__ tvel, uint32_t s,
+                                                bool active, StepParams& p, GridView& g) {
+    if (kBatched) {
+        const uint32_t sc = active ? __float_as_uint(tvel[s].w) : 0u;
+        p = scenes[sc].p;
+        g = scenes[sc].g;
+    }
+}
+
+template <bool kKahan, bool kFinalize, bool kBatched>
 __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
-                                                    StepParams p, GridView g,
+                                                    StepParams p_in, GridView g_in,
                                                     const uint32_t* __restrict__ n_dev,
                                                     float* __restrict__ rho,
-                                                    float2* __restrict__ aux) {
+                                                    float2* __restrict__ aux,
+                                                    const float4* __restrict__ tvel,
+                                                    const SceneParams* __restrict__ scenes) {
     __shared__ float4 tile[kDensityTile];
     __shared__ uint32_t red[2 * kWavesPerBlock];
     const uint32_t n = *n_dev;
@@ -120,6 +136,9 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
     if (block0 >= n) return;  // block-uniform: the grid covers the capacity
     const uint32_t s = block0 + threadIdx.x;
     const bool active = s < n;
+    StepParams p = p_in;
+    GridView g = g_in;
+    scene_constants<kBatched>(scenes, tvel, s, active, p, g);
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sum = 0.0f, comp = 0.0f;
@@ -189,16 +208,18 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
 // tile holds (x, y, z, A_j) and (vx, vy, vz, B_j) per candidate.
 // With kIntegrate = false (two-scale own-set pass) the kernel only writes the
 // acceleration, gravity included, to accel.
-template <bool kTension, bool kIntegrate>
+template <bool kTension, bool kIntegrate, bool kBatched>
 __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
                                                   const float2* __restrict__ aux,
                                                   const uint32_t* __restrict__ cs,
-                                                  const uint32_t* __restrict__ ce, StepParams p,
-                                                  GridView g, const uint32_t* __restrict__ n_dev,
+                                                  const uint32_t* __restrict__ ce,
+                                                  StepParams p_in, GridView g_in,
+                                                  const uint32_t* __restrict__ n_dev,
                                                   float dt, float4* __restrict__ pos,
                                                   float4* __restrict__ vel,
-                                                  float4* __restrict__ accel) {
+                                                  float4* __restrict__ accel,
+                                                  const SceneParams* __restrict__ scenes) {
     __shared__ float4 tp[kForceTile];
     __shared__ float4 tv[kForceTile];
     __shared__ uint32_t list[kListCap][kBlock];
@@ -208,6 +229,9 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
     if (block0 >= n) return;  // block-uniform: the grid covers the capacity
     const uint32_t s = block0 + threadIdx.x;
     const bool active = s < n;
+    StepParams p = p_in;
+    GridView g = g_in;
+    scene_constants<kBatched>(scenes, tvel, s, active, p, g);
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float2 ai = active ? aux[s] : make_float2(0.f, 1.f);
@@ -386,22 +410,22 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
             break;
         case kPassDensity:
             if (p.flags & SPH_FLAG_KAHAN)
-                hipLaunchKernelGGL((k_density<true, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
+                hipLaunchKernelGGL((k_density<true, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
             else
-                hipLaunchKernelGGL((k_density<false, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
+                hipLaunchKernelGGL((k_density<false, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
             break;
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (p.tens != 0.0f)
-                hipLaunchKernelGGL((k_force<true, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_force<true, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
                                    b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g,
-                                   b.n_dev, dt, b.pos, b.vel, acc);
+                                   b.n_dev, dt, b.pos, b.vel, acc, nullptr);
             else
-                hipLaunchKernelGGL((k_force<false, true>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                hipLaunchKernelGGL((k_force<false, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
                                    b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g,
-                                   b.n_dev, dt, b.pos, b.vel, acc);
+                                   b.n_dev, dt, b.pos, b.vel, acc, nullptr);
             break;
         }
         default:
@@ -424,11 +448,11 @@ int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t st
     if (p.n == 0) return SPH_OK;
     const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
     if (p.flags & SPH_FLAG_KAHAN)
-        hipLaunchKernelGGL((k_density<true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
+        hipLaunchKernelGGL((k_density<true, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
     else
-        hipLaunchKernelGGL((k_density<false, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux);
+        hipLaunchKernelGGL((k_density<false, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
+                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -437,13 +461,13 @@ int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& 
                      float4* accel_own, hipStream_t stream) {
     if (p.n == 0) return SPH_OK;
     if (p.tens != 0.0f)
-        hipLaunchKernelGGL((k_force<true, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
+        hipLaunchKernelGGL((k_force<true, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
                            b.tvel, b.aux, b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f, nullptr,
-                           nullptr, accel_own);
+                           nullptr, accel_own, nullptr);
     else
-        hipLaunchKernelGGL((k_force<false, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
+        hipLaunchKernelGGL((k_force<false, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
                            b.tvel, b.aux, b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f, nullptr,
-                           nullptr, accel_own);
+                           nullptr, accel_own, nullptr);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -460,6 +484,52 @@ int launch_pack_state(const float* pxyz, const float* vxyz, uint32_t n, float4* 
 int launch_export_aos(const float4* src, uint32_t n, float* dst, hipStream_t stream) {
     if (n == 0) return SPH_OK;
     hipLaunchKernelGGL(k_export_aos, blocks_for(n), dim3(kBlock), 0, stream, src, n, dst);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
+                      const SceneParams* scenes, bool kahan, bool tension, float dt,
+                      uint32_t key_bits, bool write_accel, hipStream_t stream) {
+    if (cap.n == 0) return SPH_OK;
+    const GridView g0 = make_grid_view(cap.ox, cap.oy, cap.oz, cap.spacing, cap.dx, cap.dy, cap.dz,
+                                       cap.h);
+    switch (pass) {
+        case kPassHash: {
+            const int rc = hash_hist_cells_batched(cap, b.n_dev, b.pos, b.vel, scenes, b.keys,
+                                                   key_bits, b.sort_temp, b.sort_temp_bytes,
+                                                   b.cell_start, b.cell_end, stream);
+            if (rc != SPH_OK) return rc;
+            break;
+        }
+        case kPassSort:
+        case kPassReorder:
+            return launch_pass(pass, cap, b, dt, key_bits, write_accel, stream);
+        case kPassDensity:
+            if (kahan)
+                hipLaunchKernelGGL((k_density<true, true, true>), blocks_for(cap.n), dim3(kBlock), 0,
+                                   stream, b.tpos, b.cell_start, b.cell_end, cap, g0, b.n_dev,
+                                   b.rho, b.aux, b.tvel, scenes);
+            else
+                hipLaunchKernelGGL((k_density<false, true, true>), blocks_for(cap.n), dim3(kBlock),
+                                   0, stream, b.tpos, b.cell_start, b.cell_end, cap, g0, b.n_dev,
+                                   b.rho, b.aux, b.tvel, scenes);
+            break;
+        case kPassForce: {
+            float4* acc = write_accel ? b.accel : nullptr;
+            if (tension)
+                hipLaunchKernelGGL((k_force<true, true, true>), blocks_for(cap.n), dim3(kBlock), 0,
+                                   stream, b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, cap,
+                                   g0, b.n_dev, dt, b.pos, b.vel, acc, scenes);
+            else
+                hipLaunchKernelGGL((k_force<false, true, true>), blocks_for(cap.n), dim3(kBlock), 0,
+                                   stream, b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, cap,
+                                   g0, b.n_dev, dt, b.pos, b.vel, acc, scenes);
+            break;
+        }
+        default:
+            return set_error(SPH_ERR_INVALID_ARGUMENT, "unknown pass");
+    }
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }

@@ -121,3 +121,64 @@ def two_scale_dam_break(n=63, seed=3, name=None) -> TwoScaleScene:
 
 def two_scale_250k() -> TwoScaleScene:
     return two_scale_dam_break(63, seed=3, name="two_scale_250k")
+
+
+@dataclass
+class BatchScenes:
+    name: str
+    scenes: list          # list[Scene]
+    stiffness: np.ndarray
+    viscosity: np.ndarray
+    dt: float = DT
+
+    @property
+    def counts(self) -> np.ndarray:
+        return np.array([s.n for s in self.scenes], np.int64)
+
+    @property
+    def n(self) -> int:
+        return int(self.counts.sum())
+
+
+def truncated_dam_break(count: int, seed: int, *, stiffness=STIFFNESS, viscosity=VISCOSITY,
+                        velocity_sigma=0.0, name=None) -> Scene:
+    """A dam break of exactly ``count`` particles: the first ``count`` points of
+    an a x a x ceil(count / a^2) lattice, a = ceil(count^(1/3)) (the top z layer
+    is partial), in the cubic domain [0, a / 32]^3."""
+    count = int(count)
+    a = int(np.ceil(round(count ** (1.0 / 3.0), 9)))
+    while a ** 3 < count:
+        a += 1
+    nz = -(-count // (a * a))
+    sc = dam_break((a, a, nz), seed=seed, stiffness=stiffness, viscosity=viscosity,
+                   velocity_sigma=velocity_sigma, name=name)
+    hi = (a / REFERENCE_LATTICE,) * 3
+    cfg = SolverConfiguration.FromConstants(H, particle_mass(), REST_DENSITY, stiffness, viscosity,
+                                            (0.0, 0.0, 0.0), hi, kahan=True)
+    return Scene(sc.name, cfg, np.ascontiguousarray(sc.positions[:count]),
+                 np.ascontiguousarray(sc.velocities[:count]))
+
+
+def batched_dam_breaks(n_scenes=64, seed=4, n_min=4096, n_max=262144, k_range=(1.0, 5.0),
+                       mu_range=(1.0, 8.0), name=None) -> BatchScenes:
+    """Independent dam breaks of uneven size: N_s = round(exp U[ln n_min,
+    ln n_max]), stiffness U[k_range], viscosity U[mu_range], drawn in that order
+    per scene from numpy's PCG64 seeded with ``seed``; scene s's lattice jitter
+    uses seed 1000 * seed + s."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scenes, ks, mus = [], [], []
+    for s in range(int(n_scenes)):
+        cnt = int(round(float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))))
+        k = float(np.float32(rng.uniform(*k_range)))
+        mu = float(np.float32(rng.uniform(*mu_range)))
+        scenes.append(truncated_dam_break(cnt, 1000 * seed + s, stiffness=k, viscosity=mu,
+                                          name=f"scene_{s}"))
+        ks.append(k)
+        mus.append(mu)
+    return BatchScenes(name or f"batched_{n_scenes}", scenes, np.array(ks, np.float32),
+                       np.array(mus, np.float32))
+
+
+def batched_64() -> BatchScenes:
+    """64 scenes, N_s log-uniform in [4096, 262144], seed 4."""
+    return batched_dam_breaks(64, seed=4, name="batched_64")

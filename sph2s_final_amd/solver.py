@@ -350,3 +350,100 @@ class TwoScaleSolver:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class BatchSolver:
+    """Independent single-scale scenes advanced together: one segmented
+    hash/sort and one launch per pass for every scene, replayed as a hipGraph.
+    ``configurations[s]`` holds scene s's own grid, stiffness, viscosity and
+    domain; ``positions[s]`` / ``velocities[s]`` its (N_s, 3) state.  Each
+    scene's results equal a ``Solver`` running that scene alone."""
+
+    def __init__(self, configurations, positions, velocities=None):
+        cfgs = list(configurations)
+        if not cfgs or len(positions) != len(cfgs):
+            raise ValueError("one configuration and one position array per scene")
+        pos = [_f32(p) for p in positions]
+        self.counts = np.array([p.shape[0] for p in pos], np.uint32)
+        allpos = np.ascontiguousarray(np.concatenate(pos, axis=0))
+        allvel = None
+        if velocities is not None:
+            allvel = np.ascontiguousarray(np.concatenate([_f32(v) for v in velocities], axis=0))
+            if allvel.shape != allpos.shape:
+                raise ValueError("velocities must match positions")
+        arr = (C.SphConfig * len(cfgs))(*cfgs)
+        self._h = c_void_p()
+        C.check(C.lib().sph_batch_create(arr, len(cfgs), self.counts.ctypes.data_as(c_void_p),
+                                         allpos.ctypes.data_as(c_void_p),
+                                         None if allvel is None else allvel.ctypes.data_as(c_void_p),
+                                         byref(self._h)))
+        self.configurations = cfgs
+        ns, n, nc = c_uint32(), c_uint32(), c_uint32()
+        self.part_offsets = np.zeros(len(cfgs) + 1, np.uint32)
+        self.cell_offsets = np.zeros(len(cfgs) + 1, np.uint32)
+        C.check(C.lib().sph_batch_info(self._h, byref(ns), byref(n), byref(nc),
+                                       self.part_offsets.ctypes.data_as(c_void_p),
+                                       self.cell_offsets.ctypes.data_as(c_void_p)))
+        self.n_scenes, self.n, self.num_cells = int(ns.value), int(n.value), int(nc.value)
+
+    def Advance(self, dt: float, steps: int = 1) -> None:
+        C.check(C.lib().sph_batch_advance(self._h, float(dt), int(steps)))
+
+    def advance_eager(self, dt: float, steps: int = 1) -> None:
+        C.check(C.lib().sph_batch_advance_eager(self._h, float(dt), int(steps)))
+
+    def synchronize(self) -> None:
+        C.check(C.lib().sph_batch_synchronize(self._h))
+
+    def set_debug(self, flags: int) -> None:
+        C.check(C.lib().sph_batch_set_debug(self._h, int(flags)))
+
+    def device_errors(self) -> int:
+        bits = c_uint32()
+        C.check(C.lib().sph_batch_device_errors(self._h, byref(bits)))
+        return int(bits.value)
+
+    def profile_passes(self, dt: float, steps: int) -> np.ndarray:
+        ms = (ctypes.c_float * len(C.PASS_NAMES))()
+        C.check(C.lib().sph_batch_profile_passes(self._h, float(dt), int(steps), ms))
+        return np.array(ms, np.float64)
+
+    def download(self, field: int) -> np.ndarray:
+        """One field over the whole batch, scene-major (``part_offsets`` /
+        ``cell_offsets`` give each scene's slice)."""
+        if field in (C.FIELD_POSITION, C.FIELD_VELOCITY, C.FIELD_ACCEL):
+            out = np.empty((self.n, 3), np.float32)
+        elif field == C.FIELD_DENSITY:
+            out = np.empty(self.n, np.float32)
+        elif field in (C.FIELD_ID, C.FIELD_TAG, C.FIELD_SORTED_HASH, C.FIELD_SORTED_INDEX):
+            out = np.empty(self.n, np.uint32)
+        elif field in (C.FIELD_CELL_START, C.FIELD_CELL_END):
+            out = np.empty(self.num_cells, np.uint32)
+        else:
+            raise ValueError(f"unknown field {field}")
+        C.check(C.lib().sph_batch_download(self._h, int(field), out.ctypes.data_as(c_void_p),
+                                           out.nbytes))
+        return out
+
+    def scene_slice(self, s: int) -> slice:
+        return slice(int(self.part_offsets[s]), int(self.part_offsets[s + 1]))
+
+    def cell_slice(self, s: int) -> slice:
+        return slice(int(self.cell_offsets[s]), int(self.cell_offsets[s + 1]))
+
+    def close(self) -> None:
+        if self._h:
+            C.lib().sph_batch_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
