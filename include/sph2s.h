@@ -255,6 +255,41 @@ int sph_batch_profile_passes(sph_batch* b, float dt, uint32_t n_steps, float* ms
  * returns each particle's scene index; CELL_* cover all scenes' cells). */
 int sph_batch_download(sph_batch* b, int field, void* host_dst, size_t bytes);
 
+/* ---- differentiable batched rollout -----------------------------------------
+ * The reference has no differentiation; this differentiates n_steps of the
+ * batched step (Solver::Advance, SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1082-1198,
+ * with the wall clamp) by hand-written adjoint kernels with the neighbour
+ * topology frozen within each step.  The state is checkpointed every
+ * checkpoint_every steps; backward recomputes one segment at a time.
+ * Arrays are device pointers, AoS 3 * n in global particle order (scene-major,
+ * as sph_batch_create's input).  Tension must be 0.
+ * All calls are ordered after the work already on `stream` (a hipStream_t,
+ * NULL = the null stream) and `stream` is ordered after them; they return
+ * before the work completes unless host outputs are requested. */
+typedef struct sph_rollout sph_rollout;
+int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes,
+                       const uint32_t* counts, float dt, uint32_t n_steps,
+                       uint32_t checkpoint_every, sph_rollout** out);
+int sph_rollout_destroy(sph_rollout* r);
+int sph_rollout_info(const sph_rollout* r, uint32_t* n_particles,
+                     uint32_t* n_scenes, size_t* device_bytes);
+/* Runs the rollout from (pos0, vel0) (vel0 may be NULL) with per-scene
+ * stiffness and viscosity (host arrays of n_scenes; NULL keeps the current
+ * values) and writes the final positions by particle to final_pos (may be
+ * NULL).  Keeps the checkpoints for one sph_rollout_backward. */
+int sph_rollout_forward(sph_rollout* r, const float* pos0, const float* vel0,
+                        const float* stiffness, const float* viscosity,
+                        float* final_pos, void* stream);
+/* Given dL/d(final_pos), writes dL/dpos0 and dL/dvel0 (device, may be NULL)
+ * and dL/dstiffness, dL/dviscosity per scene (host doubles, may be NULL;
+ * requesting them synchronises).  Consumes the forward's state. */
+int sph_rollout_backward(sph_rollout* r, const float* grad_final,
+                         float* grad_pos0, float* grad_vel0,
+                         double* grad_stiffness, double* grad_viscosity,
+                         void* stream);
+int sph_rollout_synchronize(sph_rollout* r);
+int sph_rollout_device_errors(sph_rollout* r, uint32_t* bits);
+
 /* ---- standalone stable LSD radix sort (replaces thrust::sort_by_key,
  *      SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115) ----------------------- */
 /* Temporary device bytes needed to sort n pairs. */

@@ -53,6 +53,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_cell_ranges.argtypes = [vp, c_uint32, c_uint32, vp, vp]
         L.oracle_step.argtypes = [P, vp, vp, c_uint32, c_double, vp, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_rollout.argtypes = [P, vp, vp, c_uint32, c_double, c_uint32, vp, vp, vp]
+        L.oracle_rollout_loss_grad.argtypes = [P, vp, vp, c_uint32, c_double, c_uint32, vp, vp,
+                                               vp, vp, vp, vp]
+        L.oracle_rollout_loss_grad.restype = c_double
         L.oracle_density_bruteforce.argtypes = [P, vp, c_uint32, vp]
         L.oracle_accel_bruteforce.argtypes = [P, vp, vp, vp, c_uint32, vp]
         for f in ("oracle_poly6", "oracle_spiky_grad_mag", "oracle_visc_laplacian"):
@@ -145,6 +148,32 @@ def rollout(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, steps:
     lib().oracle_rollout(ctypes.byref(p), _p(pos), _p(vel), n, float(dt), int(steps), _p(x),
                          _p(v), _p(ids))
     return x, v, ids
+
+
+def rollout_loss_grad(p: OracleParams, pos0, vel0, dt: float, steps: int, target,
+                      grads: bool = True) -> dict:
+    """FP64 rollout and the adjoint of L = mean((x_T - target)^2) over the 3n
+    components (particles matched by input index): returns loss, x_final and,
+    with ``grads``, dL/dpos0, dL/dvel0, dL/dstiffness, dL/dviscosity."""
+    pos0 = np.ascontiguousarray(pos0, np.float64)
+    vel0 = np.ascontiguousarray(vel0, np.float64)
+    target = np.ascontiguousarray(target, np.float64)
+    n = pos0.shape[0]
+    out = dict(x_final=np.empty((n, 3)))
+    gk, gm = ctypes.c_double(), ctypes.c_double()
+    if grads:
+        out["g_pos0"] = np.empty((n, 3))
+        out["g_vel0"] = np.empty((n, 3))
+    loss = lib().oracle_rollout_loss_grad(
+        ctypes.byref(p), _p(pos0), _p(vel0), n, float(dt), int(steps), _p(target),
+        _p(out["x_final"]), _p(out["g_pos0"]) if grads else None,
+        _p(out["g_vel0"]) if grads else None, ctypes.byref(gk) if grads else None,
+        ctypes.byref(gm) if grads else None)
+    out["loss"] = float(loss)
+    if grads:
+        out["g_stiffness"] = gk.value
+        out["g_viscosity"] = gm.value
+    return out
 
 
 def density_bruteforce(p: OracleParams, pos: np.ndarray) -> np.ndarray:

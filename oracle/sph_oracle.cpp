@@ -865,3 +865,169 @@ void oracle_ts_get(const oracle_ts* t, int res, int field, void* out) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Differentiable rollout (FP64): reverse-mode adjoint of n steps of the
+// single-scale step above with the pair topology frozen within each step.
+// Per step, with A_i = k (rho_i - rho0) / rho_i^2, B_i = 1 / rho_i,
+// F = (h - r)^2 / r, C45 = m c45, Vc = mu m c45 and the force pairs 0 < r < h:
+//   a_i = C45 sum_j (A_i + A_j) F x_ij + Vc B_i sum_j (h - r) B_j (v_j - v_i) + g
+// (Solver.cu:231-309,601-711 divided by rho_i), then v' = v + dt a,
+// x' = x + dt v' and the wall clamp (Solver.cu:713-755).  The adjoint sweeps
+// the steps backwards: clamp and integration per particle, then one pair pass
+// for the adjoints of A, B, the force's position and velocity terms and the
+// parameters, then one pair pass for the density's position term.  Pair sums
+// are taken as symmetric (j in i's stencil iff i in j's).  Tension is not
+// differentiated (the rollout is defined for tension 0).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StepRecord {
+    State s;                       // sorted pre-step state
+    Neighbourhood nb;
+    std::vector<double> rho, acc;
+};
+
+template <class F>
+void for_each_pair(const oracle_params& p, const State& s, const Neighbourhood& nb, uint32_t i,
+                   F&& f) {
+    const double* xi = &s.x[3 * i];
+    for_each_stencil_cell(p, xi, [&](uint32_t c) {
+        const uint32_t b = nb.start[c], e = nb.end[c];
+        if (b == kEmpty) return;
+        for (uint32_t j = b; j < e; ++j) {
+            const double xij[3] = {xi[0] - s.x[3 * j], xi[1] - s.x[3 * j + 1],
+                                   xi[2] - s.x[3 * j + 2]};
+            const double r = std::sqrt(xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2]);
+            if (r != 0.0 && r < p.h) f(j, xij, r);
+        }
+    });
+}
+
+// One step's adjoint: (gx, gv) hold dL/dx', dL/dv' in the step's sorted order
+// on entry and dL/dx, dL/dv in the step's input order on exit.
+void step_adjoint(const oracle_params& p, const StepRecord& R, double dt, std::vector<double>& gx,
+                  std::vector<double>& gv, double& gk, double& gmu) {
+    const uint32_t n = static_cast<uint32_t>(R.s.id.size());
+    const Coeffs cf(p.h);
+    const double C45 = p.mass * cf.c45, Vc = p.viscosity * p.mass * cf.c45;
+    const double C6 = p.mass * cf.c6, h = p.h, h2 = p.h * p.h;
+    std::vector<double> ga(3 * n), gxd(3 * n), gvd(3 * n), A(n), B(n), grho(n);
+    // Clamp and symplectic Euler, per particle and axis.
+    for (uint32_t i = 0; i < n; ++i) {
+        A[i] = p.stiffness * (R.rho[i] - p.rest_density) / (R.rho[i] * R.rho[i]);
+        B[i] = 1.0 / R.rho[i];
+        for (int d = 0; d < 3; ++d) {
+            const size_t k = 3 * static_cast<size_t>(i) + d;
+            const double vp = R.s.v[k] + dt * R.acc[k];
+            const double xp = R.s.x[k] + dt * vp;
+            double gxp = gx[k], gvp = gv[k];
+            if (p.wall_clamp && (xp < p.lo[d] || xp > p.hi[d])) {
+                gxp = 0.0;
+                gvp = -p.restitution * gv[k];
+            }
+            gvp += dt * gxp;
+            gxd[k] = gxp;
+            gvd[k] = gvp;
+            ga[k] = dt * gvp;
+        }
+    }
+    // Force pair pass.
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* vi = &R.s.v[3 * i];
+        const double* ai = &ga[3 * i];
+        double sumA = 0.0, S1 = 0.0, S2 = 0.0, gxs[3] = {0, 0, 0}, gvs[3] = {0, 0, 0};
+        for_each_pair(p, R.s, R.nb, i, [&](uint32_t j, const double* xij, double r) {
+            const double* vj = &R.s.v[3 * j];
+            const double* aj = &ga[3 * j];
+            const double hr = h - r, F = hr * hr / r, Fr = -hr * (h + r) / (r * r * r);
+            double dA[3], dv[3];
+            for (int d = 0; d < 3; ++d) {
+                dA[d] = ai[d] - aj[d];
+                dv[d] = vj[d] - vi[d];
+            }
+            const double xdA = xij[0] * dA[0] + xij[1] * dA[1] + xij[2] * dA[2];
+            const double dvdA = dv[0] * dA[0] + dv[1] * dA[1] + dv[2] * dA[2];
+            sumA += F * xdA;
+            S1 += hr * B[j] * (dv[0] * ai[0] + dv[1] * ai[1] + dv[2] * ai[2]);
+            S2 += hr * B[j] * (dv[0] * aj[0] + dv[1] * aj[1] + dv[2] * aj[2]);
+            const double pa = A[i] + A[j];
+            const double cx = C45 * pa * Fr * xdA - Vc * B[i] * B[j] * dvdA / r;
+            for (int d = 0; d < 3; ++d) {
+                gxs[d] += C45 * pa * F * dA[d] + cx * xij[d];
+                gvs[d] -= Vc * B[i] * B[j] * hr * dA[d];
+            }
+        });
+        const double gA = C45 * sumA, gB = Vc * (S1 - S2);
+        grho[i] = gA * p.stiffness * (2.0 * p.rest_density - R.rho[i]) * B[i] * B[i] * B[i] -
+                  gB * B[i] * B[i];
+        gk += gA * (R.rho[i] - p.rest_density) * B[i] * B[i];
+        gmu += p.mass * cf.c45 * B[i] * S1;
+        for (int d = 0; d < 3; ++d) {
+            gxd[3 * i + d] += gxs[d];
+            gvd[3 * i + d] += gvs[d];
+        }
+    }
+    // Density pair pass, then back to the step's input order.
+    for (uint32_t i = 0; i < n; ++i) {
+        double gxs[3] = {0, 0, 0};
+        for_each_pair(p, R.s, R.nb, i, [&](uint32_t j, const double* xij, double r) {
+            const double a = h2 - r * r;
+            const double c = -6.0 * C6 * a * a * (grho[i] + grho[j]);
+            for (int d = 0; d < 3; ++d) gxs[d] += c * xij[d];
+        });
+        const uint32_t q = R.nb.perm[i];
+        for (int d = 0; d < 3; ++d) {
+            gx[3 * q + d] = gxd[3 * i + d] + gxs[d];
+            gv[3 * q + d] = gvd[3 * i + d];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+double oracle_rollout_loss_grad(const oracle_params* p, const double* pos0, const double* vel0,
+                                uint32_t n, double dt, uint32_t n_steps, const double* target,
+                                double* x_final, double* g_pos0, double* g_vel0,
+                                double* g_stiffness, double* g_viscosity) {
+    State s;
+    s.x.assign(pos0, pos0 + 3 * static_cast<size_t>(n));
+    s.v.assign(3 * static_cast<size_t>(n), 0.0);
+    if (vel0) s.v.assign(vel0, vel0 + 3 * static_cast<size_t>(n));
+    s.id.resize(n);
+    std::iota(s.id.begin(), s.id.end(), 0u);
+    const bool grads = g_pos0 || g_vel0 || g_stiffness || g_viscosity;
+    std::vector<StepRecord> rec(grads ? n_steps : 0);
+    Neighbourhood nb;
+    std::vector<double> r, a;
+    for (uint32_t t = 0; t < n_steps; ++t) {
+        build_neighbourhood(*p, s, nb);
+        densities(*p, s, nb, r);
+        accelerations(*p, s, nb, r, a);
+        if (grads) rec[t] = StepRecord{s, nb, r, a};
+        integrate(*p, s, a, dt);
+    }
+    // L = sum (x_T - target)^2 / (3n), particles matched by id.
+    double loss = 0.0;
+    std::vector<double> gx(3 * static_cast<size_t>(n)), gv(3 * static_cast<size_t>(n), 0.0);
+    const double scale = 1.0 / (3.0 * n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            const double e = s.x[3 * i + d] - target[3 * s.id[i] + d];
+            loss += e * e * scale;
+            gx[3 * i + d] = 2.0 * e * scale;
+            if (x_final) x_final[3 * s.id[i] + d] = s.x[3 * i + d];
+        }
+    if (!grads) return loss;
+    double gk = 0.0, gmu = 0.0;
+    for (uint32_t t = n_steps; t-- > 0;) step_adjoint(*p, rec[t], dt, gx, gv, gk, gmu);
+    if (g_pos0) std::copy(gx.begin(), gx.end(), g_pos0);
+    if (g_vel0) std::copy(gv.begin(), gv.end(), g_vel0);
+    if (g_stiffness) *g_stiffness = gk;
+    if (g_viscosity) *g_viscosity = gmu;
+    return loss;
+}
+
+}  // extern "C"

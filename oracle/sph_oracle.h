@@ -132,3 +132,24 @@ void oracle_ts_get(const oracle_ts* t, int res, int field, void* out);
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- differentiable rollout (FP64) ------------------------------------------
+ * n_steps single-scale steps from (pos0, vel0) (AoS 3n FP64, vel0 may be NULL)
+ * and the loss L = sum_{i,d} (x_T[i,d] - target[i,d])^2 / (3n), particles
+ * matched by input index.  Returns L; x_final (3n, by input index) and the
+ * gradients dL/dpos0, dL/dvel0 (3n), dL/dstiffness, dL/dviscosity are
+ * written where non-NULL (all NULL: forward only).  Reverse-mode adjoint with
+ * the pair topology frozen within each step; tension is not differentiated. */
+double oracle_rollout_loss_grad(const oracle_params* p, const double* pos0,
+                                const double* vel0, uint32_t n, double dt,
+                                uint32_t n_steps, const double* target,
+                                double* x_final, double* g_pos0, double* g_vel0,
+                                double* g_stiffness, double* g_viscosity);
+
+#ifdef __cplusplus
+}
+#endif

@@ -132,6 +132,17 @@ SIGNATURES = [
     ("sph_batch_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_batch_profile_passes", c_int, [c_void_p, c_float, c_uint32, POINTER(c_float)]),
     ("sph_batch_download", c_int, [c_void_p, c_int, c_void_p, c_size_t]),
+    ("sph_rollout_create", c_int, [POINTER(SphConfig), c_uint32, c_void_p, c_float, c_uint32,
+                                   c_uint32, POINTER(c_void_p)]),
+    ("sph_rollout_destroy", c_int, [c_void_p]),
+    ("sph_rollout_info", c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32),
+                                 POINTER(c_size_t)]),
+    ("sph_rollout_forward", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p]),
+    ("sph_rollout_backward", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    ("sph_rollout_synchronize", c_int, [c_void_p]),
+    ("sph_rollout_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_radix_sort_temp_bytes", c_size_t, [c_uint32]),
     ("sph_radix_sort_pairs", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32,
                                      c_void_p, c_size_t, c_void_p]),

@@ -20,7 +20,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libsph2s.so"
-SOURCES = ["api.hip", "radix_sort.hip", "step.hip", "two_scale.hip", "batch.hip"]
+SOURCES = ["api.hip", "radix_sort.hip", "step.hip", "two_scale.hip", "batch.hip", "adjoint.hip"]
 HEADERS = ["internal.hpp", "neighbors.hpp"]
 
 

@@ -17,26 +17,9 @@
 #include <new>
 #include <vector>
 
-#include "neighbors.hpp"
+#include "batch.hpp"
 
 using namespace sph;
-
-struct sph_batch {
-    uint32_t n_scenes = 0, n = 0, total_cells = 0, key_bits = 1;
-    std::vector<uint32_t> part_offset;   // n_scenes + 1
-    std::vector<uint32_t> cell_offset;   // n_scenes + 1
-    std::vector<sph_config> cfg;
-    StepParams cap{};
-    bool kahan = true, tension = false;
-    StepBuffers buf{};
-    SceneParams* scenes = nullptr;       // device table
-    hipStream_t stream = nullptr;
-    void* arena = nullptr;
-    hipGraphExec_t graph = nullptr;
-    float graph_dt = 0.0f;
-    bool graph_accel = false;
-    uint32_t debug = 0;
-};
 
 namespace {
 
@@ -49,20 +32,11 @@ void destroy_graph(sph_batch* b) {
     }
 }
 
-int launch_batch_step(sph_batch* b, float dt, bool accel) {
-    for (int pass = 0; pass < kNumPasses; ++pass) {
-        const int rc = launch_batch_pass(pass, b->cap, b->buf, b->scenes, b->kahan, b->tension, dt,
-                                         b->key_bits, accel, b->stream);
-        if (rc != SPH_OK) return rc;
-    }
-    return SPH_OK;
-}
-
 int capture(sph_batch* b, float dt, bool accel) {
     destroy_graph(b);
     hipGraph_t g = nullptr;
     SPH_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = launch_batch_step(b, dt, accel);
+    const int rc = batch_step(b, b->buf, dt, accel, b->stream);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc != SPH_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -80,12 +54,21 @@ int capture(sph_batch* b, float dt, bool accel) {
     return SPH_OK;
 }
 
+SceneParams scene_params(const sph_config& c, uint32_t count, uint32_t cell_offset) {
+    SceneParams sp;
+    sp.p = make_step_params(c, count);
+    const StepParams& p = sp.p;
+    sp.g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
+    sp.g.cell_offset = cell_offset;
+    return sp;
+}
+
 }  // namespace
 
-extern "C" {
+namespace sph {
 
-int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* counts,
-                     const float* pos, const float* vel, sph_batch** out) {
+int batch_alloc(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* counts,
+                sph_batch** out) {
     if (!out || !cfgs || !counts || n_scenes == 0)
         return set_error(SPH_ERR_INVALID_ARGUMENT, "bad batch arguments");
     *out = nullptr;
@@ -99,7 +82,7 @@ int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* 
         n += counts[s];
         cells += sph_grid_num_cells(&c.grid);
     }
-    if (n == 0 || !pos) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch: no particles");
+    if (n == 0) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch: no particles");
     if (n >= (1ull << 30) || cells >= (1ull << 31))
         return set_error(SPH_ERR_INVALID_ARGUMENT, "batch: too many particles or cells");
     int dcount = 0;
@@ -116,15 +99,12 @@ int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* 
     b->cfg.assign(cfgs, cfgs + n_scenes);
     b->part_offset.assign(n_scenes + 1, 0u);
     b->cell_offset.assign(n_scenes + 1, 0u);
-    std::vector<SceneParams> table(n_scenes);
+    b->table.resize(n_scenes);
     b->kahan = false;
     for (uint32_t s = 0; s < n_scenes; ++s) {
         b->part_offset[s + 1] = b->part_offset[s] + counts[s];
         b->cell_offset[s + 1] = b->cell_offset[s] + sph_grid_num_cells(&cfgs[s].grid);
-        table[s].p = make_step_params(cfgs[s], counts[s]);
-        const StepParams& p = table[s].p;
-        table[s].g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
-        table[s].g.cell_offset = b->cell_offset[s];
+        b->table[s] = scene_params(cfgs[s], counts[s], b->cell_offset[s]);
         b->kahan = b->kahan || (cfgs[s].flags & SPH_FLAG_KAHAN) != 0u;
         b->tension = b->tension || cfgs[s].tension != 0.0f;
     }
@@ -136,7 +116,7 @@ int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* 
     const size_t sizes[] = {al(16 * N), al(16 * N), al(16 * N), al(16 * N), al(4 * N),
                             al(16 * N), al(4 * N),  al(4 * N),  al(4 * N),  al(4 * C),
                             al(4 * C),  al(sort_temp), al(8 * N), al(4),
-                            al(sizeof(SceneParams) * n_scenes)};
+                            al(sizeof(SceneParams) * n_scenes), al(4 * (n_scenes + 1))};
     size_t total = 0;
     for (size_t v : sizes) total += v;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -173,8 +153,74 @@ int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* 
     b->buf.aux = static_cast<float2*>(take());
     b->buf.n_dev = static_cast<uint32_t*>(take());
     b->scenes = static_cast<SceneParams*>(take());
+    b->part_offset_dev = static_cast<uint32_t*>(take());
 
+    e = hipMemsetAsync(b->arena, 0, total, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->scenes, b->table.data(), sizeof(SceneParams) * n_scenes,
+                           hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->part_offset_dev, b->part_offset.data(), 4 * (n_scenes + 1),
+                           hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->buf.n_dev, &b->n, 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->buf.cell_start, 0xFF, 4 * C, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->buf.cell_end, 0xFF, 4 * C, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        sph_batch_destroy(b);
+        return hip_error(e, "batch upload", __FILE__, __LINE__);
+    }
+    *out = b;
+    return SPH_OK;
+}
+
+int batch_step(const sph_batch* b, const StepBuffers& buf, float dt, bool accel,
+               hipStream_t stream) {
+    for (int pass = 0; pass < kNumPasses; ++pass) {
+        const int rc = launch_batch_pass(pass, b->cap, buf, b->scenes, b->kahan, b->tension, dt,
+                                         b->key_bits, accel, stream);
+        if (rc != SPH_OK) return rc;
+    }
+    return SPH_OK;
+}
+
+int batch_set_parameters(sph_batch* b, const float* stiffness, const float* viscosity) {
+    if (!stiffness && !viscosity) return SPH_OK;
+    for (uint32_t s = 0; s < b->n_scenes; ++s) {
+        if (stiffness) {
+            if (!std::isfinite(stiffness[s]))
+                return set_error(SPH_ERR_INVALID_ARGUMENT, "stiffness must be finite");
+            b->cfg[s].stiffness = stiffness[s];
+        }
+        if (viscosity) {
+            if (!std::isfinite(viscosity[s]))
+                return set_error(SPH_ERR_INVALID_ARGUMENT, "viscosity must be finite");
+            b->cfg[s].viscosity = viscosity[s];
+        }
+        b->table[s] = scene_params(b->cfg[s], b->part_offset[s + 1] - b->part_offset[s],
+                                   b->cell_offset[s]);
+    }
+    SPH_HIP(hipMemcpyAsync(b->scenes, b->table.data(), sizeof(SceneParams) * b->n_scenes,
+                           hipMemcpyHostToDevice, b->stream));
+    // The table is read from device memory by the captured step, so a graph
+    // stays valid; the host copy must outlive the asynchronous copy.
+    SPH_HIP(hipStreamSynchronize(b->stream));
+    return SPH_OK;
+}
+
+}  // namespace sph
+
+extern "C" {
+
+int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* counts,
+                     const float* pos, const float* vel, sph_batch** out) {
+    if (!pos) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch: no particles");
+    sph_batch* b = nullptr;
+    const int rc = batch_alloc(cfgs, n_scenes, counts, &b);
+    if (rc != SPH_OK) return rc;
     // Host-side packing: pos.w = global particle id, vel.w = scene id.
+    const size_t N = b->n;
     std::vector<float4> p4(N), v4(N);
     for (uint32_t s = 0; s < n_scenes; ++s)
         for (uint32_t i = b->part_offset[s]; i < b->part_offset[s + 1]; ++i) {
@@ -186,16 +232,8 @@ int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* 
             v4[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], fsc)
                         : make_float4(0.f, 0.f, 0.f, fsc);
         }
-    e = hipMemsetAsync(b->arena, 0, total, b->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(b->buf.pos, p4.data(), 16 * N, hipMemcpyHostToDevice, b->stream);
+    hipError_t e = hipMemcpyAsync(b->buf.pos, p4.data(), 16 * N, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(b->buf.vel, v4.data(), 16 * N, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->scenes, table.data(), sizeof(SceneParams) * n_scenes,
-                           hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->buf.n_dev, &b->n, 4, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b->buf.cell_start, 0xFF, 4 * C, b->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(b->buf.cell_end, 0xFF, 4 * C, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
         sph_batch_destroy(b);
@@ -233,7 +271,7 @@ int sph_batch_advance_eager(sph_batch* b, float dt, uint32_t n_steps) {
     if (!(dt > 0.0f)) return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
     const bool accel = (b->debug & SPH_DEBUG_ACCEL) != 0u;
     for (uint32_t i = 0; i < n_steps; ++i) {
-        const int rc = launch_batch_step(b, dt, accel);
+        const int rc = batch_step(b, b->buf, dt, accel, b->stream);
         if (rc != SPH_OK) return rc;
     }
     return SPH_OK;

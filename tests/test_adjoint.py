@@ -1,0 +1,183 @@
+"""GPU adjoint of the batched rollout against the FP64 oracle adjoint.
+
+The oracle adjoint is pinned by central finite differences and by torch
+float64 autograd (tests/test_adjoint_oracle.py).  Here the HIP rollout is run
+forward in FP32 from the same initial state and seeded with the oracle's
+dL/dx_T, and its gradients are compared with the oracle's:
+  * dL/dpos0 and dL/dvel0: max |g - g*| <= 1e-3 max |g*| per scene;
+  * dL/dstiffness and dL/dviscosity: |g - g*| <= 1e-3 |g*| per scene;
+on one step and on 20 steps at 16^3 = 4,096 particles, and on a batch of
+uneven scenes.  The forward is bit-identical to the plain batched step, and
+the gradients do not depend on the checkpoint interval (bit-identical).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from sph2s_final_amd import FIELD_ID, FIELD_POSITION, BatchSolver, scenes
+from sph2s_final_amd.autograd import Rollout, rollout
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to("cuda")
+
+
+def _scene(n, seed, sigma, k=None, mu=None):
+    sc = scenes.dam_break((n, n, n), seed=seed, stiffness=k or scenes.STIFFNESS,
+                          viscosity=mu or scenes.VISCOSITY)
+    sc.velocities = scenes.gaussian_velocities(sc.n, sigma, 40 + seed)
+    return sc
+
+
+def _oracle_grads(sc, steps, target):
+    p = O.params_from_config(sc.config)
+    return O.rollout_loss_grad(p, sc.positions.astype(np.float64),
+                               sc.velocities.astype(np.float64), sc.dt, steps, target)
+
+
+def _seed(ref, target):
+    n = target.shape[0]
+    return 2.0 * (ref["x_final"] - target) / (3.0 * n)
+
+
+def _compare(name, got, ref):
+    if np.ndim(ref) == 0:
+        err = abs(got - ref) / abs(ref)
+    else:
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"  {name}: err {err:.2e}", flush=True)
+    assert err <= TOL, name
+
+
+def _gpu_grads(sc_list, steps, seed, every=20):
+    counts = [sc.n for sc in sc_list]
+    with Rollout([sc.config for sc in sc_list], counts, sc_list[0].dt, steps, every) as r:
+        pos = _dev(np.concatenate([sc.positions for sc in sc_list]))
+        vel = _dev(np.concatenate([sc.velocities for sc in sc_list]))
+        x = r.forward(pos, vel)
+        gp, gv, gk, gm = r.backward(_dev(seed))
+        torch.cuda.synchronize()
+        assert r.device_errors() == 0
+        return x.cpu().numpy(), gp.cpu().numpy(), gv.cpu().numpy(), gk, gm
+
+
+def test_rollout_forward_equals_batched_step():
+    bs = scenes.batched_dam_breaks(3, seed=5, n_min=1000, n_max=6000)
+    print(f"case rollout_forward counts={bs.counts.tolist()}", flush=True)
+    steps = 7
+    with Rollout([s.config for s in bs.scenes], bs.counts, bs.dt, steps, 3) as r, \
+            BatchSolver([s.config for s in bs.scenes], [s.positions for s in bs.scenes]) as b:
+        x = r.forward(_dev(np.concatenate([s.positions for s in bs.scenes])))
+        b.Advance(bs.dt, steps)
+        b.synchronize()
+        ids = b.download(FIELD_ID)
+        ref = np.empty_like(b.download(FIELD_POSITION))
+        ref[ids] = b.download(FIELD_POSITION)
+        assert np.array_equal(x.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("steps,sigma", [(1, 0.3), (20, 0.05)])
+def test_gradients_match_oracle(steps, sigma):
+    sc = _scene(16, 2, sigma)
+    print(f"case adjoint n={sc.n} steps={steps}", flush=True)
+    target = sc.positions.astype(np.float64) + 0.005 * np.random.default_rng(3).standard_normal(
+        (sc.n, 3))
+    ref = _oracle_grads(sc, steps, target)
+    x, gp, gv, gk, gm = _gpu_grads([sc], steps, _seed(ref, target))
+    print(f"  x_T max diff {np.max(np.abs(x - ref['x_final'])):.2e}", flush=True)
+    _compare("g_pos0", gp, ref["g_pos0"])
+    _compare("g_vel0", gv, ref["g_vel0"])
+    _compare("g_stiffness", gk[0], ref["g_stiffness"])
+    _compare("g_viscosity", gm[0], ref["g_viscosity"])
+
+
+def test_gradients_with_hidden_parameter_targets():
+    # The differentiable-rollout task at 4,096 particles and 20 steps: targets
+    # from hidden stiffness/viscosity, MSE on final positions.
+    steps = 20
+    sc = _scene(16, 4, 0.05)
+    ph = O.params_from_config(sc.config)
+    ph.stiffness, ph.viscosity = 1.7, 6.2
+    target = O.rollout_loss_grad(ph, sc.positions.astype(np.float64),
+                                 sc.velocities.astype(np.float64), sc.dt, steps,
+                                 np.zeros((sc.n, 3)), grads=False)["x_final"]
+    ref = _oracle_grads(sc, steps, target)
+    _, gp, gv, gk, gm = _gpu_grads([sc], steps, _seed(ref, target))
+    _compare("g_vel0", gv, ref["g_vel0"])
+    _compare("g_pos0", gp, ref["g_pos0"])
+    _compare("g_stiffness", gk[0], ref["g_stiffness"])
+    _compare("g_viscosity", gm[0], ref["g_viscosity"])
+
+
+def test_batched_gradients_match_oracle_per_scene():
+    steps = 10
+    scs = [_scene(10, 6, 0.2, k=1.5, mu=7.0), _scene(16, 7, 0.2, k=4.5, mu=2.0),
+           _scene(13, 8, 0.2, k=3.0, mu=4.0)]
+    print(f"case batched adjoint counts={[s.n for s in scs]}", flush=True)
+    rng = np.random.default_rng(9)
+    targets = [s.positions + 0.005 * rng.standard_normal((s.n, 3)) for s in scs]
+    refs = [_oracle_grads(s, steps, t) for s, t in zip(scs, targets)]
+    seed = np.concatenate([_seed(r, t) for r, t in zip(refs, targets)])
+    _, gp, gv, gk, gm = _gpu_grads(scs, steps, seed, every=4)
+    off = np.concatenate([[0], np.cumsum([s.n for s in scs])])
+    for i, ref in enumerate(refs):
+        sl = slice(off[i], off[i + 1])
+        print(f" scene {i}", flush=True)
+        _compare("g_pos0", gp[sl], ref["g_pos0"])
+        _compare("g_vel0", gv[sl], ref["g_vel0"])
+        _compare("g_stiffness", gk[i], ref["g_stiffness"])
+        _compare("g_viscosity", gm[i], ref["g_viscosity"])
+
+
+def test_checkpoint_interval_does_not_change_gradients():
+    sc = _scene(12, 3, 0.1)
+    steps = 12
+    seed = np.random.default_rng(2).standard_normal((sc.n, 3)) * 1e-4
+    outs = [_gpu_grads([sc], steps, seed, every=k) for k in (1, 5, 12)]
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+
+
+def test_autograd_function_matches_direct_backward():
+    bs = [_scene(10, 11, 0.05), _scene(12, 12, 0.05)]
+    steps = 6
+    counts = [s.n for s in bs]
+    with Rollout([s.config for s in bs], counts, bs[0].dt, steps, 3) as r:
+        pos = _dev(np.concatenate([s.positions for s in bs])).requires_grad_(True)
+        vel = _dev(np.concatenate([s.velocities for s in bs])).requires_grad_(True)
+        k = torch.tensor([2.0, 4.0], requires_grad=True)
+        mu = torch.tensor([3.0, 6.0], requires_grad=True)
+        target = pos.detach() + 0.01
+        x = rollout(r, pos, vel, k, mu)
+        loss = ((x - target) ** 2).mean()
+        loss.backward()
+        g_auto = [pos.grad.cpu().numpy(), vel.grad.cpu().numpy(), k.grad.numpy(),
+                  mu.grad.numpy()]
+        # Direct call with the same seed.
+        seed = (2.0 * (x.detach() - target) / x.numel()).contiguous()
+        x2 = r.forward(pos.detach(), vel.detach(), k.detach(), mu.detach())
+        assert torch.equal(x2, x.detach())
+        gp, gv, gk, gm = r.backward(seed)
+        torch.cuda.synchronize()
+        assert np.array_equal(g_auto[0], gp.cpu().numpy())
+        assert np.array_equal(g_auto[1], gv.cpu().numpy())
+        assert np.allclose(g_auto[2], gk, rtol=1e-6) and np.allclose(g_auto[3], gm, rtol=1e-6)
+        # And against the oracle, scene by scene.
+        off = np.concatenate([[0], np.cumsum(counts)])
+        tgt = target.cpu().numpy().astype(np.float64)
+        for i, sc in enumerate(bs):
+            p = O.params_from_config(sc.config)
+            p.stiffness, p.viscosity = float(k[i].detach()), float(mu[i].detach())
+            sl = slice(off[i], off[i + 1])
+            ref = O.rollout_loss_grad(p, sc.positions.astype(np.float64),
+                                      sc.velocities.astype(np.float64), sc.dt, steps, tgt[sl])
+            scale = counts[i] / sum(counts)  # the torch loss averages over all scenes
+            _compare("g_vel0", g_auto[1][sl], ref["g_vel0"] * scale)
+            _compare("g_stiffness", g_auto[2][i], ref["g_stiffness"] * scale)
+            _compare("g_viscosity", g_auto[3][i], ref["g_viscosity"] * scale)
