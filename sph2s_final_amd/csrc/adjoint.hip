@@ -30,7 +30,8 @@
 // recomputes each K-step segment from its checkpoint, keeping every step's
 // sorted state, accelerations, densities, permutation and cell ranges
 // (64 B per particle + 8 B per cell per step), then runs the adjoint passes
-// over the segment in reverse.  Parameter gradients accumulate per slot in
+// over the segment in reverse.  The forward writes the last segment's records
+// itself, so that segment is never recomputed.  Parameter gradients accumulate per slot in
 // FP64 (a slot always belongs to the same scene) and are reduced per scene in
 // a fixed order, so the result is deterministic.
 #include <algorithm>
@@ -39,6 +40,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "pairwalk.hpp"
 
 using namespace sph;
 
@@ -146,7 +148,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_integrate(
     gvd[s] = V;
 }
 
-// Force-pair adjoint; see the file comment for the terms.
+#ifndef SPH_ADJ_FORCE_TILE
+#define SPH_ADJ_FORCE_TILE 512
+#endif
+#ifndef SPH_ADJ_DENSITY_TILE
+#define SPH_ADJ_DENSITY_TILE 1024
+#endif
+constexpr int kAdjForceTile = SPH_ADJ_FORCE_TILE;
+constexpr int kAdjDensityTile = SPH_ADJ_DENSITY_TILE;
+constexpr int kWaves = kBlock / 64;
+
+// Force-pair adjoint; see the file comment for the terms.  Candidates are
+// staged in LDS as (x, A), (v, B) and abar.
 __global__ __launch_bounds__(kBlock) void k_adj_force(
     const float4* __restrict__ tpos, const float4* __restrict__ tvel,
     const float2* __restrict__ aux, const float* __restrict__ rho,
@@ -154,54 +167,88 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
     const uint32_t* __restrict__ ce, const SceneParams* __restrict__ scenes,
     const uint32_t* __restrict__ n_dev, float* __restrict__ grho, float4* __restrict__ gxd,
     float4* __restrict__ gvd, double* __restrict__ kacc, double* __restrict__ macc) {
+    __shared__ float4 tx[kAdjForceTile];
+    __shared__ float4 tv[kAdjForceTile];
+    __shared__ float4 ta[kAdjForceTile];
+    __shared__ uint32_t red[2 * kWaves];
     const uint32_t n = *n_dev;
-    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const float4 xi = tpos[s], vi = tvel[s], bi = abar[s];
-    const float2 ABi = aux[s];
-    const SceneParams& sp = scene_for(scenes, vi);
-    const StepParams& p = sp.p;
-    const GridView& g = sp.g;
-    const float h = p.h, h2 = p.h2;
+    const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
+    if (block0 >= n) return;  // block-uniform: the grid covers the capacity
+    const uint32_t s = block0 + threadIdx.x;
+    const bool active = s < n;
+    const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const SceneParams& sp = scenes[active ? __float_as_uint(vi.w) : 0u];
+    const GridView g = sp.g;
+    const float h = sp.p.h, h2 = sp.p.h2;
+    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 bi = active ? abar[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float2 ABi = active ? aux[s] : make_float2(0.f, 1.f);
+    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sumA = 0.f, S1 = 0.f, S2 = 0.f;
     float gPx = 0.f, gPy = 0.f, gPz = 0.f;   // sum (A_i + A_j)[F dA + F'/r (x.dA) x]
     float gVx = 0.f, gVy = 0.f, gVz = 0.f;   // sum B_j (dv.dA) x / r
     float gWx = 0.f, gWy = 0.f, gWz = 0.f;   // sum (h - r) B_j dA
-    for_each_run(g, cs, ce, n, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-        for (uint32_t j = b; j < e; ++j) {
-            const float4 xj = tpos[j];
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < h2 && r2 > 0.0f)) continue;
-            const float4 vj = tvel[j], bj = abar[j];
-            const float2 ABj = aux[j];
-            const float inv_r = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * inv_r;
-            const float hr = h - r;
-            const float F = hr * hr * inv_r;
-            const float Fr = -hr * (h + r) * inv_r * inv_r * inv_r;
-            const float ax = bi.x - bj.x, ay = bi.y - bj.y, az = bi.z - bj.z;
-            const float ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
-            const float xdA = dx * ax + dy * ay + dz * az;
-            const float udA = ux * ax + uy * ay + uz * az;
-            sumA += F * xdA;
-            const float wB = hr * ABj.y;
-            S1 += wB * (ux * bi.x + uy * bi.y + uz * bi.z);
-            S2 += wB * (ux * bj.x + uy * bj.y + uz * bj.z);
-            const float pa = ABi.x + ABj.x;
-            const float cP = pa * Fr * xdA, pF = pa * F;
-            gPx += pF * ax + cP * dx;
-            gPy += pF * ay + cP * dy;
-            gPz += pF * az + cP * dz;
-            const float cV = ABj.y * udA * inv_r;
-            gVx += cV * dx;
-            gVy += cV * dy;
-            gVz += cV * dz;
-            gWx += wB * ax;
-            gWy += wB * ay;
-            gWz += wB * az;
-        }
-    });
+
+    // xa = (x_j, A_j), vb = (v_j, B_j), bj = abar_j; requires 0 < r < h.
+    auto pair = [&](float dx, float dy, float dz, float r2, const float4& xa, const float4& vb,
+                    const float4& bj) {
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * inv_r;
+        const float hr = h - r;
+        const float F = hr * hr * inv_r;
+        const float Fr = -hr * (h + r) * inv_r * inv_r * inv_r;
+        const float ax = bi.x - bj.x, ay = bi.y - bj.y, az = bi.z - bj.z;
+        const float ux = vb.x - vi.x, uy = vb.y - vi.y, uz = vb.z - vi.z;
+        const float xdA = dx * ax + dy * ay + dz * az;
+        const float udA = ux * ax + uy * ay + uz * az;
+        sumA += F * xdA;
+        const float wB = hr * vb.w;
+        S1 += wB * (ux * bi.x + uy * bi.y + uz * bi.z);
+        S2 += wB * (ux * bj.x + uy * bj.y + uz * bj.z);
+        const float pa = ABi.x + xa.w;
+        const float cP = pa * Fr * xdA, pF = pa * F;
+        gPx += pF * ax + cP * dx;
+        gPy += pF * ay + cP * dy;
+        gPz += pF * az + cP * dz;
+        const float cV = vb.w * udA * inv_r;
+        gVx += cV * dx;
+        gVy += cV * dy;
+        gVz += cV * dz;
+        gWx += wB * ax;
+        gWy += wB * ay;
+        gWz += wB * az;
+    };
+    walk_stencil_staged<kBlock, kAdjForceTile>(
+        g, pr, cs, ce, n, active, red,
+        [&](uint32_t k, uint32_t j) {
+            const float4 x = tpos[j], v = tvel[j];
+            const float2 ab = aux[j];
+            tx[k] = make_float4(x.x, x.y, x.z, ab.x);
+            tv[k] = make_float4(v.x, v.y, v.z, ab.y);
+            ta[k] = abar[j];
+        },
+        [&](uint32_t t0, uint32_t t1, uint32_t) {
+            for (uint32_t t = t0; t < t1; ++t) {
+                const float4 xa = tx[t];
+                const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < h2 && r2 > 0.0f) pair(dx, dy, dz, r2, xa, tv[t], ta[t]);
+            }
+        },
+        [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = tpos[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (!(r2 < h2 && r2 > 0.0f)) continue;
+                const float4 vj = tvel[j];
+                const float2 ab = aux[j];
+                pair(dx, dy, dz, r2, make_float4(xj.x, xj.y, xj.z, ab.x),
+                     make_float4(vj.x, vj.y, vj.z, ab.y), abar[j]);
+            }
+        });
+    if (!active) return;
+    const StepParams& p = sp.p;
     const float Bi = ABi.y, ri = rho[s];
     const float gA = p.c45m * sumA, gB = p.visc * (S1 - S2);
     grho[s] = gA * p.stiffness * (2.0f * p.rest_density - ri) * Bi * Bi * Bi - gB * Bi * Bi;
@@ -220,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
 }
 
 // Density-pair adjoint, then the scatter to the step's input order.
+// Candidates are staged in LDS as (x, dL/drho).
 __global__ __launch_bounds__(kBlock) void k_adj_density(
     const float4* __restrict__ tpos, const float4* __restrict__ tvel,
     const float* __restrict__ grho, const uint32_t* __restrict__ cs,
@@ -227,26 +275,50 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
     const SceneParams* __restrict__ scenes, const uint32_t* __restrict__ n_dev,
     const float4* __restrict__ gxd, const float4* __restrict__ gvd, float4* __restrict__ gx,
     float4* __restrict__ gv) {
+    __shared__ float4 tile[kAdjDensityTile];
+    __shared__ uint32_t red[2 * kWaves];
     const uint32_t n = *n_dev;
-    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const float4 xi = tpos[s];
-    const SceneParams& sp = scene_for(scenes, tvel[s]);
-    const float h2 = sp.p.h2, gi = grho[s];
+    const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
+    if (block0 >= n) return;  // block-uniform: the grid covers the capacity
+    const uint32_t s = block0 + threadIdx.x;
+    const bool active = s < n;
+    const SceneParams& sp = scenes[active ? __float_as_uint(tvel[s].w) : 0u];
+    const GridView g = sp.g;
+    const float h2 = sp.p.h2;
+    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float gi = active ? grho[s] : 0.0f;
+    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for_each_run(sp.g, cs, ce, n, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-        for (uint32_t j = b; j < e; ++j) {
-            const float4 xj = tpos[j];
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (!(r2 < h2 && r2 > 0.0f)) continue;
-            const float a = h2 - r2;
-            const float c = a * a * (gi + grho[j]);
-            sx += c * dx;
-            sy += c * dy;
-            sz += c * dz;
-        }
-    });
+    auto pair = [&](float dx, float dy, float dz, float r2, float gj) {
+        const float a = h2 - r2;
+        const float c = a * a * (gi + gj);
+        sx += c * dx;
+        sy += c * dy;
+        sz += c * dz;
+    };
+    walk_stencil_staged<kBlock, kAdjDensityTile>(
+        g, pr, cs, ce, n, active, red,
+        [&](uint32_t k, uint32_t j) {
+            const float4 x = tpos[j];
+            tile[k] = make_float4(x.x, x.y, x.z, grho[j]);
+        },
+        [&](uint32_t t0, uint32_t t1, uint32_t) {
+            for (uint32_t t = t0; t < t1; ++t) {
+                const float4 q = tile[t];
+                const float dx = xi.x - q.x, dy = xi.y - q.y, dz = xi.z - q.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < h2 && r2 > 0.0f) pair(dx, dy, dz, r2, q.w);
+            }
+        },
+        [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = tpos[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < h2 && r2 > 0.0f) pair(dx, dy, dz, r2, grho[j]);
+            }
+        });
+    if (!active) return;
     const float k6 = -6.0f * sp.p.c6m;
     float4 X = gxd[s];
     X.x += k6 * sx;
@@ -421,7 +493,8 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     r->n_ckpt = (n_steps + r->every - 1) / r->every;
     const size_t N = r->b->n, C = r->b->total_cells, S = r->b->n_scenes;
     const uint32_t K = n_steps ? r->every : 0u;
-    std::vector<size_t> sizes = {al(16 * N * r->n_ckpt), al(16 * N * r->n_ckpt)};
+    const size_t n_stored = r->n_ckpt ? r->n_ckpt - 1 : 0;  // the last segment keeps records
+    std::vector<size_t> sizes = {al(16 * N * n_stored), al(16 * N * n_stored)};
     for (uint32_t k = 0; k < K; ++k)
         for (size_t v : {16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 4 * N, 4 * C, 4 * C})
             sizes.push_back(al(v));
@@ -492,16 +565,26 @@ int sph_rollout_forward(sph_rollout* r, const float* pos0, const float* vel0,
     hipLaunchKernelGGL(k_pack_batch, blocks_for(n), dim3(kBlock), 0, b->stream, pos0, vel0,
                        b->part_offset_dev, b->n_scenes, n, b->buf.pos, b->buf.vel);
     SPH_HIP(hipGetLastError());
+    // The last segment runs through the segment-record graphs, so the backward
+    // sweep starts from records the forward already wrote.
+    const uint32_t last0 = r->n_ckpt ? (r->n_ckpt - 1) * r->every : 0u;
     for (uint32_t t = 0; t < r->n_steps; ++t) {
-        if (t % r->every == 0) {
+        if (t % r->every == 0 && t < last0) {
             const size_t o = static_cast<size_t>(t / r->every) * n;
             SPH_HIP(hipMemcpyAsync(r->ckpt_pos + o, b->buf.pos, 16 * size_t(n),
                                    hipMemcpyDeviceToDevice, b->stream));
             SPH_HIP(hipMemcpyAsync(r->ckpt_vel + o, b->buf.vel, 16 * size_t(n),
                                    hipMemcpyDeviceToDevice, b->stream));
         }
-        rc = sph_batch_advance(b, r->dt, 1);
-        if (rc != SPH_OK) return rc;
+        if (t >= last0) {
+            RolloutSlot& sl = r->slot[t - last0];
+            rc = slot_graph(r, sl);
+            if (rc != SPH_OK) return rc;
+            SPH_HIP(hipGraphLaunch(sl.graph, b->stream));
+        } else {
+            rc = sph_batch_advance(b, r->dt, 1);
+            if (rc != SPH_OK) return rc;
+        }
     }
     if (final_pos) {
         hipLaunchKernelGGL(k_export_by_id, blocks_for(n), dim3(kBlock), 0, b->stream, b->buf.pos,
@@ -529,24 +612,26 @@ int sph_rollout_backward(sph_rollout* r, const float* grad_final, float* grad_po
     SPH_HIP(hipMemsetAsync(r->macc, 0, 8 * size_t(n), b->stream));
     for (uint32_t c = r->n_ckpt; c-- > 0;) {
         const uint32_t t0 = c * r->every, t1 = std::min(r->n_steps, t0 + r->every);
-        const size_t o = static_cast<size_t>(c) * n;
-        SPH_HIP(hipMemcpyAsync(b->buf.pos, r->ckpt_pos + o, 16 * size_t(n),
-                               hipMemcpyDeviceToDevice, b->stream));
-        SPH_HIP(hipMemcpyAsync(b->buf.vel, r->ckpt_vel + o, 16 * size_t(n),
-                               hipMemcpyDeviceToDevice, b->stream));
-        for (uint32_t t = t0; t < t1; ++t) {
-            RolloutSlot& s = r->slot[t - t0];
-            rc = slot_graph(r, s);
-            if (rc != SPH_OK) return rc;
-            SPH_HIP(hipGraphLaunch(s.graph, b->stream));
+        if (c + 1 < r->n_ckpt) {  // the last segment's records come from the forward
+            const size_t o = static_cast<size_t>(c) * n;
+            SPH_HIP(hipMemcpyAsync(b->buf.pos, r->ckpt_pos + o, 16 * size_t(n),
+                                   hipMemcpyDeviceToDevice, b->stream));
+            SPH_HIP(hipMemcpyAsync(b->buf.vel, r->ckpt_vel + o, 16 * size_t(n),
+                                   hipMemcpyDeviceToDevice, b->stream));
+            for (uint32_t t = t0; t < t1; ++t) {
+                RolloutSlot& s = r->slot[t - t0];
+                rc = slot_graph(r, s);
+                if (rc != SPH_OK) return rc;
+                SPH_HIP(hipGraphLaunch(s.graph, b->stream));
+            }
         }
         for (uint32_t t = t1; t-- > t0;) {
             rc = adjoint_step(r, r->slot[t - t0]);
             if (rc != SPH_OK) return rc;
         }
     }
-    // The state now holds the last recomputed segment's end; the final state
-    // the forward produced is no longer resident.
+    // The state now holds a recomputed segment's end (or, with one segment,
+    // the forward's final state, whose records the adjoint has consumed).
     r->have_forward = false;
     if (grad_pos0)
         hipLaunchKernelGGL(k_unpack3, blocks_for(n), dim3(kBlock), 0, b->stream, r->gx, n,

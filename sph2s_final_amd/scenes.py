@@ -182,3 +182,28 @@ def batched_dam_breaks(n_scenes=64, seed=4, n_min=4096, n_max=262144, k_range=(1
 def batched_64() -> BatchScenes:
     """64 scenes, N_s log-uniform in [4096, 262144], seed 4."""
     return batched_dam_breaks(64, seed=4, name="batched_64")
+
+
+def rollout_batch(n_scenes=16, shape=(64, 32, 32), seed=5, sigma=0.05, k_range=(1.0, 5.0),
+                  mu_range=(1.0, 8.0), name=None) -> BatchScenes:
+    """Scenes of the differentiable-rollout task: ``n_scenes`` dam breaks of
+    one lattice shape (domain shape / 32), initial velocities N(0, sigma^2) per
+    component, and hidden per-scene stiffness and viscosity drawn from numpy's
+    PCG64(seed) (stiffness then viscosity per scene).  The configurations carry
+    the shared constants' stiffness and viscosity (the starting guess); the
+    hidden values are in ``stiffness`` / ``viscosity``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scs, ks, mus = [], [], []
+    for s in range(int(n_scenes)):
+        ks.append(float(np.float32(rng.uniform(*k_range))))
+        mus.append(float(np.float32(rng.uniform(*mu_range))))
+        sc = dam_break(shape, seed=1000 * seed + s, name=f"scene_{s}")
+        sc.velocities = gaussian_velocities(sc.n, sigma, 7000 + 1000 * seed + s)
+        scs.append(sc)
+    return BatchScenes(name or f"rollout_{n_scenes}x{shape[0] * shape[1] * shape[2]}", scs,
+                       np.array(ks, np.float32), np.array(mus, np.float32))
+
+
+def rollout_16x64k() -> BatchScenes:
+    """16 scenes x 65,536 particles (64 x 32 x 32), seed 5."""
+    return rollout_batch(16, (64, 32, 32), seed=5, name="rollout_16x64k")
