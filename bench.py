@@ -16,6 +16,14 @@ Prints ONE JSON line with:
               eager launches give bitwise-identical states)
   cpu_baseline the FP32 OpenMP CPU step (a port of the same algorithm) on the
               same scene, timed for a bounded number of steps
+  adjoint     the differentiable rollout (16 scenes x 65,536 particles, 200
+              steps, checkpoint every 20): forward and forward+adjoint times and
+              particle-updates/s, and their ratio
+  batched     64 uneven scenes (log-uniform 4,096..262,144 particles) in one
+              batched step vs the same scenes as 64 single-scene solvers (each on
+              its own stream, and serialised)
+  two_scale   the two-scale dam break (63^3 low-resolution particles, 1->8
+              split / 8->1 merge region over 20% of the domain)
 """
 from __future__ import annotations
 
@@ -75,6 +83,122 @@ def cpu_baseline(scene, budget_s: float, max_steps: int):
     }
 
 
+def _timed(fn, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_adjoint(torch, steps: int, every: int) -> dict:
+    """Forward vs forward+adjoint of the differentiable batched rollout."""
+    import numpy as np
+    from sph2s_final_amd import scenes
+    from sph2s_final_amd.autograd import Rollout
+    bs = scenes.rollout_16x64k()
+    pos = torch.from_numpy(np.concatenate([s.positions for s in bs.scenes])).cuda()
+    vel = torch.from_numpy(np.concatenate([s.velocities for s in bs.scenes])).cuda()
+    hidden_k, hidden_mu = torch.from_numpy(bs.stiffness), torch.from_numpy(bs.viscosity)
+    guess_k = torch.full((len(bs.scenes),), scenes.STIFFNESS)
+    guess_mu = torch.full((len(bs.scenes),), scenes.VISCOSITY)
+    with Rollout([s.config for s in bs.scenes], bs.counts, bs.dt, steps, every) as r:
+        target = r.forward(pos, vel, hidden_k, hidden_mu)
+        x = r.forward(pos, vel, guess_k, guess_mu)   # warm-up (graph capture)
+        r.backward((2.0 * (x - target) / x.numel()).contiguous())
+        res = {}
+        for _ in range(2):
+            start, mid, stop = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            start.record()
+            x = r.forward(pos, vel, guess_k, guess_mu)
+            mid.record()
+            seed = (2.0 * (x - target) / x.numel()).contiguous()
+            gp, gv, gk, gm = r.backward(seed)
+            stop.record()
+            torch.cuda.synchronize()
+            fwd, tot = start.elapsed_time(mid), start.elapsed_time(stop)
+            if not res or tot < res["fwd_adjoint_ms"]:
+                res = {"fwd_ms": fwd, "fwd_adjoint_ms": tot}
+        n = bs.n
+        res.update({
+            "workload": bs.name, "particles": n, "scenes": len(bs.scenes), "steps": steps,
+            "checkpoint_every": every, "device_bytes": r.device_bytes,
+            "ratio": res["fwd_adjoint_ms"] / res["fwd_ms"],
+            "fwd_pu_s": n * steps / (res["fwd_ms"] / 1e3),
+            "fwd_adjoint_pu_s": n * steps / (res["fwd_adjoint_ms"] / 1e3),
+            "loss": float(((x - target) ** 2).mean()),
+            "grads_finite": bool(np.isfinite(gk).all() and np.isfinite(gm).all()
+                                 and torch.isfinite(gv).all().item()),
+            "device_errors": r.device_errors(),
+        })
+    return res
+
+
+def bench_batched(torch, steps: int) -> dict:
+    """64 uneven scenes: one batched step vs 64 single-scene solvers."""
+    from sph2s_final_amd import BatchSolver, ParticleData, Solver, scenes
+    bs = scenes.batched_64()
+    n, dt = bs.n, bs.dt
+    b = BatchSolver([s.config for s in bs.scenes], [s.positions for s in bs.scenes],
+                    [s.velocities for s in bs.scenes])
+    b.Advance(dt, 1)
+    b.synchronize()
+    tb = _timed(lambda: (b.Advance(dt, steps), b.synchronize()), torch)
+    errs = b.device_errors()
+    b.close()
+    singles = [Solver(ParticleData(s.n, s.positions, s.velocities), None, s.config)
+               for s in bs.scenes]
+    for s in singles:
+        s.Advance(dt, 1)
+    for s in singles:
+        s.synchronize()
+
+    def streams():
+        for _ in range(steps):
+            for s in singles:
+                s.Advance(dt, 1)
+        for s in singles:
+            s.synchronize()
+
+    def serial():
+        for _ in range(steps):
+            for s in singles:
+                s.Advance(dt, 1)
+                s.synchronize()
+
+    ts = _timed(streams, torch)
+    tser = _timed(serial, torch)
+    for s in singles:
+        s.close()
+    return {"workload": bs.name, "scenes": len(bs.scenes), "particles": n,
+            "steps": steps, "pu_s": n * steps / tb, "ms_per_step": tb / steps * 1e3,
+            "one_per_launch_streams_pu_s": n * steps / ts,
+            "one_per_launch_serial_pu_s": n * steps / tser,
+            "speedup_vs_streams": ts / tb, "speedup_vs_serial": tser / tb,
+            "device_errors": errs,
+            "note": "one-per-launch: each scene its own solver and graph; 'streams' "
+                    "launches all 64 per step before waiting, 'serial' waits after each"}
+
+
+def bench_two_scale(torch, steps: int) -> dict:
+    from sph2s_final_amd import TwoScaleSolver, scenes
+    sc = scenes.two_scale_250k()
+    t = TwoScaleSolver(sc.config, sc.region_lo, sc.region_hi, sc.positions, sc.velocities,
+                       max_low=sc.n, max_high=8 * sc.n)
+    t.Advance(sc.dt, 1)
+    t.synchronize()
+    c0 = t.counts()
+    el = _timed(lambda: (t.Advance(sc.dt, steps), t.synchronize()), torch)
+    c1 = t.counts()
+    errs = t.device_errors()
+    t.close()
+    active = 0.5 * ((c0[0] + c0[1]) + (c1[2] + c1[3]))  # mean particles per step (end points)
+    return {"workload": sc.name, "steps": steps, "ms_per_step": el / steps * 1e3,
+            "pu_s": active * steps / el, "counts_start": list(c0[:2]),
+            "counts_end": list(c1[:2]), "device_errors": errs,
+            "note": "pu_s = mean of the start and end active counts (low + high) x steps / t"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +209,12 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
     ap.add_argument("--cpu-max-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--checkpoint-every", type=int, default=20)
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--two-scale-steps", type=int, default=500)
+    ap.add_argument("--only-main", action="store_true",
+                    help="skip the adjoint, batched and two-scale measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run plain 256 MiB device copies (scales rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -185,6 +315,11 @@ def main() -> int:
         "device_errors": dev_errors,
         "profiled_steps": prof_steps,
     }
+    if not args.only_main:
+        out["adjoint"] = bench_adjoint(torch, args.adjoint_steps, args.checkpoint_every)
+        out["value_fwd_adjoint"] = out["adjoint"]["fwd_adjoint_pu_s"]
+        out["batched"] = bench_batched(torch, args.batched_steps)
+        out["two_scale"] = bench_two_scale(torch, args.two_scale_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
