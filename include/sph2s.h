@@ -185,6 +185,26 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
 #define SPH_DEVERR_CAPACITY  2u   /* two-scale: a split/merge exceeded a capacity */
 int sph_device_errors(sph_solver* s, uint32_t* bits);
 
+/* ---- launch variants and the autotuner -------------------------------------
+ * The reference launches fixed block shapes (Solver.cu:1082-1198).  Here the
+ * density and force passes have launch variants, selected by a tune word:
+ *   bits 0-3   density LDS tile: 0 = 1024, 1 = 512, 2 = 1536 particles
+ *   bits 4-7   force LDS tile / neighbour-list capacity: 0 = 768/8, 1 = 512/8,
+ *              2 = 1024/8, 3 = 768/12, 4 = 768/4, 5 = 512/12
+ *   bits 8-11  density grid, bits 12-15 force grid: 0 = one 256-thread block
+ *              per 256 particles; 1..4 = persistent grid of 1, 2, 4, 8 blocks
+ *              per CU looping over those blocks
+ * All variants give bit-identical results.  sph_autotune times the variants
+ * on the solver's current state (reps launches each), keeps the fastest and
+ * returns its tune word; ms[4] (may be NULL) receives the default and tuned
+ * milliseconds of density and force.  The state is unchanged; the solver's
+ * intermediate fields (density, sorted arrays) then hold the tuning pass's
+ * values. */
+#define SPH_TUNE_DEFAULT 0u
+int sph_autotune(sph_solver* s, float dt, uint32_t reps, uint32_t* tune, float* ms);
+int sph_set_tune(sph_solver* s, uint32_t tune);
+int sph_get_tune(const sph_solver* s, uint32_t* tune);
+
 /* ---- the two-scale solver (SPH2S_FINAL_TWO_SCALE/Solver.h:109-144) ---------
  * A low-resolution set (h0, m0 from cfg) and a high-resolution set (h1 = h0/2,
  * m1 = m0/8, TWO_SCALE/Solver.h:36-43), each with its own grid over the
@@ -254,6 +274,10 @@ int sph_batch_profile_passes(sph_batch* b, float dt, uint32_t n_steps, float* ms
 /* Host copy of one field over the whole batch (fields as sph_download; TAG
  * returns each particle's scene index; CELL_* cover all scenes' cells). */
 int sph_batch_download(sph_batch* b, int field, void* host_dst, size_t bytes);
+/* The autotuner and tune words of sph_autotune, for a batch. */
+int sph_batch_autotune(sph_batch* b, float dt, uint32_t reps, uint32_t* tune, float* ms);
+int sph_batch_set_tune(sph_batch* b, uint32_t tune);
+int sph_batch_get_tune(const sph_batch* b, uint32_t* tune);
 
 /* ---- differentiable batched rollout -----------------------------------------
  * The reference has no differentiation; this differentiates n_steps of the

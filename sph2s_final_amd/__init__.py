@@ -9,6 +9,7 @@ from ._capi import (LibraryNotBuilt, SPHError, lib, FIELD_POSITION, FIELD_VELOCI
                     FIELD_DENSITY, FIELD_ID, FIELD_SORTED_HASH, FIELD_SORTED_INDEX,
                     FIELD_CELL_START, FIELD_CELL_END, FIELD_ACCEL, FIELD_TAG, SPH_DEBUG_ACCEL,
                     SPH_EMPTY_CELL, SPH_FLAG_KAHAN, SPH_FLAG_WALL_CLAMP)
+from . import autotune  # noqa: F401
 from .solver import (BatchSolver, Grid, ParticleData, Solver, SolverConfiguration,  # noqa: F401
                      TwoScaleSolver)
 

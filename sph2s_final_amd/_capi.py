@@ -107,6 +107,9 @@ SIGNATURES = [
     ("sph_set_debug", c_int, [c_void_p, c_uint32]),
     ("sph_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_profile_passes", c_int, [c_void_p, c_float, c_uint32, POINTER(c_float)]),
+    ("sph_autotune", c_int, [c_void_p, c_float, c_uint32, POINTER(c_uint32), POINTER(c_float)]),
+    ("sph_set_tune", c_int, [c_void_p, c_uint32]),
+    ("sph_get_tune", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph2_create", c_int, [POINTER(SphConfig), POINTER(c_float), POINTER(c_float), c_void_p,
                             c_void_p, c_uint32, c_uint32, c_uint32, POINTER(c_void_p)]),
     ("sph2_destroy", c_int, [c_void_p]),
@@ -132,6 +135,10 @@ SIGNATURES = [
     ("sph_batch_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_batch_profile_passes", c_int, [c_void_p, c_float, c_uint32, POINTER(c_float)]),
     ("sph_batch_download", c_int, [c_void_p, c_int, c_void_p, c_size_t]),
+    ("sph_batch_autotune", c_int, [c_void_p, c_float, c_uint32, POINTER(c_uint32),
+                                   POINTER(c_float)]),
+    ("sph_batch_set_tune", c_int, [c_void_p, c_uint32]),
+    ("sph_batch_get_tune", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_rollout_create", c_int, [POINTER(SphConfig), c_uint32, c_void_p, c_float, c_uint32,
                                    c_uint32, POINTER(c_void_p)]),
     ("sph_rollout_destroy", c_int, [c_void_p]),

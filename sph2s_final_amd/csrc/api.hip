@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "neighbors.hpp"
 
 namespace sph {
 
@@ -93,6 +93,7 @@ struct sph_solver {
     float graph_dt = 0.0f;
     bool graph_accel = false;
     uint32_t debug = 0;
+    uint32_t tune = 0;           // launch variant of the neighbour passes
 };
 
 namespace {
@@ -162,7 +163,8 @@ int upload_state(sph_solver* s, const float* pos, const float* vel, uint32_t n) 
 
 int capture_graph(sph_solver* s, float dt, bool accel) {
     destroy_graph(s);
-    const StepParams p = make_step_params(s->cfg, s->n);
+    StepParams p = make_step_params(s->cfg, s->n);
+    p.tune = s->tune;
     hipGraph_t g = nullptr;
     SPH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = launch_step(p, s->buf, dt, s->key_bits, accel, s->stream);
@@ -424,12 +426,45 @@ int sph_advance_eager(sph_solver* s, float dt, uint32_t n_steps) {
     if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
     if (!(dt > 0.0f) || !std::isfinite(dt))
         return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
-    const StepParams p = make_step_params(s->cfg, s->n);
+    StepParams p = make_step_params(s->cfg, s->n);
+    p.tune = s->tune;
     const bool accel = (s->debug & SPH_DEBUG_ACCEL) != 0u;
     for (uint32_t i = 0; i < n_steps; ++i) {
         int rc = launch_step(p, s->buf, dt, s->key_bits, accel, s->stream);
         if (rc != SPH_OK) return rc;
     }
+    return SPH_OK;
+}
+
+int sph_autotune(sph_solver* s, float dt, uint32_t reps, uint32_t* tune, float* ms) {
+    if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    if (!(dt > 0.0f) || !std::isfinite(dt))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    StepParams p = make_step_params(s->cfg, s->n);
+    uint32_t best = 0;
+    const int rc = tune_neighbour_passes(p, s->buf, nullptr, (s->cfg.flags & SPH_FLAG_KAHAN) != 0u,
+                                         p.tens != 0.0f, dt, s->key_bits, reps, s->stream, &best,
+                                         ms);
+    if (rc != SPH_OK) return rc;
+    s->tune = best;
+    destroy_graph(s);
+    if (tune) *tune = best;
+    return SPH_OK;
+}
+
+int sph_set_tune(sph_solver* s, uint32_t tune) {
+    if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    if (!tune_word_valid(tune)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
+    if (tune != s->tune) {
+        s->tune = tune;
+        destroy_graph(s);
+    }
+    return SPH_OK;
+}
+
+int sph_get_tune(const sph_solver* s, uint32_t* tune) {
+    if (!s || !tune) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    *tune = s->tune;
     return SPH_OK;
 }
 
@@ -471,7 +506,8 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
     if (!s || !ms_out) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
     if (!(dt > 0.0f)) return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
     static_assert(kNumPasses == SPH_NUM_PASSES, "pass count");
-    const StepParams p = make_step_params(s->cfg, s->n);
+    StepParams p = make_step_params(s->cfg, s->n);
+    p.tune = s->tune;
     const bool accel = (s->debug & SPH_DEBUG_ACCEL) != 0u;
     hipEvent_t ev[kNumPasses + 1] = {};
     for (auto& e : ev) SPH_HIP(hipEventCreate(&e));

@@ -277,6 +277,36 @@ int sph_batch_advance_eager(sph_batch* b, float dt, uint32_t n_steps) {
     return SPH_OK;
 }
 
+int sph_batch_autotune(sph_batch* b, float dt, uint32_t reps, uint32_t* tune, float* ms) {
+    if (!b) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch is null");
+    if (!(dt > 0.0f) || !std::isfinite(dt))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
+    uint32_t best = 0;
+    const int rc = tune_neighbour_passes(b->cap, b->buf, b->scenes, b->kahan, b->tension, dt,
+                                         b->key_bits, reps, b->stream, &best, ms);
+    if (rc != SPH_OK) return rc;
+    b->cap.tune = best;
+    destroy_graph(b);
+    if (tune) *tune = best;
+    return SPH_OK;
+}
+
+int sph_batch_set_tune(sph_batch* b, uint32_t tune) {
+    if (!b) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch is null");
+    if (!tune_word_valid(tune)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
+    if (tune != b->cap.tune) {
+        b->cap.tune = tune;
+        destroy_graph(b);
+    }
+    return SPH_OK;
+}
+
+int sph_batch_get_tune(const sph_batch* b, uint32_t* tune) {
+    if (!b || !tune) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    *tune = b->cap.tune;
+    return SPH_OK;
+}
+
 int sph_batch_synchronize(sph_batch* b) {
     if (!b) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch is null");
     SPH_HIP(hipStreamSynchronize(b->stream));

@@ -46,6 +46,7 @@ struct StepParams {
     uint32_t flags;         // SPH_FLAG_*
     uint32_t n;             // launch capacity (particles); the live count is in device memory
     uint32_t num_cells;
+    uint32_t tune;          // launch variant of the neighbour passes (0 = defaults; step.hip)
 };
 
 StepParams make_step_params(const sph_config& cfg, uint32_t n);

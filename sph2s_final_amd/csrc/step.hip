@@ -70,6 +70,38 @@ constexpr int kForceTile = SPH_FORCE_TILE;
 constexpr int kListCap = SPH_LIST_CAP;
 constexpr int kWavesPerBlock = kBlock / 64;
 
+// Launch variants of the neighbour passes, selected at run time by the tune
+// word (StepParams::tune; 0 = the defaults above).  Every variant computes
+// bit-identical results: the tile size only decides which runs are read from
+// LDS instead of global memory, the list capacity only when queued pairs are
+// evaluated, and the grid shape only which block processes which particles.
+//   bits 0-3   density LDS tile:   1024 (default), 512, 1536 particles
+//   bits 4-7   force tile / list:  768/8 (default), 512/8, 1024/8, 768/12,
+//                                  768/4, 512/12
+//   bits 8-11  density grid:       one block per 256 particles, or a
+//   bits 12-15 force grid:         persistent grid of 1, 2, 4, 8 blocks per CU
+//                                  looping over the 256-particle blocks
+constexpr uint32_t kPersistPerCu[] = {0, 1, 2, 4, 8};
+const SceneParams* const kNoScenes = nullptr;
+constexpr uint32_t kNumCus = 256;
+
+struct LaunchShape {
+    dim3 grid;
+    uint32_t nvb;  // 256-particle blocks to cover
+};
+
+inline LaunchShape launch_shape(uint32_t cap_n, uint32_t persist_idx) {
+    const uint32_t nvb = cap_n ? (cap_n + kBlock - 1) / kBlock : 1u;
+    uint32_t g = nvb;
+    if (persist_idx > 0 && persist_idx < 5) {
+        g = std::min(nvb, kPersistPerCu[persist_idx] * kNumCus);
+        // A multiple of the XCD count keeps each virtual block on the XCD the
+        // chunked mapping assigns it to.
+        if (g > kNumXcds) g -= g % kNumXcds;
+    }
+    return {dim3(g), nvb};
+}
+
 // The neighbour passes walk the nine grid rows (cy + dy, cz + dz),
 // dy, dz in {-1, 0, 1}, as block-wide phases.  In each phase every lane finds
 // its candidate run in that row (neighbors.hpp), the block takes the union of
@@ -119,21 +151,15 @@ __device__ __forceinline__ void scene_constants(const SceneParams* __restrict__ 
     }
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched>
-__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
-                                                    const uint32_t* __restrict__ cs,
-                                                    const uint32_t* __restrict__ ce,
-                                                    StepParams p_in, GridView g_in,
-                                                    const uint32_t* __restrict__ n_dev,
-                                                    float* __restrict__ rho,
-                                                    float2* __restrict__ aux,
-                                                    const float4* __restrict__ tvel,
-                                                    const SceneParams* __restrict__ scenes) {
-    __shared__ float4 tile[kDensityTile];
-    __shared__ uint32_t red[2 * kWavesPerBlock];
-    const uint32_t n = *n_dev;
-    const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
-    if (block0 >= n) return;  // block-uniform: the grid covers the capacity
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile>
+__device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float4* tile,
+                                              uint32_t* red, const float4* __restrict__ tpos,
+                                              const uint32_t* __restrict__ cs,
+                                              const uint32_t* __restrict__ ce,
+                                              const StepParams& p_in, const GridView& g_in,
+                                              float* __restrict__ rho, float2* __restrict__ aux,
+                                              const float4* __restrict__ tvel,
+                                              const SceneParams* __restrict__ scenes) {
     const uint32_t s = block0 + threadIdx.x;
     const bool active = s < n;
     StepParams p = p_in;
@@ -159,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
             uint32_t ub, ue;
             block_run_union<kBlock>(b, e, red, ub, ue);
             // Loaded range [ub, seg_e); empty when every lane's run is empty.
-            const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kDensityTile)) : 0u;
+            const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) tile[k] = tpos[ub + k];
@@ -194,6 +220,28 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
     }
 }
 
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile>
+__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
+                                                    const uint32_t* __restrict__ cs,
+                                                    const uint32_t* __restrict__ ce,
+                                                    StepParams p_in, GridView g_in,
+                                                    const uint32_t* __restrict__ n_dev,
+                                                    float* __restrict__ rho,
+                                                    float2* __restrict__ aux,
+                                                    const float4* __restrict__ tvel,
+                                                    const SceneParams* __restrict__ scenes,
+                                                    uint32_t nvb) {
+    __shared__ float4 tile[kTile];
+    __shared__ uint32_t red[2 * kWavesPerBlock];
+    const uint32_t n = *n_dev;
+    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
+        if (block0 < n)  // block-uniform
+            density_block<kKahan, kFinalize, kBatched, kTile>(block0, n, tile, red, tpos, cs, ce,
+                                                              p_in, g_in, rho, aux, tvel, scenes);
+    }
+}
+
 // a_i = m c45 sum_j (A_i + A_j)(h-r)^2/r x_ij            (pressure)
 //     + (mu m c45 / rho_i) sum_j B_j (h-r)(v_j - v_i)     (viscosity)
 //     - (sigma m c6 / rho_i) sum_j (h^2-r^2)^3 x_ij       (tension)
@@ -208,25 +256,19 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
 // tile holds (x, y, z, A_j) and (vx, vy, vz, B_j) per candidate.
 // With kIntegrate = false (two-scale own-set pass) the kernel only writes the
 // acceleration, gravity included, to accel.
-template <bool kTension, bool kIntegrate, bool kBatched>
-__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
-                                                  const float4* __restrict__ tvel,
-                                                  const float2* __restrict__ aux,
-                                                  const uint32_t* __restrict__ cs,
-                                                  const uint32_t* __restrict__ ce,
-                                                  StepParams p_in, GridView g_in,
-                                                  const uint32_t* __restrict__ n_dev,
-                                                  float dt, float4* __restrict__ pos,
-                                                  float4* __restrict__ vel,
-                                                  float4* __restrict__ accel,
-                                                  const SceneParams* __restrict__ scenes) {
-    __shared__ float4 tp[kForceTile];
-    __shared__ float4 tv[kForceTile];
-    __shared__ uint32_t list[kListCap][kBlock];
-    __shared__ uint32_t red[2 * kWavesPerBlock];
-    const uint32_t n = *n_dev;
-    const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
-    if (block0 >= n) return;  // block-uniform: the grid covers the capacity
+template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap>
+__device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4* tp, float4* tv,
+                                            uint32_t (*list)[kBlock], uint32_t* red,
+                                            const float4* __restrict__ tpos,
+                                            const float4* __restrict__ tvel,
+                                            const float2* __restrict__ aux,
+                                            const uint32_t* __restrict__ cs,
+                                            const uint32_t* __restrict__ ce,
+                                            const StepParams& p_in, const GridView& g_in,
+                                            float dt, float4* __restrict__ pos,
+                                            float4* __restrict__ vel,
+                                            float4* __restrict__ accel,
+                                            const SceneParams* __restrict__ scenes) {
     const uint32_t s = block0 + threadIdx.x;
     const bool active = s < n;
     StepParams p = p_in;
@@ -292,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
             if (active) probe_row_run(g, pr, cs, ce, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
             block_run_union<kBlock>(b, e, red, ub, ue);
-            const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kForceTile)) : 0u;
+            const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) {
@@ -314,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
                                 ++cnt;
                             }
                         }
-                        if (__ballot(cnt > kListCap - 4) != 0ull) flush();
+                        if (__ballot(cnt > kCap - 4) != 0ull) flush();
                     }
                 } else {
                     global_run(b, e);
@@ -361,6 +403,92 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
     pos[s] = make_float4(x, y, z, xi.w);
     vel[s] = make_float4(vx, vy, vz, vi.w);
     if (accel) accel[s] = make_float4(ax, ay, az, 0.0f);
+}
+
+template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap>
+__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
+                                                  const float4* __restrict__ tvel,
+                                                  const float2* __restrict__ aux,
+                                                  const uint32_t* __restrict__ cs,
+                                                  const uint32_t* __restrict__ ce,
+                                                  StepParams p_in, GridView g_in,
+                                                  const uint32_t* __restrict__ n_dev,
+                                                  float dt, float4* __restrict__ pos,
+                                                  float4* __restrict__ vel,
+                                                  float4* __restrict__ accel,
+                                                  const SceneParams* __restrict__ scenes,
+                                                  uint32_t nvb) {
+    static_assert(kCap >= 4, "the scan queues up to 4 candidates between flush checks");
+    __shared__ float4 tp[kTile];
+    __shared__ float4 tv[kTile];
+    __shared__ uint32_t list[kCap][kBlock];
+    __shared__ uint32_t red[2 * kWavesPerBlock];
+    const uint32_t n = *n_dev;
+    for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
+        if (block0 < n)  // block-uniform
+            force_block<kTension, kIntegrate, kBatched, kTile, kCap>(
+                block0, n, tp, tv, list, red, tpos, tvel, aux, cs, ce, p_in, g_in, dt, pos, vel,
+                accel, scenes);
+    }
+}
+
+// Variant dispatch.  Only the passes of the single-scale and batched steps
+// (density with aux, force with integration and no tension) are tunable; the
+// other instantiations use the defaults.
+template <bool kKahan, bool kFinalize, bool kBatched, class... A>
+void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
+    const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
+    const LaunchShape sh = launch_shape(cap_n, kFinalize ? ((tune >> 8) & 0xFu) : 0u);
+    if constexpr (kFinalize) {
+        if (v == 1) {
+            hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, 512>), sh.grid,
+                               dim3(kBlock), 0, st, args..., sh.nvb);
+            return;
+        }
+        if (v == 2) {
+            hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, 1536>), sh.grid,
+                               dim3(kBlock), 0, st, args..., sh.nvb);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, kDensityTile>), sh.grid,
+                       dim3(kBlock), 0, st, args..., sh.nvb);
+}
+
+template <bool kTension, bool kIntegrate, bool kBatched, class... A>
+void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
+    constexpr bool kTunable = kIntegrate && !kTension;
+    const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
+    const LaunchShape sh = launch_shape(cap_n, kTunable ? ((tune >> 12) & 0xFu) : 0u);
+    if constexpr (kTunable) {
+        switch (v) {
+            case 1:
+                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 512, 8>), sh.grid,
+                                   dim3(kBlock), 0, st, args..., sh.nvb);
+                return;
+            case 2:
+                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 1024, 8>), sh.grid,
+                                   dim3(kBlock), 0, st, args..., sh.nvb);
+                return;
+            case 3:
+                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 768, 12>), sh.grid,
+                                   dim3(kBlock), 0, st, args..., sh.nvb);
+                return;
+            case 4:
+                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 768, 4>), sh.grid,
+                                   dim3(kBlock), 0, st, args..., sh.nvb);
+                return;
+            case 5:
+                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 512, 12>), sh.grid,
+                                   dim3(kBlock), 0, st, args..., sh.nvb);
+                return;
+            default:
+                break;
+        }
+    }
+    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, kForceTile, kListCap>), sh.grid,
+                       dim3(kBlock), 0, st, args..., sh.nvb);
 }
 
 __global__ void k_pack_state(const float* __restrict__ pxyz, const float* __restrict__ vxyz,
@@ -410,22 +538,24 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
             break;
         case kPassDensity:
             if (p.flags & SPH_FLAG_KAHAN)
-                hipLaunchKernelGGL((k_density<true, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
+                launch_density<true, true, false>(p.tune, p.n, stream, b.tpos, b.cell_start,
+                                                  b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel,
+                                                  kNoScenes);
             else
-                hipLaunchKernelGGL((k_density<false, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
+                launch_density<false, true, false>(p.tune, p.n, stream, b.tpos, b.cell_start,
+                                                   b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel,
+                                                   kNoScenes);
             break;
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (p.tens != 0.0f)
-                hipLaunchKernelGGL((k_force<true, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g,
-                                   b.n_dev, dt, b.pos, b.vel, acc, nullptr);
+                launch_force<true, true, false>(p.tune, p.n, stream, b.tpos, b.tvel, b.aux,
+                                                b.cell_start, b.cell_end, p, g, b.n_dev, dt, b.pos,
+                                                b.vel, acc, kNoScenes);
             else
-                hipLaunchKernelGGL((k_force<false, true, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, p, g,
-                                   b.n_dev, dt, b.pos, b.vel, acc, nullptr);
+                launch_force<false, true, false>(p.tune, p.n, stream, b.tpos, b.tvel, b.aux,
+                                                 b.cell_start, b.cell_end, p, g, b.n_dev, dt,
+                                                 b.pos, b.vel, acc, kNoScenes);
             break;
         }
         default:
@@ -448,11 +578,11 @@ int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t st
     if (p.n == 0) return SPH_OK;
     const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
     if (p.flags & SPH_FLAG_KAHAN)
-        hipLaunchKernelGGL((k_density<true, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
+        launch_density<true, false, false>(0u, p.n, stream, b.tpos, b.cell_start, b.cell_end, p, g,
+                                           b.n_dev, b.rho, b.aux, b.tvel, kNoScenes);
     else
-        hipLaunchKernelGGL((k_density<false, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream,
-                           b.tpos, b.cell_start, b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel, nullptr);
+        launch_density<false, false, false>(0u, p.n, stream, b.tpos, b.cell_start, b.cell_end, p,
+                                            g, b.n_dev, b.rho, b.aux, b.tvel, kNoScenes);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -460,14 +590,15 @@ int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t st
 int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& b,
                      float4* accel_own, hipStream_t stream) {
     if (p.n == 0) return SPH_OK;
+    float4* const no_state = nullptr;
     if (p.tens != 0.0f)
-        hipLaunchKernelGGL((k_force<true, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
-                           b.tvel, b.aux, b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f, nullptr,
-                           nullptr, accel_own, nullptr);
+        launch_force<true, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
+                                         b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
+                                         accel_own, kNoScenes);
     else
-        hipLaunchKernelGGL((k_force<false, false, false>), blocks_for(p.n), dim3(kBlock), 0, stream, b.tpos,
-                           b.tvel, b.aux, b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f, nullptr,
-                           nullptr, accel_own, nullptr);
+        launch_force<false, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
+                                          b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
+                                          accel_own, kNoScenes);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -507,24 +638,24 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
             return launch_pass(pass, cap, b, dt, key_bits, write_accel, stream);
         case kPassDensity:
             if (kahan)
-                hipLaunchKernelGGL((k_density<true, true, true>), blocks_for(cap.n), dim3(kBlock), 0,
-                                   stream, b.tpos, b.cell_start, b.cell_end, cap, g0, b.n_dev,
-                                   b.rho, b.aux, b.tvel, scenes);
+                launch_density<true, true, true>(cap.tune, cap.n, stream, b.tpos, b.cell_start,
+                                                 b.cell_end, cap, g0, b.n_dev, b.rho, b.aux,
+                                                 b.tvel, scenes);
             else
-                hipLaunchKernelGGL((k_density<false, true, true>), blocks_for(cap.n), dim3(kBlock),
-                                   0, stream, b.tpos, b.cell_start, b.cell_end, cap, g0, b.n_dev,
-                                   b.rho, b.aux, b.tvel, scenes);
+                launch_density<false, true, true>(cap.tune, cap.n, stream, b.tpos, b.cell_start,
+                                                  b.cell_end, cap, g0, b.n_dev, b.rho, b.aux,
+                                                  b.tvel, scenes);
             break;
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (tension)
-                hipLaunchKernelGGL((k_force<true, true, true>), blocks_for(cap.n), dim3(kBlock), 0,
-                                   stream, b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, cap,
-                                   g0, b.n_dev, dt, b.pos, b.vel, acc, scenes);
+                launch_force<true, true, true>(cap.tune, cap.n, stream, b.tpos, b.tvel, b.aux,
+                                               b.cell_start, b.cell_end, cap, g0, b.n_dev, dt,
+                                               b.pos, b.vel, acc, scenes);
             else
-                hipLaunchKernelGGL((k_force<false, true, true>), blocks_for(cap.n), dim3(kBlock), 0,
-                                   stream, b.tpos, b.tvel, b.aux, b.cell_start, b.cell_end, cap,
-                                   g0, b.n_dev, dt, b.pos, b.vel, acc, scenes);
+                launch_force<false, true, true>(cap.tune, cap.n, stream, b.tpos, b.tvel, b.aux,
+                                                b.cell_start, b.cell_end, cap, g0, b.n_dev, dt,
+                                                b.pos, b.vel, acc, scenes);
             break;
         }
         default:
@@ -532,6 +663,105 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
     }
     SPH_HIP(hipGetLastError());
     return SPH_OK;
+}
+
+}  // namespace sph
+
+namespace sph {
+
+bool tune_word_valid(uint32_t tune) {
+    return (tune >> 16) == 0u && (tune & 0xFu) <= 2u && ((tune >> 4) & 0xFu) <= 5u &&
+           ((tune >> 8) & 0xFu) <= 4u && ((tune >> 12) & 0xFu) <= 4u;
+}
+
+int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams* scenes,
+                          bool kahan, bool tension, float dt, uint32_t key_bits, uint32_t reps,
+                          hipStream_t stream, uint32_t* tune_out, float* ms_out) {
+    if (p.n == 0) return set_error(SPH_ERR_INVALID_ARGUMENT, "autotune: no particles");
+    reps = std::max(reps, 1u);
+    auto run = [&](int pass, uint32_t tune) {
+        p.tune = tune;
+        return scenes ? launch_batch_pass(pass, p, b, scenes, kahan, tension, dt, key_bits, false,
+                                          stream)
+                      : launch_pass(pass, p, b, dt, key_bits, false, stream);
+    };
+    // The force pass integrates into pos/vel: keep the state to restore it.
+    void* saved = nullptr;
+    const size_t bytes = 16 * static_cast<size_t>(p.n);
+    SPH_HIP(hipMalloc(&saved, 2 * bytes));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SPH_OK;
+    auto fail = [&](int code) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(saved);
+        return code;
+    };
+    hipError_t e = hipMemcpyAsync(saved, b.pos, bytes, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(static_cast<char*>(saved) + bytes, b.vel, bytes,
+                           hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) return fail(hip_error(e, "autotune setup", __FILE__, __LINE__));
+    for (int pass = kPassHash; pass <= kPassReorder && rc == SPH_OK; ++pass) rc = run(pass, 0u);
+    if (rc != SPH_OK) return fail(rc);
+    // Mean milliseconds of one launch of `pass` with tune word t (after one
+    // untimed launch).
+    auto time_pass = [&](int pass, uint32_t t, float& ms) -> int {
+        int r = run(pass, t);
+        if (r != SPH_OK) return r;
+        SPH_HIP(hipEventRecord(e0, stream));
+        for (uint32_t i = 0; i < reps && r == SPH_OK; ++i) r = run(pass, t);
+        if (r != SPH_OK) return r;
+        SPH_HIP(hipEventRecord(e1, stream));
+        SPH_HIP(hipEventSynchronize(e1));
+        SPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= static_cast<float>(reps);
+        return SPH_OK;
+    };
+    // Density: tile first, then the grid shape for the best tile.
+    float d_def = 0.f, d_best = 0.f, ms = 0.f;
+    uint32_t d_tile = 0, d_grid = 0;
+    for (uint32_t v = 0; v <= 2 && rc == SPH_OK; ++v) {
+        rc = time_pass(kPassDensity, v, ms);
+        if (v == 0) d_def = d_best = ms;
+        else if (ms < d_best) { d_best = ms; d_tile = v; }
+    }
+    for (uint32_t gsh = 1; gsh <= 4 && rc == SPH_OK; ++gsh) {
+        rc = time_pass(kPassDensity, d_tile | (gsh << 8), ms);
+        if (ms < d_best) { d_best = ms; d_grid = gsh; }
+    }
+    if (rc != SPH_OK) return fail(rc);
+    const uint32_t dword = d_tile | (d_grid << 8);
+    rc = run(kPassDensity, dword);  // aux for the force pass
+    // Force: variant first, then the grid shape (tension runs the default).
+    float f_def = 0.f, f_best = 0.f;
+    uint32_t f_var = 0, f_grid = 0;
+    for (uint32_t v = 0; v <= (tension ? 0u : 5u) && rc == SPH_OK; ++v) {
+        rc = time_pass(kPassForce, dword | (v << 4), ms);
+        if (v == 0) f_def = f_best = ms;
+        else if (ms < f_best) { f_best = ms; f_var = v; }
+    }
+    for (uint32_t gsh = 1; gsh <= (tension ? 0u : 4u) && rc == SPH_OK; ++gsh) {
+        rc = time_pass(kPassForce, dword | (f_var << 4) | (gsh << 12), ms);
+        if (ms < f_best) { f_best = ms; f_grid = gsh; }
+    }
+    if (rc != SPH_OK) return fail(rc);
+    e = hipMemcpyAsync(b.pos, saved, bytes, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b.vel, static_cast<char*>(saved) + bytes, bytes,
+                           hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return fail(hip_error(e, "autotune restore", __FILE__, __LINE__));
+    *tune_out = dword | (f_var << 4) | (f_grid << 12);
+    if (ms_out) {
+        ms_out[0] = d_def;
+        ms_out[1] = d_best;
+        ms_out[2] = f_def;
+        ms_out[3] = f_best;
+    }
+    return fail(SPH_OK);
 }
 
 }  // namespace sph

@@ -181,6 +181,24 @@ class Solver:
         C.check(C.lib().sph_device_errors(self._h, byref(bits)))
         return int(bits.value)
 
+    def autotune(self, dt: float, reps: int = 5):
+        """Times the neighbour-pass launch variants on the current state and
+        keeps the fastest (results are bit-identical for every variant).
+        Returns (tune word, [density default ms, density tuned ms, force
+        default ms, force tuned ms])."""
+        t = c_uint32()
+        ms = (ctypes.c_float * 4)()
+        C.check(C.lib().sph_autotune(self._h, float(dt), int(reps), byref(t), ms))
+        return int(t.value), [float(v) for v in ms]
+
+    def set_tune(self, tune: int) -> None:
+        C.check(C.lib().sph_set_tune(self._h, int(tune)))
+
+    def get_tune(self) -> int:
+        t = c_uint32()
+        C.check(C.lib().sph_get_tune(self._h, byref(t)))
+        return int(t.value)
+
     def set_state(self, positions, velocities=None) -> None:
         pos = _f32(positions)
         vel = None if velocities is None else _f32(velocities)
@@ -402,6 +420,21 @@ class BatchSolver:
         bits = c_uint32()
         C.check(C.lib().sph_batch_device_errors(self._h, byref(bits)))
         return int(bits.value)
+
+    def autotune(self, dt: float, reps: int = 5):
+        """As Solver.autotune, over the whole batch."""
+        t = c_uint32()
+        ms = (ctypes.c_float * 4)()
+        C.check(C.lib().sph_batch_autotune(self._h, float(dt), int(reps), byref(t), ms))
+        return int(t.value), [float(v) for v in ms]
+
+    def set_tune(self, tune: int) -> None:
+        C.check(C.lib().sph_batch_set_tune(self._h, int(tune)))
+
+    def get_tune(self) -> int:
+        t = c_uint32()
+        C.check(C.lib().sph_batch_get_tune(self._h, byref(t)))
+        return int(t.value)
 
     def profile_passes(self, dt: float, steps: int) -> np.ndarray:
         ms = (ctypes.c_float * len(C.PASS_NAMES))()
