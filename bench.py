@@ -139,10 +139,13 @@ def bench_batched(torch, steps: int) -> dict:
     from sph2s_final_amd import BatchSolver, ParticleData, Solver, scenes
     bs = scenes.batched_64()
     n, dt = bs.n, bs.dt
+    from sph2s_final_amd import autotune
     b = BatchSolver([s.config for s in bs.scenes], [s.positions for s in bs.scenes],
                     [s.velocities for s in bs.scenes])
     b.Advance(dt, 1)
     b.synchronize()
+    word, tune_ms = b.autotune(dt, 5)
+    b.Advance(dt, 1)
     tb = _timed(lambda: (b.Advance(dt, steps), b.synchronize()), torch)
     errs = b.device_errors()
     b.close()
@@ -176,6 +179,9 @@ def bench_batched(torch, steps: int) -> dict:
             "one_per_launch_serial_pu_s": n * steps / tser,
             "speedup_vs_streams": ts / tb, "speedup_vs_serial": tser / tb,
             "device_errors": errs,
+            "tune": {"word": word, "launch": autotune.describe(word),
+                     "density_ms_default_tuned": tune_ms[:2],
+                     "force_ms_default_tuned": tune_ms[2:]},
             "note": "one-per-launch: each scene its own solver and graph; 'streams' "
                     "launches all 64 per step before waiting, 'serial' waits after each"}
 
