@@ -18,8 +18,10 @@ from __future__ import annotations
 import json
 import os
 
-DENSITY_TILES = (1024, 512, 1536)
-FORCE_VARIANTS = ((768, 8), (512, 8), (1024, 8), (768, 12), (768, 4), (512, 12))
+DENSITY_TILES = (1536, 512, 1024, 2048, 768, 1280)
+# (LDS tile, list capacity, list entry bits)
+FORCE_VARIANTS = ((640, 16, 16), (512, 8, 32), (1024, 8, 32), (768, 16, 16), (768, 24, 16),
+                  (512, 32, 16), (512, 16, 16), (512, 24, 16), (1024, 16, 16), (768, 8, 32))
 GRID_BLOCKS_PER_CU = (None, 1, 2, 4, 8)  # None: one block per 256 particles
 
 
@@ -28,7 +30,7 @@ def describe(tune: int) -> dict:
     d, f = tune & 0xF, (tune >> 4) & 0xF
     dg, fg = (tune >> 8) & 0xF, (tune >> 12) & 0xF
     return {"density_tile": DENSITY_TILES[d], "force_tile": FORCE_VARIANTS[f][0],
-            "force_list_cap": FORCE_VARIANTS[f][1],
+            "force_list_cap": FORCE_VARIANTS[f][1], "force_list_bits": FORCE_VARIANTS[f][2],
             "density_blocks_per_cu": GRID_BLOCKS_PER_CU[dg],
             "force_blocks_per_cu": GRID_BLOCKS_PER_CU[fg], "block_threads": 256}
 

@@ -57,13 +57,13 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
 }
 
 #ifndef SPH_DENSITY_TILE
-#define SPH_DENSITY_TILE 1024
+#define SPH_DENSITY_TILE 1536
 #endif
 #ifndef SPH_FORCE_TILE
-#define SPH_FORCE_TILE 768
+#define SPH_FORCE_TILE 640
 #endif
 #ifndef SPH_LIST_CAP
-#define SPH_LIST_CAP 8
+#define SPH_LIST_CAP 16
 #endif
 constexpr int kDensityTile = SPH_DENSITY_TILE;
 constexpr int kForceTile = SPH_FORCE_TILE;
@@ -75,9 +75,10 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // bit-identical results: the tile size only decides which runs are read from
 // LDS instead of global memory, the list capacity only when queued pairs are
 // evaluated, and the grid shape only which block processes which particles.
-//   bits 0-3   density LDS tile:   1024 (default), 512, 1536 particles
-//   bits 4-7   force tile / list:  768/8 (default), 512/8, 1024/8, 768/12,
-//                                  768/4, 512/12
+//   bits 0-3   density LDS tile: 1536 (default), 512, 1024, 2048, 768, 1280
+//   bits 4-7   force tile / list capacity / list entry:  640/16/u16 (default),
+//              512/8/u32, 1024/8/u32, 768/16/u16, 768/24/u16, 512/32/u16,
+//              512/16/u16, 512/24/u16, 1024/16/u16, 768/8/u32
 //   bits 8-11  density grid:       one block per 256 particles, or a
 //   bits 12-15 force grid:         persistent grid of 1, 2, 4, 8 blocks per CU
 //                                  looping over the 256-particle blocks
@@ -119,14 +120,20 @@ __device__ __forceinline__ float poly6_term(const float4& xi, const float4& xj, 
     return (r2 < h2) ? a * a * a : 0.0f;
 }
 
+// A run's terms are added one at a time in index order, whether the run is
+// read from global memory or staged in LDS.
+constexpr int kScanGroup = 8;  // force scan: candidates loaded together before use
+
 template <class Load>
 __device__ __forceinline__ float poly6_run(const float4& xi, float h2, uint32_t m, Load&& load) {
     float rs = 0.0f;
     uint32_t k = 0;
     for (; k + 4 <= m; k += 4) {
         const float4 q0 = load(k), q1 = load(k + 1), q2 = load(k + 2), q3 = load(k + 3);
-        rs += poly6_term(xi, q0, h2) + poly6_term(xi, q1, h2) + poly6_term(xi, q2, h2) +
-              poly6_term(xi, q3, h2);
+        rs += poly6_term(xi, q0, h2);
+        rs += poly6_term(xi, q1, h2);
+        rs += poly6_term(xi, q2, h2);
+        rs += poly6_term(xi, q3, h2);
     }
     for (; k < m; ++k) rs += poly6_term(xi, load(k), h2);
     return rs;
@@ -190,14 +197,14 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) tile[k] = tpos[ub + k];
             __syncthreads();
-            if (b < e) {
-                if (b >= ub && e <= seg_e) {
-                    const float4* t = tile + (b - ub);
-                    accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
-                } else {
-                    const float4* t = tpos + b;
-                    accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
-                }
+            const bool staged = b < e && b >= ub && e <= seg_e;
+            if (staged) {
+                const float4* t = tile + (b - ub);
+                accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
+            }
+            if (b < e && !staged) {
+                const float4* t = tpos + b;
+                accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
         }
     }
@@ -237,8 +244,8 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
         if (block0 < n)  // block-uniform
-            density_block<kKahan, kFinalize, kBatched, kTile>(block0, n, tile, red, tpos, cs, ce,
-                                                              p_in, g_in, rho, aux, tvel, scenes);
+            density_block<kKahan, kFinalize, kBatched, kTile>(
+                block0, n, tile, red, tpos, cs, ce, p_in, g_in, rho, aux, tvel, scenes);
     }
 }
 
@@ -256,9 +263,9 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
 // tile holds (x, y, z, A_j) and (vx, vy, vz, B_j) per candidate.
 // With kIntegrate = false (two-scale own-set pass) the kernel only writes the
 // acceleration, gravity included, to accel.
-template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap>
+template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList>
 __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4* tp, float4* tv,
-                                            uint32_t (*list)[kBlock], uint32_t* red,
+                                            TList (*list)[kBlock], uint32_t* red,
                                             const float4* __restrict__ tpos,
                                             const float4* __restrict__ tvel,
                                             const float2* __restrict__ aux,
@@ -310,10 +317,19 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
         const float r2 = dx * dx + dy * dy + dz * dz;
         return (r2 < p.h2) && (r2 > 0.0f);
     };
+    // Queued pairs in queue order; two list entries are fetched per step so
+    // their tile reads overlap.
     auto flush = [&]() {
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t t = list[k][threadIdx.x];
-            pair(tp[t], tv[t]);
+        uint32_t k = 0;
+        for (; k + 2 <= cnt; k += 2) {
+            const uint32_t t0 = list[k][threadIdx.x], t1 = list[k + 1][threadIdx.x];
+            const float4 a0 = tp[t0], b0 = tv[t0], a1 = tp[t1], b1 = tv[t1];
+            pair(a0, b0);
+            pair(a1, b1);
+        }
+        if (k < cnt) {
+            const uint32_t t0 = list[k][threadIdx.x];
+            pair(tp[t0], tv[t0]);
         }
         cnt = 0;
     };
@@ -345,23 +361,27 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
                 tv[k] = make_float4(v.x, v.y, v.z, a.y);
             }
             __syncthreads();
-            if (b < e) {
-                if (b >= ub && e <= seg_e) {
-                    const uint32_t t0 = b - ub, t1 = e - ub;
-                    for (uint32_t t = t0; t < t1; t += 4) {
+            const bool staged = b < e && b >= ub && e <= seg_e;
+            if (staged) {
+                // Whole groups of candidates are loaded before any test (the
+                // tile is padded); every candidate's index is written at the
+                // queue tail and the tail advances only for in-range ones.
+                constexpr int kG = kCap >= 16 ? kScanGroup : kScanGroup / 2;
+                const uint32_t t0 = b - ub, t1 = e - ub;
+                for (uint32_t t = t0; t < t1; t += kG) {
+                    float4 q[kG];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            if (t + u < t1 && in_range(tp[t + u])) {
-                                list[cnt][threadIdx.x] = t + u;
-                                ++cnt;
-                            }
-                        }
-                        if (__ballot(cnt > kCap - 4) != 0ull) flush();
+                    for (int u = 0; u < kG; ++u) q[u] = tp[t + u];
+#pragma unroll
+                    for (int u = 0; u < kG; ++u) {
+                        const bool hit = (t + u < t1) && in_range(q[u]);
+                        list[cnt][threadIdx.x] = static_cast<TList>(t + u);
+                        cnt += hit ? 1u : 0u;
                     }
-                } else {
-                    global_run(b, e);
+                    if (__ballot(cnt > kCap - kG) != 0ull) flush();
                 }
             }
+            if (b < e && !staged) global_run(b, e);
             flush();
         }
     }
@@ -405,7 +425,7 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
     if (accel) accel[s] = make_float4(ax, ay, az, 0.0f);
 }
 
-template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap>
+template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList>
 __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
                                                   const float2* __restrict__ aux,
@@ -418,16 +438,17 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
                                                   float4* __restrict__ accel,
                                                   const SceneParams* __restrict__ scenes,
                                                   uint32_t nvb) {
-    static_assert(kCap >= 4, "the scan queues up to 4 candidates between flush checks");
-    __shared__ float4 tp[kTile];
+    static_assert(kCap >= kScanGroup, "a scan group queues up to kCap / 2 candidates");
+    static_assert(kTile + kScanGroup <= 65536 || sizeof(TList) == 4, "list entries index the tile");
+    __shared__ float4 tp[kTile + kScanGroup];  // padded for whole-group loads
     __shared__ float4 tv[kTile];
-    __shared__ uint32_t list[kCap][kBlock];
+    __shared__ TList list[kCap][kBlock];
     __shared__ uint32_t red[2 * kWavesPerBlock];
     const uint32_t n = *n_dev;
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
         if (block0 < n)  // block-uniform
-            force_block<kTension, kIntegrate, kBatched, kTile, kCap>(
+            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList>(
                 block0, n, tp, tv, list, red, tpos, tvel, aux, cs, ce, p_in, g_in, dt, pos, vel,
                 accel, scenes);
     }
@@ -440,20 +461,21 @@ template <bool kKahan, bool kFinalize, bool kBatched, class... A>
 void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
     const LaunchShape sh = launch_shape(cap_n, kFinalize ? ((tune >> 8) & 0xFu) : 0u);
+#define SPH_DENSITY_VARIANT(TILE)                                                               \
+    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE>), sh.grid, dim3(kBlock),  \
+                       0, st, args..., sh.nvb)
     if constexpr (kFinalize) {
-        if (v == 1) {
-            hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, 512>), sh.grid,
-                               dim3(kBlock), 0, st, args..., sh.nvb);
-            return;
-        }
-        if (v == 2) {
-            hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, 1536>), sh.grid,
-                               dim3(kBlock), 0, st, args..., sh.nvb);
-            return;
+        switch (v) {
+            case 1: SPH_DENSITY_VARIANT(512); return;
+            case 2: SPH_DENSITY_VARIANT(1024); return;
+            case 3: SPH_DENSITY_VARIANT(2048); return;
+            case 4: SPH_DENSITY_VARIANT(768); return;
+            case 5: SPH_DENSITY_VARIANT(1280); return;
+            default: break;
         }
     }
-    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, kDensityTile>), sh.grid,
-                       dim3(kBlock), 0, st, args..., sh.nvb);
+    SPH_DENSITY_VARIANT(kDensityTile);
+#undef SPH_DENSITY_VARIANT
 }
 
 template <bool kTension, bool kIntegrate, bool kBatched, class... A>
@@ -461,34 +483,25 @@ void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     constexpr bool kTunable = kIntegrate && !kTension;
     const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
     const LaunchShape sh = launch_shape(cap_n, kTunable ? ((tune >> 12) & 0xFu) : 0u);
+#define SPH_FORCE_VARIANT(TILE, CAP, TLIST)                                                     \
+    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST>), sh.grid,   \
+                       dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kTunable) {
         switch (v) {
-            case 1:
-                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 512, 8>), sh.grid,
-                                   dim3(kBlock), 0, st, args..., sh.nvb);
-                return;
-            case 2:
-                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 1024, 8>), sh.grid,
-                                   dim3(kBlock), 0, st, args..., sh.nvb);
-                return;
-            case 3:
-                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 768, 12>), sh.grid,
-                                   dim3(kBlock), 0, st, args..., sh.nvb);
-                return;
-            case 4:
-                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 768, 4>), sh.grid,
-                                   dim3(kBlock), 0, st, args..., sh.nvb);
-                return;
-            case 5:
-                hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, 512, 12>), sh.grid,
-                                   dim3(kBlock), 0, st, args..., sh.nvb);
-                return;
-            default:
-                break;
+            case 1: SPH_FORCE_VARIANT(512, 8, uint32_t); return;
+            case 2: SPH_FORCE_VARIANT(1024, 8, uint32_t); return;
+            case 3: SPH_FORCE_VARIANT(768, 16, uint16_t); return;
+            case 4: SPH_FORCE_VARIANT(768, 24, uint16_t); return;
+            case 5: SPH_FORCE_VARIANT(512, 32, uint16_t); return;
+            case 6: SPH_FORCE_VARIANT(512, 16, uint16_t); return;
+            case 7: SPH_FORCE_VARIANT(512, 24, uint16_t); return;
+            case 8: SPH_FORCE_VARIANT(1024, 16, uint16_t); return;
+            case 9: SPH_FORCE_VARIANT(768, 8, uint32_t); return;
+            default: break;
         }
     }
-    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, kForceTile, kListCap>), sh.grid,
-                       dim3(kBlock), 0, st, args..., sh.nvb);
+    SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t);
+#undef SPH_FORCE_VARIANT
 }
 
 __global__ void k_pack_state(const float* __restrict__ pxyz, const float* __restrict__ vxyz,
@@ -670,7 +683,7 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
 namespace sph {
 
 bool tune_word_valid(uint32_t tune) {
-    return (tune >> 16) == 0u && (tune & 0xFu) <= 2u && ((tune >> 4) & 0xFu) <= 5u &&
+    return (tune >> 16) == 0u && (tune & 0xFu) <= 5u && ((tune >> 4) & 0xFu) <= 9u &&
            ((tune >> 8) & 0xFu) <= 4u && ((tune >> 12) & 0xFu) <= 4u;
 }
 
@@ -724,7 +737,7 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
     // Density: tile first, then the grid shape for the best tile.
     float d_def = 0.f, d_best = 0.f, ms = 0.f;
     uint32_t d_tile = 0, d_grid = 0;
-    for (uint32_t v = 0; v <= 2 && rc == SPH_OK; ++v) {
+    for (uint32_t v = 0; v <= 5 && rc == SPH_OK; ++v) {
         rc = time_pass(kPassDensity, v, ms);
         if (v == 0) d_def = d_best = ms;
         else if (ms < d_best) { d_best = ms; d_tile = v; }
@@ -739,7 +752,7 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
     // Force: variant first, then the grid shape (tension runs the default).
     float f_def = 0.f, f_best = 0.f;
     uint32_t f_var = 0, f_grid = 0;
-    for (uint32_t v = 0; v <= (tension ? 0u : 5u) && rc == SPH_OK; ++v) {
+    for (uint32_t v = 0; v <= (tension ? 0u : 9u) && rc == SPH_OK; ++v) {
         rc = time_pass(kPassForce, dword | (v << 4), ms);
         if (v == 0) f_def = f_best = ms;
         else if (ms < f_best) { f_best = ms; f_var = v; }

@@ -181,6 +181,13 @@ class Solver:
         C.check(C.lib().sph_device_errors(self._h, byref(bits)))
         return int(bits.value)
 
+    def profile_passes(self, dt: float, steps: int) -> np.ndarray:
+        """Runs ``steps`` eager steps with hipEvents around each pass; returns
+        the summed milliseconds per pass (PASS_NAMES order)."""
+        ms = (ctypes.c_float * len(C.PASS_NAMES))()
+        C.check(C.lib().sph_profile_passes(self._h, float(dt), int(steps), ms))
+        return np.array(ms, np.float64)
+
     def autotune(self, dt: float, reps: int = 5):
         """Times the neighbour-pass launch variants on the current state and
         keeps the fastest (results are bit-identical for every variant).
