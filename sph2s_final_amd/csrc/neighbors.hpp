@@ -51,19 +51,26 @@ __device__ __forceinline__ float slab_gap(float f, int c, float sp) {
 }
 
 // Contiguous particle run [b, e) covering cells lo..hi of a row starting at
-// hash `row`.  Cells with a start always have an end <= n; min() keeps the
-// loop bound in range regardless.
+// hash `row`.  The starts and ends of three cells are loaded together (a
+// stencil row of cells of edge h spans at most three; indices past hi repeat
+// hi, which min/max absorb).  Empty cells hold SPH_EMPTY_CELL as start, so
+// the run's start is the minimum start and its end the maximum end over the
+// non-empty cells.  Cells with a start always have an end <= n; min() keeps
+// the loop bound in range regardless.
 __device__ __forceinline__ void row_run(const uint32_t* __restrict__ cs,
                                         const uint32_t* __restrict__ ce, uint32_t row, int lo,
                                         int hi, uint32_t n, uint32_t& b, uint32_t& e) {
     b = kEmptyCell;
     e = 0u;
-    for (int cx = lo; cx <= hi; ++cx) {
-        const uint32_t st = cs[row + cx];
-        if (st != kEmptyCell) {
-            b = min(b, st);
-            e = max(e, ce[row + cx]);
-        }
+    for (int cx = lo; cx <= hi; cx += 3) {
+        const uint32_t i0 = row + cx;
+        const uint32_t i1 = row + min(cx + 1, hi);
+        const uint32_t i2 = row + min(cx + 2, hi);
+        const uint32_t s0 = cs[i0], s1 = cs[i1], s2 = cs[i2];
+        const uint32_t e0 = ce[i0], e1 = ce[i1], e2 = ce[i2];
+        b = min(b, min(s0, min(s1, s2)));
+        e = max(e, max(s0 != kEmptyCell ? e0 : 0u,
+                       max(s1 != kEmptyCell ? e1 : 0u, s2 != kEmptyCell ? e2 : 0u)));
     }
     e = min(e, n);
 }

@@ -374,7 +374,7 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
                     for (int u = 0; u < kG; ++u) q[u] = tp[t + u];
 #pragma unroll
                     for (int u = 0; u < kG; ++u) {
-                        const bool hit = (t + u < t1) && in_range(q[u]);
+                        const bool hit = (t + u < t1) & in_range(q[u]);  // no short circuit
                         list[cnt][threadIdx.x] = static_cast<TList>(t + u);
                         cnt += hit ? 1u : 0u;
                     }
