@@ -128,6 +128,13 @@ template <class Load>
 __device__ __forceinline__ float poly6_run(const float4& xi, float h2, uint32_t m, Load&& load) {
     float rs = 0.0f;
     uint32_t k = 0;
+    for (; k + 8 <= m; k += 8) {
+        float4 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = load(k + u);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rs += poly6_term(xi, q[u], h2);
+    }
     for (; k + 4 <= m; k += 4) {
         const float4 q0 = load(k), q1 = load(k + 1), q2 = load(k + 2), q3 = load(k + 3);
         rs += poly6_term(xi, q0, h2);
@@ -312,10 +319,13 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
             Tz += w * dz;
         }
     };
+    // 0 < r2 < h2 as one unsigned compare: r2 >= +0 is a sum of squares, and
+    // for non-negative floats the bit patterns order like the values.
+    const uint32_t h2_bits_m1 = __float_as_uint(p.h2) - 1u;
     auto in_range = [&](const float4& xj) {
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        return (r2 < p.h2) && (r2 > 0.0f);
+        return __float_as_uint(r2) - 1u < h2_bits_m1;
     };
     // Queued pairs in queue order; two list entries are fetched per step so
     // their tile reads overlap.
