@@ -340,7 +340,9 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
         align_up(4 * N),  align_up(4 * N),  align_up(4 * N),                     // keys skeys perm
         align_up(4 * C),  align_up(4 * C),  align_up(sort_temp),                 // cells, sort temp
         align_up(8 * N),                                                         // aux
-        align_up(4)};                                                            // live count
+        align_up(4),                                                             // live count
+        align_up(16 * N), align_up(16 * N), align_up(8 * N),                    // ipos ivel iaux
+        align_up(4 * N),  align_up(16 * C)};                                     // i2r xbins
     size_t total = 0;
     for (size_t b : sizes) total += b;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -375,6 +377,11 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
     s->buf.sort_temp = take(11);
     s->buf.aux = static_cast<float2*>(take(12));
     s->buf.n_dev = static_cast<uint32_t*>(take(13));
+    s->buf.ipos = static_cast<float4*>(take(14));
+    s->buf.ivel = static_cast<float4*>(take(15));
+    s->buf.iaux = static_cast<float2*>(take(16));
+    s->buf.i2r = static_cast<uint32_t*>(take(17));
+    s->buf.xbins = static_cast<uint4*>(take(18));
     s->buf.sort_temp_bytes = sort_temp;
 
     e = hipMemsetAsync(s->arena, 0, total, s->stream);

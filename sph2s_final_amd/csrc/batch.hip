@@ -116,7 +116,8 @@ int batch_alloc(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* count
     const size_t sizes[] = {al(16 * N), al(16 * N), al(16 * N), al(16 * N), al(4 * N),
                             al(16 * N), al(4 * N),  al(4 * N),  al(4 * N),  al(4 * C),
                             al(4 * C),  al(sort_temp), al(8 * N), al(4),
-                            al(sizeof(SceneParams) * n_scenes), al(4 * (n_scenes + 1))};
+                            al(sizeof(SceneParams) * n_scenes), al(4 * (n_scenes + 1)),
+                            al(16 * N), al(16 * N), al(8 * N), al(4 * N), al(16 * C)};
     size_t total = 0;
     for (size_t v : sizes) total += v;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -154,6 +155,11 @@ int batch_alloc(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* count
     b->buf.n_dev = static_cast<uint32_t*>(take());
     b->scenes = static_cast<SceneParams*>(take());
     b->part_offset_dev = static_cast<uint32_t*>(take());
+    b->buf.ipos = static_cast<float4*>(take());
+    b->buf.ivel = static_cast<float4*>(take());
+    b->buf.iaux = static_cast<float2*>(take());
+    b->buf.i2r = static_cast<uint32_t*>(take());
+    b->buf.xbins = static_cast<uint4*>(take());
 
     e = hipMemsetAsync(b->arena, 0, total, b->stream);
     if (e == hipSuccess)

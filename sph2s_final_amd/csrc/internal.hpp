@@ -118,6 +118,15 @@ struct StepBuffers {
     void*  sort_temp;
     size_t sort_temp_bytes;
     uint32_t* n_dev;    // live particle count (device word)
+    // Internal x-order of the neighbour passes (all NULL: they run in the
+    // sorted order).  Within each cell the particles are ordered by x, so a
+    // stencil row's candidates can be trimmed to the x-window; results are
+    // written back at the sorted slots.
+    float4* ipos;       // sorted state in the internal order
+    float4* ivel;
+    float2* iaux;       // aux in the internal order (density -> force)
+    uint32_t* i2r;      // internal slot -> sorted slot
+    uint4* xbins;       // per cell: cumulative counts of its 8 x-bins (16 bits each)
 };
 
 // Passes of one step, in launch order.

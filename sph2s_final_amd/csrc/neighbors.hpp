@@ -187,6 +187,78 @@ __device__ __forceinline__ void probe_row_run(const GridView& g, const Probe& pr
     }
 }
 
+// x-bins: each cell's edge is cut into 8 bins; the internal order keeps a
+// cell's particles sorted by x, so each bin is a contiguous range of it.
+// t is the position along x in cell units relative to the cell's low face.
+__device__ __forceinline__ int xbin_of(float t) {
+    return min(7, max(0, static_cast<int>(floorf(t * 8.0f))));
+}
+
+// Particles of the cell in bins 0..k (the table packs the 8 cumulative counts
+// as 16-bit pairs; a last count of 0xFFFF marks a cell too large to bin).
+__device__ __forceinline__ uint32_t xbin_count(const uint4& v, int k) {
+    const uint32_t w = (k < 2) ? v.x : (k < 4) ? v.y : (k < 6) ? v.z : v.w;
+    return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+// probe_row_run in the internal x-order: the run additionally starts at the
+// first x-bin of its first cell, and ends after the last x-bin of its last
+// cell, that the x-window [x - w, x + w] reaches.  w carries the pruning
+// margin h (1 + 1e-3), far above the rounding of bin positions, so no
+// particle with r < h is dropped.
+__device__ __forceinline__ void probe_row_run_x(const GridView& g, const Probe& pr,
+                                                const uint32_t* __restrict__ cs,
+                                                const uint32_t* __restrict__ ce,
+                                                const uint4* __restrict__ bins, uint32_t n,
+                                                int ry, int rz, uint32_t& b, uint32_t& e) {
+    b = kEmptyCell;
+    e = 0u;
+    if (ry < pr.sy || ry > pr.ey || rz < pr.sz || rz > pr.ez) return;
+    const float gy = slab_gap(pr.fy, ry, g.spacing);
+    const float gz = slab_gap(pr.fz, rz, g.spacing);
+    const float d2 = gy * gy + gz * gz;
+    if (d2 >= g.prune_h2) return;
+    const float w = __builtin_amdgcn_sqrtf(g.prune_h2 - d2);
+    const float xl = (pr.fx - w) * g.inv_spacing, xh = (pr.fx + w) * g.inv_spacing;
+    const int lo = max(pr.sx, static_cast<int>(floorf(xl)));
+    const int hi = min(pr.ex, static_cast<int>(floorf(xh)));
+    if (lo > hi) return;
+    const uint32_t row = g.cell_offset + static_cast<uint32_t>(g.dx) * (ry + g.dy * rz);
+    row_run(cs, ce, row, lo, hi, n, b, e);
+    if (b >= e) {
+        b = kEmptyCell;
+        e = 0u;
+        return;
+    }
+    const uint32_t s_lo = cs[row + lo], s_hi = cs[row + hi];
+    if (s_lo != kEmptyCell) {  // the run starts in cell lo
+        const uint4 t = bins[row + lo];
+        const int k = xbin_of(xl - static_cast<float>(lo));
+        if (k > 0 && xbin_count(t, 7) != 0xFFFFu) b = s_lo + xbin_count(t, k - 1);
+    }
+    if (s_hi != kEmptyCell) {  // the run ends in cell hi
+        const uint4 t = bins[row + hi];
+        const int k = xbin_of(xh - static_cast<float>(hi));
+        if (xbin_count(t, 7) != 0xFFFFu) e = min(e, s_hi + xbin_count(t, k));
+    }
+    if (b >= e) {
+        b = kEmptyCell;
+        e = 0u;
+    }
+}
+
+template <bool kX>
+__device__ __forceinline__ void probe_row(const GridView& g, const Probe& pr,
+                                          const uint32_t* __restrict__ cs,
+                                          const uint32_t* __restrict__ ce,
+                                          const uint4* __restrict__ bins, uint32_t n, int ry,
+                                          int rz, uint32_t& b, uint32_t& e) {
+    if (kX)
+        probe_row_run_x(g, pr, cs, ce, bins, n, ry, rz, b, e);
+    else
+        probe_row_run(g, pr, cs, ce, n, ry, rz, b, e);
+}
+
 // Block-wide [min b, max e) over the lanes' runs (empty runs contribute
 // b = SPH_EMPTY_CELL, e = 0).  red must hold 2 * (blockDim / 64) words.
 template <int kBlockThreads>

@@ -56,6 +56,117 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
     tvel[s] = vel[j];
 }
 
+// Internal x-order: slot s of the sorted order (cell c, run [b, e)) moves to
+// b + rank, its rank by (x, sorted slot) among the cell's particles, so each
+// cell's particles are sorted by x (deterministically: ties keep the sorted
+// order).  Cells are short (tens to a few hundred particles), so the rank is
+// a direct count over the cell.  The block stages
+// the x of every particle in its cells' runs in LDS (lanes of one cell then
+// read the same LDS word each step); runs beyond the staged range are read
+// from global memory.
+constexpr int kXorderTile = 2048;
+
+template <bool kBatched>
+__global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ skeys,
+                                                   const float4* __restrict__ tpos,
+                                                   const float4* __restrict__ tvel,
+                                                   const uint32_t* __restrict__ cell_start,
+                                                   const uint32_t* __restrict__ cell_end,
+                                                   const uint32_t* __restrict__ n_dev,
+                                                   StepParams p,
+                                                   const SceneParams* __restrict__ scenes,
+                                                   float4* __restrict__ ipos,
+                                                   float4* __restrict__ ivel,
+                                                   uint32_t* __restrict__ i2r,
+                                                   uint4* __restrict__ bins) {
+    __shared__ float xs_tile[kXorderTile];
+    __shared__ uint32_t red[2 * (kBlock / 64)];
+    const uint32_t n = *n_dev;
+    const uint32_t block0 = blockIdx.x * kBlock;
+    if (block0 >= n) return;  // block-uniform
+    const uint32_t s = block0 + threadIdx.x;
+    const bool active = s < n;
+    uint32_t key = 0, b = kEmptyCell, e = 0u;
+    if (active) {
+        key = min(skeys[s], p.num_cells - 1);
+        // s lies in its cell's run, so the run is non-empty and b <= s < e <= n.
+        b = min(cell_start[key], s);
+        e = max(min(cell_end[key], n), s + 1);
+    }
+    uint32_t ub, ue;
+    block_run_union<kBlock>(b, e, red, ub, ue);
+    const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kXorderTile)) : 0u;
+    for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) xs_tile[k] = tpos[ub + k].x;
+    __syncthreads();
+    if (!active) return;
+    const float4 xs = tpos[s];
+    const float4 vs = tvel[s];
+    const bool staged = e <= ub + seg_n;
+    uint32_t rank = 0;
+    if (staged) {
+        for (uint32_t j = b; j < e; ++j) {
+            const float xj = xs_tile[j - ub];
+            rank += ((xj < xs.x) | ((xj == xs.x) & (j < s))) ? 1u : 0u;
+        }
+    } else {
+        for (uint32_t j = b; j < e; ++j) {
+            const float xj = tpos[j].x;
+            rank += ((xj < xs.x) | ((xj == xs.x) & (j < s))) ? 1u : 0u;
+        }
+    }
+    const uint32_t q = b + rank;
+    ipos[q] = xs;
+    ivel[q] = vs;
+    i2r[q] = s;
+}
+
+// Per cell, the cumulative counts of its 8 x-bins (packed 16-bit), from the
+// internal order: bins are non-decreasing along a cell's x-sorted particles,
+// so the particle at internal slot q (local index l = q - start) holds the
+// count l + 1 for every bin from its own up to (not including) the next
+// particle's bin, or up to the last bin if it is the cell's last particle;
+// the cell's first particle also writes 0 for the bins below its own.  Every
+// entry is written by exactly one particle.  Cells of more than 0xFFFE
+// particles get the all-ones marker (no trimming).
+template <bool kBatched>
+__global__ __launch_bounds__(kBlock) void k_xbins(const uint32_t* __restrict__ skeys,
+                                                  const float4* __restrict__ ipos,
+                                                  const float4* __restrict__ ivel,
+                                                  const uint32_t* __restrict__ cell_start,
+                                                  const uint32_t* __restrict__ cell_end,
+                                                  const uint32_t* __restrict__ n_dev,
+                                                  StepParams p,
+                                                  const SceneParams* __restrict__ scenes,
+                                                  uint4* __restrict__ bins) {
+    const uint32_t n = *n_dev;
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t key = min(skeys[q], p.num_cells - 1);
+    const uint32_t b = min(cell_start[key], q), e = max(min(cell_end[key], n), q + 1);
+    uint16_t* t = reinterpret_cast<uint16_t*>(bins + key);
+    if (e - b > 0xFFFEu) {
+        if (q == b)
+            for (int k = 0; k < 8; ++k) t[k] = 0xFFFFu;
+        return;
+    }
+    float ox = p.ox, inv = 1.0f / p.spacing;
+    uint32_t local = key, dx = static_cast<uint32_t>(p.dx);
+    if (kBatched) {
+        const GridView& g = scenes[__float_as_uint(ivel[q].w)].g;
+        ox = g.ox;
+        inv = g.inv_spacing;
+        dx = static_cast<uint32_t>(g.dx);
+        local = key - g.cell_offset;
+    }
+    const float cx = static_cast<float>(local % dx);
+    const int kq = xbin_of((ipos[q].x - ox) * inv - cx);
+    const int kn = (q + 1 < e) ? max(kq, xbin_of((ipos[q + 1].x - ox) * inv - cx)) : 8;
+    if (q == b)
+        for (int k = 0; k < kq; ++k) t[k] = 0u;
+    const uint16_t cnt = static_cast<uint16_t>(q - b + 1);
+    for (int k = kq; k < kn; ++k) t[k] = cnt;
+}
+
 #ifndef SPH_DENSITY_TILE
 #define SPH_DENSITY_TILE 1536
 #endif
@@ -84,6 +195,14 @@ constexpr int kWavesPerBlock = kBlock / 64;
 //                                  looping over the 256-particle blocks
 constexpr uint32_t kPersistPerCu[] = {0, 1, 2, 4, 8};
 const SceneParams* const kNoScenes = nullptr;
+
+// Internal x-order of a neighbour pass (kX): the pass reads the internal-order
+// state, trims rows with the cells' x-bins and writes results at i2r[slot].
+struct XOrder {
+    const uint32_t* i2r;
+    const uint4* bins;
+    float2* iaux;  // density: internal-order copy of aux for the force pass
+};
 constexpr uint32_t kNumCus = 256;
 
 struct LaunchShape {
@@ -165,7 +284,7 @@ __device__ __forceinline__ void scene_constants(const SceneParams* __restrict__ 
     }
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched, int kTile>
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX>
 __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float4* tile,
                                               uint32_t* red, const float4* __restrict__ tpos,
                                               const uint32_t* __restrict__ cs,
@@ -173,7 +292,8 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
                                               const StepParams& p_in, const GridView& g_in,
                                               float* __restrict__ rho, float2* __restrict__ aux,
                                               const float4* __restrict__ tvel,
-                                              const SceneParams* __restrict__ scenes) {
+                                              const SceneParams* __restrict__ scenes,
+                                              const XOrder& xo) {
     const uint32_t s = block0 + threadIdx.x;
     const bool active = s < n;
     StepParams p = p_in;
@@ -195,7 +315,7 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row_run(g, pr, cs, ce, n, pr.cy + dy, pr.cz + dz, b, e);
+            if (active) probe_row<kX>(g, pr, cs, ce, xo.bins, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
             block_run_union<kBlock>(b, e, red, ub, ue);
             // Loaded range [ub, seg_e); empty when every lane's run is empty.
@@ -220,21 +340,24 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
         for (int ry = pr.sy; ry <= pr.ey; ++ry) {
             if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
             uint32_t b, e;
-            probe_row_run(g, pr, cs, ce, n, ry, rz, b, e);
+            probe_row<kX>(g, pr, cs, ce, xo.bins, n, ry, rz, b, e);
             if (b < e) {
                 const float4* t = tpos + b;
                 accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
         }
     const float r = p.c6m * sum;
-    rho[s] = r;
+    const uint32_t o = kX ? xo.i2r[s] : s;
+    rho[o] = r;
     if (kFinalize) {
         const float inv = __builtin_amdgcn_rcpf(r);
-        aux[s] = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
+        const float2 ab = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
+        aux[o] = ab;
+        if (kX) xo.iaux[s] = ab;
     }
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched, int kTile>
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX>
 __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
@@ -244,15 +367,15 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
                                                     float2* __restrict__ aux,
                                                     const float4* __restrict__ tvel,
                                                     const SceneParams* __restrict__ scenes,
-                                                    uint32_t nvb) {
+                                                    XOrder xo, uint32_t nvb) {
     __shared__ float4 tile[kTile];
     __shared__ uint32_t red[2 * kWavesPerBlock];
     const uint32_t n = *n_dev;
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
         if (block0 < n)  // block-uniform
-            density_block<kKahan, kFinalize, kBatched, kTile>(
-                block0, n, tile, red, tpos, cs, ce, p_in, g_in, rho, aux, tvel, scenes);
+            density_block<kKahan, kFinalize, kBatched, kTile, kX>(
+                block0, n, tile, red, tpos, cs, ce, p_in, g_in, rho, aux, tvel, scenes, xo);
     }
 }
 
@@ -270,7 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ t
 // tile holds (x, y, z, A_j) and (vx, vy, vz, B_j) per candidate.
 // With kIntegrate = false (two-scale own-set pass) the kernel only writes the
 // acceleration, gravity included, to accel.
-template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList>
+template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
+          bool kX>
 __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4* tp, float4* tv,
                                             TList (*list)[kBlock], uint32_t* red,
                                             const float4* __restrict__ tpos,
@@ -282,7 +406,8 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
                                             float dt, float4* __restrict__ pos,
                                             float4* __restrict__ vel,
                                             float4* __restrict__ accel,
-                                            const SceneParams* __restrict__ scenes) {
+                                            const SceneParams* __restrict__ scenes,
+                                            const XOrder& xo) {
     const uint32_t s = block0 + threadIdx.x;
     const bool active = s < n;
     StepParams p = p_in;
@@ -357,7 +482,7 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row_run(g, pr, cs, ce, n, pr.cy + dy, pr.cz + dz, b, e);
+            if (active) probe_row<kX>(g, pr, cs, ce, xo.bins, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
             block_run_union<kBlock>(b, e, red, ub, ue);
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
@@ -400,7 +525,7 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
         for (int ry = pr.sy; ry <= pr.ey; ++ry) {
             if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
             uint32_t b, e;
-            probe_row_run(g, pr, cs, ce, n, ry, rz, b, e);
+            probe_row<kX>(g, pr, cs, ce, xo.bins, n, ry, rz, b, e);
             global_run(b, e);
         }
 
@@ -414,8 +539,9 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
         ay -= tc * Ty;
         az -= tc * Tz;
     }
+    const uint32_t o = kX ? xo.i2r[s] : s;
     if (!kIntegrate) {
-        accel[s] = make_float4(ax, ay, az, 0.0f);
+        accel[o] = make_float4(ax, ay, az, 0.0f);
         return;
     }
     // integrateD (Solver.cu:713-755): v += dt a; x += dt v; then the wall clamp.
@@ -430,12 +556,13 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
         if (z < p.loz) { z = p.loz; vz = -e * vz; }
         if (z > p.hiz) { z = p.hiz; vz = -e * vz; }
     }
-    pos[s] = make_float4(x, y, z, xi.w);
-    vel[s] = make_float4(vx, vy, vz, vi.w);
-    if (accel) accel[s] = make_float4(ax, ay, az, 0.0f);
+    pos[o] = make_float4(x, y, z, xi.w);
+    vel[o] = make_float4(vx, vy, vz, vi.w);
+    if (accel) accel[o] = make_float4(ax, ay, az, 0.0f);
 }
 
-template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList>
+template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
+          bool kX>
 __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
                                                   const float2* __restrict__ aux,
@@ -447,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
                                                   float4* __restrict__ vel,
                                                   float4* __restrict__ accel,
                                                   const SceneParams* __restrict__ scenes,
-                                                  uint32_t nvb) {
+                                                  XOrder xo, uint32_t nvb) {
     static_assert(kCap >= kScanGroup, "a scan group queues up to kCap / 2 candidates");
     static_assert(kTile + kScanGroup <= 65536 || sizeof(TList) == 4, "list entries index the tile");
     __shared__ float4 tp[kTile + kScanGroup];  // padded for whole-group loads
@@ -458,22 +585,24 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
         const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
         if (block0 < n)  // block-uniform
-            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList>(
+            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList, kX>(
                 block0, n, tp, tv, list, red, tpos, tvel, aux, cs, ce, p_in, g_in, dt, pos, vel,
-                accel, scenes);
+                accel, scenes, xo);
     }
 }
 
 // Variant dispatch.  Only the passes of the single-scale and batched steps
 // (density with aux, force with integration and no tension) are tunable; the
-// other instantiations use the defaults.
+// other instantiations use the defaults.  The single-scale and batched passes
+// (density with aux, force with integration) run in the internal x-order; the
+// two-scale own-set passes (neither) in the sorted order.
 template <bool kKahan, bool kFinalize, bool kBatched, class... A>
 void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
     const LaunchShape sh = launch_shape(cap_n, kFinalize ? ((tune >> 8) & 0xFu) : 0u);
 #define SPH_DENSITY_VARIANT(TILE)                                                               \
-    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE>), sh.grid, dim3(kBlock),  \
-                       0, st, args..., sh.nvb)
+    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kFinalize>), sh.grid,     \
+                       dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kFinalize) {
         switch (v) {
             case 1: SPH_DENSITY_VARIANT(512); return;
@@ -494,8 +623,8 @@ void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
     const LaunchShape sh = launch_shape(cap_n, kTunable ? ((tune >> 12) & 0xFu) : 0u);
 #define SPH_FORCE_VARIANT(TILE, CAP, TLIST)                                                     \
-    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST>), sh.grid,   \
-                       dim3(kBlock), 0, st, args..., sh.nvb)
+    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kIntegrate>), \
+                       sh.grid, dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kTunable) {
         switch (v) {
             case 1: SPH_FORCE_VARIANT(512, 8, uint32_t); return;
@@ -558,27 +687,38 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
             hipLaunchKernelGGL(k_reorder, blocks_for(p.n), dim3(kBlock), 0, stream, b.skeys,
                                b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start, b.cell_end,
                                b.n_dev, p.num_cells);
+            if (b.ipos) {
+                hipLaunchKernelGGL(k_xorder<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.skeys, b.tpos, b.tvel, b.cell_start, b.cell_end, b.n_dev, p,
+                                   kNoScenes, b.ipos, b.ivel, b.i2r, b.xbins);
+                hipLaunchKernelGGL(k_xbins<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.skeys, b.ipos, b.ivel, b.cell_start, b.cell_end, b.n_dev, p,
+                                   kNoScenes, b.xbins);
+            }
             break;
         case kPassDensity:
+            if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
             if (p.flags & SPH_FLAG_KAHAN)
-                launch_density<true, true, false>(p.tune, p.n, stream, b.tpos, b.cell_start,
-                                                  b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel,
-                                                  kNoScenes);
+                launch_density<true, true, false>(p.tune, p.n, stream, b.ipos, b.cell_start,
+                                                  b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.ivel,
+                                                  kNoScenes, XOrder{b.i2r, b.xbins, b.iaux});
             else
-                launch_density<false, true, false>(p.tune, p.n, stream, b.tpos, b.cell_start,
-                                                   b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.tvel,
-                                                   kNoScenes);
+                launch_density<false, true, false>(p.tune, p.n, stream, b.ipos, b.cell_start,
+                                                   b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.ivel,
+                                                   kNoScenes, XOrder{b.i2r, b.xbins, b.iaux});
             break;
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
+            if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
+            const XOrder xo{b.i2r, b.xbins, nullptr};
             if (p.tens != 0.0f)
-                launch_force<true, true, false>(p.tune, p.n, stream, b.tpos, b.tvel, b.aux,
+                launch_force<true, true, false>(p.tune, p.n, stream, b.ipos, b.ivel, b.iaux,
                                                 b.cell_start, b.cell_end, p, g, b.n_dev, dt, b.pos,
-                                                b.vel, acc, kNoScenes);
+                                                b.vel, acc, kNoScenes, xo);
             else
-                launch_force<false, true, false>(p.tune, p.n, stream, b.tpos, b.tvel, b.aux,
+                launch_force<false, true, false>(p.tune, p.n, stream, b.ipos, b.ivel, b.iaux,
                                                  b.cell_start, b.cell_end, p, g, b.n_dev, dt,
-                                                 b.pos, b.vel, acc, kNoScenes);
+                                                 b.pos, b.vel, acc, kNoScenes, xo);
             break;
         }
         default:
@@ -602,10 +742,10 @@ int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t st
     const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
     if (p.flags & SPH_FLAG_KAHAN)
         launch_density<true, false, false>(0u, p.n, stream, b.tpos, b.cell_start, b.cell_end, p, g,
-                                           b.n_dev, b.rho, b.aux, b.tvel, kNoScenes);
+                                           b.n_dev, b.rho, b.aux, b.tvel, kNoScenes, XOrder{});
     else
         launch_density<false, false, false>(0u, p.n, stream, b.tpos, b.cell_start, b.cell_end, p,
-                                            g, b.n_dev, b.rho, b.aux, b.tvel, kNoScenes);
+                                            g, b.n_dev, b.rho, b.aux, b.tvel, kNoScenes, XOrder{});
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -617,11 +757,11 @@ int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& 
     if (p.tens != 0.0f)
         launch_force<true, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
                                          b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
-                                         accel_own, kNoScenes);
+                                         accel_own, kNoScenes, XOrder{});
     else
         launch_force<false, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
                                           b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
-                                          accel_own, kNoScenes);
+                                          accel_own, kNoScenes, XOrder{});
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -657,28 +797,44 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
             break;
         }
         case kPassSort:
-        case kPassReorder:
             return launch_pass(pass, cap, b, dt, key_bits, write_accel, stream);
+        case kPassReorder: {
+            if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
+            StepBuffers sorted_only = b;
+            sorted_only.ipos = nullptr;
+            const int rc = launch_pass(pass, cap, sorted_only, dt, key_bits, write_accel, stream);
+            if (rc != SPH_OK) return rc;
+            hipLaunchKernelGGL(k_xorder<true>, blocks_for(cap.n), dim3(kBlock), 0, stream, b.skeys,
+                               b.tpos, b.tvel, b.cell_start, b.cell_end, b.n_dev, cap, scenes,
+                               b.ipos, b.ivel, b.i2r, b.xbins);
+            hipLaunchKernelGGL(k_xbins<true>, blocks_for(cap.n), dim3(kBlock), 0, stream, b.skeys,
+                               b.ipos, b.ivel, b.cell_start, b.cell_end, b.n_dev, cap, scenes,
+                               b.xbins);
+            break;
+        }
         case kPassDensity:
+            if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
             if (kahan)
-                launch_density<true, true, true>(cap.tune, cap.n, stream, b.tpos, b.cell_start,
+                launch_density<true, true, true>(cap.tune, cap.n, stream, b.ipos, b.cell_start,
                                                  b.cell_end, cap, g0, b.n_dev, b.rho, b.aux,
-                                                 b.tvel, scenes);
+                                                 b.ivel, scenes, XOrder{b.i2r, b.xbins, b.iaux});
             else
-                launch_density<false, true, true>(cap.tune, cap.n, stream, b.tpos, b.cell_start,
+                launch_density<false, true, true>(cap.tune, cap.n, stream, b.ipos, b.cell_start,
                                                   b.cell_end, cap, g0, b.n_dev, b.rho, b.aux,
-                                                  b.tvel, scenes);
+                                                  b.ivel, scenes, XOrder{b.i2r, b.xbins, b.iaux});
             break;
         case kPassForce: {
+            if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
             float4* acc = write_accel ? b.accel : nullptr;
+            const XOrder xo{b.i2r, b.xbins, nullptr};
             if (tension)
-                launch_force<true, true, true>(cap.tune, cap.n, stream, b.tpos, b.tvel, b.aux,
+                launch_force<true, true, true>(cap.tune, cap.n, stream, b.ipos, b.ivel, b.iaux,
                                                b.cell_start, b.cell_end, cap, g0, b.n_dev, dt,
-                                               b.pos, b.vel, acc, scenes);
+                                               b.pos, b.vel, acc, scenes, xo);
             else
-                launch_force<false, true, true>(cap.tune, cap.n, stream, b.tpos, b.tvel, b.aux,
+                launch_force<false, true, true>(cap.tune, cap.n, stream, b.ipos, b.ivel, b.iaux,
                                                 b.cell_start, b.cell_end, cap, g0, b.n_dev, dt,
-                                                b.pos, b.vel, acc, scenes);
+                                                b.pos, b.vel, acc, scenes, xo);
             break;
         }
         default:
