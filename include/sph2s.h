@@ -188,12 +188,12 @@ int sph_device_errors(sph_solver* s, uint32_t* bits);
 /* ---- launch variants and the autotuner -------------------------------------
  * The reference launches fixed block shapes (Solver.cu:1082-1198).  Here the
  * density and force passes have launch variants, selected by a tune word:
- *   bits 0-3   density LDS tile: 0 = 1536, 1 = 512, 2 = 1024, 3 = 2048, 4 = 768,
+ *   bits 0-3   density LDS tile: 0 = 1024, 1 = 512, 2 = 1536, 3 = 2048, 4 = 768,
  *              5 = 1280 particles
  *   bits 4-7   force LDS tile / neighbour-list capacity / entry width:
- *              0 = 640/16/16-bit, 1 = 512/8/32, 2 = 1024/8/32, 3 = 768/16/16,
- *              4 = 768/24/16, 5 = 512/32/16, 6 = 512/16/16, 7 = 512/24/16,
- *              8 = 1024/16/16, 9 = 768/8/32
+ *              0 = 768/16/16-bit, 1 = 640/16/16, 2 = 768/24/16, 3 = 768/8/32,
+ *              4 = 512/16/16, 5 = 1024/16/16, 6 = 512/8/32, 7 = 640/24/16,
+ *              8 = 1024/8/32, 9 = 512/24/16
  *   bits 8-11  density grid, bits 12-15 force grid: 0 = one 256-thread block
  *              per 256 particles; 1..4 = persistent grid of 1, 2, 4, 8 blocks
  *              per CU looping over those blocks

@@ -168,10 +168,10 @@ __global__ __launch_bounds__(kBlock) void k_xbins(const uint32_t* __restrict__ s
 }
 
 #ifndef SPH_DENSITY_TILE
-#define SPH_DENSITY_TILE 1536
+#define SPH_DENSITY_TILE 1024
 #endif
 #ifndef SPH_FORCE_TILE
-#define SPH_FORCE_TILE 640
+#define SPH_FORCE_TILE 768
 #endif
 #ifndef SPH_LIST_CAP
 #define SPH_LIST_CAP 16
@@ -186,10 +186,10 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // bit-identical results: the tile size only decides which runs are read from
 // LDS instead of global memory, the list capacity only when queued pairs are
 // evaluated, and the grid shape only which block processes which particles.
-//   bits 0-3   density LDS tile: 1536 (default), 512, 1024, 2048, 768, 1280
-//   bits 4-7   force tile / list capacity / list entry:  640/16/u16 (default),
-//              512/8/u32, 1024/8/u32, 768/16/u16, 768/24/u16, 512/32/u16,
-//              512/16/u16, 512/24/u16, 1024/16/u16, 768/8/u32
+//   bits 0-3   density LDS tile: 1024 (default), 512, 1536, 2048, 768, 1280
+//   bits 4-7   force tile / list capacity / list entry:  768/16/u16 (default),
+//              640/16/u16, 768/24/u16, 768/8/u32, 512/16/u16, 1024/16/u16,
+//              512/8/u32, 640/24/u16, 1024/8/u32, 512/24/u16
 //   bits 8-11  density grid:       one block per 256 particles, or a
 //   bits 12-15 force grid:         persistent grid of 1, 2, 4, 8 blocks per CU
 //                                  looping over the 256-particle blocks
@@ -497,11 +497,11 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
             }
             __syncthreads();
             const bool staged = b < e && b >= ub && e <= seg_e;
+            constexpr int kG = kCap >= 16 ? kScanGroup : kScanGroup / 2;
             if (staged) {
                 // Whole groups of candidates are loaded before any test (the
                 // tile is padded); every candidate's index is written at the
                 // queue tail and the tail advances only for in-range ones.
-                constexpr int kG = kCap >= 16 ? kScanGroup : kScanGroup / 2;
                 const uint32_t t0 = b - ub, t1 = e - ub;
                 for (uint32_t t = t0; t < t1; t += kG) {
                     float4 q[kG];
@@ -606,7 +606,7 @@ void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     if constexpr (kFinalize) {
         switch (v) {
             case 1: SPH_DENSITY_VARIANT(512); return;
-            case 2: SPH_DENSITY_VARIANT(1024); return;
+            case 2: SPH_DENSITY_VARIANT(1536); return;
             case 3: SPH_DENSITY_VARIANT(2048); return;
             case 4: SPH_DENSITY_VARIANT(768); return;
             case 5: SPH_DENSITY_VARIANT(1280); return;
@@ -627,15 +627,15 @@ void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
                        sh.grid, dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kTunable) {
         switch (v) {
-            case 1: SPH_FORCE_VARIANT(512, 8, uint32_t); return;
-            case 2: SPH_FORCE_VARIANT(1024, 8, uint32_t); return;
-            case 3: SPH_FORCE_VARIANT(768, 16, uint16_t); return;
-            case 4: SPH_FORCE_VARIANT(768, 24, uint16_t); return;
-            case 5: SPH_FORCE_VARIANT(512, 32, uint16_t); return;
-            case 6: SPH_FORCE_VARIANT(512, 16, uint16_t); return;
-            case 7: SPH_FORCE_VARIANT(512, 24, uint16_t); return;
-            case 8: SPH_FORCE_VARIANT(1024, 16, uint16_t); return;
-            case 9: SPH_FORCE_VARIANT(768, 8, uint32_t); return;
+            case 1: SPH_FORCE_VARIANT(640, 16, uint16_t); return;
+            case 2: SPH_FORCE_VARIANT(768, 24, uint16_t); return;
+            case 3: SPH_FORCE_VARIANT(768, 8, uint32_t); return;
+            case 4: SPH_FORCE_VARIANT(512, 16, uint16_t); return;
+            case 5: SPH_FORCE_VARIANT(1024, 16, uint16_t); return;
+            case 6: SPH_FORCE_VARIANT(512, 8, uint32_t); return;
+            case 7: SPH_FORCE_VARIANT(640, 24, uint16_t); return;
+            case 8: SPH_FORCE_VARIANT(1024, 8, uint32_t); return;
+            case 9: SPH_FORCE_VARIANT(512, 24, uint16_t); return;
             default: break;
         }
     }
