@@ -26,11 +26,17 @@
 //                    written at the step's input slot perm[i].
 // The pair sums use that j is a neighbour of i iff i is one of j's.
 //
+// The adjoint passes run in the forward step's internal x-order (cells sorted
+// by x inside, stencil rows trimmed with the cells' x-bins), like the forward
+// neighbour passes; the adjoint of the step's outputs is gathered from the
+// sorted slots and the result scattered to the step's input slots
+// perm[i2r[q]].
+//
 // Memory: the state is checkpointed every K steps; the backward sweep
 // recomputes each K-step segment from its checkpoint, keeping every step's
-// sorted state, accelerations, densities, permutation and cell ranges
-// (64 B per particle + 8 B per cell per step), then runs the adjoint passes
-// over the segment in reverse.  The forward writes the last segment's records
+// x-ordered state, accelerations, densities, A/B factors, permutations, cell
+// ranges and x-bin tables (68 B per particle + 24 B per cell per step), then
+// runs the adjoint passes over the segment in reverse.  The forward writes the last segment's records
 // itself, so that segment is never recomputed.  Parameter gradients accumulate per slot in
 // FP64 (a slot always belongs to the same scene) and are reduced per scene in
 // a fixed order, so the result is deterministic.
@@ -112,15 +118,18 @@ __device__ __forceinline__ const SceneParams& scene_for(const SceneParams* __res
 // Clamp and symplectic Euler.  The pre-clamp position is recomputed with the
 // forward kernel's expressions from the stored sorted state and acceleration.
 __global__ __launch_bounds__(kBlock) void k_adj_integrate(
-    const float4* __restrict__ tpos, const float4* __restrict__ tvel,
-    const float4* __restrict__ accel, const float4* __restrict__ gx,
-    const float4* __restrict__ gv, const SceneParams* __restrict__ scenes,
-    const uint32_t* __restrict__ n_dev, float dt, float4* __restrict__ abar,
-    float4* __restrict__ gxd, float4* __restrict__ gvd) {
+    const float4* __restrict__ ipos, const float4* __restrict__ ivel,
+    const float4* __restrict__ accel, const uint32_t* __restrict__ i2r,
+    const float4* __restrict__ gx, const float4* __restrict__ gv,
+    const SceneParams* __restrict__ scenes, const uint32_t* __restrict__ n_dev, float dt,
+    float4* __restrict__ abar, float4* __restrict__ gxd, float4* __restrict__ gvd) {
     const uint32_t n = *n_dev;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= n) return;
-    const float4 xi = tpos[s], vi = tvel[s], ai = accel[s], gxi = gx[s], gvi = gv[s];
+    // s: internal slot; the step's outputs (and their adjoints) sit at its
+    // sorted slot i2r[s].
+    const uint32_t o = min(i2r[s], n - 1);
+    const float4 xi = ipos[s], vi = ivel[s], ai = accel[o], gxi = gx[o], gvi = gv[o];
     const StepParams& p = scene_for(scenes, vi).p;
     const bool clamp = (p.flags & SPH_FLAG_WALL_CLAMP) != 0u;
     const float e = p.restitution;
@@ -157,19 +166,32 @@ __global__ __launch_bounds__(kBlock) void k_adj_integrate(
 constexpr int kAdjForceTile = SPH_ADJ_FORCE_TILE;
 constexpr int kAdjDensityTile = SPH_ADJ_DENSITY_TILE;
 constexpr int kWaves = kBlock / 64;
+constexpr int kAdjGroup = 8;   // candidates loaded together before any test
+constexpr int kAdjQueue = 16;  // per-lane queue of in-range candidates (force adjoint)
+
+// 0 < r2 < h2 as one unsigned compare (r2 >= +0; non-negative floats order
+// like their bit patterns).
+__device__ __forceinline__ bool pair_in_range(float r2, uint32_t h2_bits_m1) {
+    return __float_as_uint(r2) - 1u < h2_bits_m1;
+}
 
 // Force-pair adjoint; see the file comment for the terms.  Candidates are
-// staged in LDS as (x, A), (v, B) and abar.
+// staged in LDS as (x, A), (v, B) and abar.  A lane first queues the tile
+// indices of its in-range candidates (whole groups of candidates are loaded
+// before any test) and evaluates the queued pair terms, in queue order, when
+// some lane's queue is nearly full and at the end of each staged run.
 __global__ __launch_bounds__(kBlock) void k_adj_force(
     const float4* __restrict__ tpos, const float4* __restrict__ tvel,
     const float2* __restrict__ aux, const float* __restrict__ rho,
-    const float4* __restrict__ abar, const uint32_t* __restrict__ cs,
-    const uint32_t* __restrict__ ce, const SceneParams* __restrict__ scenes,
+    const uint32_t* __restrict__ i2r, const float4* __restrict__ abar,
+    const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+    const uint4* __restrict__ bins, const SceneParams* __restrict__ scenes,
     const uint32_t* __restrict__ n_dev, float* __restrict__ grho, float4* __restrict__ gxd,
     float4* __restrict__ gvd, double* __restrict__ kacc, double* __restrict__ macc) {
-    __shared__ float4 tx[kAdjForceTile];
+    __shared__ float4 tx[kAdjForceTile + kAdjGroup];  // padded for whole-group loads
     __shared__ float4 tv[kAdjForceTile];
     __shared__ float4 ta[kAdjForceTile];
+    __shared__ uint16_t queue[kAdjQueue][kBlock];
     __shared__ uint32_t red[2 * kWaves];
     const uint32_t n = *n_dev;
     const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
@@ -184,6 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
     const float4 bi = active ? abar[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float2 ABi = active ? aux[s] : make_float2(0.f, 1.f);
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
+    const uint32_t h2_bits_m1 = __float_as_uint(h2) - 1u;
     float sumA = 0.f, S1 = 0.f, S2 = 0.f;
     float gPx = 0.f, gPy = 0.f, gPz = 0.f;   // sum (A_i + A_j)[F dA + F'/r (x.dA) x]
     float gVx = 0.f, gVy = 0.f, gVz = 0.f;   // sum B_j (dv.dA) x / r
@@ -218,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
         gWy += wB * ay;
         gWz += wB * az;
     };
-    walk_stencil_staged<kBlock, kAdjForceTile>(
+    walk_stencil_staged<kBlock, kAdjForceTile, true>(
         g, pr, cs, ce, n, active, red,
         [&](uint32_t k, uint32_t j) {
             const float4 x = tpos[j], v = tvel[j];
@@ -228,28 +251,49 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
             ta[k] = abar[j];
         },
         [&](uint32_t t0, uint32_t t1, uint32_t) {
-            for (uint32_t t = t0; t < t1; ++t) {
-                const float4 xa = tx[t];
-                const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < h2 && r2 > 0.0f) pair(dx, dy, dz, r2, xa, tv[t], ta[t]);
+            uint32_t cnt = 0;
+            auto flush = [&]() {
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const uint32_t t = queue[k][threadIdx.x];
+                    const float4 xa = tx[t];
+                    const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    pair(dx, dy, dz, r2, xa, tv[t], ta[t]);
+                }
+                cnt = 0;
+            };
+            for (uint32_t t = t0; t < t1; t += kAdjGroup) {
+                float4 q[kAdjGroup];
+#pragma unroll
+                for (int u = 0; u < kAdjGroup; ++u) q[u] = tx[t + u];
+#pragma unroll
+                for (int u = 0; u < kAdjGroup; ++u) {
+                    const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const bool hit = (t + u < t1) & pair_in_range(r2, h2_bits_m1);
+                    queue[cnt][threadIdx.x] = static_cast<uint16_t>(t + u);
+                    cnt += hit ? 1u : 0u;
+                }
+                if (__ballot(cnt > kAdjQueue - kAdjGroup) != 0ull) flush();
             }
+            flush();
         },
         [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
                 const float4 xj = tpos[j];
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                if (!(r2 < h2 && r2 > 0.0f)) continue;
+                if (!pair_in_range(r2, h2_bits_m1)) continue;
                 const float4 vj = tvel[j];
                 const float2 ab = aux[j];
                 pair(dx, dy, dz, r2, make_float4(xj.x, xj.y, xj.z, ab.x),
                      make_float4(vj.x, vj.y, vj.z, ab.y), abar[j]);
             }
-        });
+        },
+        bins);
     if (!active) return;
     const StepParams& p = sp.p;
-    const float Bi = ABi.y, ri = rho[s];
+    const float Bi = ABi.y, ri = rho[min(i2r[s], n - 1)];
     const float gA = p.c45m * sumA, gB = p.visc * (S1 - S2);
     grho[s] = gA * p.stiffness * (2.0f * p.rest_density - ri) * Bi * Bi * Bi - gB * Bi * Bi;
     kacc[s] += static_cast<double>(gA * (ri - p.rest_density) * Bi * Bi);
@@ -271,11 +315,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
 __global__ __launch_bounds__(kBlock) void k_adj_density(
     const float4* __restrict__ tpos, const float4* __restrict__ tvel,
     const float* __restrict__ grho, const uint32_t* __restrict__ cs,
-    const uint32_t* __restrict__ ce, const uint32_t* __restrict__ perm,
+    const uint32_t* __restrict__ ce, const uint4* __restrict__ bins,
+    const uint32_t* __restrict__ i2r, const uint32_t* __restrict__ perm,
     const SceneParams* __restrict__ scenes, const uint32_t* __restrict__ n_dev,
     const float4* __restrict__ gxd, const float4* __restrict__ gvd, float4* __restrict__ gx,
     float4* __restrict__ gv) {
-    __shared__ float4 tile[kAdjDensityTile];
+    __shared__ float4 tile[kAdjDensityTile + kAdjGroup];  // padded for whole-group loads
     __shared__ uint32_t red[2 * kWaves];
     const uint32_t n = *n_dev;
     const uint32_t block0 = xcd_local_block(blockIdx.x, gridDim.x) * kBlock;
@@ -287,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
     const float h2 = sp.p.h2;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float gi = active ? grho[s] : 0.0f;
+    const uint32_t h2_bits_m1 = __float_as_uint(h2) - 1u;
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sx = 0.f, sy = 0.f, sz = 0.f;
     auto pair = [&](float dx, float dy, float dz, float r2, float gj) {
@@ -296,18 +342,23 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
         sy += c * dy;
         sz += c * dz;
     };
-    walk_stencil_staged<kBlock, kAdjDensityTile>(
+    walk_stencil_staged<kBlock, kAdjDensityTile, true>(
         g, pr, cs, ce, n, active, red,
         [&](uint32_t k, uint32_t j) {
             const float4 x = tpos[j];
             tile[k] = make_float4(x.x, x.y, x.z, grho[j]);
         },
         [&](uint32_t t0, uint32_t t1, uint32_t) {
-            for (uint32_t t = t0; t < t1; ++t) {
-                const float4 q = tile[t];
-                const float dx = xi.x - q.x, dy = xi.y - q.y, dz = xi.z - q.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < h2 && r2 > 0.0f) pair(dx, dy, dz, r2, q.w);
+            for (uint32_t t = t0; t < t1; t += kAdjGroup) {
+                float4 q[kAdjGroup];
+#pragma unroll
+                for (int u = 0; u < kAdjGroup; ++u) q[u] = tile[t + u];
+#pragma unroll
+                for (int u = 0; u < kAdjGroup; ++u) {
+                    const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    if ((t + u < t1) & pair_in_range(r2, h2_bits_m1)) pair(dx, dy, dz, r2, q[u].w);
+                }
             }
         },
         [&](uint32_t b, uint32_t e) {
@@ -315,16 +366,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
                 const float4 xj = tpos[j];
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < h2 && r2 > 0.0f) pair(dx, dy, dz, r2, grho[j]);
+                if (pair_in_range(r2, h2_bits_m1)) pair(dx, dy, dz, r2, grho[j]);
             }
-        });
+        },
+        bins);
     if (!active) return;
     const float k6 = -6.0f * sp.p.c6m;
     float4 X = gxd[s];
     X.x += k6 * sx;
     X.y += k6 * sy;
     X.z += k6 * sz;
-    const uint32_t q = min(perm[s], n - 1);
+    const uint32_t q = min(perm[min(i2r[s], n - 1)], n - 1);
     gx[q] = X;
     gv[q] = gvd[s];
 }
@@ -365,10 +417,11 @@ size_t al(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
 // One recomputed step's record inside a segment.
 struct RolloutSlot {
-    float4 *tpos, *tvel, *accel;
-    float* rho;
-    float2* aux;
-    uint32_t *perm, *cell_start, *cell_end;
+    float4 *ipos, *ivel, *accel;  // internal x-order state; accelerations at sorted slots
+    float* rho;                   // sorted slots
+    float2* iaux;                 // internal order
+    uint32_t *perm, *i2r, *cell_start, *cell_end;
+    uint4* xbins;
     hipGraphExec_t graph = nullptr;
 };
 
@@ -405,11 +458,13 @@ int leave(sph_rollout* r, void* stream) {
 
 StepBuffers slot_buffers(const sph_rollout* r, const RolloutSlot& s) {
     StepBuffers sb = r->b->buf;
-    sb.tpos = s.tpos;
-    sb.tvel = s.tvel;
+    sb.ipos = s.ipos;
+    sb.ivel = s.ivel;
+    sb.iaux = s.iaux;
+    sb.i2r = s.i2r;
+    sb.xbins = s.xbins;
     sb.accel = s.accel;
     sb.rho = s.rho;
-    sb.aux = s.aux;
     sb.perm = s.perm;
     sb.cell_start = s.cell_start;
     sb.cell_end = s.cell_end;
@@ -441,15 +496,15 @@ int adjoint_step(sph_rollout* r, const RolloutSlot& s) {
     sph_batch* b = r->b;
     const uint32_t n = b->n;
     hipStream_t st = b->stream;
-    hipLaunchKernelGGL(k_adj_integrate, blocks_for(n), dim3(kBlock), 0, st, s.tpos, s.tvel,
-                       s.accel, r->gx, r->gv, b->scenes, b->buf.n_dev, r->dt, r->abar, r->gxd,
-                       r->gvd);
-    hipLaunchKernelGGL(k_adj_force, blocks_for(n), dim3(kBlock), 0, st, s.tpos, s.tvel, s.aux,
-                       s.rho, r->abar, s.cell_start, s.cell_end, b->scenes, b->buf.n_dev, r->grho,
-                       r->gxd, r->gvd, r->kacc, r->macc);
-    hipLaunchKernelGGL(k_adj_density, blocks_for(n), dim3(kBlock), 0, st, s.tpos, s.tvel, r->grho,
-                       s.cell_start, s.cell_end, s.perm, b->scenes, b->buf.n_dev, r->gxd, r->gvd,
-                       r->gx, r->gv);
+    hipLaunchKernelGGL(k_adj_integrate, blocks_for(n), dim3(kBlock), 0, st, s.ipos, s.ivel,
+                       s.accel, s.i2r, r->gx, r->gv, b->scenes, b->buf.n_dev, r->dt, r->abar,
+                       r->gxd, r->gvd);
+    hipLaunchKernelGGL(k_adj_force, blocks_for(n), dim3(kBlock), 0, st, s.ipos, s.ivel, s.iaux,
+                       s.rho, s.i2r, r->abar, s.cell_start, s.cell_end, s.xbins, b->scenes,
+                       b->buf.n_dev, r->grho, r->gxd, r->gvd, r->kacc, r->macc);
+    hipLaunchKernelGGL(k_adj_density, blocks_for(n), dim3(kBlock), 0, st, s.ipos, s.ivel, r->grho,
+                       s.cell_start, s.cell_end, s.xbins, s.i2r, s.perm, b->scenes, b->buf.n_dev,
+                       r->gxd, r->gvd, r->gx, r->gv);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -496,7 +551,7 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     const size_t n_stored = r->n_ckpt ? r->n_ckpt - 1 : 0;  // the last segment keeps records
     std::vector<size_t> sizes = {al(16 * N * n_stored), al(16 * N * n_stored)};
     for (uint32_t k = 0; k < K; ++k)
-        for (size_t v : {16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 4 * N, 4 * C, 4 * C})
+        for (size_t v : {16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 4 * N, 4 * N, 4 * C, 4 * C, 16 * C})
             sizes.push_back(al(v));
     for (size_t v : {16 * N, 16 * N, 16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 8 * N, 16 * S})
         sizes.push_back(al(v));
@@ -521,14 +576,16 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     r->ckpt_vel = static_cast<float4*>(take());
     r->slot.resize(K);
     for (auto& s : r->slot) {
-        s.tpos = static_cast<float4*>(take());
-        s.tvel = static_cast<float4*>(take());
+        s.ipos = static_cast<float4*>(take());
+        s.ivel = static_cast<float4*>(take());
         s.accel = static_cast<float4*>(take());
         s.rho = static_cast<float*>(take());
-        s.aux = static_cast<float2*>(take());
+        s.iaux = static_cast<float2*>(take());
         s.perm = static_cast<uint32_t*>(take());
+        s.i2r = static_cast<uint32_t*>(take());
         s.cell_start = static_cast<uint32_t*>(take());
         s.cell_end = static_cast<uint32_t*>(take());
+        s.xbins = static_cast<uint4*>(take());
     }
     r->gx = static_cast<float4*>(take());
     r->gv = static_cast<float4*>(take());

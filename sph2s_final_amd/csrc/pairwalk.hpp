@@ -15,21 +15,25 @@
 
 namespace sph {
 
-// stage(k, j): write candidate j (global sorted index) into LDS slot k.
+// stage(k, j): write candidate j (global index) into LDS slot k.
 // tile_run(t0, t1, base): scan LDS slots [t0, t1) (global index base + t).
 // global_run(b, e): scan global indices [b, e).
+// kX: the indices are of the internal x-order (cells sorted by x inside) and
+// rows are trimmed with the cells' x-bin tables `bins` (probe_row_run_x).
 // Every lane of the block must call this (it synchronises the block);
 // inactive lanes pass active = false.
-template <int kBlockThreads, int kTile, class Stage, class TileRun, class GlobalRun>
+template <int kBlockThreads, int kTile, bool kX = false, class Stage, class TileRun,
+          class GlobalRun>
 __device__ __forceinline__ void walk_stencil_staged(const GridView& g, const Probe& pr,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce, uint32_t n,
                                                     bool active, uint32_t* red, Stage&& stage,
-                                                    TileRun&& tile_run, GlobalRun&& global_run) {
+                                                    TileRun&& tile_run, GlobalRun&& global_run,
+                                                    const uint4* __restrict__ bins = nullptr) {
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
             uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row_run(g, pr, cs, ce, n, pr.cy + dy, pr.cz + dz, b, e);
+            if (active) probe_row<kX>(g, pr, cs, ce, bins, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
             block_run_union<kBlockThreads>(b, e, red, ub, ue);
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
@@ -50,7 +54,7 @@ __device__ __forceinline__ void walk_stencil_staged(const GridView& g, const Pro
         for (int ry = pr.sy; ry <= pr.ey; ++ry) {
             if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
             uint32_t b, e;
-            probe_row_run(g, pr, cs, ce, n, ry, rz, b, e);
+            probe_row<kX>(g, pr, cs, ce, bins, n, ry, rz, b, e);
             if (b < e) global_run(b, e);
         }
 }
