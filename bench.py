@@ -91,7 +91,7 @@ def _timed(fn, torch):
     return time.perf_counter() - t0
 
 
-def bench_adjoint(torch, steps: int, every: int) -> dict:
+def bench_adjoint(torch, steps: int, every: int, record_all: bool = True) -> dict:
     """Forward vs forward+adjoint of the differentiable batched rollout."""
     import numpy as np
     from sph2s_final_amd import scenes
@@ -131,6 +131,10 @@ def bench_adjoint(torch, steps: int, every: int) -> dict:
                                  and torch.isfinite(gv).all().item()),
             "device_errors": r.device_errors(),
         })
+    if record_all and every < steps:
+        # The same rollout keeping every step's record (no recomputation):
+        # about 76 MB per step at this size, 15 GB for 200 steps.
+        res["record_every_step"] = bench_adjoint(torch, steps, steps, record_all=False)
     return res
 
 

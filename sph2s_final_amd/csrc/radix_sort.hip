@@ -25,9 +25,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kItems = 8;
+constexpr int kItems = 16;
 constexpr int kWaveKeys = 64 * kItems;
 constexpr int kTile = kBlock * kItems;
+// The histogram kernel reads its keys in smaller tiles (more blocks in flight).
+constexpr int kHistItems = 8;
+constexpr int kHistTile = kBlock * kHistItems;
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 4;
 constexpr uint32_t kFlagAggregate = 1u << 30;
@@ -129,13 +132,13 @@ __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __rest
                                                  int passes, uint32_t key_bits,
                                                  uint32_t* __restrict__ ghist) {
     const uint32_t n = *n_dev;
-    if (blockIdx.x * kTile >= n) return;  // block-uniform: the grid covers the capacity
+    if (blockIdx.x * kHistTile >= n) return;  // block-uniform: the grid covers the capacity
     __shared__ uint32_t cnt[kMaxPasses][kRadix];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t base = blockIdx.x * kTile + wave * kWaveKeys;
-    for (int r = 0; r < kItems; ++r) {
+    const uint32_t base = blockIdx.x * kHistTile + wave * (64 * kHistItems);
+    for (int r = 0; r < kHistItems; ++r) {
         const uint32_t idx = base + r * 64 + lane;
         const bool valid = idx < n;
         const uint32_t key = valid ? src(idx) : 0u;
@@ -322,7 +325,8 @@ int prepare_hist(Src src, uint32_t cap, const uint32_t* n_src, uint32_t n_host, 
     const uint32_t prep_blocks = std::min<uint32_t>((prep_words + kBlock - 1) / kBlock, 2048u);
     hipLaunchKernelGGL(k_prepare, dim3(prep_blocks), dim3(kBlock), 0, stream, L.header, zero_words,
                        fill_a, fill_b, fill_words, n_src, n_host);
-    hipLaunchKernelGGL(k_hist<Src>, dim3(tiles), dim3(kBlock), 0, stream, src,
+    const uint32_t hist_blocks = std::max(1u, (cap + kHistTile - 1) / kHistTile);
+    hipLaunchKernelGGL(k_hist<Src>, dim3(hist_blocks), dim3(kBlock), 0, stream, src,
                        L.header + kNWord, passes, key_bits, L.header);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
