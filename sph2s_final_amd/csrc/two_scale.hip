@@ -58,19 +58,57 @@ __device__ __forceinline__ float child_offset(int k, int axis, float d) {
 }
 
 // ---------------------------------------------------------------------------
+// Occupancy of the low-resolution grid by high-resolution particles, so that a
+// low particle whose neighbourhood holds none skips the walk over the high
+// grid (most low particles lie away from the refinement region).
+__global__ void k_clear_words(uint32_t* __restrict__ w, uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        w[i] = 0u;
+}
+
+__global__ void k_mark_occupancy(const float4* __restrict__ pos, const uint32_t* __restrict__ n_dev,
+                                 GridView g, uint32_t* __restrict__ occ) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const float4 x = pos[i];
+    occ[cell_hash(x.x, x.y, x.z, g.ox, g.oy, g.oz, g.spacing, g.dx, g.dy, g.dz)] = 1u;
+}
+
+// True when a low-grid cell within reach of (x, y, z) -- the cells overlapping
+// the box x +/- 1.001 h0 (a margin far above rounding) -- holds a high particle.
+__device__ __forceinline__ bool high_nearby(const uint32_t* __restrict__ occ, const GridView& ga,
+                                            float x, float y, float z) {
+    const float r = ga.h * 1.001f;
+    const int sx = cell_coord_off(x, -r, ga.ox, ga.spacing, ga.dx);
+    const int ex = cell_coord_off(x, r, ga.ox, ga.spacing, ga.dx);
+    const int sy = cell_coord_off(y, -r, ga.oy, ga.spacing, ga.dy);
+    const int ey = cell_coord_off(y, r, ga.oy, ga.spacing, ga.dy);
+    const int sz = cell_coord_off(z, -r, ga.oz, ga.spacing, ga.dz);
+    const int ez = cell_coord_off(z, r, ga.oz, ga.spacing, ga.dz);
+    uint32_t any = 0u;
+    for (int cz = sz; cz <= ez; ++cz)
+        for (int cy = sy; cy <= ey; ++cy) {
+            const uint32_t row = static_cast<uint32_t>(ga.dx) * (cy + ga.dy * cz);
+            for (int cx = sx; cx <= ex; ++cx) any |= occ[row + cx];
+        }
+    return any != 0u;
+}
+
+// ---------------------------------------------------------------------------
 // complementary density: rho_i += m_B sum_{j in B, r < h0} 1/2 (W_h0 + W_h1),
 // then A_i, B_i for the force passes.
 __global__ __launch_bounds__(kBlock) void k_cross_density(
     const float4* __restrict__ tpos_a, const uint32_t* __restrict__ n_a,
     const float4* __restrict__ tpos_b, const uint32_t* __restrict__ cs_b,
     const uint32_t* __restrict__ ce_b, const uint32_t* __restrict__ n_b, GridView gb,
-    CrossParams c, float* __restrict__ rho_a, float2* __restrict__ aux_a) {
+    CrossParams c, const uint32_t* __restrict__ occ, GridView ga, float* __restrict__ rho_a,
+    float2* __restrict__ aux_a) {
     const uint32_t na = *n_a, nb = *n_b;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= na) return;
     const float4 xi = tpos_a[s];
     float sum = 0.0f;
-    if (nb > 0) {
+    if (nb > 0 && (!occ || high_nearby(occ, ga, xi.x, xi.y, xi.z))) {
         for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
                 const float4 xj = tpos_b[j];
@@ -101,8 +139,9 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
     const uint32_t* __restrict__ n_a, const float4* __restrict__ tpos_b,
     const float4* __restrict__ tvel_b, const float2* __restrict__ aux_b,
     const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
-    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, float dt,
-    float4* __restrict__ pos_a, float4* __restrict__ vel_a, float4* __restrict__ acc_out) {
+    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, const uint32_t* __restrict__ occ,
+    GridView ga, float dt, float4* __restrict__ pos_a, float4* __restrict__ vel_a,
+    float4* __restrict__ acc_out) {
     const uint32_t na = *n_a, nb = *n_b;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= na) return;
@@ -110,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
     const float4 vi = tvel_a[s];
     const float2 ai = aux_a[s];
     float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f;
-    if (nb > 0) {
+    if (nb > 0 && (!occ || high_nearby(occ, ga, xi.x, xi.y, xi.z))) {
         for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
                 const float4 xj = tpos_b[j];
@@ -491,6 +530,7 @@ struct sph_two_scale {
     uint32_t* grp_cnt = nullptr;
     uint32_t* tot = nullptr;     // 6 words
     uint32_t* error_word = nullptr;
+    uint32_t* occ = nullptr;     // per low-grid cell: 1 when it holds a high particle
     hipStream_t stream = nullptr;
     void* arena = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -596,6 +636,14 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, dt, S.key_bits, false, st);
         if (rc != SPH_OK) return rc;
     }
+    // Occupancy of the low grid by the (sorted) high particles.
+    {
+        const uint32_t C0 = t->set[0].num_cells;
+        hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 1024u)),
+                           dim3(kBlock), 0, st, t->occ, C0);
+        hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[1].cap), dim3(kBlock), 0, st,
+                           t->set[1].buf.tpos, t->set[1].buf.n_dev, G[0], t->occ);
+    }
     // Own-set densities, then the complementary sums (which also finalise A, B).
     for (int r = 1; r >= 0; --r) {
         int rc = launch_density_own(P[r], t->set[r].buf, st);
@@ -604,9 +652,11 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
+        // Low particles (r = 0) skip the walk when no high particle is near.
+        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
         hipLaunchKernelGGL(k_cross_density, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
                            A.buf.n_dev, B.buf.tpos, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                           X[1 - r], cross_params(t, r), A.buf.rho, A.buf.aux);
+                           X[1 - r], cross_params(t, r), occ, X[0], A.buf.rho, A.buf.aux);
     }
     // Own-set forces (no integration), then complementary forces + integration.
     for (int r = 1; r >= 0; --r) {
@@ -618,17 +668,18 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         TwoScaleSet& B = t->set[1 - r];
         const CrossParams c = cross_params(t, r);
         float4* acc = accel_out ? A.buf.accel : nullptr;
+        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
         if (c.tension != 0.0f)
             hipLaunchKernelGGL(k_cross_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0, st,
                                A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev, B.buf.tpos,
                                B.buf.tvel, B.buf.aux, B.buf.cell_start, B.buf.cell_end,
-                               B.buf.n_dev, X[1 - r], c, dt, A.buf.pos, A.buf.vel, acc);
+                               B.buf.n_dev, X[1 - r], c, occ, X[0], dt, A.buf.pos, A.buf.vel, acc);
         else
             hipLaunchKernelGGL(k_cross_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
                                st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev,
                                B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
-                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, dt, A.buf.pos, A.buf.vel,
-                               acc);
+                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, occ, X[0], dt, A.buf.pos,
+                               A.buf.vel, acc);
     }
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
@@ -735,6 +786,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
     add(4 * static_cast<size_t>(t->num_groups));    // grp_cnt
     add(64);                                        // tot[8] + error word
+    add(4 * static_cast<size_t>(t->set[0].num_cells));  // occupancy of the low grid
 
     hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -782,6 +834,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     t->grp_cnt = static_cast<uint32_t*>(take());
     t->tot = static_cast<uint32_t*>(take());
     t->error_word = t->tot + 8;
+    t->occ = static_cast<uint32_t*>(take());
 
     e = hipMemsetAsync(t->arena, 0, total, t->stream);
     if (e != hipSuccess) {
