@@ -160,6 +160,8 @@ int sph_get_views(sph_solver* s, sph_views* out);
 #define SPH_FIELD_CELL_END      7
 #define SPH_FIELD_ACCEL         8
 #define SPH_FIELD_TAG           9   /* two-scale high set: (slot << 29) | group */
+#define SPH_FIELD_BLEND        10   /* two-scale, blended coupling: lambda (float) */
+#define SPH_FIELD_STATE        11   /* two-scale, blended coupling: state word (u32) */
 int sph_download(sph_solver* s, int field, void* host_dst, size_t bytes);
 
 /* Renderer contract: positions as 3 floats per particle
@@ -244,10 +246,73 @@ int sph2_set_debug(sph_two_scale* t, uint32_t flags);
 int sph2_counts(sph_two_scale* t, uint32_t counts[4]);
 int sph2_device_errors(sph_two_scale* t, uint32_t* bits);
 /* Host copy of one field of resolution 0 (low) or 1 (high).  POSITION,
- * VELOCITY, ID, TAG: current (post split/merge) state; DENSITY, ACCEL,
- * SORTED_*: the last step's sorted order; CELL_*: the last step's grid. */
+ * VELOCITY, ID, TAG, BLEND, STATE: current (post split/merge) state; DENSITY,
+ * ACCEL, SORTED_*: the last step's sorted order; CELL_*: the last step's grid.
+ * BLEND and STATE exist under SPH2_COUPLING_BLENDED only. */
 int sph2_download(sph_two_scale* t, int resolution, int field, void* host_dst,
                   size_t bytes);
+
+/* Coupling options of the two-scale solver.
+ *
+ * coupling SPH2_COUPLING_FIXED (the default): blend weight 1 everywhere,
+ * instantaneous split (parent removed) and merge (children removed), as
+ * described above.
+ *
+ * coupling SPH2_COUPLING_BLENDED: the reference's own two-scale coupling.
+ * Every particle carries a blend weight lambda (SPH2S_FINAL_TWO_SCALE/
+ * Solver.h:89-91) and a state word (state & 3: 0 settled, 1 fading out,
+ * 2 fading in, 3 split requested; state & 4: high set).  Every neighbour term
+ * of the own-set and complementary density and force sums is weighted by the
+ * neighbour's lambda (Solver.cu:215-499); after the integration lambda moves
+ * by -/+ blend_increment in states 1/2, a fading-out particle is removed when
+ * lambda <= 0 and a fading-in one settles (state & 4, lambda = 1) when
+ * lambda >= 1 (integrateD, Solver.cu:1240-1289).  A settled low particle
+ * inside the region, or one in state 3 (sph2_mark_split), splits into 8
+ * children with lambda 0 in state 6 and fades out (state 1) instead of being
+ * removed.  Merging is the reverse: a group of 8 settled siblings outside the
+ * region expanded by h0/2 emits a low particle at their centroid with their
+ * mean velocity, lambda 0, state 2, and the siblings fade out.  (The
+ * reference has no merge.)  A split waits while a child of the same group is
+ * alive, a merge while a low particle with the group's id is alive, so ids
+ * stay unique.
+ *
+ * split_offsets SPH2_SPLIT_LATTICE: children at x +/- h1/4 per axis;
+ * SPH2_SPLIT_REFERENCE: at x + r d_k with d_k the 8 diagonals (+/-1, +/-1,
+ * +/-1)/sqrt(3) and r = 1/2 (3 m0 / (4 pi rho))^(1/3), rho the parent's
+ * density of the step (insertHighResParticles, Solver.cu:541-586; the
+ * standalone sph2_apply_split_merge uses the rest density).  Child k takes the
+ * sign + on axis d when bit d of k is set.
+ * merge: 0 = never merge (the reference), 1 = merge as above.
+ * velocity_cap c > 0: after v += dt a, |v| is limited to c h_r / dt with h_r
+ * the particle's own radius (Solver.cu:1212-1223, c = 0.8 there; |v| = 0 is
+ * left alone rather than divided by); 0 disables the cap.
+ * blend_increment: lambda's change per step (SolverConfiguration::
+ * BlendIncrement, Solver.h:27,48,60; 0.0125 in SPH2S_FINAL_TWO_SCALE/
+ * main.cpp:228-240), in (0, 1].
+ * Switching to the blended coupling sets every particle to lambda = 1,
+ * settled. */
+#define SPH2_COUPLING_FIXED    0u
+#define SPH2_COUPLING_BLENDED  1u
+#define SPH2_SPLIT_LATTICE     0u
+#define SPH2_SPLIT_REFERENCE   1u
+typedef struct sph2_options {
+    uint32_t coupling;         /* SPH2_COUPLING_*                          */
+    uint32_t split_offsets;    /* SPH2_SPLIT_*                             */
+    uint32_t merge;            /* 0 or 1                                   */
+    float    blend_increment;  /* lambda change per step (blended)         */
+    float    velocity_cap;     /* c of |v| <= c h / dt; 0 = off            */
+} sph2_options;
+int sph2_set_options(sph_two_scale* t, const sph2_options* options);
+int sph2_get_options(const sph_two_scale* t, sph2_options* options);
+/* Replace one set's blend weights and state words (blended coupling; n must
+ * equal the set's current count; either array may be NULL to keep it). */
+int sph2_set_blend(sph_two_scale* t, int resolution, const float* lambda,
+                   const uint32_t* state, uint32_t n);
+/* Request the split of the listed low-resolution particles (by id): settled
+ * ones get state 3 and split at the end of the next step.  The reference sets
+ * this state from its screen-space surface pass (SSFRenderer.cu:196-218 and
+ * the mark in Solver.cu:1077-1082).  Blended coupling only; n <= max_low. */
+int sph2_mark_split(sph_two_scale* t, const uint32_t* ids, uint32_t n);
 
 /* ---- batched independent scenes --------------------------------------------
  * The reference runs one scene per process through a global __constant__

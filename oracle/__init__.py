@@ -32,6 +32,11 @@ class OracleParams(Structure):
     ]
 
 
+class OracleTsOptions(Structure):
+    _fields_ = [("coupling", c_uint32), ("split_offsets", c_uint32), ("merge", c_uint32),
+                ("blend_increment", c_float), ("velocity_cap", c_float)]
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
@@ -241,7 +246,9 @@ class OracleTwoScale:
 
     FIELDS = {"pos": (0, np.float64, 3), "vel": (1, np.float64, 3), "id": (2, np.uint32, 1),
               "tag": (3, np.uint32, 1), "rho": (4, np.float64, 1), "acc": (5, np.float64, 3),
-              "hash": (6, np.uint32, 1), "perm": (7, np.uint32, 1)}
+              "hash": (6, np.uint32, 1), "perm": (7, np.uint32, 1),
+              "blend": (10, np.float32, 1), "state": (11, np.uint32, 1)}
+    CURRENT = ("pos", "vel", "id", "tag", "blend", "state")
 
     def __init__(self, p_low: OracleParams, grid_high, region_lo, region_hi, max_low: int):
         L = lib()
@@ -256,6 +263,9 @@ class OracleTwoScale:
             L.oracle_ts_split_merge.argtypes = [vp]
             L.oracle_ts_counts.argtypes = [vp, vp]
             L.oracle_ts_get.argtypes = [vp, c_int, c_int, vp]
+            L.oracle_ts_set_options.argtypes = [vp, POINTER(OracleTsOptions)]
+            L.oracle_ts_set_blend.argtypes = [vp, c_int, vp, vp]
+            L.oracle_ts_mark_split.argtypes = [vp, vp, c_uint32]
             L._ts_bound = True
         g = OracleGrid()
         for d in range(3):
@@ -285,6 +295,22 @@ class OracleTwoScale:
                                   arr(vel_h, np.float32), arr(ids_h, np.uint32),
                                   arr(tags_h, np.uint32), nh)
 
+    def set_options(self, coupling=0, split_offsets=0, merge=1, blend_increment=0.0125,
+                    velocity_cap=0.0):
+        o = OracleTsOptions(int(coupling), int(split_offsets), int(merge), float(blend_increment),
+                            float(velocity_cap))
+        lib().oracle_ts_set_options(self._h, ctypes.byref(o))
+
+    def set_blend(self, res: int, lam=None, state=None):
+        la = None if lam is None else np.ascontiguousarray(lam, np.float32)
+        st = None if state is None else np.ascontiguousarray(state, np.uint32)
+        lib().oracle_ts_set_blend(self._h, int(res), None if la is None else _p(la),
+                                  None if st is None else _p(st))
+
+    def mark_split(self, ids):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        lib().oracle_ts_mark_split(self._h, _p(ids) if len(ids) else None, len(ids))
+
     def step(self, dt: float) -> None:
         lib().oracle_ts_step(self._h, float(dt))
 
@@ -303,7 +329,7 @@ class OracleTwoScale:
             lib().oracle_ts_get(self._h, res, 8 if name == "start" else 9, _p(out))
             return out
         fid, dt, k = self.FIELDS[name]
-        n = c[res] if name in ("pos", "vel", "id", "tag") else c[2 + res]
+        n = c[res] if name in self.CURRENT else c[2 + res]
         out = np.empty((n, k) if k > 1 else n, dt)
         lib().oracle_ts_get(self._h, res, fid, _p(out))
         return out

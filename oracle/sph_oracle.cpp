@@ -538,11 +538,26 @@ void cpu_solver_destroy(cpu_solver* s) { delete s; }
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Two-scale step (FP64).  Restates TWO_SCALE/Solver.cu:1568-1614 with the
-// coupling and split/merge rules of the HIP implementation: complementary
-// averaged-kernel sums (TWO_SCALE/Solver.cu:252-290, 394-499) with blend weight
-// 1, split of low particles inside the region into 8 children at +/- h1/4,
-// merge of sibling groups that are all outside the region expanded by h0/2.
+// Two-scale step (FP64).  Restates TWO_SCALE/Solver.cu:1568-1614 under two
+// couplings.  Fixed (coupling 0, the HIP default): complementary
+// averaged-kernel sums (TWO_SCALE/Solver.cu:252-290, 394-499) with blend
+// weight 1, split of low particles inside the region into 8 children, merge of
+// sibling groups that are all outside the region expanded by h0/2.  Blended
+// (coupling 1): every neighbour term of both the own-set and the complementary
+// sums is weighted by the neighbour's blend weight lambda_j
+// (TWO_SCALE/Solver.cu:215-499), and integrateD's state machine runs after
+// the integration (Solver.cu:1240-1289): state & 3 is the mode (0 settled,
+// 1 fading out, 2 fading in, 3 split requested), state & 4 marks the high set;
+// lambda moves by -inc / +inc in modes 1 / 2 (FP32 arithmetic, as the
+// reference keeps lambda in float); a particle in mode 1 with lambda <= 0 is
+// removed; one in mode 2 reaching lambda >= 1 settles (mode 0, lambda = 1).
+// A low particle in mode 3, or settled and inside the region, splits: 8
+// children with lambda 0 in mode 2 (state 6), and the parent fades out
+// (state 1) instead of being removed.  The faded merge is the reverse: a group
+// of 8 settled siblings outside the expanded region emits a low particle at
+// their centroid with lambda 0 in mode 2, and the siblings fade out.  A split
+// waits while a live child of the same group exists, and a merge while a live
+// low particle with the group's id exists, so ids stay unique.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -550,6 +565,8 @@ struct TsSet {
     oracle_params p;
     std::vector<double> x, v;            // state, AoS
     std::vector<uint32_t> id, tag;
+    std::vector<float> lam;              // blend weight lambda
+    std::vector<uint32_t> state;         // state word
     // last step, sorted order
     std::vector<double> rho, acc;
     Neighbourhood nb;
@@ -578,13 +595,45 @@ void sort_set(TsSet& S) {
     st.id = S.id;
     const uint32_t n = static_cast<uint32_t>(S.id.size());
     build_neighbourhood(S.p, st, S.nb);
-    std::vector<uint32_t> tag(n);
-    for (uint32_t i = 0; i < n; ++i) tag[i] = S.tag[S.nb.perm[i]];
+    std::vector<uint32_t> tag(n), state(n);
+    std::vector<float> lam(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t j = S.nb.perm[i];
+        tag[i] = S.tag[j];
+        lam[i] = S.lam[j];
+        state[i] = S.state[j];
+    }
     S.x = std::move(st.x);
     S.v = std::move(st.v);
     S.id = std::move(st.id);
     S.tag = std::move(tag);
+    S.lam = std::move(lam);
+    S.state = std::move(state);
     S.n_step = n;
+}
+
+// v += dt a, the velocity cap |v| <= vmax (vmax > 0; Solver.cu:1212-1223,
+// without the reference's 0/0 at |v| = 0), x += dt v, then the wall clamp.
+void integrate_ts(const oracle_params& box, TsSet& A, double dt, float vmax) {
+    const size_t n = A.id.size();
+    for (size_t i = 0; i < n; ++i) {
+        double v[3], x[3];
+        for (int d = 0; d < 3; ++d) v[d] = A.v[3 * i + d] + dt * A.acc[3 * i + d];
+        if (vmax > 0.0f) {
+            const double nv = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            if (nv > vmax)
+                for (int d = 0; d < 3; ++d) v[d] *= static_cast<double>(vmax) / nv;
+        }
+        for (int d = 0; d < 3; ++d) {
+            x[d] = A.x[3 * i + d] + dt * v[d];
+            if (box.wall_clamp) {
+                if (x[d] < box.lo[d]) { x[d] = box.lo[d]; v[d] = -box.restitution * v[d]; }
+                if (x[d] > box.hi[d]) { x[d] = box.hi[d]; v[d] = -box.restitution * v[d]; }
+            }
+            A.v[3 * i + d] = v[d];
+            A.x[3 * i + d] = x[d];
+        }
+    }
 }
 
 }  // namespace
@@ -593,67 +642,21 @@ struct oracle_ts {
     TsSet set[2];
     float rlo[3], rhi[3], margin_f, child_d_f;
     uint32_t max_low = 0;
+    oracle_ts_options opt{0u, 0u, 1u, 0.0125f, 0.0f};
+    std::vector<double> split_rho;   // low densities the split offsets use (sorted order)
 };
 
-extern "C" {
+namespace {
 
-oracle_ts* oracle_ts_create(const oracle_params* p_low, const oracle_grid* grid_high,
-                            const float region_lo[3], const float region_hi[3],
-                            uint32_t max_low) {
-    auto* t = new oracle_ts;
-    t->set[0].p = *p_low;
-    t->set[1].p = *p_low;
-    oracle_params& ph = t->set[1].p;
-    ph.h_f32 = 0.5f * p_low->h_f32;
-    ph.h = static_cast<double>(ph.h_f32);
-    ph.mass = static_cast<double>(static_cast<float>(p_low->mass) / 8.0f);
-    ph.grid = *grid_high;
-    for (int d = 0; d < 3; ++d) {
-        t->rlo[d] = region_lo[d];
-        t->rhi[d] = region_hi[d];
-    }
-    t->margin_f = 0.5f * p_low->h_f32;
-    t->child_d_f = 0.25f * ph.h_f32;
-    t->max_low = max_low;
-    return t;
-}
+inline bool ts_blended(const oracle_ts* t) { return t->opt.coupling == 1u; }
 
-void oracle_ts_destroy(oracle_ts* t) { delete t; }
-
-void oracle_ts_set_state(oracle_ts* t, const float* pos_l, const float* vel_l,
-                         const uint32_t* ids_l, uint32_t n_l, const float* pos_h,
-                         const float* vel_h, const uint32_t* ids_h, const uint32_t* tags_h,
-                         uint32_t n_h) {
-    const float* P[2] = {pos_l, pos_h};
-    const float* V[2] = {vel_l, vel_h};
-    const uint32_t* I[2] = {ids_l, ids_h};
-    const uint32_t N[2] = {n_l, n_h};
-    for (int r = 0; r < 2; ++r) {
-        TsSet& S = t->set[r];
-        const uint32_t n = N[r];
-        S.x.assign(3 * static_cast<size_t>(n), 0.0);
-        S.v.assign(3 * static_cast<size_t>(n), 0.0);
-        S.id.resize(n);
-        S.tag.assign(n, 0u);
-        for (size_t i = 0; i < 3 * static_cast<size_t>(n); ++i) {
-            S.x[i] = P[r][i];
-            if (V[r]) S.v[i] = V[r][i];
-        }
-        for (uint32_t i = 0; i < n; ++i) {
-            S.id[i] = I[r] ? I[r][i] : i;
-            if (r == 1) S.tag[i] = tags_h[i];
-        }
-    }
-}
-
-void oracle_ts_step(oracle_ts* t, double dt) {
+// Fixed coupling: own-set density and acceleration of the single-scale step,
+// plus the complementary sums with blend weight 1.
+void ts_forces_fixed(oracle_ts* t) {
     TsSet& L = t->set[0];
     TsSet& H = t->set[1];
-    sort_set(H);
-    sort_set(L);
     const double h0 = L.p.h, h1 = H.p.h;
     const Coeffs k0(h0), k1(h1);
-    // own-set densities
     for (int r = 0; r < 2; ++r) {
         TsSet& S = t->set[r];
         State st;
@@ -693,7 +696,6 @@ void oracle_ts_step(oracle_ts* t, double dt) {
     }
     L.rho = rho_new[0];
     H.rho = rho_new[1];
-    // forces: own set (with gravity) + complementary, then integrate
     for (int r = 0; r < 2; ++r) {
         TsSet& A = t->set[r];
         TsSet& B = t->set[1 - r];
@@ -738,23 +740,245 @@ void oracle_ts_step(oracle_ts* t, double dt) {
             for (int d = 0; d < 3; ++d) A.acc[3 * i + d] += f[d];
         }
     }
+}
+
+// Blended coupling: computeDensitiesPressuresD and computeAccelerationsD of
+// TWO_SCALE/Solver.cu:732-1048 (without the boundary particles, whose role
+// the wall clamp takes): every term weighted by the neighbour's lambda.
+void ts_forces_blended(oracle_ts* t) {
+    TsSet& L = t->set[0];
+    TsSet& H = t->set[1];
+    const double h0 = L.p.h, h1 = H.p.h;
+    const Coeffs k0(h0), k1(h1);
+    // densities: m_A sum_A lambda_j W_hA + m_B sum_{B, r < h0} lambda_j 1/2 (W_h0 + W_h1)
     for (int r = 0; r < 2; ++r) {
         TsSet& A = t->set[r];
-        State st;
-        st.x = A.x;
-        st.v = A.v;
-        st.id = A.id;
-        integrate(L.p, st, A.acc, dt);  // clamp box and restitution of the low configuration
-        A.x = std::move(st.x);
-        A.v = std::move(st.v);
+        TsSet& B = t->set[1 - r];
+        const Coeffs ka(A.p.h);
+        const double ha = A.p.h;
+        const uint32_t na = static_cast<uint32_t>(A.id.size());
+        A.rho.assign(na, 0.0);
+        for (uint32_t i = 0; i < na; ++i) {
+            const double* xi = &A.x[3 * i];
+            double own = 0.0, cross = 0.0;
+            for_each_stencil_cell(A.p, xi, [&](uint32_t c) {
+                const uint32_t b = A.nb.start[c], e = A.nb.end[c];
+                if (b == kEmpty) return;
+                for (uint32_t j = b; j < e; ++j) {
+                    const double dx = xi[0] - A.x[3 * j], dy = xi[1] - A.x[3 * j + 1],
+                                 dz = xi[2] - A.x[3 * j + 2];
+                    const double rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    if (rr < ha) {
+                        const double a = ha * ha - rr * rr;
+                        own += static_cast<double>(A.lam[j]) * ka.c6 * a * a * a;
+                    }
+                }
+            });
+            if (!B.id.empty())
+                for_each_cell_in(B.p.grid, L.p.h_f32, xi, [&](uint32_t c) {
+                    const uint32_t b = B.nb.start[c], e = B.nb.end[c];
+                    if (b == kEmpty) return;
+                    for (uint32_t j = b; j < e; ++j) {
+                        const double dx = xi[0] - B.x[3 * j], dy = xi[1] - B.x[3 * j + 1],
+                                     dz = xi[2] - B.x[3 * j + 2];
+                        const double rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+                        if (rr < h0) {
+                            const double a0 = h0 * h0 - rr * rr, a1 = h1 * h1 - rr * rr;
+                            const double w0 = k0.c6 * a0 * a0 * a0;
+                            const double w1 = (rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0;
+                            cross += static_cast<double>(B.lam[j]) * 0.5 * (w0 + w1);
+                        }
+                    }
+                });
+            A.rho[i] = A.p.mass * own + B.p.mass * cross;
+        }
     }
+    // accelerations: f_i / rho_i + g, f of computeAccelerationCell(Compl)
+    for (int r = 0; r < 2; ++r) {
+        TsSet& A = t->set[r];
+        TsSet& B = t->set[1 - r];
+        const Coeffs ka(A.p.h);
+        const uint32_t na = static_cast<uint32_t>(A.id.size());
+        A.acc.assign(3 * static_cast<size_t>(na), 0.0);
+        for (uint32_t i = 0; i < na; ++i) {
+            const double* xi = &A.x[3 * i];
+            const double* vi = &A.v[3 * i];
+            const double rhoi = A.rho[i];
+            const double pi = A.p.stiffness * (rhoi - A.p.rest_density);
+            double f[3] = {0.0, 0.0, 0.0};
+            for_each_stencil_cell(A.p, xi, [&](uint32_t c) {
+                const uint32_t b = A.nb.start[c], e = A.nb.end[c];
+                if (b == kEmpty) return;
+                for (uint32_t j = b; j < e; ++j) {
+                    double fj[3] = {0.0, 0.0, 0.0};
+                    pair_force(A.p, ka, xi, vi, rhoi, pi, &A.x[3 * j], &A.v[3 * j], A.rho[j], fj);
+                    for (int d = 0; d < 3; ++d) f[d] += static_cast<double>(A.lam[j]) * fj[d];
+                }
+            });
+            if (!B.id.empty())
+                for_each_cell_in(B.p.grid, L.p.h_f32, xi, [&](uint32_t c) {
+                    const uint32_t b = B.nb.start[c], e = B.nb.end[c];
+                    if (b == kEmpty) return;
+                    for (uint32_t j = b; j < e; ++j) {
+                        const double xij[3] = {xi[0] - B.x[3 * j], xi[1] - B.x[3 * j + 1],
+                                               xi[2] - B.x[3 * j + 2]};
+                        const double rr = std::sqrt(xij[0] * xij[0] + xij[1] * xij[1] +
+                                                    xij[2] * xij[2]);
+                        if (!(rr != 0.0 && rr < h0)) continue;
+                        const double lj = static_cast<double>(B.lam[j]);
+                        const double rhoj = B.rho[j];
+                        const double pj = A.p.stiffness * (rhoj - A.p.rest_density);
+                        // Solver.cu:394-499: averaged spiky gradient, Laplacian and Poly6.
+                        const double coeffP =
+                            -rhoi * B.p.mass * (pi / (rhoi * rhoi) + pj / (rhoj * rhoj));
+                        const double gmag = -0.5 * (k0.c45 * (h0 - rr) * (h0 - rr) +
+                                                    ((rr < h1) ? k1.c45 * (h1 - rr) * (h1 - rr)
+                                                               : 0.0));
+                        const double coeffV = A.p.viscosity * B.p.mass / rhoj;
+                        const double lap = 0.5 * (k0.c45 * (h0 - rr) +
+                                                  ((rr < h1) ? k1.c45 * (h1 - rr) : 0.0));
+                        const double a0 = h0 * h0 - rr * rr, a1 = h1 * h1 - rr * rr;
+                        const double w = 0.5 * (k0.c6 * a0 * a0 * a0 +
+                                                ((rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0));
+                        const double coeffT = -w * B.p.mass * A.p.tension;
+                        for (int d = 0; d < 3; ++d) {
+                            f[d] += lj * coeffP * gmag * xij[d] / rr;
+                            f[d] += lj * coeffV * (B.v[3 * j + d] - vi[d]) * lap;
+                            f[d] += lj * coeffT * xij[d];
+                        }
+                    }
+                });
+            for (int d = 0; d < 3; ++d) A.acc[3 * i + d] = f[d] / rhoi + A.p.gravity[d];
+        }
+    }
+}
+
+// integrateD's blend update (Solver.cu:1240-1262) in FP32.
+void blend_update(TsSet& A, float inc) {
+    for (size_t i = 0; i < A.id.size(); ++i) {
+        const uint32_t mode = A.state[i] & 3u;
+        const float c = (mode == 1u) ? -1.0f : (mode == 2u) ? 1.0f : 0.0f;
+        float lam = A.lam[i] + c * inc;
+        if (mode == 2u && lam >= 1.0f) {
+            A.state[i] &= 4u;
+            lam = 1.0f;
+        }
+        A.lam[i] = lam;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+oracle_ts* oracle_ts_create(const oracle_params* p_low, const oracle_grid* grid_high,
+                            const float region_lo[3], const float region_hi[3],
+                            uint32_t max_low) {
+    auto* t = new oracle_ts;
+    t->set[0].p = *p_low;
+    t->set[1].p = *p_low;
+    oracle_params& ph = t->set[1].p;
+    ph.h_f32 = 0.5f * p_low->h_f32;
+    ph.h = static_cast<double>(ph.h_f32);
+    ph.mass = static_cast<double>(static_cast<float>(p_low->mass) / 8.0f);
+    ph.grid = *grid_high;
+    for (int d = 0; d < 3; ++d) {
+        t->rlo[d] = region_lo[d];
+        t->rhi[d] = region_hi[d];
+    }
+    t->margin_f = 0.5f * p_low->h_f32;
+    t->child_d_f = 0.25f * ph.h_f32;
+    t->max_low = max_low;
+    return t;
+}
+
+void oracle_ts_destroy(oracle_ts* t) { delete t; }
+
+void oracle_ts_set_state(oracle_ts* t, const float* pos_l, const float* vel_l,
+                         const uint32_t* ids_l, uint32_t n_l, const float* pos_h,
+                         const float* vel_h, const uint32_t* ids_h, const uint32_t* tags_h,
+                         uint32_t n_h) {
+    const float* P[2] = {pos_l, pos_h};
+    const float* V[2] = {vel_l, vel_h};
+    const uint32_t* I[2] = {ids_l, ids_h};
+    const uint32_t N[2] = {n_l, n_h};
+    for (int r = 0; r < 2; ++r) {
+        TsSet& S = t->set[r];
+        const uint32_t n = N[r];
+        S.x.assign(3 * static_cast<size_t>(n), 0.0);
+        S.v.assign(3 * static_cast<size_t>(n), 0.0);
+        S.id.resize(n);
+        S.tag.assign(n, 0u);
+        S.lam.assign(n, 1.0f);
+        S.state.assign(n, r == 0 ? 0u : 4u);
+        for (size_t i = 0; i < 3 * static_cast<size_t>(n); ++i) {
+            S.x[i] = P[r][i];
+            if (V[r]) S.v[i] = V[r][i];
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            S.id[i] = I[r] ? I[r][i] : i;
+            if (r == 1) S.tag[i] = tags_h[i];
+        }
+    }
+}
+
+void oracle_ts_set_options(oracle_ts* t, const oracle_ts_options* o) {
+    const bool was = ts_blended(t);
+    t->opt = *o;
+    if (ts_blended(t) && !was)
+        for (int r = 0; r < 2; ++r) {
+            TsSet& S = t->set[r];
+            S.lam.assign(S.id.size(), 1.0f);
+            S.state.assign(S.id.size(), r == 0 ? 0u : 4u);
+        }
+}
+
+void oracle_ts_set_blend(oracle_ts* t, int res, const float* lambda, const uint32_t* state) {
+    TsSet& S = t->set[res];
+    for (size_t i = 0; i < S.id.size(); ++i) {
+        if (lambda) S.lam[i] = lambda[i];
+        if (state) S.state[i] = state[i];
+    }
+}
+
+void oracle_ts_mark_split(oracle_ts* t, const uint32_t* ids, uint32_t n) {
+    TsSet& L = t->set[0];
+    std::vector<uint32_t> want(ids, ids + n);
+    std::sort(want.begin(), want.end());
+    for (size_t i = 0; i < L.id.size(); ++i)
+        if (L.state[i] == 0u && std::binary_search(want.begin(), want.end(), L.id[i]))
+            L.state[i] = 3u;
+}
+
+void oracle_ts_step(oracle_ts* t, double dt) {
+    TsSet& L = t->set[0];
+    TsSet& H = t->set[1];
+    sort_set(H);
+    sort_set(L);
+    if (ts_blended(t))
+        ts_forces_blended(t);
+    else
+        ts_forces_fixed(t);
+    for (int r = 0; r < 2; ++r) {
+        TsSet& A = t->set[r];
+        // 0.8f * h / dt of Solver.cu:1214-1217, rounded as the FP32 step does.
+        const float vmax = (t->opt.velocity_cap > 0.0f)
+                               ? t->opt.velocity_cap * A.p.h_f32 / static_cast<float>(dt)
+                               : 0.0f;
+        integrate_ts(L.p, A, dt, vmax);  // clamp box and restitution of the low configuration
+        if (ts_blended(t)) blend_update(A, t->opt.blend_increment);
+    }
+    t->split_rho = L.rho;
     oracle_ts_split_merge(t);
 }
 
 void oracle_ts_split_merge(oracle_ts* t) {
     TsSet& L = t->set[0];
     TsSet& H = t->set[1];
+    const bool blended = ts_blended(t);
     const uint32_t nl = static_cast<uint32_t>(L.id.size()), nh = static_cast<uint32_t>(H.id.size());
+    // The split offsets of a standalone pass (no step before it) use rest density.
+    if (t->split_rho.size() != nl) t->split_rho.assign(nl, L.p.rest_density);
     const float m = t->margin_f;
     const float olo[3] = {t->rlo[0] - m, t->rlo[1] - m, t->rlo[2] - m};
     const float ohi[3] = {t->rhi[0] + m, t->rhi[1] + m, t->rhi[2] + m};
@@ -768,14 +992,21 @@ void oracle_ts_split_merge(oracle_ts* t) {
             if (x[d] < olo[d] || x[d] > ohi[d]) return true;
         return false;
     };
+    auto removed = [&](const TsSet& S, uint32_t i) {
+        return blended && (S.state[i] & 3u) == 1u && S.lam[i] <= 0.0f;
+    };
     constexpr uint32_t kMask = (1u << 29) - 1u;
-    // groups whose 8 children are all outside the expanded region
-    std::vector<uint32_t> cnt(t->max_low, 0u);
+    // Per group: settled children outside the expanded region, and whether a
+    // live child (bit 0) or a live low particle with the group's id (bit 1) exists.
+    std::vector<uint32_t> cnt(t->max_low, 0u), live(t->max_low, 0u);
     std::vector<double> gx(static_cast<size_t>(t->max_low) * 24, 0.0),
         gv(static_cast<size_t>(t->max_low) * 24, 0.0);
     for (uint32_t i = 0; i < nh; ++i) {
         const uint32_t g = H.tag[i] & kMask, k = H.tag[i] >> 29;
-        if (g >= t->max_low || !outside_margin(&H.x[3 * i])) continue;
+        if (g >= t->max_low || removed(H, i)) continue;
+        live[g] |= 1u;
+        if (!t->opt.merge || !outside_margin(&H.x[3 * i])) continue;
+        if (blended && (H.state[i] & 3u) != 0u) continue;
         ++cnt[g];
         for (int d = 0; d < 3; ++d) {
             gx[(static_cast<size_t>(g) * 8 + k) * 3 + d] = H.x[3 * i + d];
@@ -783,29 +1014,38 @@ void oracle_ts_split_merge(oracle_ts* t) {
         }
     }
     TsSet nL, nH;
-    auto push = [](TsSet& S, const double* x, const double* v, uint32_t id, uint32_t tag) {
+    auto push = [](TsSet& S, const double* x, const double* v, uint32_t id, uint32_t tag,
+                   float lam, uint32_t state) {
         for (int d = 0; d < 3; ++d) {
             S.x.push_back(x[d]);
             S.v.push_back(v[d]);
         }
         S.id.push_back(id);
         S.tag.push_back(tag);
+        S.lam.push_back(lam);
+        S.state.push_back(state);
     };
     std::vector<uint32_t> split;
     for (uint32_t i = 0; i < nl; ++i) {
-        if (inside(&L.x[3 * i]))
-            split.push_back(i);
-        else
-            push(nL, &L.x[3 * i], &L.v[3 * i], L.id[i], 0u);
+        if (removed(L, i)) continue;
+        live[L.id[i]] |= 2u;
+        const uint32_t mode = L.state[i] & 3u;
+        const bool s = blended ? ((mode == 3u || (mode == 0u && inside(&L.x[3 * i]))) &&
+                                  !(live[L.id[i]] & 1u))
+                               : inside(&L.x[3 * i]);
+        if (s) split.push_back(i);
+        if (!s || blended)
+            push(nL, &L.x[3 * i], &L.v[3 * i], L.id[i], 0u, L.lam[i], s ? 1u : L.state[i]);
     }
     std::vector<uint32_t> leaders;
     for (uint32_t i = 0; i < nh; ++i) {
+        if (removed(H, i)) continue;
         const uint32_t g = H.tag[i] & kMask, k = H.tag[i] >> 29;
-        const bool merged = g < t->max_low && cnt[g] == 8u;
-        if (!merged)
-            push(nH, &H.x[3 * i], &H.v[3 * i], H.id[i], H.tag[i]);
-        else if (k == 0u)
-            leaders.push_back(g);
+        const bool merged = g < t->max_low && cnt[g] == 8u && !(blended && (live[g] & 2u));
+        if (!merged || blended)
+            push(nH, &H.x[3 * i], &H.v[3 * i], H.id[i], H.tag[i], H.lam[i],
+                 merged ? ((H.state[i] & 4u) | 1u) : H.state[i]);
+        if (merged && k == 0u) leaders.push_back(g);
     }
     for (uint32_t g : leaders) {
         double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -818,26 +1058,38 @@ void oracle_ts_split_merge(oracle_ts* t) {
             x[d] /= 8.0;
             v[d] /= 8.0;
         }
-        push(nL, x, v, g, 0u);
+        push(nL, x, v, g, 0u, blended ? 0.0f : 1.0f, blended ? 2u : 0u);
     }
     const double cd = static_cast<double>(t->child_d_f);
     for (uint32_t i : split) {
         const uint32_t pid = L.id[i];
+        // insertHighResParticles (Solver.cu:541-586): r/sqrt(3) along the
+        // diagonals with r = 1/2 (3 m0 / (4 pi rho))^(1/3); else +/- h1/4.
+        const double rr = 0.5 * std::cbrt(3.0 / (4.0 * kPi) * L.p.mass / t->split_rho[i]);
+        const double d_ref = rr / std::sqrt(3.0);
         for (uint32_t k = 0; k < 8; ++k) {
             double x[3];
-            for (int d = 0; d < 3; ++d)
-                x[d] = L.x[3 * i + d] + (((k >> d) & 1u) ? cd : -cd);
-            push(nH, x, &L.v[3 * i], t->max_low + 8u * pid + k, (k << 29) | (pid & kMask));
+            for (int d = 0; d < 3; ++d) {
+                const double off = t->opt.split_offsets ? d_ref : cd;
+                x[d] = L.x[3 * i + d] + (((k >> d) & 1u) ? off : -off);
+            }
+            push(nH, x, &L.v[3 * i], t->max_low + 8u * pid + k, (k << 29) | (pid & kMask),
+                 blended ? 0.0f : 1.0f, blended ? 6u : 4u);
         }
     }
     L.x = std::move(nL.x);
     L.v = std::move(nL.v);
     L.id = std::move(nL.id);
     L.tag = std::move(nL.tag);
+    L.lam = std::move(nL.lam);
+    L.state = std::move(nL.state);
     H.x = std::move(nH.x);
     H.v = std::move(nH.v);
     H.id = std::move(nH.id);
     H.tag = std::move(nH.tag);
+    H.lam = std::move(nH.lam);
+    H.state = std::move(nH.state);
+    t->split_rho.clear();
 }
 
 void oracle_ts_counts(const oracle_ts* t, uint32_t counts[4]) {
@@ -860,6 +1112,8 @@ void oracle_ts_get(const oracle_ts* t, int res, int field, void* out) {
         case 7: std::copy(S.nb.perm.begin(), S.nb.perm.end(), static_cast<uint32_t*>(out)); break;
         case 8: std::copy(S.nb.start.begin(), S.nb.start.end(), static_cast<uint32_t*>(out)); break;
         case 9: std::copy(S.nb.end.begin(), S.nb.end.end(), static_cast<uint32_t*>(out)); break;
+        case 10: std::copy(S.lam.begin(), S.lam.end(), static_cast<float*>(out)); break;
+        case 11: std::copy(S.state.begin(), S.state.end(), static_cast<uint32_t*>(out)); break;
         default: break;
     }
 }

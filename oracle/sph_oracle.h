@@ -126,8 +126,28 @@ void oracle_ts_split_merge(oracle_ts* t);
 void oracle_ts_counts(const oracle_ts* t, uint32_t counts[4]);
 /* field: 0 pos (3n f64), 1 vel (3n f64), 2 id (n u32), 3 tag (n u32),
  * 4 rho (n f64, last step sorted order), 5 acc (3n f64), 6 sorted hash (n u32),
- * 7 perm (n u32), 8 cell start, 9 cell end (u32 per cell). */
+ * 7 perm (n u32), 8 cell start, 9 cell end (u32 per cell), 10 blend weight
+ * lambda (n f32), 11 state word (n u32). */
 void oracle_ts_get(const oracle_ts* t, int res, int field, void* out);
+
+/* Coupling options.  coupling 0: blend weight 1 and instantaneous split/merge
+ * (the rules above); 1: the reference's blended coupling -- per-particle
+ * blend weights lambda on every neighbour term, the fade-in/fade-out state
+ * machine of integrateD (TWO_SCALE/Solver.cu:1163-1262) and the faded merge.
+ * split_offsets 0: children at +/- h1/4 per axis; 1: at r/sqrt(3) along the
+ * diagonals, r = 1/2 (3 m0 / (4 pi rho))^(1/3) (Solver.cu:541-586).
+ * merge 0 disables merging.  velocity_cap c > 0 limits |v| to c h_r / dt
+ * after the velocity update (Solver.cu:1212-1223); 0 disables it. */
+typedef struct oracle_ts_options {
+    uint32_t coupling, split_offsets, merge;
+    float blend_increment, velocity_cap;
+} oracle_ts_options;
+void oracle_ts_set_options(oracle_ts* t, const oracle_ts_options* o);
+/* Replaces one set's blend weights and state words (n = its current count). */
+void oracle_ts_set_blend(oracle_ts* t, int res, const float* lambda, const uint32_t* state);
+/* Low particles whose id is listed and whose state is 0 get state 3 (split
+ * requested), the reference's external mark of TWO_SCALE/Solver.cu:1077-1082. */
+void oracle_ts_mark_split(oracle_ts* t, const uint32_t* ids, uint32_t n);
 
 #ifdef __cplusplus
 }

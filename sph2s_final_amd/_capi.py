@@ -38,6 +38,13 @@ FIELD_CELL_START = 6
 FIELD_CELL_END = 7
 FIELD_ACCEL = 8
 FIELD_TAG = 9
+FIELD_BLEND = 10
+FIELD_STATE = 11
+
+SPH2_COUPLING_FIXED = 0
+SPH2_COUPLING_BLENDED = 1
+SPH2_SPLIT_LATTICE = 0
+SPH2_SPLIT_REFERENCE = 1
 
 SPH_DEVERR_SORT_WAIT = 1
 SPH_DEVERR_CAPACITY = 2
@@ -78,6 +85,11 @@ class SphViews(Structure):
         ("cell_start", c_void_p), ("cell_end", c_void_p), ("accel4", c_void_p),
         ("n", c_uint32), ("num_cells", c_uint32), ("stream", c_void_p),
     ]
+
+
+class Sph2Options(Structure):
+    _fields_ = [("coupling", c_uint32), ("split_offsets", c_uint32), ("merge", c_uint32),
+                ("blend_increment", c_float), ("velocity_cap", c_float)]
 
 
 # (name, restype, argtypes) of every exported entry point of include/sph2s.h.
@@ -123,6 +135,10 @@ SIGNATURES = [
     ("sph2_counts", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph2_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph2_download", c_int, [c_void_p, c_int, c_int, c_void_p, c_size_t]),
+    ("sph2_set_options", c_int, [c_void_p, POINTER(Sph2Options)]),
+    ("sph2_get_options", c_int, [c_void_p, POINTER(Sph2Options)]),
+    ("sph2_set_blend", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_uint32]),
+    ("sph2_mark_split", c_int, [c_void_p, c_void_p, c_uint32]),
     ("sph_batch_create", c_int, [POINTER(SphConfig), c_uint32, c_void_p, c_void_p, c_void_p,
                                  POINTER(c_void_p)]),
     ("sph_batch_destroy", c_int, [c_void_p]),

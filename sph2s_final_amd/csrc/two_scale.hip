@@ -7,11 +7,16 @@
 // computeNeighborhoods :1616-1680, the complementary density/force sums
 // :252-290, :394-499, :732-845, :847-1048, integrateD + insertHighResParticles
 // :1163-1289, :541-586) and Solver.h:16-50 (h1 = h0/2, m1 = m0/8, one grid per
-// resolution).  Departures, all deliberate:
-//   * coupling: each set sees the other through the reference's averaged
-//     kernels 1/2 (W_h0 + W_h1) (and their gradients / Laplacians) with cutoff
-//     h0 and the other set's mass, with blend weight lambda = 1 throughout (no
-//     fade-in/fade-out timeline);
+// resolution).  Two couplings:
+//   * fixed (the default): each set sees the other through the reference's
+//     averaged kernels 1/2 (W_h0 + W_h1) (and their gradients / Laplacians)
+//     with cutoff h0 and the other set's mass, with blend weight lambda = 1
+//     throughout (no fade-in/fade-out timeline) and the split/merge below;
+//   * blended: the reference's own coupling -- every neighbour term weighted
+//     by the neighbour's lambda, integrateD's fade-in/fade-out state machine
+//     and velocity cap, splits that fade the parent out and the children in,
+//     and a faded merge (k_blend_density, k_blend_force_integrate).
+// Departures of the fixed coupling, all deliberate:
 //   * split: children at x +/- h1/4 on each axis (a uniform child lattice of
 //     spacing h1/2, the low-resolution lattice's h-relative spacing), mass m1,
 //     velocity copied; the reference places them at r/sqrt(3) with
@@ -31,6 +36,7 @@
 #include <vector>
 
 #include "neighbors.hpp"
+#include "pairwalk.hpp"
 
 namespace sph {
 namespace {
@@ -50,7 +56,23 @@ struct CrossParams {
     float gx, gy, gz, restitution;
     float lox, loy, loz, hix, hiy, hiz;
     uint32_t flags;
+    float vmax;            // velocity cap c h_A / dt of the target set; 0 = off
 };
+
+// The velocity cap of integrateD (TWO_SCALE/Solver.cu:1212-1223):
+// |v| <= vmax.  The reference scales by min(|v|, vmax) / |v|, which is exactly
+// 1 when |v| <= vmax (and 0/0 at |v| = 0, which is left alone here).
+__device__ __forceinline__ void cap_velocity(float vmax, float& vx, float& vy, float& vz) {
+    if (vmax > 0.0f) {
+        const float nv = __fsqrt_rn(vx * vx + vy * vy + vz * vz);
+        if (nv > vmax) {
+            const float q = __fdiv_rn(vmax, nv);
+            vx *= q;
+            vy *= q;
+            vz *= q;
+        }
+    }
+}
 
 // x + offset of child k: bit 0 -> x, bit 1 -> y, bit 2 -> z; set = +d.
 __device__ __forceinline__ float child_offset(int k, int axis, float d) {
@@ -197,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
         az -= tc * Tz;
     }
     float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
+    cap_velocity(c.vmax, vx, vy, vz);
     float x = xi.x + dt * vx, y = xi.y + dt * vy, z = xi.z + dt * vz;
     if (c.flags & SPH_FLAG_WALL_CLAMP) {
         const float e = c.restitution;
@@ -213,12 +236,303 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
 }
 
 // ---------------------------------------------------------------------------
+// Blended coupling (TWO_SCALE/Solver.cu:215-499, 732-1083, 1163-1289): every
+// neighbour term carries the neighbour's blend weight lambda_j.  Each set's
+// own sums walk its own grid with the block-staged stencil walk; the
+// complementary sums walk the other set's grid per particle, as in the fixed
+// coupling.  Blend state words: float2 (lambda, state bits).
+
+constexpr int kBlendDensityTile = 1024;
+constexpr int kBlendForceTile = 512;
+
+// rho_i = m_A sum_{A, r < h_A} lambda_j W_hA + m_B sum_{B, r < h0} lambda_j
+// 1/2 (W_h0 + W_h1), then A_i, B_i.
+__global__ __launch_bounds__(kBlock) void k_blend_density(
+    const float4* __restrict__ tpos_a, const float2* __restrict__ tbl_a,
+    const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
+    const uint32_t* __restrict__ n_a, GridView ga, StepParams pa,
+    const float4* __restrict__ tpos_b, const float2* __restrict__ tbl_b,
+    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
+    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
+    const uint32_t* __restrict__ occ, GridView g_occ, float* __restrict__ rho_a,
+    float2* __restrict__ aux_a) {
+    __shared__ float4 tile[kBlendDensityTile];
+    __shared__ uint32_t red[2 * (kBlock / 64)];
+    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = s < na;
+    const float4 xi = active ? tpos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const Probe pr = make_probe(ga, xi.x, xi.y, xi.z);
+    float own = 0.0f;
+    auto pair = [&](const float4& xj, float lj) {
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < pa.h2) {
+            const float a = pa.h2 - r2;
+            own += lj * (a * a * a);
+        }
+    };
+    walk_stencil_staged<kBlock, kBlendDensityTile>(
+        ga, pr, cs_a, ce_a, na, active, red,
+        [&](uint32_t k, uint32_t j) {
+            const float4 x = tpos_a[j];
+            tile[k] = make_float4(x.x, x.y, x.z, tbl_a[j].x);
+        },
+        [&](uint32_t t0, uint32_t t1, uint32_t) {
+            for (uint32_t t = t0; t < t1; ++t) {
+                const float4 q = tile[t];
+                pair(q, q.w);
+            }
+        },
+        [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) pair(tpos_a[j], tbl_a[j].x);
+        });
+    if (!active) return;
+    float cross = 0.0f;
+    if (nb > 0 && (!occ || high_nearby(occ, g_occ, xi.x, xi.y, xi.z))) {
+        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = tpos_b[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < c.h02) {
+                    const float a0 = c.h02 - r2;
+                    const float a1 = c.h12 - r2;
+                    const float w0 = c.c6_0 * a0 * a0 * a0;
+                    const float w1 = (r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f;
+                    cross += tbl_b[j].x * (0.5f * (w0 + w1));
+                }
+            }
+        });
+    }
+    const float r = pa.c6m * own + c.m_other * cross;
+    rho_a[s] = r;
+    const float inv = __builtin_amdgcn_rcpf(r);
+    aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+}
+
+// Blended acceleration (own + complementary, lambda-weighted; gravity), the
+// velocity update with the cap, the position update and the wall clamp, then
+// integrateD's blend update; results at sorted slot s.
+template <bool kTension>
+__global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
+    const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
+    const float2* __restrict__ aux_a, const float2* __restrict__ tbl_a,
+    const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
+    const uint32_t* __restrict__ n_a, GridView ga, StepParams pa,
+    const float4* __restrict__ tpos_b, const float4* __restrict__ tvel_b,
+    const float2* __restrict__ aux_b, const float2* __restrict__ tbl_b,
+    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
+    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
+    const uint32_t* __restrict__ occ, GridView g_occ, float dt, float inc,
+    float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
+    float4* __restrict__ acc_out) {
+    __shared__ float4 tx[kBlendForceTile];
+    __shared__ float4 tv[kBlendForceTile];
+    __shared__ float tl[kBlendForceTile];
+    __shared__ uint32_t red[2 * (kBlock / 64)];
+    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = s < na;
+    const float4 xi = active ? tpos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = active ? tvel_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float2 ai = active ? aux_a[s] : make_float2(0.f, 1.f);
+    const Probe pr = make_probe(ga, xi.x, xi.y, xi.z);
+    float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f,
+          Tz = 0.f;
+    const uint32_t h2_bits_m1 = __float_as_uint(pa.h2) - 1u;
+    // xa = (x_j, A_j), vb = (v_j, B_j); 0 < r < h_A.
+    auto pair = [&](const float4& xa, const float4& vb, float lj) {
+        const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (__float_as_uint(r2) - 1u >= h2_bits_m1) return;
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float hr = pa.h - r2 * inv_r;
+        const float tpr = lj * (ai.x + xa.w) * hr * hr * inv_r;
+        Px += tpr * dx;
+        Py += tpr * dy;
+        Pz += tpr * dz;
+        const float tvs = lj * hr * vb.w;
+        Vx += tvs * (vb.x - vi.x);
+        Vy += tvs * (vb.y - vi.y);
+        Vz += tvs * (vb.z - vi.z);
+        if (kTension) {
+            const float a = pa.h2 - r2;
+            const float w = lj * (a * a * a);
+            Tx += w * dx;
+            Ty += w * dy;
+            Tz += w * dz;
+        }
+    };
+    walk_stencil_staged<kBlock, kBlendForceTile>(
+        ga, pr, cs_a, ce_a, na, active, red,
+        [&](uint32_t k, uint32_t j) {
+            const float4 x = tpos_a[j], v = tvel_a[j];
+            const float2 a = aux_a[j];
+            tx[k] = make_float4(x.x, x.y, x.z, a.x);
+            tv[k] = make_float4(v.x, v.y, v.z, a.y);
+            tl[k] = tbl_a[j].x;
+        },
+        [&](uint32_t t0, uint32_t t1, uint32_t) {
+            for (uint32_t t = t0; t < t1; ++t) pair(tx[t], tv[t], tl[t]);
+        },
+        [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 x = tpos_a[j], v = tvel_a[j];
+                const float2 a = aux_a[j];
+                pair(make_float4(x.x, x.y, x.z, a.x), make_float4(v.x, v.y, v.z, a.y),
+                     tbl_a[j].x);
+            }
+        });
+    if (!active) return;
+    const float vc = pa.visc * ai.y;
+    float ax = pa.c45m * Px + vc * Vx + pa.gx;
+    float ay = pa.c45m * Py + vc * Vy + pa.gy;
+    float az = pa.c45m * Pz + vc * Vz + pa.gz;
+    if (kTension) {
+        const float tc = pa.tens * ai.y;
+        ax -= tc * Tx;
+        ay -= tc * Ty;
+        az -= tc * Tz;
+    }
+    // complementary sums (Solver.cu:394-499) weighted by lambda_j
+    float Qx = 0.f, Qy = 0.f, Qz = 0.f, Ux = 0.f, Uy = 0.f, Uz = 0.f, Sx = 0.f, Sy = 0.f,
+          Sz = 0.f;
+    if (nb > 0 && (!occ || high_nearby(occ, g_occ, xi.x, xi.y, xi.z))) {
+        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = tpos_b[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 < c.h02 && r2 > 0.0f) {
+                    const float4 vj = tvel_b[j];
+                    const float2 aj = aux_b[j];
+                    const float lj = tbl_b[j].x;
+                    const float inv_r = __builtin_amdgcn_rsqf(r2);
+                    const float r = r2 * inv_r;
+                    const float hr0 = c.h0 - r;
+                    const float hr1 = fmaxf(c.h1 - r, 0.0f);
+                    const float g = 0.5f * (c.c45_0 * hr0 * hr0 + c.c45_1 * hr1 * hr1) * inv_r;
+                    const float l = 0.5f * (c.c45_0 * hr0 + c.c45_1 * hr1);
+                    const float tp = lj * (ai.x + aj.x) * g;
+                    Qx += tp * dx;
+                    Qy += tp * dy;
+                    Qz += tp * dz;
+                    const float tvv = lj * l * aj.y;
+                    Ux += tvv * (vj.x - vi.x);
+                    Uy += tvv * (vj.y - vi.y);
+                    Uz += tvv * (vj.z - vi.z);
+                    if (kTension) {
+                        const float a0 = c.h02 - r2, a1 = c.h12 - r2;
+                        const float w = lj * 0.5f *
+                                        (c.c6_0 * a0 * a0 * a0 +
+                                         ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
+                        Sx += w * dx;
+                        Sy += w * dy;
+                        Sz += w * dz;
+                    }
+                }
+            }
+        });
+    }
+    const float mv = c.m_other * c.visc * ai.y;
+    ax += c.m_other * Qx + mv * Ux;
+    ay += c.m_other * Qy + mv * Uy;
+    az += c.m_other * Qz + mv * Uz;
+    if (kTension) {
+        const float tc = c.tension * c.m_other * ai.y;
+        ax -= tc * Sx;
+        ay -= tc * Sy;
+        az -= tc * Sz;
+    }
+    float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
+    cap_velocity(c.vmax, vx, vy, vz);
+    float x = xi.x + dt * vx, y = xi.y + dt * vy, z = xi.z + dt * vz;
+    if (c.flags & SPH_FLAG_WALL_CLAMP) {
+        const float e = c.restitution;
+        if (x < c.lox) { x = c.lox; vx = -e * vx; }
+        if (x > c.hix) { x = c.hix; vx = -e * vx; }
+        if (y < c.loy) { y = c.loy; vy = -e * vy; }
+        if (y > c.hiy) { y = c.hiy; vy = -e * vy; }
+        if (z < c.loz) { z = c.loz; vz = -e * vz; }
+        if (z > c.hiz) { z = c.hiz; vz = -e * vz; }
+    }
+    pos_a[s] = make_float4(x, y, z, xi.w);
+    vel_a[s] = make_float4(vx, vy, vz, vi.w);
+    if (acc_out) acc_out[s] = make_float4(ax, ay, az, 0.0f);
+    // Blend update (Solver.cu:1242-1257): lambda += c[mode] inc with
+    // c = {0, -1, +1, 0}; a fading-in particle reaching 1 settles.  Removal
+    // (fading out, lambda <= 0) happens in the compaction.
+    const float2 bl = tbl_a[s];
+    uint32_t st = __float_as_uint(bl.y);
+    const uint32_t mode = st & 3u;
+    float lam = bl.x;
+    if (mode == 1u) lam = __fsub_rn(lam, inc);
+    if (mode == 2u) lam = __fadd_rn(lam, inc);
+    if (mode == 2u && lam >= 1.0f) {
+        st &= 4u;
+        lam = 1.0f;
+    }
+    nbl_a[s] = make_float2(lam, __uint_as_float(st));
+}
+
+__global__ void k_gather_blend(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ n_dev,
+                               const float2* __restrict__ src, float2* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    dst[i] = src[perm[i]];
+}
+
+__global__ void k_fill_blend(const uint32_t* __restrict__ n_dev, float2 value,
+                             float2* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    dst[i] = value;
+}
+
+__global__ void k_fill_float(const uint32_t* __restrict__ n_dev, float value,
+                             float* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    dst[i] = value;
+}
+
+// Low particles whose id is in the sorted list `ids` and that are settled get
+// state 3 (split requested).
+__global__ void k_mark_ids(const float4* __restrict__ pos, const uint32_t* __restrict__ n_dev,
+                           const uint32_t* __restrict__ ids, uint32_t n_ids,
+                           float2* __restrict__ bl) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_dev) return;
+    const uint32_t id = __float_as_uint(pos[i].w);
+    uint32_t lo = 0, hi = n_ids;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ids[mid] < id)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo < n_ids && ids[lo] == id) {
+        const float2 b = bl[i];
+        if (__float_as_uint(b.y) == 0u) bl[i] = make_float2(b.x, __uint_as_float(3u));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // split / merge bookkeeping
 
 struct Region {
     float lox, loy, loz, hix, hiy, hiz;
     float margin;      // merge hysteresis
-    float child_d;     // child offset per axis
+    float child_d;     // child offset per axis (lattice offsets)
+};
+
+// How a split places its children and whether groups merge.
+struct SplitParams {
+    uint32_t merge;        // 0: never merge
+    uint32_t ref_offsets;  // 1: r/sqrt(3) diagonals with r from the parent's density
+    float m0;              // low-resolution mass
 };
 
 __device__ __forceinline__ bool inside(const Region& r, float x, float y, float z) {
@@ -233,24 +547,23 @@ __device__ __forceinline__ bool outside_margin(const Region& r, float x, float y
 // High-resolution particles carry (slot << 29) | group in vel.w.
 constexpr uint32_t kGroupMask = (1u << 29) - 1u;
 
-// Low set: split flag.  High set: children outside the expanded region record
-// their state in their group's slots and count themselves.
-__global__ __launch_bounds__(kBlock) void k_split_mark(const float4* __restrict__ pos_l,
-                                                       const uint32_t* __restrict__ n_l, Region rg,
-                                                       uint32_t* __restrict__ flag_l) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= *n_l) return;
-    const float4 x = pos_l[i];
-    flag_l[i] = inside(rg, x.x, x.y, x.z) ? 1u : 0u;
+// Blended coupling: a fading-out particle whose weight reached 0 is removed.
+__device__ __forceinline__ bool blend_removed(const float2& b) {
+    return (__float_as_uint(b.y) & 3u) == 1u && b.x <= 0.0f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_merge_mark(const float4* __restrict__ pos_h,
-                                                       const float4* __restrict__ vel_h,
-                                                       const uint32_t* __restrict__ n_h, Region rg,
-                                                       float4* __restrict__ grp_pos,
-                                                       float4* __restrict__ grp_vel,
-                                                       uint32_t* __restrict__ grp_cnt,
-                                                       uint32_t num_groups) {
+// Flags, per particle of either set: bit 0 = not kept, bit 1 = emits (a low
+// particle its 8 children, a high merge leader the merged particle), bit 2
+// (blended high set) = starts fading out.
+//
+// High set: children outside the expanded region record their state in their
+// group's slots and count themselves (blended: only settled ones, and every
+// live child marks its group in grp_live bit 0).
+__global__ __launch_bounds__(kBlock) void k_merge_mark(
+    const float4* __restrict__ pos_h, const float4* __restrict__ vel_h,
+    const float2* __restrict__ bl_h, const uint32_t* __restrict__ n_h, Region rg, uint32_t merge,
+    float4* __restrict__ grp_pos, float4* __restrict__ grp_vel, uint32_t* __restrict__ grp_cnt,
+    uint32_t* __restrict__ grp_live, uint32_t num_groups) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *n_h) return;
     const float4 x = pos_h[i];
@@ -258,33 +571,84 @@ __global__ __launch_bounds__(kBlock) void k_merge_mark(const float4* __restrict_
     const uint32_t tag = __float_as_uint(v.w);
     const uint32_t g = tag & kGroupMask, k = tag >> 29;
     if (g >= num_groups) return;
-    if (outside_margin(rg, x.x, x.y, x.z)) {
+    if (bl_h) {
+        const float2 b = bl_h[i];
+        if (blend_removed(b)) return;
+        atomicOr(&grp_live[g], 1u);
+        if ((__float_as_uint(b.y) & 3u) != 0u) return;
+    }
+    if (merge && outside_margin(rg, x.x, x.y, x.z)) {
         grp_pos[static_cast<size_t>(g) * 8 + k] = x;
         grp_vel[static_cast<size_t>(g) * 8 + k] = v;
         atomicAdd(&grp_cnt[g], 1u);
     }
 }
 
-// flag_h: bit 0 = merged away, bit 1 = emits the merged particle (slot 0).
+// Low set.  Fixed: inside the region -> split and removed (3).  Blended:
+// removed (1); or split (2) when settled inside the region or requested
+// (state 3) and no child of its group is alive; every live particle marks
+// its id in grp_live bit 1.
+__global__ __launch_bounds__(kBlock) void k_split_mark(const float4* __restrict__ pos_l,
+                                                       const float2* __restrict__ bl_l,
+                                                       const uint32_t* __restrict__ n_l, Region rg,
+                                                       uint32_t* __restrict__ grp_live,
+                                                       uint32_t num_groups,
+                                                       uint32_t* __restrict__ flag_l) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= *n_l) return;
+    const float4 x = pos_l[i];
+    const bool in = inside(rg, x.x, x.y, x.z);
+    if (!bl_l) {
+        flag_l[i] = in ? 3u : 0u;
+        return;
+    }
+    const float2 b = bl_l[i];
+    if (blend_removed(b)) {
+        flag_l[i] = 1u;
+        return;
+    }
+    const uint32_t id = __float_as_uint(x.w);
+    uint32_t children = 0u;
+    if (id < num_groups) children = atomicOr(&grp_live[id], 2u) & 1u;
+    const uint32_t mode = __float_as_uint(b.y) & 3u;
+    const bool want = mode == 3u || (mode == 0u && in);
+    flag_l[i] = (want && !children) ? 2u : 0u;
+}
+
+// High set.  Fixed: a group with 8 children outside the expanded region
+// merges (1, leader 1 | 2).  Blended: removed (1); a group of 8 settled
+// children outside, with no live low particle of its id, starts fading out
+// (4, leader 4 | 2).
 __global__ __launch_bounds__(kBlock) void k_merge_flags(const float4* __restrict__ vel_h,
+                                                        const float2* __restrict__ bl_h,
                                                         const uint32_t* __restrict__ n_h,
                                                         const uint32_t* __restrict__ grp_cnt,
+                                                        const uint32_t* __restrict__ grp_live,
                                                         uint32_t num_groups,
                                                         uint32_t* __restrict__ flag_h) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= *n_h) return;
     const uint32_t tag = __float_as_uint(vel_h[i].w);
     const uint32_t g = tag & kGroupMask, k = tag >> 29;
-    const bool merge = g < num_groups && grp_cnt[g] == 8u;
-    flag_h[i] = merge ? (1u | ((k == 0u) ? 2u : 0u)) : 0u;
+    const uint32_t lead = (k == 0u) ? 2u : 0u;
+    if (!bl_h) {
+        const bool merge = g < num_groups && grp_cnt[g] == 8u;
+        flag_h[i] = merge ? (1u | lead) : 0u;
+        return;
+    }
+    if (blend_removed(bl_h[i])) {
+        flag_h[i] = 1u;
+        return;
+    }
+    const bool merge = g < num_groups && grp_cnt[g] == 8u && !(grp_live[g] & 2u);
+    flag_h[i] = merge ? (4u | lead) : 0u;
 }
 
-// Per-block counts of two predicates over the flags of one set.
-// set L: c0 = kept (flag == 0), c1 = split (flag == 1)
-// set H: c0 = kept (bit 0 clear), c1 = merge leaders (bit 1 set)
+// Per-block counts over the flags of one set: c0 = kept (bit 0 clear),
+// c1 = emitting (bit 1 set).
 __global__ __launch_bounds__(kBlock) void k_block_counts(const uint32_t* __restrict__ flag,
                                                          const uint32_t* __restrict__ n_dev,
-                                                         int high, uint32_t* __restrict__ counts,
+                                                         uint32_t* __restrict__ counts,
                                                          uint32_t nblocks) {
     __shared__ uint32_t c0s, c1s;
     if (threadIdx.x == 0) c0s = c1s = 0u;
@@ -294,8 +658,8 @@ __global__ __launch_bounds__(kBlock) void k_block_counts(const uint32_t* __restr
     bool c0 = false, c1 = false;
     if (i < n) {
         const uint32_t f = flag[i];
-        c0 = high ? !(f & 1u) : (f == 0u);
-        c1 = high ? ((f & 2u) != 0u) : (f == 1u);
+        c0 = !(f & 1u);
+        c1 = (f & 2u) != 0u;
     }
     const uint64_t b0 = __ballot(c0), b1 = __ballot(c1);
     if ((threadIdx.x & 63) == 0) {
@@ -375,60 +739,83 @@ __device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t* wsum) {
     return off + below;
 }
 
-// Low set: survivors to new_l[keep rank]; split particles emit 8 children to
+// Low set: kept particles to new_l[keep rank] (blended: a splitting parent
+// starts fading out); emitting ones write 8 children to
 // new_h[keepH + 8 * split rank + k].  ids: child id = id_base + 8 * pid + k,
-// group = pid, slot = k.
+// group = pid, slot = k.  Child offsets: +/- child_d per axis, or (reference
+// offsets) r/sqrt(3) per axis with r = 1/2 (3 m0 / (4 pi rho))^(1/3)
+// (insertHighResParticles, TWO_SCALE/Solver.cu:541-586).  Blend state words
+// (bl_l != NULL): children start at lambda 0, state 6.
 __global__ __launch_bounds__(kBlock) void k_scatter_low(
     const float4* __restrict__ pos_l, const float4* __restrict__ vel_l,
+    const float* __restrict__ rho_l, const float2* __restrict__ bl_l,
     const uint32_t* __restrict__ flag_l, const uint32_t* __restrict__ n_l,
     const uint32_t* __restrict__ cl, uint32_t nbl, const uint32_t* __restrict__ tot,
-    Region rg, uint32_t id_base, uint32_t cap_l, uint32_t cap_h, float4* __restrict__ new_pos_l,
-    float4* __restrict__ new_vel_l, float4* __restrict__ new_pos_h,
-    float4* __restrict__ new_vel_h) {
+    Region rg, SplitParams sp, uint32_t id_base, uint32_t cap_l, uint32_t cap_h,
+    uint32_t* __restrict__ grp_live, uint32_t num_groups, float4* __restrict__ new_pos_l,
+    float4* __restrict__ new_vel_l, float2* __restrict__ new_bl_l,
+    float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
+    float2* __restrict__ new_bl_h) {
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = *n_l;
     const bool live = i < n;
-    const uint32_t f = live ? flag_l[i] : 2u;
-    const uint32_t rk = block_rank(live && f == 0u, wsum);
-    const uint32_t rs = block_rank(live && f == 1u, wsum);
+    const uint32_t f = live ? flag_l[i] : 1u;
+    const uint32_t rk = block_rank(live && !(f & 1u), wsum);
+    const uint32_t rs = block_rank(live && (f & 2u), wsum);
     if (!live) return;
     const float4 x = pos_l[i];
     const float4 v = vel_l[i];
-    if (f == 0u) {
+    const uint32_t pid = __float_as_uint(x.w);
+    if (grp_live && pid < num_groups) grp_live[pid] = 0u;
+    if (!(f & 1u)) {
         const uint32_t dst = cl[blockIdx.x] + rk;
         if (dst < cap_l) {
             new_pos_l[dst] = x;
             new_vel_l[dst] = make_float4(v.x, v.y, v.z, 0.0f);
+            if (bl_l) {
+                const float2 b = bl_l[i];
+                new_bl_l[dst] = (f & 2u) ? make_float2(b.x, __uint_as_float(1u)) : b;
+            }
         }
-    } else {
+    }
+    if (f & 2u) {
         const uint32_t base = tot[2] + 8u * (cl[nbl + blockIdx.x] + rs);
-        const uint32_t pid = __float_as_uint(x.w);
+        float d = rg.child_d;
+        if (sp.ref_offsets) {
+            // 3 / (4 pi) and 1/sqrt(3) as the reference's float constants
+            const float r = 0.5f * cbrtf(0.238732414637843f * sp.m0 / rho_l[i]);
+            d = r * 0.57735026919f;
+        }
         for (int k = 0; k < 8; ++k) {
             const uint32_t dst = base + k;
             if (dst >= cap_h) break;
-            new_pos_h[dst] = make_float4(x.x + child_offset(k, 0, rg.child_d),
-                                         x.y + child_offset(k, 1, rg.child_d),
-                                         x.z + child_offset(k, 2, rg.child_d),
+            new_pos_h[dst] = make_float4(x.x + child_offset(k, 0, d), x.y + child_offset(k, 1, d),
+                                         x.z + child_offset(k, 2, d),
                                          __uint_as_float(id_base + 8u * pid + k));
             new_vel_h[dst] = make_float4(v.x, v.y, v.z,
                                          __uint_as_float((static_cast<uint32_t>(k) << 29) |
                                                          (pid & kGroupMask)));
+            if (bl_l) new_bl_h[dst] = make_float2(0.0f, __uint_as_float(6u));
         }
     }
 }
 
-// High set: survivors to new_h[keep rank]; merge leaders emit the merged
-// particle (centroid, mean velocity, id = group) to new_l[keepL + lead rank].
-// Every high particle clears its group's counter for the next step.
+// High set: kept particles to new_h[keep rank] (blended: merging siblings
+// start fading out); merge leaders emit the merged particle (centroid, mean
+// velocity, id = group; blended: lambda 0, state 2) to
+// new_l[keepL + lead rank].  Every high particle clears its group's counter
+// and live bits for the next step.
 __global__ __launch_bounds__(kBlock) void k_scatter_high(
     const float4* __restrict__ pos_h, const float4* __restrict__ vel_h,
-    const uint32_t* __restrict__ flag_h, const uint32_t* __restrict__ n_h,
-    const uint32_t* __restrict__ ch, uint32_t nbh, const uint32_t* __restrict__ tot,
-    const float4* __restrict__ grp_pos, const float4* __restrict__ grp_vel,
-    uint32_t* __restrict__ grp_cnt, uint32_t num_groups, uint32_t cap_l, uint32_t cap_h,
+    const float2* __restrict__ bl_h, const uint32_t* __restrict__ flag_h,
+    const uint32_t* __restrict__ n_h, const uint32_t* __restrict__ ch, uint32_t nbh,
+    const uint32_t* __restrict__ tot, const float4* __restrict__ grp_pos,
+    const float4* __restrict__ grp_vel, uint32_t* __restrict__ grp_cnt,
+    uint32_t* __restrict__ grp_live, uint32_t num_groups, uint32_t cap_l, uint32_t cap_h,
     float4* __restrict__ new_pos_l, float4* __restrict__ new_vel_l,
-    float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h) {
+    float2* __restrict__ new_bl_l, float4* __restrict__ new_pos_h,
+    float4* __restrict__ new_vel_h, float2* __restrict__ new_bl_h) {
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = *n_h;
@@ -439,14 +826,24 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
     if (!live) return;
     const float4 v = vel_h[i];
     const uint32_t g = __float_as_uint(v.w) & kGroupMask;
-    if (g < num_groups) grp_cnt[g] = 0u;
+    if (g < num_groups) {
+        grp_cnt[g] = 0u;
+        if (grp_live) grp_live[g] = 0u;
+    }
     if (!(f & 1u)) {
         const uint32_t dst = ch[blockIdx.x] + rk;
         if (dst < cap_h) {
             new_pos_h[dst] = pos_h[i];
             new_vel_h[dst] = v;
+            if (bl_h) {
+                const float2 b = bl_h[i];
+                new_bl_h[dst] =
+                    (f & 4u) ? make_float2(b.x, __uint_as_float((__float_as_uint(b.y) & 4u) | 1u))
+                             : b;
+            }
         }
-    } else if (f & 2u) {
+    }
+    if (f & 2u) {
         const uint32_t dst = tot[0] + ch[nbh + blockIdx.x] + rl;
         if (dst < cap_l) {
             float sx = 0.f, sy = 0.f, sz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
@@ -462,6 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
             }
             new_pos_l[dst] = make_float4(0.125f * sx, 0.125f * sy, 0.125f * sz, __uint_as_float(g));
             new_vel_l[dst] = make_float4(0.125f * ux, 0.125f * uy, 0.125f * uz, 0.0f);
+            if (bl_h) new_bl_l[dst] = make_float2(0.0f, __uint_as_float(2u));
         }
     }
 }
@@ -491,12 +889,14 @@ __global__ void k_pack_two_scale(const float* __restrict__ pxyz, const float* __
 }
 
 __global__ void k_copy_state(const float4* __restrict__ sp, const float4* __restrict__ sv,
-                             const uint32_t* __restrict__ n_dev, float4* __restrict__ dp,
-                             float4* __restrict__ dv) {
+                             const float2* __restrict__ sb, const uint32_t* __restrict__ n_dev,
+                             float4* __restrict__ dp, float4* __restrict__ dv,
+                             float2* __restrict__ db) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *n_dev) return;
     dp[i] = sp[i];
     dv[i] = sv[i];
+    if (sb) db[i] = sb[i];
 }
 
 inline dim3 blocks_for(uint32_t n) { return dim3((n + kBlock - 1) / kBlock > 0 ? (n + kBlock - 1) / kBlock : 1); }
@@ -517,6 +917,11 @@ struct TwoScaleSet {
     float4* accel = nullptr;
     uint32_t* flag = nullptr;
     uint32_t* counts = nullptr;   // 2 * nblocks
+    // Blend state words (lambda, state bits) of the blended coupling: the
+    // compacted state, its sorted copy and the step's updated words (sorted).
+    float2* state_blend = nullptr;
+    float2* tblend = nullptr;
+    float2* nblend = nullptr;
 };
 
 struct sph_two_scale {
@@ -531,6 +936,8 @@ struct sph_two_scale {
     uint32_t* tot = nullptr;     // 6 words
     uint32_t* error_word = nullptr;
     uint32_t* occ = nullptr;     // per low-grid cell: 1 when it holds a high particle
+    uint32_t* grp_live = nullptr;  // per group: bit 0 live child, bit 1 live low particle (blended)
+    sph2_options opt{SPH2_COUPLING_FIXED, SPH2_SPLIT_LATTICE, 1u, 0.0125f, 0.0f};
     hipStream_t stream = nullptr;
     void* arena = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -577,7 +984,15 @@ CrossParams cross_params(const sph_two_scale* t, int target) {
     c.hiy = lo.domain_hi[1];
     c.hiz = lo.domain_hi[2];
     c.flags = lo.flags;
+    c.vmax = 0.0f;
     return c;
+}
+
+bool blended(const sph_two_scale* t) { return t->opt.coupling == SPH2_COUPLING_BLENDED; }
+
+// c h_A / dt in FP32, as integrateD computes 0.8f * h / dt (Solver.cu:1214-1217).
+float velocity_limit(const sph_two_scale* t, int target, float dt) {
+    return t->opt.velocity_cap > 0.0f ? t->opt.velocity_cap * t->set[target].cfg.h / dt : 0.0f;
 }
 
 // The integrated states -> split/merge -> compacted states and new counts.
@@ -586,29 +1001,83 @@ int launch_split_merge(sph_two_scale* t) {
     TwoScaleSet& H = t->set[1];
     hipStream_t st = t->stream;
     const uint32_t nbl = blocks_for(L.cap).x, nbh = blocks_for(H.cap).x;
-    hipLaunchKernelGGL(k_split_mark, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.n_dev,
-                       t->region, L.flag);
-    hipLaunchKernelGGL(k_merge_mark, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel,
-                       H.buf.n_dev, t->region, t->grp_pos, t->grp_vel, t->grp_cnt, t->num_groups);
-    hipLaunchKernelGGL(k_merge_flags, dim3(nbh), dim3(kBlock), 0, st, H.buf.vel, H.buf.n_dev,
-                       t->grp_cnt, t->num_groups, H.flag);
-    hipLaunchKernelGGL(k_block_counts, dim3(nbl), dim3(kBlock), 0, st, L.flag, L.buf.n_dev, 0,
+    // Blended coupling: the step's updated blend words; fixed: none (NULL).
+    const bool bl = blended(t);
+    const float2* bl_l = bl ? L.nblend : nullptr;
+    const float2* bl_h = bl ? H.nblend : nullptr;
+    uint32_t* live = bl ? t->grp_live : nullptr;
+    SplitParams sp{};
+    sp.merge = t->opt.merge;
+    sp.ref_offsets = t->opt.split_offsets == SPH2_SPLIT_REFERENCE ? 1u : 0u;
+    sp.m0 = L.cfg.mass;
+    // High marks first: a split waits for its group's live children.
+    hipLaunchKernelGGL(k_merge_mark, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel, bl_h,
+                       H.buf.n_dev, t->region, sp.merge, t->grp_pos, t->grp_vel, t->grp_cnt,
+                       live, t->num_groups);
+    hipLaunchKernelGGL(k_split_mark, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, bl_l, L.buf.n_dev,
+                       t->region, live, t->num_groups, L.flag);
+    hipLaunchKernelGGL(k_merge_flags, dim3(nbh), dim3(kBlock), 0, st, H.buf.vel, bl_h,
+                       H.buf.n_dev, t->grp_cnt, live, t->num_groups, H.flag);
+    hipLaunchKernelGGL(k_block_counts, dim3(nbl), dim3(kBlock), 0, st, L.flag, L.buf.n_dev,
                        L.counts, nbl);
-    hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, st, H.flag, H.buf.n_dev, 1,
+    hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, st, H.flag, H.buf.n_dev,
                        H.counts, nbh);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
                        L.cap, H.cap, t->tot, t->error_word);
-    hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel, L.flag,
-                       L.buf.n_dev, L.counts, nbl, t->tot, t->region, t->id_base, L.cap, H.cap,
-                       L.state_pos, L.state_vel, H.state_pos, H.state_vel);
-    hipLaunchKernelGGL(k_scatter_high, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel,
+    hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel,
+                       L.buf.rho, bl_l, L.flag, L.buf.n_dev, L.counts, nbl, t->tot, t->region, sp,
+                       t->id_base, L.cap, H.cap, live, t->num_groups, L.state_pos, L.state_vel,
+                       L.state_blend, H.state_pos, H.state_vel, H.state_blend);
+    hipLaunchKernelGGL(k_scatter_high, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel, bl_h,
                        H.flag, H.buf.n_dev, H.counts, nbh, t->tot, t->grp_pos, t->grp_vel,
-                       t->grp_cnt, t->num_groups, L.cap, H.cap, L.state_pos, L.state_vel,
-                       H.state_pos, H.state_vel);
+                       t->grp_cnt, live, t->num_groups, L.cap, H.cap, L.state_pos, L.state_vel,
+                       L.state_blend, H.state_pos, H.state_vel, H.state_blend);
     hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(64), 0, st, t->tot, L.buf.n_dev,
                        H.buf.n_dev);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
+}
+
+// Blended coupling: lambda-weighted densities of both sets, then the
+// lambda-weighted forces with integration and the blend update, then the
+// split/merge stage.
+int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView* G,
+                          const GridView* X, float dt, bool accel_out) {
+    hipStream_t st = t->stream;
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& A = t->set[r];
+        TwoScaleSet& B = t->set[1 - r];
+        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
+        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+                           A.tblend, A.buf.cell_start, A.buf.cell_end, A.buf.n_dev, G[r], P[r],
+                           B.buf.tpos, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
+                           X[1 - r], cross_params(t, r), occ, X[0], A.buf.rho, A.buf.aux);
+    }
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& A = t->set[r];
+        TwoScaleSet& B = t->set[1 - r];
+        CrossParams c = cross_params(t, r);
+        c.vmax = velocity_limit(t, r, dt);
+        float4* acc = accel_out ? A.buf.accel : nullptr;
+        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
+        const float inc = t->opt.blend_increment;
+        if (P[r].tens != 0.0f || c.tension != 0.0f)
+            hipLaunchKernelGGL(k_blend_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0,
+                               st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
+                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
+                               B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
+                               X[1 - r], c, occ, X[0], dt, inc, A.buf.pos, A.buf.vel, A.nblend,
+                               acc);
+        else
+            hipLaunchKernelGGL(k_blend_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
+                               st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
+                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
+                               B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
+                               X[1 - r], c, occ, X[0], dt, inc, A.buf.pos, A.buf.vel, A.nblend,
+                               acc);
+    }
+    SPH_HIP(hipGetLastError());
+    return launch_split_merge(t);
 }
 
 int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
@@ -635,6 +1104,9 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         if (rc == SPH_OK) rc = launch_pass(kPassSort, P[r], b, dt, S.key_bits, false, st);
         if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, dt, S.key_bits, false, st);
         if (rc != SPH_OK) return rc;
+        if (blended(t))
+            hipLaunchKernelGGL(k_gather_blend, blocks_for(S.cap), dim3(kBlock), 0, st, S.buf.perm,
+                               S.buf.n_dev, S.state_blend, S.tblend);
     }
     // Occupancy of the low grid by the (sorted) high particles.
     {
@@ -644,6 +1116,7 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[1].cap), dim3(kBlock), 0, st,
                            t->set[1].buf.tpos, t->set[1].buf.n_dev, G[0], t->occ);
     }
+    if (blended(t)) return launch_blended_passes(t, P, G, X, dt, accel_out);
     // Own-set densities, then the complementary sums (which also finalise A, B).
     for (int r = 1; r >= 0; --r) {
         int rc = launch_density_own(P[r], t->set[r].buf, st);
@@ -666,7 +1139,8 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
-        const CrossParams c = cross_params(t, r);
+        CrossParams c = cross_params(t, r);
+        c.vmax = velocity_limit(t, r, dt);
         float4* acc = accel_out ? A.buf.accel : nullptr;
         const uint32_t* occ = (r == 0) ? t->occ : nullptr;
         if (c.tension != 0.0f)
@@ -781,12 +1255,14 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         add(16 * N);                               // own-set accel
         add(4 * N);                                // flags
         add(8 * nb);                               // block counts
+        for (int k = 0; k < 3; ++k) add(8 * N);    // state_blend tblend nblend
     }
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
     add(4 * static_cast<size_t>(t->num_groups));    // grp_cnt
     add(64);                                        // tot[8] + error word
     add(4 * static_cast<size_t>(t->set[0].num_cells));  // occupancy of the low grid
+    add(4 * static_cast<size_t>(t->num_groups));    // grp_live
 
     hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -828,6 +1304,9 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         S.accel = static_cast<float4*>(take());
         S.flag = static_cast<uint32_t*>(take());
         S.counts = static_cast<uint32_t*>(take());
+        S.state_blend = static_cast<float2*>(take());
+        S.tblend = static_cast<float2*>(take());
+        S.nblend = static_cast<float2*>(take());
     }
     t->grp_pos = static_cast<float4*>(take());
     t->grp_vel = static_cast<float4*>(take());
@@ -835,6 +1314,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     t->tot = static_cast<uint32_t*>(take());
     t->error_word = t->tot + 8;
     t->occ = static_cast<uint32_t*>(take());
+    t->grp_live = static_cast<uint32_t*>(take());
 
     e = hipMemsetAsync(t->arena, 0, total, t->stream);
     if (e != hipSuccess) {
@@ -851,6 +1331,19 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
 }
 
 namespace {
+
+// Every particle of set r settled with lambda = 1 (state 0 low, 4 high).
+int reset_blend(sph_two_scale* t, int r) {
+    TwoScaleSet& S = t->set[r];
+    const uint32_t state = (r == 0) ? 0u : 4u;
+    float2 w;
+    w.x = 1.0f;
+    std::memcpy(&w.y, &state, 4);
+    hipLaunchKernelGGL(k_fill_blend, blocks_for(S.cap), dim3(kBlock), 0, t->stream, S.buf.n_dev, w,
+                       S.state_blend);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
 
 // Uploads one resolution's state (AoS host arrays) into its compacted-state
 // buffers, staging through the sorted-copy buffers.
@@ -873,7 +1366,7 @@ int upload_set(sph_two_scale* t, int r, const float* pos, const float* vel, cons
                        vel ? dv : nullptr, ids ? di : nullptr, tags ? dt : nullptr, n, S.state_pos,
                        S.state_vel);
     SPH_HIP(hipGetLastError());
-    return SPH_OK;
+    return reset_blend(t, r);
 }
 
 }  // namespace
@@ -921,10 +1414,83 @@ int sph2_apply_split_merge(sph_two_scale* t) {
     for (int r = 0; r < 2; ++r) {
         TwoScaleSet& S = t->set[r];
         hipLaunchKernelGGL(k_copy_state, blocks_for(S.cap), dim3(kBlock), 0, t->stream, S.state_pos,
-                           S.state_vel, S.buf.n_dev, S.buf.pos, S.buf.vel);
+                           S.state_vel, blended(t) ? S.state_blend : nullptr, S.buf.n_dev,
+                           S.buf.pos, S.buf.vel, S.nblend);
     }
+    // No step ran on this state: reference split offsets use the rest density.
+    hipLaunchKernelGGL(k_fill_float, blocks_for(t->set[0].cap), dim3(kBlock), 0, t->stream,
+                       t->set[0].buf.n_dev, t->set[0].cfg.rest_density, t->set[0].buf.rho);
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
+}
+
+int sph2_set_options(sph_two_scale* t, const sph2_options* o) {
+    if (!t || !o) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    if (o->coupling > SPH2_COUPLING_BLENDED || o->split_offsets > SPH2_SPLIT_REFERENCE ||
+        o->merge > 1u)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "options: unknown coupling/split/merge value");
+    if (!(o->blend_increment > 0.0f && o->blend_increment <= 1.0f))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "options: blend_increment must be in (0, 1]");
+    if (!(o->velocity_cap >= 0.0f) || !std::isfinite(o->velocity_cap))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "options: velocity_cap must be >= 0");
+    const bool was = blended(t);
+    t->opt = *o;
+    if (blended(t) && !was) {
+        for (int r = 0; r < 2; ++r) {
+            const int rc = reset_blend(t, r);
+            if (rc != SPH_OK) return rc;
+        }
+        SPH_HIP(hipMemsetAsync(t->grp_live, 0, 4ull * t->num_groups, t->stream));
+    }
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    destroy_graph2(t);
+    return SPH_OK;
+}
+
+int sph2_get_options(const sph_two_scale* t, sph2_options* o) {
+    if (!t || !o) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    *o = t->opt;
+    return SPH_OK;
+}
+
+int sph2_set_blend(sph_two_scale* t, int resolution, const float* lambda, const uint32_t* state,
+                   uint32_t n) {
+    if (!t || resolution < 0 || resolution > 1)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "bad argument");
+    if (!blended(t)) return set_error(SPH_ERR_INVALID_ARGUMENT, "blend words need the blended coupling");
+    uint32_t c[4];
+    int rc = sph2_counts(t, c);
+    if (rc != SPH_OK) return rc;
+    if (n != c[resolution]) return set_error(SPH_ERR_INVALID_ARGUMENT, "n must equal the set's count");
+    if (n == 0) return SPH_OK;
+    TwoScaleSet& S = t->set[resolution];
+    std::vector<float2> w(n);
+    SPH_HIP(hipMemcpyAsync(w.data(), S.state_blend, 8ull * n, hipMemcpyDeviceToHost, t->stream));
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        if (lambda) w[i].x = lambda[i];
+        if (state) std::memcpy(&w[i].y, &state[i], 4);
+    }
+    SPH_HIP(hipMemcpyAsync(S.state_blend, w.data(), 8ull * n, hipMemcpyHostToDevice, t->stream));
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
+}
+
+int sph2_mark_split(sph_two_scale* t, const uint32_t* ids, uint32_t n) {
+    if (!t || (n > 0 && !ids)) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!blended(t)) return set_error(SPH_ERR_INVALID_ARGUMENT, "split marks need the blended coupling");
+    TwoScaleSet& L = t->set[0];
+    if (n > L.cap) return set_error(SPH_ERR_INVALID_ARGUMENT, "more ids than low-resolution capacity");
+    if (n == 0) return SPH_OK;
+    std::vector<uint32_t> sorted(ids, ids + n);
+    std::sort(sorted.begin(), sorted.end());
+    // The low set's hash-key buffer is free between steps: stage the ids there.
+    SPH_HIP(hipMemcpyAsync(L.buf.keys, sorted.data(), 4ull * n, hipMemcpyHostToDevice, t->stream));
+    hipLaunchKernelGGL(k_mark_ids, blocks_for(L.cap), dim3(kBlock), 0, t->stream, L.state_pos,
+                       L.buf.n_dev, L.buf.keys, n, L.state_blend);
+    SPH_HIP(hipGetLastError());
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
 }
 
 int sph2_synchronize(sph_two_scale* t) {
@@ -981,12 +1547,17 @@ int sph2_download(sph_two_scale* t, int resolution, int field, void* dst, size_t
     const float4* vec = nullptr;
     const void* src = nullptr;
     size_t n = 0, need = 0;
-    int mode = 0;  // 0 raw copy, 1 xyz of float4, 2 w bits of float4
+    int mode = 0;  // 0 raw copy, 1 xyz of float4, 2 w bits of float4, 3/4 lambda/state
     switch (field) {
         case SPH_FIELD_POSITION: vec = S.state_pos; n = n_now; need = 12 * n; mode = 1; break;
         case SPH_FIELD_VELOCITY: vec = S.state_vel; n = n_now; need = 12 * n; mode = 1; break;
         case SPH_FIELD_ID: vec = S.state_pos; n = n_now; need = 4 * n; mode = 2; break;
         case SPH_FIELD_TAG: vec = S.state_vel; n = n_now; need = 4 * n; mode = 2; break;
+        case SPH_FIELD_BLEND:
+        case SPH_FIELD_STATE:
+            if (!blended(t))
+                return set_error(SPH_ERR_INVALID_ARGUMENT, "blend words need the blended coupling");
+            n = n_now; need = 4 * n; mode = (field == SPH_FIELD_BLEND) ? 3 : 4; break;
         case SPH_FIELD_DENSITY: src = S.buf.rho; n = n_step; need = 4 * n; break;
         case SPH_FIELD_SORTED_HASH: src = S.buf.skeys; n = n_step; need = 4 * n; break;
         case SPH_FIELD_SORTED_INDEX: src = S.buf.perm; n = n_step; need = 4 * n; break;
@@ -1003,6 +1574,14 @@ int sph2_download(sph_two_scale* t, int resolution, int field, void* dst, size_t
     if (mode == 0) {
         SPH_HIP(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, t->stream));
         SPH_HIP(hipStreamSynchronize(t->stream));
+        return SPH_OK;
+    }
+    if (mode >= 3) {  // one lane of the blend words
+        std::vector<float> w(2 * n);
+        SPH_HIP(hipMemcpyAsync(w.data(), S.state_blend, 8 * n, hipMemcpyDeviceToHost, t->stream));
+        SPH_HIP(hipStreamSynchronize(t->stream));
+        auto* o = static_cast<float*>(dst);
+        for (size_t i = 0; i < n; ++i) std::memcpy(&o[i], &w[2 * i + (mode == 3 ? 0 : 1)], 4);
         return SPH_OK;
     }
     std::vector<float> tmp(4 * n);

@@ -321,6 +321,42 @@ class TwoScaleSolver:
         C.check(C.lib().sph2_set_state(self._h, ptr(pl), ptr(vl), ptr(il), len(pl), ptr(ph),
                                        ptr(vh), ptr(ih), ptr(th), nh))
 
+    def set_options(self, coupling: int = C.SPH2_COUPLING_FIXED,
+                    split_offsets: int = C.SPH2_SPLIT_LATTICE, merge: bool = True,
+                    blend_increment: float = 0.0125, velocity_cap: float = 0.0) -> None:
+        """Coupling options (sph2_set_options): fixed or blended coupling
+        (the reference's lambda blending and fade timeline), lattice or
+        reference split offsets, merging on/off, the blend increment per step
+        (SolverConfiguration::BlendIncrement) and the velocity cap factor c of
+        |v| <= c h / dt (0 = off)."""
+        o = C.Sph2Options(int(coupling), int(split_offsets), int(bool(merge)),
+                          float(blend_increment), float(velocity_cap))
+        C.check(C.lib().sph2_set_options(self._h, byref(o)))
+
+    def get_options(self) -> dict:
+        o = C.Sph2Options()
+        C.check(C.lib().sph2_get_options(self._h, byref(o)))
+        return {"coupling": int(o.coupling), "split_offsets": int(o.split_offsets),
+                "merge": bool(o.merge), "blend_increment": float(o.blend_increment),
+                "velocity_cap": float(o.velocity_cap)}
+
+    def set_blend(self, resolution: int, lam=None, state=None) -> None:
+        """Replace one set's blend weights and/or state words (blended coupling)."""
+        n = self.counts()[resolution]
+        la = None if lam is None else np.ascontiguousarray(lam, np.float32)
+        st = None if state is None else np.ascontiguousarray(state, np.uint32)
+        for a in (la, st):
+            if a is not None and a.shape != (n,):
+                raise ValueError(f"expected {n} entries")
+        ptr = lambda a: None if a is None else a.ctypes.data_as(c_void_p)  # noqa: E731
+        C.check(C.lib().sph2_set_blend(self._h, int(resolution), ptr(la), ptr(st), n))
+
+    def mark_split(self, ids) -> None:
+        """Request the split of the low-resolution particles with these ids."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        C.check(C.lib().sph2_mark_split(self._h, ids.ctypes.data_as(c_void_p) if len(ids) else None,
+                                        len(ids)))
+
     def counts(self) -> tuple[int, int, int, int]:
         """(low, high) now and (low, high) the last step ran with."""
         c = (c_uint32 * 4)()
@@ -339,8 +375,10 @@ class TwoScaleSolver:
             out = np.empty((n_now, 3), np.float32)
         elif field == C.FIELD_ACCEL:
             out = np.empty((n_step, 3), np.float32)
-        elif field in (C.FIELD_ID, C.FIELD_TAG):
+        elif field in (C.FIELD_ID, C.FIELD_TAG, C.FIELD_STATE):
             out = np.empty(n_now, np.uint32)
+        elif field == C.FIELD_BLEND:
+            out = np.empty(n_now, np.float32)
         elif field == C.FIELD_DENSITY:
             out = np.empty(n_step, np.float32)
         elif field in (C.FIELD_SORTED_HASH, C.FIELD_SORTED_INDEX):
