@@ -23,7 +23,8 @@ Prints ONE JSON line with:
               batched step vs the same scenes as 64 single-scene solvers (each on
               its own stream, and serialised)
   two_scale   the two-scale dam break (63^3 low-resolution particles, 1->8
-              split / 8->1 merge region over 20% of the domain)
+              split / 8->1 merge region over 20% of the domain), under the
+              fixed coupling and under the reference's blended coupling
 """
 from __future__ import annotations
 
@@ -190,23 +191,38 @@ def bench_batched(torch, steps: int) -> dict:
                     "launches all 64 per step before waiting, 'serial' waits after each"}
 
 
-def bench_two_scale(torch, steps: int) -> dict:
-    from sph2s_final_amd import TwoScaleSolver, scenes
+def bench_two_scale(torch, steps: int, blended: bool = False) -> dict:
+    """Two-scale dam break; blended = the reference's lambda-blended coupling
+    (fade timeline with increment 0.0125, velocity cap 0.8 h / dt, r/sqrt(3)
+    split offsets) instead of the fixed coupling."""
+    from sph2s_final_amd import (FIELD_BLEND, SPH2_COUPLING_BLENDED, SPH2_SPLIT_REFERENCE,
+                                 TwoScaleSolver, scenes)
     sc = scenes.two_scale_250k()
     t = TwoScaleSolver(sc.config, sc.region_lo, sc.region_hi, sc.positions, sc.velocities,
                        max_low=sc.n, max_high=8 * sc.n)
+    if blended:
+        t.set_options(coupling=SPH2_COUPLING_BLENDED, split_offsets=SPH2_SPLIT_REFERENCE,
+                      blend_increment=0.0125, velocity_cap=0.8)
     t.Advance(sc.dt, 1)
     t.synchronize()
     c0 = t.counts()
     el = _timed(lambda: (t.Advance(sc.dt, steps), t.synchronize()), torch)
     c1 = t.counts()
     errs = t.device_errors()
-    t.close()
+    out = {"workload": sc.name + ("_blended" if blended else ""), "steps": steps,
+           "ms_per_step": el / steps * 1e3}
     active = 0.5 * ((c0[0] + c0[1]) + (c1[2] + c1[3]))  # mean particles per step (end points)
-    return {"workload": sc.name, "steps": steps, "ms_per_step": el / steps * 1e3,
-            "pu_s": active * steps / el, "counts_start": list(c0[:2]),
-            "counts_end": list(c1[:2]), "device_errors": errs,
-            "note": "pu_s = mean of the start and end active counts (low + high) x steps / t"}
+    out.update({"pu_s": active * steps / el, "counts_start": list(c0[:2]),
+                "counts_end": list(c1[:2]), "device_errors": errs,
+                "note": "pu_s = mean of the start and end active counts (low + high) x steps / t"})
+    if blended:
+        import numpy as np
+        m0 = float(np.float32(sc.config.mass))
+        eff = m0 * t.download(0, FIELD_BLEND).astype(np.float64).sum() + \
+            m0 / 8 * t.download(1, FIELD_BLEND).astype(np.float64).sum()
+        out["effective_mass_ratio"] = eff / (m0 * sc.n)
+    t.close()
+    return out
 
 
 def main() -> int:
@@ -330,6 +346,7 @@ def main() -> int:
         out["value_fwd_adjoint"] = out["adjoint"]["fwd_adjoint_pu_s"]
         out["batched"] = bench_batched(torch, args.batched_steps)
         out["two_scale"] = bench_two_scale(torch, args.two_scale_steps)
+        out["two_scale_blended"] = bench_two_scale(torch, args.two_scale_steps, blended=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
