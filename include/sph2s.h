@@ -71,19 +71,23 @@ typedef struct sph_config {
     float    domain_hi[3];
     sph_grid grid;           /* hashing grid (Solver.h:60)               */
     uint32_t flags;          /* SPH_FLAG_*                               */
+    float    speed_sound;    /* SpeedSound: boundary-particle walls only */
+    float    boundary_mass;  /* BoundaryParticleMass (Solver.h:38: = m)  */
 } sph_config;
 
 /* Fill a configuration from explicit physical constants; the grid is
  * MakeGrid(domain_lo, domain_hi, h); gravity (0,-9.81,0), tension 0,
- * restitution 0.5, flags WALL_CLAMP|KAHAN. */
+ * restitution 0.5, flags WALL_CLAMP|KAHAN, speed of sound 88.1472 (the
+ * reference scenes' value, SPH2S_FINAL_SINGLE_SCALE/main.cpp:243) and
+ * boundary mass = mass. */
 int sph_config_init(sph_config* out, float h, float mass, float rest_density,
                     float stiffness, float viscosity, const float domain_lo[3],
                     const float domain_hi[3]);
 
 /* SolverConfiguration::MakeConfiguration (SPH2S_FINAL_SINGLE_SCALE/Solver.h:15-48):
  * h = (0.75 V avg / (pi N))^(1/3), mass = rho0 V / N, grid over [origin,end]
- * at spacing h.  speed_sound only matters for boundary particles and is
- * ignored here. */
+ * at spacing h, boundary mass = mass; speed_sound is kept for the
+ * boundary-particle walls (sph_set_boundary). */
 int sph_make_configuration(const float origin[3], const float end[3],
                            float volume, uint32_t num_particles,
                            uint32_t avg_particles, float rest_density,
@@ -125,6 +129,21 @@ int sph_synchronize(sph_solver* s);
 /* Replace the particle state from host arrays (ids restart at 0..n-1). */
 int sph_set_state(sph_solver* s, const float* pos_xyz, const float* vel_xyz,
                   uint32_t n);
+
+/* Boundary particles: the boundaryData of Solver(fluid, boundary, cfg).
+ * The reference hashes and sorts these static wall particles once on the
+ * fluid grid in its constructor (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:
+ * 1003-1059).  From then on, every step adds each fluid particle's boundary
+ * acceleration to its acceleration (a_i = f_i / rho_i + b_i + g, :687-710):
+ *   b_i = sum_{j : r_ij < h} B(r_ij) m_b / (m + m_b) x_ij / r_ij
+ *   B(r) = 0.02 c^2 / r * (2/3 if q < 2/3; 2q - 3/2 q^2 if q < 1;
+ *          1/2 (2 - q)^2 if q < 2),  q = 2 r / h
+ * (evaluateBoundaryWeight, computeBoundaryForceCell, :169-196, :311-344).
+ * Here c = cfg.speed_sound and m_b = cfg.boundary_mass.  The reference divides
+ * by zero at r = 0; here such a pair is skipped.  pos_xyz holds AoS 3n
+ * floats.  n = 0 removes the walls.  The wall clamp of cfg.flags still
+ * applies when it is set. */
+int sph_set_boundary(sph_solver* s, const float* pos_xyz, uint32_t n);
 /* ParticleData::NumParticles. */
 int sph_num_particles(const sph_solver* s, uint32_t* n);
 int sph_get_config(const sph_solver* s, sph_config* out);
@@ -239,6 +258,11 @@ int sph2_advance(sph_two_scale* t, float dt, uint32_t n_steps);
 int sph2_advance_eager(sph_two_scale* t, float dt, uint32_t n_steps);
 /* Runs only the split/merge/compaction stage on the current state. */
 int sph2_apply_split_merge(sph_two_scale* t);
+/* Boundary particles of the two-scale solver: sorted once on the
+ * low-resolution grid.  Both sets get the boundary acceleration of
+ * sph_set_boundary with h0 and m0: the reference passes LOW_RES for both
+ * (SPH2S_FINAL_TWO_SCALE/Solver.cu:454-499, :1007-1044). */
+int sph2_set_boundary(sph_two_scale* t, const float* pos_xyz, uint32_t n);
 int sph2_synchronize(sph_two_scale* t);
 int sph2_set_debug(sph_two_scale* t, uint32_t flags);
 /* counts[0..1] = current (low, high) counts; counts[2..3] = the counts the

@@ -11,10 +11,10 @@
 //
 // Failures throw std::runtime_error with the library's message, like
 // CUDA::SafeCall (SPH2S_FINAL_TWO_SCALE/cuda.h:115-129).  Differences from the
-// reference: there is no GL interop (Map/Unmap are no-ops), dPositions points
-// at float4 (x, y, z, id) records (sph_export_positions_aos gives the packed
-// 3-float layout), and boundary particles are replaced by the configuration's
-// wall clamp.
+// reference: there is no GL interop (Map/Unmap are no-ops), and dPositions
+// points at float4 (x, y, z, id) records (sph_export_positions_aos gives the
+// packed 3-float layout).  Boundary particles (CreateParticleBoxCanvas, Union)
+// become the solver's static walls (sph_set_boundary).
 #pragma once
 
 #include <cmath>
@@ -90,6 +90,26 @@ public:
         return p;
     }
 
+    // The shell of `border` lattice layers around the grid's box, x fastest
+    // (ParticleData.cpp:178-209).
+    static ParticleData* CreateParticleBoxCanvas(const Grid& g, int border) {
+        const long nx = g.dims[0], ny = g.dims[1], nz = g.dims[2], b = border;
+        const unsigned int total = static_cast<unsigned int>((nx + 2 * b) * (ny + 2 * b) * (nz + 2 * b) -
+                                                             nx * ny * nz);
+        auto* p = new ParticleData(total);
+        p->hostPositions.reserve(3 * static_cast<size_t>(total));
+        for (long k = -b; k < nz + b; ++k)
+            for (long j = -b; j < ny + b; ++j)
+                for (long i = -b; i < nx + b; ++i) {
+                    if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) continue;
+                    p->hostPositions.push_back(g.origin[0] + i * g.spacing);
+                    p->hostPositions.push_back(g.origin[1] + j * g.spacing);
+                    p->hostPositions.push_back(g.origin[2] + k * g.spacing);
+                }
+        p->NumParticles = total;
+        return p;
+    }
+
     static ParticleData* FromArrays(const std::vector<float>& pos_xyz,
                                     const std::vector<float>& vel_xyz, unsigned int maxParticles) {
         auto* p = new ParticleData(maxParticles);
@@ -121,12 +141,18 @@ public:
     Solver(ParticleData* fluidData, ParticleData* boundaryData,
            const SolverConfiguration* configuration)
         : mFluid(fluidData) {
-        if (boundaryData && boundaryData->NumParticles)
-            throw std::runtime_error("sph2s: boundary particles are not supported (wall clamp)");
         const float* vel = fluidData->hostVelocities.empty() ? nullptr
                                                              : fluidData->hostVelocities.data();
         SafeCall(sph_create(configuration, fluidData->hostPositions.data(), vel,
                             fluidData->NumParticles, fluidData->MaxParticles, &mHandle));
+        if (boundaryData && boundaryData->NumParticles) {
+            const int rc = sph_set_boundary(mHandle, boundaryData->hostPositions.data(),
+                                            boundaryData->NumParticles);
+            if (rc != SPH_OK) {
+                sph_destroy(mHandle);
+                SafeCall(rc);
+            }
+        }
         refresh();
     }
     ~Solver() { sph_destroy(mHandle); }

@@ -32,6 +32,18 @@ class OracleParams(Structure):
     ]
 
 
+class OracleWalls(Structure):
+    _fields_ = [("pos", c_void_p), ("n", c_uint32), ("mass", c_double),
+                ("speed_sound", c_double)]
+
+
+def walls(pos, mass: float, speed_sound: float):
+    """(OracleWalls, keep-alive array) for boundary particle positions (N, 3)."""
+    arr = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    return OracleWalls(arr.ctypes.data_as(c_void_p), arr.shape[0], float(mass),
+                       float(speed_sound)), arr
+
+
 class OracleTsOptions(Structure):
     _fields_ = [("coupling", c_uint32), ("split_offsets", c_uint32), ("merge", c_uint32),
                 ("blend_increment", c_float), ("velocity_cap", c_float)]
@@ -58,6 +70,13 @@ def lib() -> ctypes.CDLL:
         L.oracle_cell_ranges.argtypes = [vp, c_uint32, c_uint32, vp, vp]
         L.oracle_step.argtypes = [P, vp, vp, c_uint32, c_double, vp, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_rollout.argtypes = [P, vp, vp, c_uint32, c_double, c_uint32, vp, vp, vp]
+        W = POINTER(OracleWalls)
+        L.oracle_step_walls.argtypes = [P, W, vp, vp, c_uint32, c_double, vp, vp, vp, vp, vp, vp,
+                                        vp, vp]
+        L.oracle_rollout_walls.argtypes = [P, W, vp, vp, c_uint32, c_double, c_uint32, vp, vp,
+                                           vp]
+        L.oracle_boundary_weight.argtypes = [c_double, c_double, c_double]
+        L.oracle_boundary_weight.restype = c_double
         L.oracle_rollout_loss_grad.argtypes = [P, vp, vp, c_uint32, c_double, c_uint32, vp, vp,
                                                vp, vp, vp, vp]
         L.oracle_rollout_loss_grad.restype = c_double
@@ -129,7 +148,8 @@ def cell_ranges(sorted_keys: np.ndarray, cells: int):
     return st, en
 
 
-def step(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float) -> dict:
+def step(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, wall=None) -> dict:
+    """One FP64 step; ``wall`` = walls(...)[0] adds boundary particles."""
     pos = np.ascontiguousarray(pos, np.float32)
     vel = np.ascontiguousarray(vel, np.float32)
     n = pos.shape[0]
@@ -137,22 +157,29 @@ def step(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float) -> dict:
     out = dict(hash=np.empty(n, np.uint32), perm=np.empty(n, np.uint32),
                cell_start=np.empty(C, np.uint32), cell_end=np.empty(C, np.uint32),
                rho=np.empty(n), acc=np.empty((n, 3)), pos=np.empty((n, 3)), vel=np.empty((n, 3)))
-    lib().oracle_step(ctypes.byref(p), _p(pos), _p(vel), n, float(dt), _p(out["hash"]),
-                      _p(out["perm"]), _p(out["cell_start"]), _p(out["cell_end"]), _p(out["rho"]),
-                      _p(out["acc"]), _p(out["pos"]), _p(out["vel"]))
+    lib().oracle_step_walls(ctypes.byref(p), None if wall is None else ctypes.byref(wall),
+                            _p(pos), _p(vel), n, float(dt), _p(out["hash"]), _p(out["perm"]),
+                            _p(out["cell_start"]), _p(out["cell_end"]), _p(out["rho"]),
+                            _p(out["acc"]), _p(out["pos"]), _p(out["vel"]))
     return out
 
 
-def rollout(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, steps: int):
+def rollout(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, steps: int,
+            wall=None):
     pos = np.ascontiguousarray(pos, np.float32)
     vel = np.ascontiguousarray(vel, np.float32)
     n = pos.shape[0]
     x = np.empty((n, 3))
     v = np.empty((n, 3))
     ids = np.empty(n, np.uint32)
-    lib().oracle_rollout(ctypes.byref(p), _p(pos), _p(vel), n, float(dt), int(steps), _p(x),
-                         _p(v), _p(ids))
+    lib().oracle_rollout_walls(ctypes.byref(p), None if wall is None else ctypes.byref(wall),
+                               _p(pos), _p(vel), n, float(dt), int(steps), _p(x), _p(v),
+                               _p(ids))
     return x, v, ids
+
+
+def boundary_weight(r: float, h: float, speed_sound: float) -> float:
+    return lib().oracle_boundary_weight(float(r), float(h), float(speed_sound))
 
 
 def rollout_loss_grad(p: OracleParams, pos0, vel0, dt: float, steps: int, target,
@@ -266,6 +293,7 @@ class OracleTwoScale:
             L.oracle_ts_set_options.argtypes = [vp, POINTER(OracleTsOptions)]
             L.oracle_ts_set_blend.argtypes = [vp, c_int, vp, vp]
             L.oracle_ts_mark_split.argtypes = [vp, vp, c_uint32]
+            L.oracle_ts_set_walls.argtypes = [vp, POINTER(OracleWalls)]
             L._ts_bound = True
         g = OracleGrid()
         for d in range(3):
@@ -306,6 +334,10 @@ class OracleTwoScale:
         st = None if state is None else np.ascontiguousarray(state, np.uint32)
         lib().oracle_ts_set_blend(self._h, int(res), None if la is None else _p(la),
                                   None if st is None else _p(st))
+
+    def set_walls(self, pos, mass: float, speed_sound: float):
+        w, keep = walls(pos, mass, speed_sound)
+        lib().oracle_ts_set_walls(self._h, ctypes.byref(w) if len(keep) else None)
 
     def mark_split(self, ids):
         ids = np.ascontiguousarray(ids, np.uint32)

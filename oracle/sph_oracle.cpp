@@ -222,6 +222,68 @@ void integrate(const oracle_params& p, State& s, const std::vector<double>& acc,
     }
 }
 
+// evaluateBoundaryWeight (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:169-196).
+double boundary_weight(double r, double h, double c) {
+    const double q = 2.0 * r / h;
+    const double coeff = 0.02 * c * c / r;
+    if (q < 2.0 / 3.0) return coeff * 2.0 / 3.0;
+    if (q < 1.0) return coeff * (2.0 * q - 1.5 * q * q);
+    if (q < 2.0) return coeff * 0.5 * (2.0 - q) * (2.0 - q);
+    return 0.0;
+}
+
+// Boundary particles sorted once on a grid, as the reference's constructor
+// does (Solver.cu:1003-1059: hash, stable sort by hash, cell ranges).
+struct WallSet {
+    std::vector<double> x;
+    Neighbourhood nb;
+    double mass = 0.0, speed_sound = 0.0;
+    uint32_t n = 0;
+};
+
+WallSet make_walls(const oracle_params& p, const oracle_walls& w) {
+    WallSet W;
+    W.n = w.n;
+    W.mass = w.mass;
+    W.speed_sound = w.speed_sound;
+    State s;
+    s.x.resize(3 * static_cast<size_t>(w.n));
+    s.v.assign(3 * static_cast<size_t>(w.n), 0.0);
+    s.id.resize(w.n);
+    for (size_t i = 0; i < 3 * static_cast<size_t>(w.n); ++i) s.x[i] = w.pos[i];
+    std::iota(s.id.begin(), s.id.end(), 0u);
+    build_neighbourhood(p, s, W.nb);
+    W.x = std::move(s.x);
+    return W;
+}
+
+// computeBoundaryForceCell over the stencil of each particle
+// (Solver.cu:311-344, 687-706): acc_i += B(r) m_b / (m + m_b) x_ij / r for
+// 0 < r < h.  p gives the grid, h and the fluid mass m.
+void add_wall_accel(const oracle_params& p, const WallSet& W, const std::vector<double>& x,
+                    std::vector<double>& acc) {
+    if (W.n == 0) return;
+    const size_t n = x.size() / 3;
+    const double ratio = W.mass / (p.mass + W.mass);
+    for (size_t i = 0; i < n; ++i) {
+        const double* xi = &x[3 * i];
+        double b[3] = {0.0, 0.0, 0.0};
+        for_each_stencil_cell(p, xi, [&](uint32_t c) {
+            const uint32_t s = W.nb.start[c], e = W.nb.end[c];
+            if (s == kEmpty) return;
+            for (uint32_t j = s; j < e; ++j) {
+                const double d[3] = {xi[0] - W.x[3 * j], xi[1] - W.x[3 * j + 1],
+                                     xi[2] - W.x[3 * j + 2]};
+                const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (!(r < p.h && r > 0.0)) continue;
+                const double wgt = boundary_weight(r, p.h, W.speed_sound) * ratio;
+                for (int k = 0; k < 3; ++k) b[k] += wgt * d[k] / r;
+            }
+        });
+        for (int k = 0; k < 3; ++k) acc[3 * i + k] += b[k];
+    }
+}
+
 State make_state(const float* pos, const float* vel, uint32_t n) {
     State s;
     s.x.resize(3 * static_cast<size_t>(n));
@@ -268,16 +330,29 @@ void oracle_cell_ranges(const uint32_t* k, uint32_t n, uint32_t C, uint32_t* sta
     }
 }
 
+double oracle_boundary_weight(double r, double h, double speed_sound) {
+    return boundary_weight(r, h, speed_sound);
+}
+
 void oracle_step(const oracle_params* p, const float* pos, const float* vel, uint32_t n,
                  double dt, uint32_t* hash_sorted, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* cell_end, double* rho, double* acc, double* pos_out,
                  double* vel_out) {
+    oracle_step_walls(p, nullptr, pos, vel, n, dt, hash_sorted, perm, cell_start, cell_end, rho,
+                      acc, pos_out, vel_out);
+}
+
+void oracle_step_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
+                       const float* vel, uint32_t n, double dt, uint32_t* hash_sorted,
+                       uint32_t* perm, uint32_t* cell_start, uint32_t* cell_end, double* rho,
+                       double* acc, double* pos_out, double* vel_out) {
     State s = make_state(pos, vel, n);
     Neighbourhood nb;
     build_neighbourhood(*p, s, nb);
     std::vector<double> r, a;
     densities(*p, s, nb, r);
     accelerations(*p, s, nb, r, a);
+    if (w && w->n) add_wall_accel(*p, make_walls(*p, *w), s.x, a);
     integrate(*p, s, a, dt);
     const uint32_t C = num_cells(p->grid);
     if (hash_sorted) std::copy(nb.hash_sorted.begin(), nb.hash_sorted.end(), hash_sorted);
@@ -293,13 +368,22 @@ void oracle_step(const oracle_params* p, const float* pos, const float* vel, uin
 void oracle_rollout(const oracle_params* p, const float* pos, const float* vel,
                     uint32_t n, double dt, uint32_t n_steps, double* pos_out,
                     double* vel_out, uint32_t* ids_out) {
+    oracle_rollout_walls(p, nullptr, pos, vel, n, dt, n_steps, pos_out, vel_out, ids_out);
+}
+
+void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
+                          const float* vel, uint32_t n, double dt, uint32_t n_steps,
+                          double* pos_out, double* vel_out, uint32_t* ids_out) {
     State s = make_state(pos, vel, n);
     Neighbourhood nb;
     std::vector<double> r, a;
+    WallSet W;
+    if (w && w->n) W = make_walls(*p, *w);
     for (uint32_t t = 0; t < n_steps; ++t) {
         build_neighbourhood(*p, s, nb);
         densities(*p, s, nb, r);
         accelerations(*p, s, nb, r, a);
+        add_wall_accel(*p, W, s.x, a);
         integrate(*p, s, a, dt);
     }
     if (pos_out) std::copy(s.x.begin(), s.x.end(), pos_out);
@@ -644,6 +728,7 @@ struct oracle_ts {
     uint32_t max_low = 0;
     oracle_ts_options opt{0u, 0u, 1u, 0.0125f, 0.0f};
     std::vector<double> split_rho;   // low densities the split offsets use (sorted order)
+    WallSet walls;                   // boundary particles on the low grid
 };
 
 namespace {
@@ -941,6 +1026,10 @@ void oracle_ts_set_blend(oracle_ts* t, int res, const float* lambda, const uint3
     }
 }
 
+void oracle_ts_set_walls(oracle_ts* t, const oracle_walls* w) {
+    t->walls = (w && w->n) ? make_walls(t->set[0].p, *w) : WallSet{};
+}
+
 void oracle_ts_mark_split(oracle_ts* t, const uint32_t* ids, uint32_t n) {
     TsSet& L = t->set[0];
     std::vector<uint32_t> want(ids, ids + n);
@@ -959,6 +1048,8 @@ void oracle_ts_step(oracle_ts* t, double dt) {
         ts_forces_blended(t);
     else
         ts_forces_fixed(t);
+    // boundary particles: low grid, h0 and m0 for both sets
+    for (int r = 0; r < 2; ++r) add_wall_accel(L.p, t->walls, t->set[r].x, t->set[r].acc);
     for (int r = 0; r < 2; ++r) {
         TsSet& A = t->set[r];
         // 0.8f * h / dt of Solver.cu:1214-1217, rounded as the FP32 step does.

@@ -73,6 +73,31 @@ void oracle_rollout(const oracle_params* p, const float* pos, const float* vel,
                     uint32_t n, double dt, uint32_t n_steps, double* pos_out,
                     double* vel_out, uint32_t* ids_out);
 
+/* Boundary particles (static walls).  Each fluid particle's acceleration gains
+ *   b_i = sum_{j : 0 < r_ij < h} B(r_ij) m_b / (m + m_b) x_ij / r_ij
+ * over the wall particles found through the fluid grid's stencil
+ * (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:311-344, 687-710; walls sorted once,
+ * :1003-1059).  The reference divides by zero at r = 0; such pairs are
+ * skipped. */
+typedef struct oracle_walls {
+    const float* pos;       /* AoS 3n FP32 wall positions */
+    uint32_t n;
+    double mass;            /* boundary particle mass m_b */
+    double speed_sound;     /* c */
+} oracle_walls;
+/* B(r) of evaluateBoundaryWeight (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:169-196)
+ * in FP64: 0.02 c^2 / r times 2/3 (q < 2/3), 2q - 3/2 q^2 (q < 1),
+ * 1/2 (2 - q)^2 (q < 2), else 0, with q = 2 r / h. */
+double oracle_boundary_weight(double r, double h, double speed_sound);
+/* oracle_step / oracle_rollout with boundary particles (w may be NULL). */
+void oracle_step_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
+                       const float* vel, uint32_t n, double dt, uint32_t* hash_sorted,
+                       uint32_t* perm, uint32_t* cell_start, uint32_t* cell_end, double* rho,
+                       double* acc, double* pos_out, double* vel_out);
+void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
+                          const float* vel, uint32_t n, double dt, uint32_t n_steps,
+                          double* pos_out, double* vel_out, uint32_t* ids_out);
+
 /* O(n^2) density and acceleration over all pairs (no grid).  Pins neighbour
  * completeness of the grid path. */
 void oracle_density_bruteforce(const oracle_params* p, const double* pos,
@@ -148,6 +173,10 @@ void oracle_ts_set_blend(oracle_ts* t, int res, const float* lambda, const uint3
 /* Low particles whose id is listed and whose state is 0 get state 3 (split
  * requested), the reference's external mark of TWO_SCALE/Solver.cu:1077-1082. */
 void oracle_ts_mark_split(oracle_ts* t, const uint32_t* ids, uint32_t n);
+/* Boundary particles of the two-scale step: sorted on the low grid, and both
+ * sets get the boundary acceleration with h0 and m0
+ * (SPH2S_FINAL_TWO_SCALE/Solver.cu:454-499, 1007-1044).  w may be NULL. */
+void oracle_ts_set_walls(oracle_ts* t, const oracle_walls* w);
 
 #ifdef __cplusplus
 }

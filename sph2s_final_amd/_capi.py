@@ -75,6 +75,7 @@ class SphConfig(Structure):
         ("gravity", c_float * 3), ("restitution", c_float),
         ("domain_lo", c_float * 3), ("domain_hi", c_float * 3),
         ("grid", SphGrid), ("flags", c_uint32),
+        ("speed_sound", c_float), ("boundary_mass", c_float),
     ]
 
 
@@ -111,6 +112,7 @@ SIGNATURES = [
     ("sph_advance_eager", c_int, [c_void_p, c_float, c_uint32]),
     ("sph_synchronize", c_int, [c_void_p]),
     ("sph_set_state", c_int, [c_void_p, c_void_p, c_void_p, c_uint32]),
+    ("sph_set_boundary", c_int, [c_void_p, c_void_p, c_uint32]),
     ("sph_num_particles", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph_get_config", c_int, [c_void_p, POINTER(SphConfig)]),
     ("sph_get_views", c_int, [c_void_p, POINTER(SphViews)]),
@@ -136,6 +138,7 @@ SIGNATURES = [
     ("sph2_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
     ("sph2_download", c_int, [c_void_p, c_int, c_int, c_void_p, c_size_t]),
     ("sph2_set_options", c_int, [c_void_p, POINTER(Sph2Options)]),
+    ("sph2_set_boundary", c_int, [c_void_p, c_void_p, c_uint32]),
     ("sph2_get_options", c_int, [c_void_p, POINTER(Sph2Options)]),
     ("sph2_set_blend", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_uint32]),
     ("sph2_mark_split", c_int, [c_void_p, c_void_p, c_uint32]),

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "neighbors.hpp"
+#include "walls.hpp"
 
 namespace sph {
 
@@ -94,6 +95,8 @@ struct sph_solver {
     bool graph_accel = false;
     uint32_t debug = 0;
     uint32_t tune = 0;           // launch variant of the neighbour passes
+    Walls walls{};               // boundary particles (none by default)
+    float4* bacc = nullptr;      // boundary acceleration per sorted slot
 };
 
 namespace {
@@ -252,6 +255,8 @@ int sph_config_init(sph_config* out, float h, float mass, float rest_density, fl
         out->domain_hi[d] = hi[d];
     }
     out->flags = SPH_FLAG_WALL_CLAMP | SPH_FLAG_KAHAN;
+    out->speed_sound = 88.1472f;
+    out->boundary_mass = mass;
     int rc = sph_make_grid(lo, hi, h, &out->grid);
     if (rc != SPH_OK) return rc;
     return validate_config(out);
@@ -261,7 +266,6 @@ int sph_make_configuration(const float origin[3], const float end[3], float volu
                            uint32_t num_particles, uint32_t avg_particles, float rest_density,
                            float bulk_modulus, float viscosity, float speed_sound, float tension,
                            sph_config* out) {
-    (void)speed_sound;
     if (!out || num_particles == 0)
         return set_error(SPH_ERR_INVALID_ARGUMENT, "bad MakeConfiguration arguments");
     // Solver.h:31-37: powf(0.75 V avg/(M_PI N), 1/3) with the double M_PI.
@@ -273,6 +277,8 @@ int sph_make_configuration(const float origin[3], const float end[3], float volu
     if (rc != SPH_OK) return rc;
     out->tension = tension;
     out->flags = SPH_FLAG_KAHAN;  // the reference has no wall clamp
+    out->speed_sound = speed_sound;
+    out->boundary_mass = mass;    // Solver.h:38
     return SPH_OK;
 }
 
@@ -404,9 +410,26 @@ int sph_destroy(sph_solver* s) {
     if (!s) return SPH_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     destroy_graph(s);
+    walls_destroy(&s->walls);
+    if (s->bacc) (void)hipFree(s->bacc);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
+    return SPH_OK;
+}
+
+int sph_set_boundary(sph_solver* s, const float* pos, uint32_t n) {
+    if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    SPH_HIP(hipStreamSynchronize(s->stream));
+    destroy_graph(s);
+    s->buf.walls = nullptr;
+    s->buf.bacc = nullptr;
+    const int rc = walls_create(&s->walls, s->cfg.grid, s->cfg.h, s->cfg.boundary_mass,
+                                s->cfg.mass, s->cfg.speed_sound, pos, n, s->stream);
+    if (rc != SPH_OK || n == 0) return rc;
+    if (!s->bacc) SPH_HIP(hipMalloc(&s->bacc, 16ull * s->max_n));
+    s->buf.walls = &s->walls;
+    s->buf.bacc = s->bacc;
     return SPH_OK;
 }
 

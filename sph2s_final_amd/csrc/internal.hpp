@@ -102,6 +102,8 @@ int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint3
                uint32_t key_bits, void* temp, hipStream_t stream);
 
 // ---- single-scale step kernels (step.hip) ----
+struct Walls;  // boundary-particle walls (walls.hpp)
+
 struct StepBuffers {
     float4* pos;        // state, sorted order of the previous step (w = id bits)
     float4* vel;
@@ -127,6 +129,10 @@ struct StepBuffers {
     float2* iaux;       // aux in the internal order (density -> force)
     uint32_t* i2r;      // internal slot -> sorted slot
     uint4* xbins;       // per cell: cumulative counts of its 8 x-bins (16 bits each)
+    // Boundary-particle walls (NULL: none); bacc holds each sorted slot's
+    // boundary acceleration, computed before the force pass adds it.
+    const Walls* walls;
+    float4* bacc;
 };
 
 // Passes of one step, in launch order.

@@ -16,6 +16,7 @@
 // of the grid the cells cx are consecutive hashes, so a stencil row of up to
 // three cells is one contiguous run of sorted particles.
 #include "neighbors.hpp"
+#include "walls.hpp"
 
 namespace sph {
 namespace {
@@ -202,6 +203,7 @@ struct XOrder {
     const uint32_t* i2r;
     const uint4* bins;
     float2* iaux;  // density: internal-order copy of aux for the force pass
+    const float4* bacc;  // force: boundary acceleration per sorted slot (or NULL)
 };
 constexpr uint32_t kNumCus = 256;
 
@@ -540,6 +542,12 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
         az -= tc * Tz;
     }
     const uint32_t o = kX ? xo.i2r[s] : s;
+    if (xo.bacc) {  // a_i = f_i / rho_i + b_i + g (Solver.cu:708-710)
+        const float4 b = xo.bacc[o];
+        ax += b.x;
+        ay += b.y;
+        az += b.z;
+    }
     if (!kIntegrate) {
         accel[o] = make_float4(ax, ay, az, 0.0f);
         return;
@@ -710,7 +718,12 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
-            const XOrder xo{b.i2r, b.xbins, nullptr};
+            const bool walls = b.walls && b.bacc;
+            if (walls) {
+                const int rc = launch_wall_accel(*b.walls, b.tpos, b.n_dev, p.n, b.bacc, stream);
+                if (rc != SPH_OK) return rc;
+            }
+            const XOrder xo{b.i2r, b.xbins, nullptr, walls ? b.bacc : nullptr};
             if (p.tens != 0.0f)
                 launch_force<true, true, false>(p.tune, p.n, stream, b.ipos, b.ivel, b.iaux,
                                                 b.cell_start, b.cell_end, p, g, b.n_dev, dt, b.pos,

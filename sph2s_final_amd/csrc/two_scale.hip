@@ -37,6 +37,7 @@
 
 #include "neighbors.hpp"
 #include "pairwalk.hpp"
+#include "walls.hpp"
 
 namespace sph {
 namespace {
@@ -162,8 +163,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
     const float4* __restrict__ tvel_b, const float2* __restrict__ aux_b,
     const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
     const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, const uint32_t* __restrict__ occ,
-    GridView ga, float dt, float4* __restrict__ pos_a, float4* __restrict__ vel_a,
-    float4* __restrict__ acc_out) {
+    GridView ga, const float4* __restrict__ bacc, float dt, float4* __restrict__ pos_a,
+    float4* __restrict__ vel_a, float4* __restrict__ acc_out) {
     const uint32_t na = *n_a, nb = *n_b;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= na) return;
@@ -217,6 +218,12 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
         ax -= tc * Tx;
         ay -= tc * Ty;
         az -= tc * Tz;
+    }
+    if (bacc) {  // boundary particles (Solver.cu:1046-1048)
+        const float4 b = bacc[s];
+        ax += b.x;
+        ay += b.y;
+        az += b.z;
     }
     float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
     cap_velocity(c.vmax, vx, vy, vz);
@@ -324,8 +331,8 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     const float2* __restrict__ aux_b, const float2* __restrict__ tbl_b,
     const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
     const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
-    const uint32_t* __restrict__ occ, GridView g_occ, float dt, float inc,
-    float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
+    const uint32_t* __restrict__ occ, GridView g_occ, const float4* __restrict__ bacc, float dt,
+    float inc, float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
     float4* __restrict__ acc_out) {
     __shared__ float4 tx[kBlendForceTile];
     __shared__ float4 tv[kBlendForceTile];
@@ -444,6 +451,12 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
         ax -= tc * Sx;
         ay -= tc * Sy;
         az -= tc * Sz;
+    }
+    if (bacc) {  // boundary particles (Solver.cu:1046-1048)
+        const float4 b = bacc[s];
+        ax += b.x;
+        ay += b.y;
+        az += b.z;
     }
     float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
     cap_velocity(c.vmax, vx, vy, vz);
@@ -937,6 +950,8 @@ struct sph_two_scale {
     uint32_t* error_word = nullptr;
     uint32_t* occ = nullptr;     // per low-grid cell: 1 when it holds a high particle
     uint32_t* grp_live = nullptr;  // per group: bit 0 live child, bit 1 live low particle (blended)
+    Walls walls{};               // boundary particles, sorted on the low grid
+    float4* bacc[2] = {nullptr, nullptr};  // per set: boundary acceleration per sorted slot
     sph2_options opt{SPH2_COUPLING_FIXED, SPH2_SPLIT_LATTICE, 1u, 0.0125f, 0.0f};
     hipStream_t stream = nullptr;
     void* arena = nullptr;
@@ -989,6 +1004,8 @@ CrossParams cross_params(const sph_two_scale* t, int target) {
 }
 
 bool blended(const sph_two_scale* t) { return t->opt.coupling == SPH2_COUPLING_BLENDED; }
+
+const float4* wall_acc(const sph_two_scale* t, int r) { return t->walls.n ? t->bacc[r] : nullptr; }
 
 // c h_A / dt in FP32, as integrateD computes 0.8f * h / dt (Solver.cu:1214-1217).
 float velocity_limit(const sph_two_scale* t, int target, float dt) {
@@ -1066,15 +1083,15 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
                                st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
                                A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
                                B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                               X[1 - r], c, occ, X[0], dt, inc, A.buf.pos, A.buf.vel, A.nblend,
-                               acc);
+                               X[1 - r], c, occ, X[0], wall_acc(t, r), dt, inc, A.buf.pos,
+                               A.buf.vel, A.nblend, acc);
         else
             hipLaunchKernelGGL(k_blend_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
                                st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
                                A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
                                B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                               X[1 - r], c, occ, X[0], dt, inc, A.buf.pos, A.buf.vel, A.nblend,
-                               acc);
+                               X[1 - r], c, occ, X[0], wall_acc(t, r), dt, inc, A.buf.pos,
+                               A.buf.vel, A.nblend, acc);
     }
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
@@ -1116,6 +1133,13 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[1].cap), dim3(kBlock), 0, st,
                            t->set[1].buf.tpos, t->set[1].buf.n_dev, G[0], t->occ);
     }
+    // Boundary accelerations of both sets (walls on the low grid).
+    if (t->walls.n)
+        for (int r = 1; r >= 0; --r) {
+            const int rc = launch_wall_accel(t->walls, t->set[r].buf.tpos, t->set[r].buf.n_dev,
+                                             t->set[r].cap, t->bacc[r], st);
+            if (rc != SPH_OK) return rc;
+        }
     if (blended(t)) return launch_blended_passes(t, P, G, X, dt, accel_out);
     // Own-set densities, then the complementary sums (which also finalise A, B).
     for (int r = 1; r >= 0; --r) {
@@ -1147,13 +1171,14 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
             hipLaunchKernelGGL(k_cross_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0, st,
                                A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev, B.buf.tpos,
                                B.buf.tvel, B.buf.aux, B.buf.cell_start, B.buf.cell_end,
-                               B.buf.n_dev, X[1 - r], c, occ, X[0], dt, A.buf.pos, A.buf.vel, acc);
+                               B.buf.n_dev, X[1 - r], c, occ, X[0], wall_acc(t, r), dt, A.buf.pos,
+                               A.buf.vel, acc);
         else
             hipLaunchKernelGGL(k_cross_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
                                st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev,
                                B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
-                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, occ, X[0], dt, A.buf.pos,
-                               A.buf.vel, acc);
+                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, occ, X[0],
+                               wall_acc(t, r), dt, A.buf.pos, A.buf.vel, acc);
     }
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
@@ -1598,10 +1623,26 @@ int sph2_download(sph_two_scale* t, int resolution, int field, void* dst, size_t
     return SPH_OK;
 }
 
+int sph2_set_boundary(sph_two_scale* t, const float* pos, uint32_t n) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    destroy_graph2(t);
+    const sph_config& lo = t->set[0].cfg;
+    const int rc = walls_create(&t->walls, lo.grid, lo.h, lo.boundary_mass, lo.mass, lo.speed_sound,
+                                pos, n, t->stream);
+    if (rc != SPH_OK || n == 0) return rc;
+    for (int r = 0; r < 2; ++r)
+        if (!t->bacc[r]) SPH_HIP(hipMalloc(&t->bacc[r], 16ull * t->set[r].cap));
+    return SPH_OK;
+}
+
 int sph2_destroy(sph_two_scale* t) {
     if (!t) return SPH_OK;
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     destroy_graph2(t);
+    walls_destroy(&t->walls);
+    for (int r = 0; r < 2; ++r)
+        if (t->bacc[r]) (void)hipFree(t->bacc[r]);
     if (t->arena) (void)hipFree(t->arena);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;

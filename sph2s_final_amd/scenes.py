@@ -14,6 +14,11 @@ Named workloads:
   two_scale_250k 63^3 = 250,047 low-resolution particles, domain [0, 1.969]^3,
                  seed 3, high-resolution region x in [0.6 L, L], y in [0, 0.5 L],
                  z in [0, L] (20% of the domain volume, the far-wall impact zone)
+  walled_dam_break_215k  the reference's own single-scale scene
+                 (SPH2S_FINAL_SINGLE_SCALE/main.cpp:161-250): a 42 x 95 x 54
+                 fluid block in a tank of boundary particles with a pillar,
+                 h from 35 neighbours, k = 30, mu = 5, sigma = 0.8, c = 88.1472,
+                 no wall clamp
 """
 from __future__ import annotations
 
@@ -88,6 +93,65 @@ def dam_break_32k() -> Scene:
 
 def dam_break_1m() -> Scene:
     return dam_break((100, 100, 100), seed=2, name="dam_break_1m")
+
+
+@dataclass
+class WallScene:
+    """A fluid block and static boundary particles (boundary-particle walls)."""
+    name: str
+    config: C.SphConfig
+    positions: np.ndarray   # (N, 3) float32 fluid
+    velocities: np.ndarray
+    walls: np.ndarray       # (M, 3) float32 boundary particles
+    dt: float = DT
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+
+def walled_dam_break(fluid_lo=(0.04, 0.04, 0.0), fluid_hi=(0.35, 0.75, 0.4),
+                     tank_lo=(0.0, 0.0, -0.1), tank_hi=(1.6, 1.5, 0.5),
+                     pillar=((0.7, 0.0, 0.22), (0.75, 1.5, 0.25)), spacing=0.0075, border=5,
+                     domain_lo=(-0.1, -0.1, -0.2), domain_hi=(1.7, 1.6, 0.6), tension=0.8,
+                     name=None) -> WallScene:
+    """initSim of SPH2S_FINAL_SINGLE_SCALE/main.cpp:161-250: the fluid is
+    CreateParticleBox over [fluid_lo, fluid_hi) at `spacing`; the walls are the
+    Union of CreateParticleBoxCanvas shells (`border` layers) around the tank
+    and around the pillar; the configuration is MakeConfiguration over the
+    domain with the fluid's grid volume, 35 neighbours, rho0 = 1000, k = 30,
+    mu = 5, c = 88.1472 and the tension coefficient.  The defaults are the
+    reference's values; smaller boxes give test-sized scenes."""
+    from .solver import Grid, ParticleData
+    fgrid = Grid.MakeGrid(fluid_lo, fluid_hi, spacing)
+    fluid = ParticleData.CreateParticleBox(fgrid)
+    shells = [ParticleData.CreateParticleBoxCanvas(Grid.MakeGrid(tank_lo, tank_hi, spacing), border)]
+    if pillar is not None:
+        shells.append(ParticleData.CreateParticleBoxCanvas(Grid.MakeGrid(pillar[0], pillar[1],
+                                                                         spacing), border))
+    walls = shells[0]
+    for extra in shells[1:]:
+        walls = ParticleData.Union(walls, extra)
+    volume = Grid.ComputeVolume(fgrid)
+    cfg = SolverConfiguration.MakeConfiguration(domain_lo, domain_hi, volume, fluid.NumParticles,
+                                                35, 1000.0, 30.0, 5.0, 88.1472, tension)
+    pos = fluid.positions
+    return WallScene(name or f"walled_dam_break_{pos.shape[0]}", cfg, pos, np.zeros_like(pos),
+                     walls.positions)
+
+
+def walled_dam_break_215k() -> WallScene:
+    return walled_dam_break(name="walled_dam_break_215k")
+
+
+def walled_tank_small() -> WallScene:
+    """A test-sized walled scene: 8 x 16 x 8 fluid particles in a tank with a
+    pillar, same spacing and constants as the reference scene."""
+    return walled_dam_break(fluid_lo=(0.015, 0.015, 0.015), fluid_hi=(0.075, 0.135, 0.075),
+                            tank_lo=(0.0, 0.0, 0.0), tank_hi=(0.2, 0.2, 0.09),
+                            pillar=((0.12, 0.0, 0.03), (0.135, 0.2, 0.045)), border=3,
+                            domain_lo=(-0.05, -0.05, -0.05), domain_hi=(0.25, 0.25, 0.14),
+                            name="walled_tank_small")
 
 
 @dataclass

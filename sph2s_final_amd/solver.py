@@ -37,6 +37,13 @@ class Grid:
     def ComputeNumGridCells(grid: C.SphGrid) -> int:
         return int(C.lib().sph_grid_num_cells(byref(grid)))
 
+    @staticmethod
+    def ComputeVolume(grid: C.SphGrid) -> float:
+        """Grid.h:48-54: the int cell count times spacing^3, in FP32."""
+        n = np.float32(int(grid.dims[0]) * int(grid.dims[1]) * int(grid.dims[2]))
+        s = np.float32(grid.spacing)
+        return float(np.float32(np.float32(np.float32(n * s) * s) * s))
+
 
 class SolverConfiguration:
     """SolverConfiguration (SPH2S_FINAL_SINGLE_SCALE/Solver.h:13-61)."""
@@ -100,6 +107,25 @@ class ParticleData:
         return ParticleData(pos.shape[0], pos)
 
     @staticmethod
+    def CreateParticleBoxCanvas(grid: C.SphGrid, border: int) -> "ParticleData":
+        """The shell of `border` lattice layers around a grid's box: every point
+        origin + (i, j, k) * spacing with -border <= i < dims + border (and
+        likewise j, k) that lies outside the box itself, x fastest
+        (ParticleData.cpp:178-209).  The reference builds its walls this way."""
+        b = int(border)
+        nx, ny, nz = (int(grid.dims[d]) for d in range(3))
+        k, j, i = np.meshgrid(np.arange(-b, nz + b), np.arange(-b, ny + b), np.arange(-b, nx + b),
+                              indexing="ij")
+        i, j, k = i.ravel(), j.ravel(), k.ravel()
+        shell = ~((i >= 0) & (i < nx) & (j >= 0) & (j < ny) & (k >= 0) & (k < nz))
+        o = np.array([grid.origin[d] for d in range(3)], dtype=np.float32)
+        sp = np.float32(grid.spacing)
+        pos = np.stack([o[0] + i[shell].astype(np.float32) * sp,
+                        o[1] + j[shell].astype(np.float32) * sp,
+                        o[2] + k[shell].astype(np.float32) * sp], axis=1).astype(np.float32)
+        return ParticleData(pos.shape[0], pos)
+
+    @staticmethod
     def Union(a: "ParticleData", b: "ParticleData") -> "ParticleData":
         """Concatenation of two particle sets (ParticleData.cpp:68-139)."""
         if a is None or b is None:
@@ -136,16 +162,15 @@ class ParticleData:
 class Solver:
     """Solver (SPH2S_FINAL_SINGLE_SCALE/Solver.h:109-132) on one MI355X.
 
-    ``boundaryData`` is accepted for signature parity; boundary-particle walls
-    are not part of this path (walls are the configuration's clamp box).
+    ``boundaryData`` holds the static wall particles (boundary-particle walls,
+    sph_set_boundary); it may be None (walls are then only the configuration's
+    clamp box, if its flag is set).
     """
 
     def __init__(self, fluidData: ParticleData, boundaryData=None,
                  configuration: C.SphConfig | None = None):
         if configuration is None:
             raise ValueError("configuration is required")
-        if boundaryData is not None and getattr(boundaryData, "NumParticles", 0):
-            raise NotImplementedError("boundary particles are not supported; use the wall clamp")
         self._h = c_void_p()
         pos = fluidData.positions if fluidData.positions is not None else np.zeros((0, 3), np.float32)
         vel = fluidData.velocities
@@ -157,6 +182,8 @@ class Solver:
         self.configuration = configuration
         self.fluid = fluidData
         fluidData._solver = self
+        if boundaryData is not None and getattr(boundaryData, "NumParticles", 0):
+            self.set_boundary(boundaryData.host_positions())
 
     # -- reference API --
     def Bind(self) -> None:
@@ -166,6 +193,15 @@ class Solver:
         C.check(C.lib().sph_advance(self._h, float(dt), int(steps)))
 
     # -- extras --
+    def set_boundary(self, positions) -> None:
+        """Static boundary particles (N, 3), hashed and sorted once on the fluid
+        grid; every step adds their boundary acceleration.  An empty array
+        removes them."""
+        pos = _f32(positions) if positions is not None and len(positions) else None
+        n = 0 if pos is None else int(pos.shape[0])
+        C.check(C.lib().sph_set_boundary(self._h, None if pos is None else
+                                         pos.ctypes.data_as(c_void_p), n))
+
     def advance_eager(self, dt: float, steps: int = 1) -> None:
         C.check(C.lib().sph_advance_eager(self._h, float(dt), int(steps)))
 
@@ -300,6 +336,14 @@ class TwoScaleSolver:
 
     def apply_split_merge(self) -> None:
         C.check(C.lib().sph2_apply_split_merge(self._h))
+
+    def set_boundary(self, positions) -> None:
+        """Static boundary particles (N, 3), sorted once on the low-resolution
+        grid; both sets get their boundary acceleration (h0, m0)."""
+        pos = _f32(positions) if positions is not None and len(positions) else None
+        n = 0 if pos is None else int(pos.shape[0])
+        C.check(C.lib().sph2_set_boundary(self._h, None if pos is None else
+                                          pos.ctypes.data_as(c_void_p), n))
 
     def synchronize(self) -> None:
         C.check(C.lib().sph2_synchronize(self._h))
