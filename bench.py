@@ -22,6 +22,8 @@ Prints ONE JSON line with:
   batched     64 uneven scenes (log-uniform 4,096..262,144 particles) in one
               batched step vs the same scenes as 64 single-scene solvers (each on
               its own stream, and serialised)
+  walls       the reference's own single-scale scene: 215,460 fluid particles
+              in a tank of 860,390 boundary particles (boundary-particle walls)
   two_scale   the two-scale dam break (63^3 low-resolution particles, 1->8
               split / 8->1 merge region over 20% of the domain), under the
               fixed coupling and under the reference's blended coupling
@@ -191,6 +193,29 @@ def bench_batched(torch, steps: int) -> dict:
                     "launches all 64 per step before waiting, 'serial' waits after each"}
 
 
+def bench_walls(torch, steps: int) -> dict:
+    """The reference's own single-scale scene: 215,460 fluid particles in a
+    tank of 860,390 boundary particles with a pillar (boundary-particle walls,
+    tension 0.8, no wall clamp)."""
+    from sph2s_final_amd import FIELD_POSITION, ParticleData, Solver, scenes
+    sc = scenes.walled_dam_break_215k()
+    s = Solver(ParticleData(sc.n, sc.positions), ParticleData(len(sc.walls), sc.walls), sc.config)
+    s.Advance(sc.dt, 20)
+    s.synchronize()
+    el = _timed(lambda: (s.Advance(sc.dt, steps), s.synchronize()), torch)
+    pass_ms = s.profile_passes(sc.dt, 20) / 20.0
+    x = s.download(FIELD_POSITION)
+    errs = s.device_errors()
+    s.close()
+    import numpy as np
+    return {"workload": sc.name, "particles": sc.n, "walls": int(len(sc.walls)), "steps": steps,
+            "ms_per_step": el / steps * 1e3, "pu_s": sc.n * steps / el,
+            "pass_ms": {k: float(v) for k, v in zip(
+                ["hash_reset", "radix_sort", "reorder_cells", "density", "walls_force_integrate"],
+                pass_ms)},
+            "state_finite": bool(np.all(np.isfinite(x))), "device_errors": errs}
+
+
 def bench_two_scale(torch, steps: int, blended: bool = False) -> dict:
     """Two-scale dam break; blended = the reference's lambda-blended coupling
     (fade timeline with increment 0.0125, velocity cap 0.8 h / dt, r/sqrt(3)
@@ -239,6 +264,7 @@ def main() -> int:
     ap.add_argument("--checkpoint-every", type=int, default=20)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--two-scale-steps", type=int, default=500)
+    ap.add_argument("--walls-steps", type=int, default=500)
     ap.add_argument("--only-main", action="store_true",
                     help="skip the adjoint, batched and two-scale measurements")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -345,6 +371,7 @@ def main() -> int:
         out["adjoint"] = bench_adjoint(torch, args.adjoint_steps, args.checkpoint_every)
         out["value_fwd_adjoint"] = out["adjoint"]["fwd_adjoint_pu_s"]
         out["batched"] = bench_batched(torch, args.batched_steps)
+        out["walls"] = bench_walls(torch, args.walls_steps)
         out["two_scale"] = bench_two_scale(torch, args.two_scale_steps)
         out["two_scale_blended"] = bench_two_scale(torch, args.two_scale_steps, blended=True)
     if not args.no_cpu_baseline:

@@ -138,3 +138,45 @@ def test_particle_box_and_union_mirror_reference():
     assert np.allclose(box.positions[1] - box.positions[0], [0.0075, 0, 0], atol=1e-7)
     u = S.ParticleData.Union(box, box)
     assert u.NumParticles == 2 * box.NumParticles and u.MaxParticles == 2 * box.MaxParticles
+
+
+def test_make_configuration_keeps_wall_constants():
+    cfg = S.SolverConfiguration.MakeConfiguration((-0.1, -0.1, -0.2), (1.7, 1.6, 0.6), 0.05,
+                                                  20000, 35, 1000.0, 30.0, 5.0, 88.1472, 0.8)
+    assert cfg.speed_sound == pytest.approx(88.1472)
+    assert cfg.boundary_mass == cfg.mass          # Solver.h:38
+    d = scenes.dam_break_32k().config
+    assert d.speed_sound == pytest.approx(88.1472) and d.boundary_mass == d.mass
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """The header-only C++ facade builds a reference-style program (fluid box,
+    canvas walls, Solver, Advance) and links against libsph2s.so."""
+    src = tmp_path / "facade.cpp"
+    src.write_text(r'''
+#include "sph2s/Solver.hpp"
+using namespace sph2s;
+int main() {
+    Grid fluidGrid = Grid::MakeGrid(make_float3(0.04f, 0.04f, 0.0f), make_float3(0.1f, 0.1f, 0.05f), 0.0075f);
+    ParticleData* fluid = ParticleData::CreateParticleBox(fluidGrid);
+    ParticleData* a = ParticleData::CreateParticleBoxCanvas(
+        Grid::MakeGrid(make_float3(0.0f, 0.0f, 0.0f), make_float3(0.2f, 0.2f, 0.1f), 0.0075f), 3);
+    ParticleData* b = ParticleData::CreateParticleBoxCanvas(
+        Grid::MakeGrid(make_float3(0.12f, 0.0f, 0.03f), make_float3(0.135f, 0.2f, 0.045f), 0.0075f), 3);
+    ParticleData* walls = ParticleData::Union(a, b);
+    SolverConfiguration c = SolverConfiguration::MakeConfiguration(
+        make_float3(-0.05f, -0.05f, -0.05f), make_float3(0.25f, 0.25f, 0.14f),
+        Grid::ComputeVolume(fluidGrid), fluid->MaxParticles, 35, 1000.0f, 30.0f, 5.0f, 88.1472f, 0.8f);
+    if (sph_device_count() == 0) return 0;
+    Solver s(fluid, walls, &c);
+    s.Advance(0.001f);
+    s.Synchronize();
+    return fluid->NumParticles > 0 ? 0 : 1;
+}
+''')
+    exe = tmp_path / "facade"
+    res = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{os.path.join(ROOT, 'include')}",
+                          str(src), "-o", str(exe), f"-L{os.path.dirname(C.LIB_PATH)}", "-lsph2s",
+                          f"-Wl,-rpath,{os.path.dirname(C.LIB_PATH)}"],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
