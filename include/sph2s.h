@@ -57,6 +57,14 @@ uint32_t sph_grid_num_cells(const sph_grid* grid);
 /* ---- configuration: replaces SolverConfiguration + __constant__ upload ---- */
 #define SPH_FLAG_WALL_CLAMP  1u  /* clamp x to [domain_lo, domain_hi], v *= -e */
 #define SPH_FLAG_KAHAN       2u  /* compensated density/force accumulation   */
+/* XSPH integrator (integrateXSPHD, SPH2S_FINAL_SINGLE_SCALE/Solver.cu:346-395,
+ * 757-853; the reference keeps it commented out, :1185-1196):
+ *   v'_i  = v_i + dt a_i
+ *   v''_i = v'_i + xsph * sum_{j : r_ij < h} (m / rho_j) W(r_ij) (v'_j - v'_i)
+ *   x'_i  = x_i + dt v''_i, then the wall clamp if set.
+ * The reference adds the x component of the correction to all three axes
+ * (:837-839); here each axis gets its own.  Single-scale solver only. */
+#define SPH_FLAG_XSPH        4u
 
 typedef struct sph_config {
     float    h;              /* EffectiveRadius (Solver.h:51)            */
@@ -73,6 +81,7 @@ typedef struct sph_config {
     uint32_t flags;          /* SPH_FLAG_*                               */
     float    speed_sound;    /* SpeedSound: boundary-particle walls only */
     float    boundary_mass;  /* BoundaryParticleMass (Solver.h:38: = m)  */
+    float    xsph;           /* XSPHCoeff epsilon (SPH_FLAG_XSPH)        */
 } sph_config;
 
 /* Fill a configuration from explicit physical constants; the grid is
@@ -87,12 +96,15 @@ int sph_config_init(sph_config* out, float h, float mass, float rest_density,
 /* SolverConfiguration::MakeConfiguration (SPH2S_FINAL_SINGLE_SCALE/Solver.h:15-48):
  * h = (0.75 V avg / (pi N))^(1/3), mass = rho0 V / N, grid over [origin,end]
  * at spacing h, boundary mass = mass; speed_sound is kept for the
- * boundary-particle walls (sph_set_boundary). */
+ * boundary-particle walls (sph_set_boundary).  The XSPH coefficient
+ * (MakeConfiguration's epsilon) is set with sph_config_set_xsph. */
 int sph_make_configuration(const float origin[3], const float end[3],
                            float volume, uint32_t num_particles,
                            uint32_t avg_particles, float rest_density,
                            float bulk_modulus, float viscosity,
                            float speed_sound, float tension, sph_config* out);
+/* cfg->xsph = epsilon and SPH_FLAG_XSPH set (epsilon > 0) or cleared (0). */
+int sph_config_set_xsph(sph_config* cfg, float epsilon);
 
 /* ---- scene generation (replaces ParticleData::CreateParticleBox,
  *      SPH2S_FINAL_SINGLE_SCALE/ParticleData.cpp:141-176) ------------------- */
@@ -181,14 +193,30 @@ int sph_get_views(sph_solver* s, sph_views* out);
 #define SPH_FIELD_TAG           9   /* two-scale high set: (slot << 29) | group */
 #define SPH_FIELD_BLEND        10   /* two-scale, blended coupling: lambda (float) */
 #define SPH_FIELD_STATE        11   /* two-scale, blended coupling: state word (u32) */
+#define SPH_FIELD_COLOR        12   /* colour value (float), SPH_DEBUG_COLOR        */
 int sph_download(sph_solver* s, int field, void* host_dst, size_t bytes);
 
 /* Renderer contract: positions as 3 floats per particle
  * (SPH2S_FINAL_TWO_SCALE/Renderer.cpp:40-42).  dst is a host pointer, or a
  * device pointer when dst_is_device != 0 (then the copy is stream-ordered). */
 int sph_export_positions_aos(sph_solver* s, float* dst, int dst_is_device);
+/* The renderer's buffers (ParticleData::dPositions, dColorValues,
+ * SPH2S_FINAL_TWO_SCALE/ParticleData.h:39-45, drawn by Renderer.cpp:38-46),
+ * copied into caller-owned device memory in the state's order: xyz gets 3n
+ * floats, color n floats (needs SPH_DEBUG_COLOR).  Either may be NULL.
+ * Stream-ordered on the solver's stream (sph_synchronize before reading). */
+int sph_export_render(sph_solver* s, float* xyz, float* color);
 
 #define SPH_DEBUG_ACCEL 1u   /* keep per-particle accelerations in accel4 */
+/* Compute each particle's colour value every step, as the two-scale
+ * reference's force pass does for dColorValues (SPH2S_FINAL_TWO_SCALE/
+ * Solver.cu:206-213, 376-387, 897-907, 1050-1075): over the particle and its
+ * own-set neighbours with 0 < r < h, psi = C (|x_ij/h|^2 - 3) exp(-|x_ij/h|^2/2)
+ * (C = 2 / (sqrt(3) pi^(1/4))), velW = sum psi v_j, psiSum = sum psi, x_c =
+ * mean position; ene = min(|velW|^2 / (psiSum^2 h), 300); colour = 0.3 where
+ * |x_i - x_c| < 0.0025, else ene / 300.  Read it with SPH_FIELD_COLOR or the
+ * render export. */
+#define SPH_DEBUG_COLOR 2u
 int sph_set_debug(sph_solver* s, uint32_t flags);
 
 /* Per-pass timing: runs n_steps steps eagerly with hipEvents recorded on the
@@ -333,10 +361,39 @@ int sph2_get_options(const sph_two_scale* t, sph2_options* options);
 int sph2_set_blend(sph_two_scale* t, int resolution, const float* lambda,
                    const uint32_t* state, uint32_t n);
 /* Request the split of the listed low-resolution particles (by id): settled
- * ones get state 3 and split at the end of the next step.  The reference sets
+ * ones (mode 0, surface bit 8 allowed) get mode 3 and split at the end of the
+ * next step.  The reference sets
  * this state from its screen-space surface pass (SSFRenderer.cu:196-218 and
  * the mark in Solver.cu:1077-1082).  Blended coupling only; n <= max_low. */
 int sph2_mark_split(sph_two_scale* t, const uint32_t* ids, uint32_t n);
+
+/* Renderer buffers of one set (ParticleData::dPositions, dColorValues,
+ * dStates of SPH2S_FINAL_TWO_SCALE/ParticleData.h:39-45) in the current
+ * (compacted) order, copied into caller-owned device memory: xyz 3n floats,
+ * color n floats (SPH_DEBUG_COLOR; children carry their parent's value from
+ * the step that made them, a merged particle its group's slot-0 value),
+ * state n int32 (the blended coupling's state words; 0 low / 4 high under the
+ * fixed coupling).  Any pointer may be NULL.  Stream-ordered on the solver's
+ * stream (sph2_synchronize before reading). */
+int sph2_export_render(sph_two_scale* t, int resolution, float* xyz, float* color,
+                       int32_t* state);
+
+/* Surface marks (blended coupling).  The reference's screen-space pass sets
+ * state bit 8 on the low particles its surface index map shows
+ * (getSurfaceIDs, SPH2S_FINAL_TWO_SCALE/SSFRenderer.cu:196-218) and, in a
+ * line it leaves commented out, requests the split of those in state 8
+ * (Solver.cu:1077-1082).
+ * sph2_mark_surface_map does exactly that from a device index map of count
+ * entries (slot + 1 of the current low set, 0 = background): bit 8, and with
+ * split != 0 mode 3 for particles whose state is 8.
+ * sph2_detect_surface is its view-independent replacement: from the last
+ * step's sorted state, a low particle whose colour-field gradient
+ *   n_i = sum_{L, 0<r<h0} (m0/rho_j) grad W_h0 + sum_{H, 0<r<h0} (m1/rho_j)
+ *         1/2 (grad W_h0 + grad W_h1)     (spiky gradients)
+ * has |n_i| h0 > threshold is at the free surface; it gets the same marks.
+ * Both synchronise. */
+int sph2_mark_surface_map(sph_two_scale* t, const uint32_t* map_dev, uint32_t count, int split);
+int sph2_detect_surface(sph_two_scale* t, float threshold, int split);
 
 /* ---- batched independent scenes --------------------------------------------
  * The reference runs one scene per process through a global __constant__

@@ -29,6 +29,7 @@ class OracleParams(Structure):
         ("gravity", c_double * 3), ("restitution", c_double),
         ("lo", c_double * 3), ("hi", c_double * 3),
         ("wall_clamp", c_int32), ("h_f32", c_float), ("grid", OracleGrid),
+        ("xsph", c_double), ("use_xsph", c_int32),
     ]
 
 
@@ -75,6 +76,8 @@ def lib() -> ctypes.CDLL:
                                         vp, vp]
         L.oracle_rollout_walls.argtypes = [P, W, vp, vp, c_uint32, c_double, c_uint32, vp, vp,
                                            vp]
+        L.oracle_energy_colors.argtypes = [P, vp, vp, c_uint32, vp, vp]
+        L.oracle_surface_indicator.argtypes = [P, vp, vp, c_uint32, vp, vp, c_uint32, vp]
         L.oracle_boundary_weight.argtypes = [c_double, c_double, c_double]
         L.oracle_boundary_weight.restype = c_double
         L.oracle_rollout_loss_grad.argtypes = [P, vp, vp, c_uint32, c_double, c_uint32, vp, vp,
@@ -118,6 +121,8 @@ def params_from_config(cfg) -> OracleParams:
     p.restitution = float(cfg.restitution)
     p.wall_clamp = 1 if (cfg.flags & 1) else 0
     p.h_f32 = cfg.h
+    p.xsph = float(getattr(cfg, "xsph", 0.0))
+    p.use_xsph = 1 if (cfg.flags & 4) else 0
     return p
 
 
@@ -176,6 +181,29 @@ def rollout(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, steps:
                                _p(pos), _p(vel), n, float(dt), int(steps), _p(x), _p(v),
                                _p(ids))
     return x, v, ids
+
+
+def energy_colors(p: OracleParams, pos, vel):
+    """(colour, |x_i - x_c|) per input particle of the Mexican-hat energy."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    n = pos.shape[0]
+    col = np.empty(n)
+    dist = np.empty(n)
+    lib().oracle_energy_colors(ctypes.byref(p), _p(pos), _p(vel), n, _p(col), _p(dist))
+    return col, dist
+
+
+def surface_indicator(p_low: OracleParams, pos_l, rho_l, pos_h, rho_h) -> np.ndarray:
+    pos_l = np.ascontiguousarray(pos_l, np.float32)
+    pos_h = np.ascontiguousarray(pos_h, np.float32).reshape(-1, 3)
+    rho_l = np.ascontiguousarray(rho_l, np.float64)
+    rho_h = np.ascontiguousarray(rho_h, np.float64)
+    out = np.empty(pos_l.shape[0])
+    lib().oracle_surface_indicator(ctypes.byref(p_low), _p(pos_l), _p(rho_l), pos_l.shape[0],
+                                   _p(pos_h) if len(pos_h) else None,
+                                   _p(rho_h) if len(rho_h) else None, pos_h.shape[0], _p(out))
+    return out
 
 
 def boundary_weight(r: float, h: float, speed_sound: float) -> float:

@@ -284,6 +284,55 @@ void add_wall_accel(const oracle_params& p, const WallSet& W, const std::vector<
     }
 }
 
+// integrateXSPHD (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:346-395, 757-853) with
+// each axis taking its own correction (the reference adds the x component to
+// all three, :837-839): v' = v + dt a for every particle, then
+// v'' = v' + eps sum_{r<h} (m / rho_j) W(r) (v'_j - v'_i), x' = x + dt v'',
+// and the wall clamp.
+void integrate_xsph(const oracle_params& p, State& s, const Neighbourhood& nb,
+                    const std::vector<double>& rho, const std::vector<double>& acc, double dt) {
+    const size_t n = s.id.size();
+    const Coeffs k(p.h);
+    std::vector<double> vp(3 * n);
+    for (size_t i = 0; i < 3 * n; ++i) vp[i] = s.v[i] + dt * acc[i];
+    std::vector<double> xo(3 * n), vo(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const double* xi = &s.x[3 * i];
+        double corr[3] = {0.0, 0.0, 0.0};
+        for_each_stencil_cell(p, xi, [&](uint32_t c) {
+            const uint32_t b = nb.start[c], e = nb.end[c];
+            if (b == kEmpty) return;
+            for (uint32_t j = b; j < e; ++j) {
+                const double dx = xi[0] - s.x[3 * j], dy = xi[1] - s.x[3 * j + 1],
+                             dz = xi[2] - s.x[3 * j + 2];
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (!(r < p.h)) continue;
+                const double a = p.h * p.h - r * r;
+                const double w = k.c6 * a * a * a * p.mass / rho[j];
+                for (int d = 0; d < 3; ++d) corr[d] += (vp[3 * j + d] - vp[3 * i + d]) * w;
+            }
+        });
+        for (int d = 0; d < 3; ++d) {
+            double v = vp[3 * i + d] + p.xsph * corr[d];
+            double x = xi[d] + dt * v;
+            if (p.wall_clamp) {
+                if (x < p.lo[d]) { x = p.lo[d]; v = -p.restitution * v; }
+                if (x > p.hi[d]) { x = p.hi[d]; v = -p.restitution * v; }
+            }
+            vo[3 * i + d] = v;
+            xo[3 * i + d] = x;
+        }
+    }
+    s.x = std::move(xo);
+    s.v = std::move(vo);
+}
+
+// evaluateMexicanHat3D (SPH2S_FINAL_TWO_SCALE/Solver.cu:206-213).
+double mexican_hat(double x, double y, double z) {
+    const double s = x * x + y * y + z * z;
+    return 0.8673250705840776 * (s - 3.0) * std::exp(-0.5 * s);
+}
+
 State make_state(const float* pos, const float* vel, uint32_t n) {
     State s;
     s.x.resize(3 * static_cast<size_t>(n));
@@ -330,6 +379,88 @@ void oracle_cell_ranges(const uint32_t* k, uint32_t n, uint32_t C, uint32_t* sta
     }
 }
 
+void oracle_energy_colors(const oracle_params* p, const float* pos, const float* vel, uint32_t n,
+                          double* color_out, double* dist_out) {
+    State s = make_state(pos, vel, n);
+    Neighbourhood nb;
+    build_neighbourhood(*p, s, nb);
+    const double h = p->h, m = p->mass;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* xi = &s.x[3 * i];
+        const double* vi = &s.v[3 * i];
+        const double psi0 = mexican_hat(0.0, 0.0, 0.0);
+        double psi_sum = psi0, msum = m;
+        double w[3] = {vi[0] * psi0, vi[1] * psi0, vi[2] * psi0};
+        double xc[3] = {m * xi[0], m * xi[1], m * xi[2]};
+        for_each_stencil_cell(*p, xi, [&](uint32_t c) {
+            const uint32_t b = nb.start[c], e = nb.end[c];
+            if (b == kEmpty) return;
+            for (uint32_t j = b; j < e; ++j) {
+                const double d[3] = {xi[0] - s.x[3 * j], xi[1] - s.x[3 * j + 1],
+                                     xi[2] - s.x[3 * j + 2]};
+                const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (!(r != 0.0 && r < h)) continue;
+                const double psi = mexican_hat(d[0] / h, d[1] / h, d[2] / h);
+                for (int k = 0; k < 3; ++k) {
+                    w[k] += s.v[3 * j + k] * psi;
+                    xc[k] += m * s.x[3 * j + k];
+                }
+                psi_sum += psi;
+                msum += m;
+            }
+        });
+        double dd = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double t = xi[k] - xc[k] / msum;
+            dd += t * t;
+        }
+        const double dist = std::sqrt(dd);
+        const double ene = std::min((w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) /
+                                        (psi_sum * psi_sum * h), 300.0);
+        const uint32_t id = s.id[i];
+        color_out[id] = (dist < 0.0025) ? 0.3 : ene / 300.0;
+        if (dist_out) dist_out[id] = dist;
+    }
+}
+
+void oracle_surface_indicator(const oracle_params* p_low, const float* pos_l, const double* rho_l,
+                              uint32_t n_l, const float* pos_h, const double* rho_h, uint32_t n_h,
+                              double* mag_out) {
+    const double h0 = p_low->h, h1 = static_cast<double>(0.5f * p_low->h_f32);
+    const double m0 = p_low->mass, m1 = static_cast<double>(static_cast<float>(p_low->mass) / 8.0f);
+    const Coeffs k0(h0), k1(h1);
+    auto spiky = [](double r, double h, double c45) { return c45 * (h - r) * (h - r) / r; };
+    for (uint32_t i = 0; i < n_l; ++i) {
+        double nv[3] = {0.0, 0.0, 0.0};
+        auto add = [&](const float* xj, double wgt) {
+            for (int k = 0; k < 3; ++k)
+                nv[k] -= wgt * (static_cast<double>(pos_l[3 * i + k]) - xj[k]);
+        };
+        for (uint32_t j = 0; j < n_l; ++j) {
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double t = static_cast<double>(pos_l[3 * i + k]) - pos_l[3 * j + k];
+                d2 += t * t;
+            }
+            const double r = std::sqrt(d2);
+            if (r > 0.0 && r < h0) add(&pos_l[3 * j], m0 / rho_l[j] * spiky(r, h0, k0.c45));
+        }
+        for (uint32_t j = 0; j < n_h; ++j) {
+            double d2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double t = static_cast<double>(pos_l[3 * i + k]) - pos_h[3 * j + k];
+                d2 += t * t;
+            }
+            const double r = std::sqrt(d2);
+            if (r > 0.0 && r < h0) {
+                const double g = spiky(r, h0, k0.c45) + ((r < h1) ? spiky(r, h1, k1.c45) : 0.0);
+                add(&pos_h[3 * j], m1 / rho_h[j] * 0.5 * g);
+            }
+        }
+        mag_out[i] = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) * h0;
+    }
+}
+
 double oracle_boundary_weight(double r, double h, double speed_sound) {
     return boundary_weight(r, h, speed_sound);
 }
@@ -353,7 +484,10 @@ void oracle_step_walls(const oracle_params* p, const oracle_walls* w, const floa
     densities(*p, s, nb, r);
     accelerations(*p, s, nb, r, a);
     if (w && w->n) add_wall_accel(*p, make_walls(*p, *w), s.x, a);
-    integrate(*p, s, a, dt);
+    if (p->use_xsph)
+        integrate_xsph(*p, s, nb, r, a, dt);
+    else
+        integrate(*p, s, a, dt);
     const uint32_t C = num_cells(p->grid);
     if (hash_sorted) std::copy(nb.hash_sorted.begin(), nb.hash_sorted.end(), hash_sorted);
     if (perm) std::copy(nb.perm.begin(), nb.perm.end(), perm);
@@ -384,7 +518,10 @@ void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const f
         densities(*p, s, nb, r);
         accelerations(*p, s, nb, r, a);
         add_wall_accel(*p, W, s.x, a);
-        integrate(*p, s, a, dt);
+        if (p->use_xsph)
+            integrate_xsph(*p, s, nb, r, a, dt);
+        else
+            integrate(*p, s, a, dt);
     }
     if (pos_out) std::copy(s.x.begin(), s.x.end(), pos_out);
     if (vel_out) std::copy(s.v.begin(), s.v.end(), vel_out);
@@ -1035,8 +1172,8 @@ void oracle_ts_mark_split(oracle_ts* t, const uint32_t* ids, uint32_t n) {
     std::vector<uint32_t> want(ids, ids + n);
     std::sort(want.begin(), want.end());
     for (size_t i = 0; i < L.id.size(); ++i)
-        if (L.state[i] == 0u && std::binary_search(want.begin(), want.end(), L.id[i]))
-            L.state[i] = 3u;
+        if ((L.state[i] & 7u) == 0u && std::binary_search(want.begin(), want.end(), L.id[i]))
+            L.state[i] |= 3u;
 }
 
 void oracle_ts_step(oracle_ts* t, double dt) {

@@ -39,6 +39,8 @@ typedef struct oracle_params {
     int32_t wall_clamp;
     float   h_f32;          /* the FP32 h used for stencil offsets          */
     oracle_grid grid;
+    double  xsph;           /* XSPH coefficient epsilon                     */
+    int32_t use_xsph;       /* 1: the XSPH integrator (integrateXSPHD)      */
 } oracle_params;
 
 /* Cell hash of each particle (FP32 positions, AoS 3n).
@@ -97,6 +99,21 @@ void oracle_step_walls(const oracle_params* p, const oracle_walls* w, const floa
 void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
                           const float* vel, uint32_t n, double dt, uint32_t n_steps,
                           double* pos_out, double* vel_out, uint32_t* ids_out);
+
+/* Colour value of every particle of a state (FP32 AoS pos/vel, n), by input
+ * index: the Mexican-hat energy of SPH2S_FINAL_TWO_SCALE/Solver.cu:206-213,
+ * 376-387, 897-907, 1050-1075 over the particle and its neighbours with
+ * 0 < r < h (p gives h, mass and grid).  dist_out (may be NULL) receives
+ * |x_i - x_c|, the distance the 0.0025 test uses. */
+void oracle_energy_colors(const oracle_params* p, const float* pos, const float* vel, uint32_t n,
+                          double* color_out, double* dist_out);
+/* Colour-field gradient magnitude |n_i| h0 of the two-scale surface detector
+ * for each low particle (inputs in any order; rho_* per particle, FP64):
+ *   n_i = sum_{L, 0<r<h0} (m0/rho_j) grad W_h0 + sum_{H, 0<r<h0} (m1/rho_j)
+ *         1/2 (grad W_h0 + grad W_h1)   (spiky gradients), all pairs. */
+void oracle_surface_indicator(const oracle_params* p_low, const float* pos_l, const double* rho_l,
+                              uint32_t n_l, const float* pos_h, const double* rho_h, uint32_t n_h,
+                              double* mag_out);
 
 /* O(n^2) density and acceleration over all pairs (no grid).  Pins neighbour
  * completeness of the grid path. */

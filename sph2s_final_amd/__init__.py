@@ -8,7 +8,8 @@ There is no CPU fallback: without the built library every call raises.
 from ._capi import (LibraryNotBuilt, SPHError, lib, FIELD_POSITION, FIELD_VELOCITY,  # noqa: F401
                     FIELD_DENSITY, FIELD_ID, FIELD_SORTED_HASH, FIELD_SORTED_INDEX,
                     FIELD_CELL_START, FIELD_CELL_END, FIELD_ACCEL, FIELD_TAG, FIELD_BLEND,
-                    FIELD_STATE, SPH_DEBUG_ACCEL, SPH_EMPTY_CELL, SPH_FLAG_KAHAN,
+                    FIELD_STATE, FIELD_COLOR, SPH_DEBUG_ACCEL, SPH_DEBUG_COLOR, SPH_FLAG_XSPH,
+                    SPH_EMPTY_CELL, SPH_FLAG_KAHAN,
                     SPH_FLAG_WALL_CLAMP, SPH2_COUPLING_FIXED, SPH2_COUPLING_BLENDED,
                     SPH2_SPLIT_LATTICE, SPH2_SPLIT_REFERENCE)
 from . import autotune  # noqa: F401

@@ -27,6 +27,8 @@ SPH_EMPTY_CELL = 0xFFFFFFFF
 SPH_FLAG_WALL_CLAMP = 1
 SPH_FLAG_KAHAN = 2
 SPH_DEBUG_ACCEL = 1
+SPH_DEBUG_COLOR = 2
+SPH_FLAG_XSPH = 4
 
 FIELD_POSITION = 0
 FIELD_VELOCITY = 1
@@ -40,6 +42,7 @@ FIELD_ACCEL = 8
 FIELD_TAG = 9
 FIELD_BLEND = 10
 FIELD_STATE = 11
+FIELD_COLOR = 12
 
 SPH2_COUPLING_FIXED = 0
 SPH2_COUPLING_BLENDED = 1
@@ -75,7 +78,7 @@ class SphConfig(Structure):
         ("gravity", c_float * 3), ("restitution", c_float),
         ("domain_lo", c_float * 3), ("domain_hi", c_float * 3),
         ("grid", SphGrid), ("flags", c_uint32),
-        ("speed_sound", c_float), ("boundary_mass", c_float),
+        ("speed_sound", c_float), ("boundary_mass", c_float), ("xsph", c_float),
     ]
 
 
@@ -142,6 +145,11 @@ SIGNATURES = [
     ("sph2_get_options", c_int, [c_void_p, POINTER(Sph2Options)]),
     ("sph2_set_blend", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_uint32]),
     ("sph2_mark_split", c_int, [c_void_p, c_void_p, c_uint32]),
+    ("sph2_export_render", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("sph2_mark_surface_map", c_int, [c_void_p, c_void_p, c_uint32, c_int]),
+    ("sph2_detect_surface", c_int, [c_void_p, c_float, c_int]),
+    ("sph_export_render", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("sph_config_set_xsph", c_int, [POINTER(SphConfig), c_float]),
     ("sph_batch_create", c_int, [POINTER(SphConfig), c_uint32, c_void_p, c_void_p, c_void_p,
                                  POINTER(c_void_p)]),
     ("sph_batch_destroy", c_int, [c_void_p]),

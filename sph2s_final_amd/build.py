@@ -21,8 +21,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_NAME = "libsph2s.so"
 SOURCES = ["api.hip", "radix_sort.hip", "step.hip", "two_scale.hip", "batch.hip", "adjoint.hip",
-           "walls.hip"]
-HEADERS = ["internal.hpp", "neighbors.hpp", "pairwalk.hpp", "batch.hpp", "walls.hpp"]
+           "walls.hip", "diagnostics.hip"]
+HEADERS = ["internal.hpp", "neighbors.hpp", "pairwalk.hpp", "batch.hpp", "walls.hpp",
+           "diagnostics.hpp"]
 
 
 def hipcc_path() -> str:

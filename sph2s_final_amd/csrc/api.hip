@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "diagnostics.hpp"
 #include "neighbors.hpp"
 #include "walls.hpp"
 
@@ -52,6 +53,8 @@ StepParams make_step_params(const sph_config& c, uint32_t n) {
     p.c45m = static_cast<float>(c.mass * c45);
     p.visc = static_cast<float>(static_cast<double>(c.viscosity) * c.mass * c45);
     p.tens = static_cast<float>(static_cast<double>(c.tension) * c.mass * c6);
+    p.xsph = static_cast<float>(static_cast<double>(c.xsph) * c.mass * c6);
+    p.mass = c.mass;
     p.gx = c.gravity[0];
     p.gy = c.gravity[1];
     p.gz = c.gravity[2];
@@ -97,6 +100,7 @@ struct sph_solver {
     uint32_t tune = 0;           // launch variant of the neighbour passes
     Walls walls{};               // boundary particles (none by default)
     float4* bacc = nullptr;      // boundary acceleration per sorted slot
+    float* color = nullptr;      // colour values (SPH_DEBUG_COLOR)
 };
 
 namespace {
@@ -282,6 +286,18 @@ int sph_make_configuration(const float origin[3], const float end[3], float volu
     return SPH_OK;
 }
 
+int sph_config_set_xsph(sph_config* cfg, float epsilon) {
+    if (!cfg) return set_error(SPH_ERR_INVALID_ARGUMENT, "config is null");
+    if (!(epsilon >= 0.0f) || !std::isfinite(epsilon))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "xsph epsilon must be >= 0");
+    cfg->xsph = epsilon;
+    if (epsilon > 0.0f)
+        cfg->flags |= SPH_FLAG_XSPH;
+    else
+        cfg->flags &= ~SPH_FLAG_XSPH;
+    return SPH_OK;
+}
+
 int sph_scene_lattice(const int32_t n[3], float spacing, const float lo[3], float jitter_frac,
                       uint64_t seed, float* pos) {
     if (!n || !lo || !pos) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
@@ -412,6 +428,7 @@ int sph_destroy(sph_solver* s) {
     destroy_graph(s);
     walls_destroy(&s->walls);
     if (s->bacc) (void)hipFree(s->bacc);
+    if (s->color) (void)hipFree(s->color);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -580,6 +597,16 @@ int sph_device_errors(sph_solver* s, uint32_t* bits) {
 
 int sph_set_debug(sph_solver* s, uint32_t flags) {
     if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    if ((flags & SPH_DEBUG_COLOR) && !s->color) {
+        SPH_HIP(hipMalloc(&s->color, 4ull * s->max_n));
+        SPH_HIP(hipMemsetAsync(s->color, 0, 4ull * s->max_n, s->stream));
+    }
+    float* const color = (flags & SPH_DEBUG_COLOR) ? s->color : nullptr;
+    if (color != s->buf.color) {
+        SPH_HIP(hipStreamSynchronize(s->stream));
+        destroy_graph(s);
+        s->buf.color = color;
+    }
     s->debug = flags;
     return SPH_OK;
 }
@@ -603,6 +630,10 @@ int sph_download(sph_solver* s, int field, void* dst, size_t bytes) {
         case SPH_FIELD_SORTED_INDEX: src = s->buf.perm; need = 4 * n; break;
         case SPH_FIELD_CELL_START: src = s->buf.cell_start; need = 4 * C; break;
         case SPH_FIELD_CELL_END: src = s->buf.cell_end; need = 4 * C; break;
+        case SPH_FIELD_COLOR:
+            if (!s->buf.color)
+                return set_error(SPH_ERR_INVALID_ARGUMENT, "colour values need SPH_DEBUG_COLOR");
+            src = s->buf.color; need = 4 * n; break;
         default: return set_error(SPH_ERR_INVALID_ARGUMENT, "unknown field");
     }
     if (bytes < need) return set_error(SPH_ERR_INVALID_ARGUMENT, "destination too small");
@@ -630,6 +661,14 @@ int sph_export_positions_aos(sph_solver* s, float* dst, int dst_is_device) {
     if (!s || !dst) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
     if (!dst_is_device) return sph_download(s, SPH_FIELD_POSITION, dst, 12 * static_cast<size_t>(s->n));
     return launch_export_aos(s->buf.pos, s->n, dst, s->stream);
+}
+
+int sph_export_render(sph_solver* s, float* xyz, float* color) {
+    if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    if (color && !s->buf.color)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "colour values need SPH_DEBUG_COLOR");
+    return launch_export_render(s->buf.pos, nullptr, s->buf.color, s->buf.n_dev, s->max_n, 0u, xyz,
+                                color, nullptr, s->stream);
 }
 
 size_t sph_radix_sort_temp_bytes(uint32_t n) { return radix_sort_temp_bytes(n); }

@@ -79,6 +79,8 @@ int batch_alloc(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* count
             !(c.grid.spacing > 0.0f) || c.grid.dims[0] < 1 || c.grid.dims[1] < 1 ||
             c.grid.dims[2] < 1)
             return set_error(SPH_ERR_INVALID_ARGUMENT, "batch: invalid scene configuration");
+        if (c.flags & SPH_FLAG_XSPH)
+            return set_error(SPH_ERR_UNSUPPORTED, "batch: the XSPH integrator is single-solver only");
         n += counts[s];
         cells += sph_grid_num_cells(&c.grid);
     }

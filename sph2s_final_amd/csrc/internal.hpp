@@ -40,6 +40,8 @@ struct StepParams {
     float c45m;             // mass * 45/(pi h^6): pressure acceleration factor
     float visc;             // mu * mass * 45/(pi h^6)
     float tens;             // sigma * mass * 315/(64 pi h^9)
+    float xsph;             // epsilon * mass * 315/(64 pi h^9) (XSPH integrator)
+    float mass;
     float gx, gy, gz;
     float restitution;
     float lox, loy, loz, hix, hiy, hiz;
@@ -133,6 +135,9 @@ struct StepBuffers {
     // boundary acceleration, computed before the force pass adds it.
     const Walls* walls;
     float4* bacc;
+    // Colour value per sorted slot (Mexican-hat energy), computed with the
+    // force pass when non-NULL.
+    float* color;
 };
 
 // Passes of one step, in launch order.

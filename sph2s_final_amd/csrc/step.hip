@@ -16,6 +16,8 @@
 // of the grid the cells cx are consecutive hashes, so a stencil row of up to
 // three cells is one contiguous run of sorted particles.
 #include "neighbors.hpp"
+#include "diagnostics.hpp"
+#include "pairwalk.hpp"
 #include "walls.hpp"
 
 namespace sph {
@@ -599,6 +601,83 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpo
     }
 }
 
+// XSPH integrator (integrateXSPHD, SINGLE_SCALE/Solver.cu:346-395, 757-853,
+// with each axis taking its own correction component): in sorted order, after
+// the force pass has written every particle's acceleration,
+//   v'_i = v_i + dt a_i,  v''_i = v'_i + eps m c6 sum_{r < h} B_j (h^2 - r^2)^3 (v'_j - v'_i),
+//   x'_i = x_i + dt v''_i, then the wall clamp.
+// The block-staged stencil walk holds (x_j, B_j) and v'_j per candidate.
+constexpr int kXsphTile = 512;
+
+__global__ __launch_bounds__(kBlock) void k_xsph_integrate(
+    const float4* __restrict__ tpos, const float4* __restrict__ tvel,
+    const float2* __restrict__ aux, const float4* __restrict__ accel,
+    const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce, StepParams p, GridView g,
+    const uint32_t* __restrict__ n_dev, float dt, float4* __restrict__ pos,
+    float4* __restrict__ vel) {
+    __shared__ float4 tx[kXsphTile];
+    __shared__ float4 tv[kXsphTile];
+    __shared__ uint32_t red[2 * kWavesPerBlock];
+    const uint32_t n = *n_dev;
+    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = s < n;
+    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) {
+        const float4 a = accel[s];
+        vi.x += dt * a.x;
+        vi.y += dt * a.y;
+        vi.z += dt * a.z;
+    }
+    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
+    float Sx = 0.f, Sy = 0.f, Sz = 0.f;
+    auto pair = [&](const float4& xb, const float4& v) {
+        const float dx = xi.x - xb.x, dy = xi.y - xb.y, dz = xi.z - xb.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < p.h2) {
+            const float a = p.h2 - r2;
+            const float w = a * a * a * xb.w;
+            Sx += w * (v.x - vi.x);
+            Sy += w * (v.y - vi.y);
+            Sz += w * (v.z - vi.z);
+        }
+    };
+    auto vprime = [&](uint32_t j) {
+        const float4 v = tvel[j], a = accel[j];
+        return make_float4(v.x + dt * a.x, v.y + dt * a.y, v.z + dt * a.z, 0.0f);
+    };
+    walk_stencil_staged<kBlock, kXsphTile>(
+        g, pr, cs, ce, n, active, red,
+        [&](uint32_t k, uint32_t j) {
+            const float4 x = tpos[j];
+            tx[k] = make_float4(x.x, x.y, x.z, aux[j].y);
+            tv[k] = vprime(j);
+        },
+        [&](uint32_t t0, uint32_t t1, uint32_t) {
+            for (uint32_t t = t0; t < t1; ++t) pair(tx[t], tv[t]);
+        },
+        [&](uint32_t b, uint32_t e) {
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 x = tpos[j];
+                pair(make_float4(x.x, x.y, x.z, aux[j].y), vprime(j));
+            }
+        });
+    if (!active) return;
+    float vx = vi.x + p.xsph * Sx, vy = vi.y + p.xsph * Sy, vz = vi.z + p.xsph * Sz;
+    float x = xi.x + dt * vx, y = xi.y + dt * vy, z = xi.z + dt * vz;
+    if (p.flags & SPH_FLAG_WALL_CLAMP) {
+        const float e = p.restitution;
+        if (x < p.lox) { x = p.lox; vx = -e * vx; }
+        if (x > p.hix) { x = p.hix; vx = -e * vx; }
+        if (y < p.loy) { y = p.loy; vy = -e * vy; }
+        if (y > p.hiy) { y = p.hiy; vy = -e * vy; }
+        if (z < p.loz) { z = p.loz; vz = -e * vz; }
+        if (z > p.hiz) { z = p.hiz; vz = -e * vz; }
+    }
+    pos[s] = make_float4(x, y, z, xi.w);
+    vel[s] = make_float4(vx, vy, vz, vi.w);
+}
+
 // Variant dispatch.  Only the passes of the single-scale and batched steps
 // (density with aux, force with integration and no tension) are tunable; the
 // other instantiations use the defaults.  The single-scale and batched passes
@@ -718,10 +797,34 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
         case kPassForce: {
             float4* acc = write_accel ? b.accel : nullptr;
             if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
+            if (b.color) {  // colour values from the force pass's inputs
+                const int rc = launch_energy_color(b.tpos, b.tvel, b.cell_start, b.cell_end,
+                                                   b.n_dev, p.n, g, p.mass, b.color, stream);
+                if (rc != SPH_OK) return rc;
+            }
             const bool walls = b.walls && b.bacc;
             if (walls) {
                 const int rc = launch_wall_accel(*b.walls, b.tpos, b.n_dev, p.n, b.bacc, stream);
                 if (rc != SPH_OK) return rc;
+            }
+            if (p.flags & SPH_FLAG_XSPH) {
+                // Accelerations of every particle first (sorted order), then
+                // the XSPH integration, which reads its neighbours' ones.
+                const XOrder xa{nullptr, nullptr, nullptr, walls ? b.bacc : nullptr};
+                float4* const no_state = nullptr;
+                if (p.tens != 0.0f)
+                    launch_force<true, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux,
+                                                     b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f,
+                                                     no_state, no_state, b.accel, kNoScenes, xa);
+                else
+                    launch_force<false, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux,
+                                                      b.cell_start, b.cell_end, p, g, b.n_dev,
+                                                      0.0f, no_state, no_state, b.accel, kNoScenes,
+                                                      xa);
+                hipLaunchKernelGGL(k_xsph_integrate, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.tpos, b.tvel, b.aux, b.accel, b.cell_start, b.cell_end, p, g,
+                                   b.n_dev, dt, b.pos, b.vel);
+                break;
             }
             const XOrder xo{b.i2r, b.xbins, nullptr, walls ? b.bacc : nullptr};
             if (p.tens != 0.0f)

@@ -35,6 +35,7 @@
 #include <new>
 #include <vector>
 
+#include "diagnostics.hpp"
 #include "neighbors.hpp"
 #include "pairwalk.hpp"
 #include "walls.hpp"
@@ -510,8 +511,8 @@ __global__ void k_fill_float(const uint32_t* __restrict__ n_dev, float value,
     dst[i] = value;
 }
 
-// Low particles whose id is in the sorted list `ids` and that are settled get
-// state 3 (split requested).
+// Low particles whose id is in the sorted list `ids` and that are settled
+// (mode 0; the surface bit 8 may be set) get mode 3 (split requested).
 __global__ void k_mark_ids(const float4* __restrict__ pos, const uint32_t* __restrict__ n_dev,
                            const uint32_t* __restrict__ ids, uint32_t n_ids,
                            float2* __restrict__ bl) {
@@ -528,7 +529,8 @@ __global__ void k_mark_ids(const float4* __restrict__ pos, const uint32_t* __res
     }
     if (lo < n_ids && ids[lo] == id) {
         const float2 b = bl[i];
-        if (__float_as_uint(b.y) == 0u) bl[i] = make_float2(b.x, __uint_as_float(3u));
+        const uint32_t st = __float_as_uint(b.y);
+        if ((st & 7u) == 0u) bl[i] = make_float2(b.x, __uint_as_float(st | 3u));
     }
 }
 
@@ -765,10 +767,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter_low(
     const uint32_t* __restrict__ flag_l, const uint32_t* __restrict__ n_l,
     const uint32_t* __restrict__ cl, uint32_t nbl, const uint32_t* __restrict__ tot,
     Region rg, SplitParams sp, uint32_t id_base, uint32_t cap_l, uint32_t cap_h,
-    uint32_t* __restrict__ grp_live, uint32_t num_groups, float4* __restrict__ new_pos_l,
-    float4* __restrict__ new_vel_l, float2* __restrict__ new_bl_l,
-    float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
-    float2* __restrict__ new_bl_h) {
+    uint32_t* __restrict__ grp_live, uint32_t num_groups, const float* __restrict__ col_l,
+    float4* __restrict__ new_pos_l, float4* __restrict__ new_vel_l, float2* __restrict__ new_bl_l,
+    float* __restrict__ new_col_l, float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
+    float2* __restrict__ new_bl_h, float* __restrict__ new_col_h) {
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = *n_l;
@@ -790,6 +792,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_low(
                 const float2 b = bl_l[i];
                 new_bl_l[dst] = (f & 2u) ? make_float2(b.x, __uint_as_float(1u)) : b;
             }
+            if (col_l) new_col_l[dst] = col_l[i];
         }
     }
     if (f & 2u) {
@@ -810,6 +813,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_low(
                                          __uint_as_float((static_cast<uint32_t>(k) << 29) |
                                                          (pid & kGroupMask)));
             if (bl_l) new_bl_h[dst] = make_float2(0.0f, __uint_as_float(6u));
+            if (col_l) new_col_h[dst] = col_l[i];
         }
     }
 }
@@ -826,9 +830,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
     const uint32_t* __restrict__ tot, const float4* __restrict__ grp_pos,
     const float4* __restrict__ grp_vel, uint32_t* __restrict__ grp_cnt,
     uint32_t* __restrict__ grp_live, uint32_t num_groups, uint32_t cap_l, uint32_t cap_h,
-    float4* __restrict__ new_pos_l, float4* __restrict__ new_vel_l,
-    float2* __restrict__ new_bl_l, float4* __restrict__ new_pos_h,
-    float4* __restrict__ new_vel_h, float2* __restrict__ new_bl_h) {
+    const float* __restrict__ col_h, float4* __restrict__ new_pos_l,
+    float4* __restrict__ new_vel_l, float2* __restrict__ new_bl_l, float* __restrict__ new_col_l,
+    float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
+    float2* __restrict__ new_bl_h, float* __restrict__ new_col_h) {
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = *n_h;
@@ -854,6 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
                     (f & 4u) ? make_float2(b.x, __uint_as_float((__float_as_uint(b.y) & 4u) | 1u))
                              : b;
             }
+            if (col_h) new_col_h[dst] = col_h[i];
         }
     }
     if (f & 2u) {
@@ -873,6 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
             new_pos_l[dst] = make_float4(0.125f * sx, 0.125f * sy, 0.125f * sz, __uint_as_float(g));
             new_vel_l[dst] = make_float4(0.125f * ux, 0.125f * uy, 0.125f * uz, 0.0f);
             if (bl_h) new_bl_l[dst] = make_float2(0.0f, __uint_as_float(2u));
+            if (col_h) new_col_l[dst] = col_h[i];  // the leader's (slot 0) value
         }
     }
 }
@@ -935,6 +942,10 @@ struct TwoScaleSet {
     float2* state_blend = nullptr;
     float2* tblend = nullptr;
     float2* nblend = nullptr;
+    // Colour values (SPH_DEBUG_COLOR): the step's sorted order and the
+    // compacted order, carried through split/merge.
+    float* tcolor = nullptr;
+    float* color = nullptr;
 };
 
 struct sph_two_scale {
@@ -958,7 +969,9 @@ struct sph_two_scale {
     hipGraphExec_t graph = nullptr;
     float graph_dt = 0.0f;
     bool graph_accel = false;
+    uint32_t graph_debug = 0;
     uint32_t debug = 0;
+    uint32_t* surf = nullptr;    // per low id: at the free surface (surface detector)
 };
 
 namespace {
@@ -1041,14 +1054,17 @@ int launch_split_merge(sph_two_scale* t) {
                        H.counts, nbh);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
                        L.cap, H.cap, t->tot, t->error_word);
+    const bool col = (t->debug & SPH_DEBUG_COLOR) != 0u;
     hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel,
                        L.buf.rho, bl_l, L.flag, L.buf.n_dev, L.counts, nbl, t->tot, t->region, sp,
-                       t->id_base, L.cap, H.cap, live, t->num_groups, L.state_pos, L.state_vel,
-                       L.state_blend, H.state_pos, H.state_vel, H.state_blend);
+                       t->id_base, L.cap, H.cap, live, t->num_groups, col ? L.tcolor : nullptr,
+                       L.state_pos, L.state_vel, L.state_blend, L.color, H.state_pos, H.state_vel,
+                       H.state_blend, H.color);
     hipLaunchKernelGGL(k_scatter_high, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel, bl_h,
                        H.flag, H.buf.n_dev, H.counts, nbh, t->tot, t->grp_pos, t->grp_vel,
-                       t->grp_cnt, live, t->num_groups, L.cap, H.cap, L.state_pos, L.state_vel,
-                       L.state_blend, H.state_pos, H.state_vel, H.state_blend);
+                       t->grp_cnt, live, t->num_groups, L.cap, H.cap, col ? H.tcolor : nullptr,
+                       L.state_pos, L.state_vel, L.state_blend, L.color, H.state_pos, H.state_vel,
+                       H.state_blend, H.color);
     hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(64), 0, st, t->tot, L.buf.n_dev,
                        H.buf.n_dev);
     SPH_HIP(hipGetLastError());
@@ -1133,6 +1149,15 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[1].cap), dim3(kBlock), 0, st,
                            t->set[1].buf.tpos, t->set[1].buf.n_dev, G[0], t->occ);
     }
+    // Colour values of both sets from the step's sorted state.
+    if (t->debug & SPH_DEBUG_COLOR)
+        for (int r = 1; r >= 0; --r) {
+            TwoScaleSet& S = t->set[r];
+            const int rc = launch_energy_color(S.buf.tpos, S.buf.tvel, S.buf.cell_start,
+                                               S.buf.cell_end, S.buf.n_dev, S.cap, G[r],
+                                               S.cfg.mass, S.tcolor, st);
+            if (rc != SPH_OK) return rc;
+        }
     // Boundary accelerations of both sets (walls on the low grid).
     if (t->walls.n)
         for (int r = 1; r >= 0; --r) {
@@ -1203,6 +1228,7 @@ int capture_two_scale(sph_two_scale* t, float dt, bool accel) {
     }
     t->graph_dt = dt;
     t->graph_accel = accel;
+    t->graph_debug = t->debug;
     return SPH_OK;
 }
 
@@ -1226,6 +1252,8 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         return set_error(SPH_ERR_INVALID_ARGUMENT, "two-scale ids need n < 2^29");
     if (!(cfg->h > 0.0f) || !(cfg->mass > 0.0f) || !(cfg->rest_density > 0.0f))
         return set_error(SPH_ERR_INVALID_ARGUMENT, "config: h, mass, rest_density must be > 0");
+    if (cfg->flags & SPH_FLAG_XSPH)
+        return set_error(SPH_ERR_UNSUPPORTED, "two-scale: the XSPH integrator is single-scale only");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
         (void)hipGetLastError();
@@ -1281,6 +1309,8 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         add(4 * N);                                // flags
         add(8 * nb);                               // block counts
         for (int k = 0; k < 3; ++k) add(8 * N);    // state_blend tblend nblend
+        add(4 * N);                                // tcolor
+        add(4 * N);                                // color
     }
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
@@ -1288,6 +1318,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     add(64);                                        // tot[8] + error word
     add(4 * static_cast<size_t>(t->set[0].num_cells));  // occupancy of the low grid
     add(4 * static_cast<size_t>(t->num_groups));    // grp_live
+    add(4 * static_cast<size_t>(t->num_groups));    // surf
 
     hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1332,6 +1363,8 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         S.state_blend = static_cast<float2*>(take());
         S.tblend = static_cast<float2*>(take());
         S.nblend = static_cast<float2*>(take());
+        S.tcolor = static_cast<float*>(take());
+        S.color = static_cast<float*>(take());
     }
     t->grp_pos = static_cast<float4*>(take());
     t->grp_vel = static_cast<float4*>(take());
@@ -1340,6 +1373,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     t->error_word = t->tot + 8;
     t->occ = static_cast<uint32_t*>(take());
     t->grp_live = static_cast<uint32_t*>(take());
+    t->surf = static_cast<uint32_t*>(take());
 
     e = hipMemsetAsync(t->arena, 0, total, t->stream);
     if (e != hipSuccess) {
@@ -1415,7 +1449,7 @@ int sph2_advance(sph_two_scale* t, float dt, uint32_t n_steps) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
     if (n_steps == 0) return SPH_OK;
     const bool accel = (t->debug & SPH_DEBUG_ACCEL) != 0u;
-    if (!t->graph || t->graph_dt != dt || t->graph_accel != accel) {
+    if (!t->graph || t->graph_dt != dt || t->graph_accel != accel || t->graph_debug != t->debug) {
         const int rc = capture_two_scale(t, dt, accel);
         if (rc != SPH_OK) return rc;
     }
@@ -1518,6 +1552,66 @@ int sph2_mark_split(sph_two_scale* t, const uint32_t* ids, uint32_t n) {
     return SPH_OK;
 }
 
+int sph2_export_render(sph_two_scale* t, int resolution, float* xyz, float* color,
+                       int32_t* state) {
+    if (!t || resolution < 0 || resolution > 1)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "bad argument");
+    if (color && !(t->debug & SPH_DEBUG_COLOR))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "colour values need SPH_DEBUG_COLOR");
+    TwoScaleSet& S = t->set[resolution];
+    return launch_export_render(S.state_pos, blended(t) ? S.state_blend : nullptr, S.color,
+                                S.buf.n_dev, S.cap, resolution == 0 ? 0u : 4u, xyz, color, state,
+                                t->stream);
+}
+
+int sph2_detect_surface(sph_two_scale* t, float threshold, int split) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    if (!blended(t)) return set_error(SPH_ERR_INVALID_ARGUMENT, "surface marks need the blended coupling");
+    if (!(threshold >= 0.0f) || !std::isfinite(threshold))
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "threshold must be >= 0");
+    uint32_t c[4];
+    int rc = sph2_counts(t, c);
+    if (rc != SPH_OK) return rc;
+    if (c[2] == 0) return set_error(SPH_ERR_INVALID_ARGUMENT, "no step has run on the low set");
+    TwoScaleSet& L = t->set[0];
+    TwoScaleSet& H = t->set[1];
+    const StepParams pl = make_step_params(L.cfg, L.cap), ph = make_step_params(H.cfg, H.cap);
+    const GridView gl = make_grid_view(pl.ox, pl.oy, pl.oz, pl.spacing, pl.dx, pl.dy, pl.dz, pl.h);
+    const GridView gh = make_grid_view(ph.ox, ph.oy, ph.oz, ph.spacing, ph.dx, ph.dy, ph.dz, pl.h);
+    const CrossParams cp = cross_params(t, 0);
+    SurfaceParams sp{};
+    sp.h0 = cp.h0;
+    sp.h1 = cp.h1;
+    sp.c45_0 = cp.c45_0;
+    sp.c45_1 = cp.c45_1;
+    sp.m0 = L.cfg.mass;
+    sp.m1 = H.cfg.mass;
+    sp.threshold = threshold;
+    // The last step's sorted state: the neighbour passes ran on c[2] / c[3]
+    // particles, whose counts the step's n_dev no longer holds.
+    uint32_t* n_step = t->tot + 6;  // tot[6], tot[7]
+    rc = launch_surface_detect(L.buf.tpos, L.buf.aux, L.buf.cell_start, L.buf.cell_end, n_step,
+                               L.cap, gl, H.buf.tpos, H.buf.aux, H.buf.cell_start, H.buf.cell_end,
+                               n_step + 1, gh, sp, t->surf, t->num_groups, t->stream);
+    if (rc == SPH_OK)
+        rc = launch_apply_surface(L.state_pos, L.buf.n_dev, L.cap, t->surf, t->num_groups,
+                                  split ? 1u : 0u, L.state_blend, t->stream);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
+}
+
+int sph2_mark_surface_map(sph_two_scale* t, const uint32_t* map, uint32_t count, int split) {
+    if (!t || (count > 0 && !map)) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!blended(t)) return set_error(SPH_ERR_INVALID_ARGUMENT, "surface marks need the blended coupling");
+    TwoScaleSet& L = t->set[0];
+    const int rc = launch_mark_surface_map(map, count, L.buf.n_dev, split ? 1u : 0u, L.state_blend,
+                                           t->stream);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
+}
+
 int sph2_synchronize(sph_two_scale* t) {
     if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
     SPH_HIP(hipStreamSynchronize(t->stream));
@@ -1578,6 +1672,10 @@ int sph2_download(sph_two_scale* t, int resolution, int field, void* dst, size_t
         case SPH_FIELD_VELOCITY: vec = S.state_vel; n = n_now; need = 12 * n; mode = 1; break;
         case SPH_FIELD_ID: vec = S.state_pos; n = n_now; need = 4 * n; mode = 2; break;
         case SPH_FIELD_TAG: vec = S.state_vel; n = n_now; need = 4 * n; mode = 2; break;
+        case SPH_FIELD_COLOR:
+            if (!(t->debug & SPH_DEBUG_COLOR))
+                return set_error(SPH_ERR_INVALID_ARGUMENT, "colour values need SPH_DEBUG_COLOR");
+            src = S.color; n = n_now; need = 4 * n; break;
         case SPH_FIELD_BLEND:
         case SPH_FIELD_STATE:
             if (!blended(t))

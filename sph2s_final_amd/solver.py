@@ -77,6 +77,13 @@ class SolverConfiguration:
         cfg.flags = (C.SPH_FLAG_WALL_CLAMP if wall_clamp else 0) | (C.SPH_FLAG_KAHAN if kahan else 0)
         return cfg
 
+    @staticmethod
+    def SetXSPH(cfg: C.SphConfig, epsilon: float) -> C.SphConfig:
+        """XSPHCoeff epsilon; > 0 switches the single-scale step to the XSPH
+        integrator (integrateXSPHD, Solver.cu:757-853, per-axis correction)."""
+        C.check(C.lib().sph_config_set_xsph(byref(cfg), float(epsilon)))
+        return cfg
+
 
 class ParticleData:
     """ParticleData (SPH2S_FINAL_SINGLE_SCALE/ParticleData.h:9-53).
@@ -211,6 +218,19 @@ class Solver:
     def set_debug(self, flags: int) -> None:
         C.check(C.lib().sph_set_debug(self._h, int(flags)))
 
+    def render_tensors(self, color: bool = False):
+        """The renderer's buffers as torch CUDA tensors (DLPack-exchangeable):
+        positions (N, 3) and, with ``color`` (needs SPH_DEBUG_COLOR), the
+        colour values (N,), in the state's order."""
+        import torch
+        n = self.NumParticles
+        xyz = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        col = torch.empty((n,), dtype=torch.float32, device="cuda") if color else None
+        C.check(C.lib().sph_export_render(self._h, c_void_p(xyz.data_ptr()) if n else None,
+                                          c_void_p(col.data_ptr()) if (color and n) else None))
+        self.synchronize()
+        return (xyz, col) if color else xyz
+
     def device_errors(self) -> int:
         """Device-side error bits since the last query (SPH_DEVERR_*); synchronises."""
         bits = c_uint32()
@@ -266,7 +286,7 @@ class Solver:
         cells = int(self.views().num_cells)
         if field in (C.FIELD_POSITION, C.FIELD_VELOCITY, C.FIELD_ACCEL):
             out = np.empty((n, 3), np.float32)
-        elif field == C.FIELD_DENSITY:
+        elif field in (C.FIELD_DENSITY, C.FIELD_COLOR):
             out = np.empty(n, np.float32)
         elif field in (C.FIELD_ID, C.FIELD_SORTED_HASH, C.FIELD_SORTED_INDEX):
             out = np.empty(n, np.uint32)
@@ -395,6 +415,32 @@ class TwoScaleSolver:
         ptr = lambda a: None if a is None else a.ctypes.data_as(c_void_p)  # noqa: E731
         C.check(C.lib().sph2_set_blend(self._h, int(resolution), ptr(la), ptr(st), n))
 
+    def render_tensors(self, resolution: int, color: bool = False):
+        """(positions (N, 3), colour (N,) or None, state words (N,) int32) of one
+        set as torch CUDA tensors in the current order (sph2_export_render)."""
+        import torch
+        n = self.counts()[resolution]
+        xyz = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        col = torch.empty((n,), dtype=torch.float32, device="cuda") if color else None
+        st = torch.empty((n,), dtype=torch.int32, device="cuda")
+        if n:
+            C.check(C.lib().sph2_export_render(self._h, int(resolution), c_void_p(xyz.data_ptr()),
+                                               c_void_p(col.data_ptr()) if color else None,
+                                               c_void_p(st.data_ptr())))
+        self.synchronize()
+        return xyz, col, st
+
+    def mark_surface_map(self, index_map, split: bool = True) -> None:
+        """getSurfaceIDs: ``index_map`` is a CUDA uint32/int32 tensor of
+        (low slot + 1) entries, 0 = background (blended coupling)."""
+        m = index_map.contiguous()
+        C.check(C.lib().sph2_mark_surface_map(self._h, c_void_p(m.data_ptr()), int(m.numel()),
+                                              int(bool(split))))
+
+    def detect_surface(self, threshold: float, split: bool = True) -> None:
+        """Colour-field surface detector on the last step's state (blended coupling)."""
+        C.check(C.lib().sph2_detect_surface(self._h, float(threshold), int(bool(split))))
+
     def mark_split(self, ids) -> None:
         """Request the split of the low-resolution particles with these ids."""
         ids = np.ascontiguousarray(ids, np.uint32)
@@ -421,7 +467,7 @@ class TwoScaleSolver:
             out = np.empty((n_step, 3), np.float32)
         elif field in (C.FIELD_ID, C.FIELD_TAG, C.FIELD_STATE):
             out = np.empty(n_now, np.uint32)
-        elif field == C.FIELD_BLEND:
+        elif field in (C.FIELD_BLEND, C.FIELD_COLOR):
             out = np.empty(n_now, np.float32)
         elif field == C.FIELD_DENSITY:
             out = np.empty(n_step, np.float32)
