@@ -183,6 +183,9 @@ int walls_create(Walls* w, const sph_grid& grid, float h, float boundary_mass, f
         return code;
     };
     hipError_t e = hipMemcpyAsync(xyz, pos_xyz, 12ull * n, hipMemcpyHostToDevice, st);
+    // The sort's error word is sticky (the sort never clears it) and the
+    // arena is fresh: clear it so that the check below sees this sort only.
+    if (e == hipSuccess) e = hipMemsetAsync(radix_sort_error_word(sort_temp), 0, 4, st);
     if (e != hipSuccess) return fail(hip_error(e, "hipMemcpyAsync", __FILE__, __LINE__));
     hipLaunchKernelGGL(k_wall_pack, wall_blocks(n), dim3(kWallBlock), 0, st, xyz, n, w->grid, upos,
                        keys);
@@ -199,7 +202,8 @@ int walls_create(Walls* w, const sph_grid& grid, float h, float boundary_mass, f
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(hip_error(e, "wall setup", __FILE__, __LINE__));
     uint32_t sort_err = 0;
-    e = hipMemcpy(&sort_err, radix_sort_error_word(sort_temp), 4, hipMemcpyDeviceToHost);
+    e = hipMemcpyAsync(&sort_err, radix_sort_error_word(sort_temp), 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(hip_error(e, "hipMemcpy", __FILE__, __LINE__));
     if (sort_err) return fail(set_error(SPH_ERR_HIP, "boundary sort: look-back wait gave up"));
     return SPH_OK;
