@@ -240,7 +240,7 @@ int sph_device_errors(sph_solver* s, uint32_t* bits);
  *   bits 0-3   density LDS tile: 0 = 1024, 1 = 512, 2 = 1536, 3 = 2048, 4 = 768,
  *              5 = 1280 particles
  *   bits 4-7   force LDS tile / neighbour-list capacity / entry width:
- *              0 = 768/16/16-bit, 1 = 640/16/16, 2 = 768/24/16, 3 = 768/8/32,
+ *              0 = 640/16/16-bit, 1 = 768/16/16, 2 = 768/24/16, 3 = 768/8/32,
  *              4 = 512/16/16, 5 = 1024/16/16, 6 = 512/8/32, 7 = 640/24/16,
  *              8 = 1024/8/32, 9 = 512/24/16
  *   bits 8-11  density grid, bits 12-15 force grid: 0 = one 256-thread block

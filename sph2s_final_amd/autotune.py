@@ -20,7 +20,7 @@ import os
 
 DENSITY_TILES = (1024, 512, 1536, 2048, 768, 1280)
 # (LDS tile, list capacity, list entry bits)
-FORCE_VARIANTS = ((768, 16, 16), (640, 16, 16), (768, 24, 16), (768, 8, 32), (512, 16, 16),
+FORCE_VARIANTS = ((640, 16, 16), (768, 16, 16), (768, 24, 16), (768, 8, 32), (512, 16, 16),
                   (1024, 16, 16), (512, 8, 32), (640, 24, 16), (1024, 8, 32), (512, 24, 16))
 GRID_BLOCKS_PER_CU = (None, 1, 2, 4, 8)  # None: one block per 256 particles
 

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_xbins(const uint32_t* __restrict__ s
 #define SPH_DENSITY_TILE 1024
 #endif
 #ifndef SPH_FORCE_TILE
-#define SPH_FORCE_TILE 768
+#define SPH_FORCE_TILE 640
 #endif
 #ifndef SPH_LIST_CAP
 #define SPH_LIST_CAP 16
@@ -190,8 +190,8 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // LDS instead of global memory, the list capacity only when queued pairs are
 // evaluated, and the grid shape only which block processes which particles.
 //   bits 0-3   density LDS tile: 1024 (default), 512, 1536, 2048, 768, 1280
-//   bits 4-7   force tile / list capacity / list entry:  768/16/u16 (default),
-//              640/16/u16, 768/24/u16, 768/8/u32, 512/16/u16, 1024/16/u16,
+//   bits 4-7   force tile / list capacity / list entry:  640/16/u16 (default),
+//              768/16/u16, 768/24/u16, 768/8/u32, 512/16/u16, 1024/16/u16,
 //              512/8/u32, 640/24/u16, 1024/8/u32, 512/24/u16
 //   bits 8-11  density grid:       one block per 256 particles, or a
 //   bits 12-15 force grid:         persistent grid of 1, 2, 4, 8 blocks per CU
@@ -361,8 +361,19 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
     }
 }
 
+// Occupancy targets of the single-scale neighbour passes (waves per SIMD):
+// the density pass fits 6 waves in 84 VGPRs, the force pass 5 in 102 with
+// the 640-candidate tile (LDS 28.8 KB per block); the hints make the
+// compiler allocate within those budgets.  Other instantiations (batched
+// scenes, tension, two-scale own-set passes) keep the compiler's choice.
+constexpr int density_waves(bool batched) { return batched ? 1 : 6; }
+constexpr int force_waves(bool batched, bool tension, bool integrate) {
+    return (!batched && !tension && integrate) ? 5 : 1;
+}
+
 template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX>
-__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ tpos,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(density_waves(kBatched))))
+void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
                                                     StepParams p_in, GridView g_in,
@@ -573,7 +584,9 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
 
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
           bool kX>
-__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ tpos,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(force_waves(kBatched, kTension, kIntegrate))))
+void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
                                                   const float2* __restrict__ aux,
                                                   const uint32_t* __restrict__ cs,
@@ -714,7 +727,7 @@ void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
                        sh.grid, dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kTunable) {
         switch (v) {
-            case 1: SPH_FORCE_VARIANT(640, 16, uint16_t); return;
+            case 1: SPH_FORCE_VARIANT(768, 16, uint16_t); return;
             case 2: SPH_FORCE_VARIANT(768, 24, uint16_t); return;
             case 3: SPH_FORCE_VARIANT(768, 8, uint32_t); return;
             case 4: SPH_FORCE_VARIANT(512, 16, uint16_t); return;
