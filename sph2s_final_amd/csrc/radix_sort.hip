@@ -25,11 +25,17 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kItems = 16;
+#ifndef SPH_SORT_ITEMS
+#define SPH_SORT_ITEMS 16
+#endif
+#ifndef SPH_HIST_ITEMS
+#define SPH_HIST_ITEMS 8
+#endif
+constexpr int kItems = SPH_SORT_ITEMS;
 constexpr int kWaveKeys = 64 * kItems;
 constexpr int kTile = kBlock * kItems;
 // The histogram kernel reads its keys in smaller tiles (more blocks in flight).
-constexpr int kHistItems = 8;
+constexpr int kHistItems = SPH_HIST_ITEMS;
 constexpr int kHistTile = kBlock * kHistItems;
 constexpr int kRadix = 256;
 constexpr int kMaxPasses = 4;

@@ -236,9 +236,19 @@ inline LaunchShape launch_shape(uint32_t cap_n, uint32_t persist_idx) {
 // tile) is read from global memory instead, and so are the rare extra rows a
 // reference stencil reaches when rounding puts a bound two cells away.
 
+// The pair arithmetic below is written with explicit fused multiply-adds and
+// contraction off, so every kernel that evaluates a pair rounds it the same
+// way whatever the surrounding code: the launch variants of a pass, which
+// inline it into different code, give identical sums.
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
 __device__ __forceinline__ float poly6_term(const float4& xi, const float4& xj, float h2) {
+#pragma clang fp contract(off)
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r2 = dist2(dx, dy, dz);
     const float a = h2 - r2;
     return (r2 < h2) ? a * a * a : 0.0f;
 }
@@ -394,6 +404,90 @@ void k_density(const float4* __restrict__ tpos,
     }
 }
 
+// Pair sums of the force pass for one target i (position xi, velocity vi,
+// A_i): pressure P, viscosity V and tension T.
+template <bool kTension>
+struct PairSums {
+    float Px = 0.f, Py = 0.f, Pz = 0.f;
+    float Vx = 0.f, Vy = 0.f, Vz = 0.f;
+    float Tx = 0.f, Ty = 0.f, Tz = 0.f;
+
+    // xa = (x_j, A_j), vb = (v_j, B_j); requires 0 < r_ij < h.
+    __device__ __forceinline__ void add(const float4& xi, const float4& vi, float ai, float h,
+                                        float h2, const float4& xa, const float4& vb) {
+#pragma clang fp contract(off)
+        const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
+        const float r2 = dist2(dx, dy, dz);
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float hr = __builtin_fmaf(-r2, inv_r, h);
+        const float tpr = (ai + xa.w) * hr * hr * inv_r;
+        Px = __builtin_fmaf(tpr, dx, Px);
+        Py = __builtin_fmaf(tpr, dy, Py);
+        Pz = __builtin_fmaf(tpr, dz, Pz);
+        const float tvs = hr * vb.w;
+        Vx = __builtin_fmaf(tvs, vb.x - vi.x, Vx);
+        Vy = __builtin_fmaf(tvs, vb.y - vi.y, Vy);
+        Vz = __builtin_fmaf(tvs, vb.z - vi.z, Vz);
+        if (kTension) {
+            const float a = h2 - r2;
+            const float w = a * a * a;
+            Tx = __builtin_fmaf(w, dx, Tx);
+            Ty = __builtin_fmaf(w, dy, Ty);
+            Tz = __builtin_fmaf(w, dz, Tz);
+        }
+    }
+};
+
+// The force pass's result for target i at sorted slot o: the acceleration
+// (plus the boundary acceleration bacc[o] when bacc is non-NULL), then, with
+// kIntegrate, integrateD (Solver.cu:713-755): v += dt a; x += dt v; and the
+// wall clamp.  Without kIntegrate only the acceleration is written.
+template <bool kTension, bool kIntegrate>
+__device__ __forceinline__ void force_finish(const PairSums<kTension>& S, const StepParams& p,
+                                             const float4& xi, const float4& vi,
+                                             const float2& ai, uint32_t o,
+                                             const float4* __restrict__ bacc, float dt,
+                                             float4* __restrict__ pos, float4* __restrict__ vel,
+                                             float4* __restrict__ accel) {
+#pragma clang fp contract(off)
+    const float vc = p.visc * ai.y;
+    float ax = __builtin_fmaf(vc, S.Vx, p.c45m * S.Px) + p.gx;
+    float ay = __builtin_fmaf(vc, S.Vy, p.c45m * S.Py) + p.gy;
+    float az = __builtin_fmaf(vc, S.Vz, p.c45m * S.Pz) + p.gz;
+    if (kTension) {
+        const float tc = p.tens * ai.y;
+        ax = __builtin_fmaf(-tc, S.Tx, ax);
+        ay = __builtin_fmaf(-tc, S.Ty, ay);
+        az = __builtin_fmaf(-tc, S.Tz, az);
+    }
+    if (bacc) {  // a_i = f_i / rho_i + b_i + g (Solver.cu:708-710)
+        const float4 b = bacc[o];
+        ax += b.x;
+        ay += b.y;
+        az += b.z;
+    }
+    if (!kIntegrate) {
+        accel[o] = make_float4(ax, ay, az, 0.0f);
+        return;
+    }
+    float vx = __builtin_fmaf(dt, ax, vi.x), vy = __builtin_fmaf(dt, ay, vi.y),
+          vz = __builtin_fmaf(dt, az, vi.z);
+    float x = __builtin_fmaf(dt, vx, xi.x), y = __builtin_fmaf(dt, vy, xi.y),
+          z = __builtin_fmaf(dt, vz, xi.z);
+    if (p.flags & SPH_FLAG_WALL_CLAMP) {
+        const float e = p.restitution;
+        if (x < p.lox) { x = p.lox; vx = -e * vx; }
+        if (x > p.hix) { x = p.hix; vx = -e * vx; }
+        if (y < p.loy) { y = p.loy; vy = -e * vy; }
+        if (y > p.hiy) { y = p.hiy; vy = -e * vy; }
+        if (z < p.loz) { z = p.loz; vz = -e * vz; }
+        if (z > p.hiz) { z = p.hiz; vz = -e * vz; }
+    }
+    pos[o] = make_float4(x, y, z, xi.w);
+    vel[o] = make_float4(vx, vy, vz, vi.w);
+    if (accel) accel[o] = make_float4(ax, ay, az, 0.0f);
+}
+
 // a_i = m c45 sum_j (A_i + A_j)(h-r)^2/r x_ij            (pressure)
 //     + (mu m c45 / rho_i) sum_j B_j (h-r)(v_j - v_i)     (viscosity)
 //     - (sigma m c6 / rho_i) sum_j (h^2-r^2)^3 x_ij       (tension)
@@ -432,39 +526,18 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
     const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float2 ai = active ? aux[s] : make_float2(0.f, 1.f);
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
-    float Px = 0.f, Py = 0.f, Pz = 0.f;
-    float Vx = 0.f, Vy = 0.f, Vz = 0.f;
-    float Tx = 0.f, Ty = 0.f, Tz = 0.f;
+    PairSums<kTension> S;
     uint32_t cnt = 0;
 
-    // xa = (x_j, A_j), vb = (v_j, B_j); requires 0 < r_ij < h.
     auto pair = [&](const float4& xa, const float4& vb) {
-        const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float inv_r = __builtin_amdgcn_rsqf(r2);
-        const float hr = p.h - r2 * inv_r;
-        const float tpr = (ai.x + xa.w) * hr * hr * inv_r;
-        Px += tpr * dx;
-        Py += tpr * dy;
-        Pz += tpr * dz;
-        const float tvs = hr * vb.w;
-        Vx += tvs * (vb.x - vi.x);
-        Vy += tvs * (vb.y - vi.y);
-        Vz += tvs * (vb.z - vi.z);
-        if (kTension) {
-            const float a = p.h2 - r2;
-            const float w = a * a * a;
-            Tx += w * dx;
-            Ty += w * dy;
-            Tz += w * dz;
-        }
+        S.add(xi, vi, ai.x, p.h, p.h2, xa, vb);
     };
     // 0 < r2 < h2 as one unsigned compare: r2 >= +0 is a sum of squares, and
     // for non-negative floats the bit patterns order like the values.
     const uint32_t h2_bits_m1 = __float_as_uint(p.h2) - 1u;
     auto in_range = [&](const float4& xj) {
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = dist2(dx, dy, dz);
         return __float_as_uint(r2) - 1u < h2_bits_m1;
     };
     // Queued pairs in queue order; two list entries are fetched per step so
@@ -544,42 +617,8 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
             global_run(b, e);
         }
 
-    const float vc = p.visc * ai.y;
-    float ax = p.c45m * Px + vc * Vx + p.gx;
-    float ay = p.c45m * Py + vc * Vy + p.gy;
-    float az = p.c45m * Pz + vc * Vz + p.gz;
-    if (kTension) {
-        const float tc = p.tens * ai.y;
-        ax -= tc * Tx;
-        ay -= tc * Ty;
-        az -= tc * Tz;
-    }
     const uint32_t o = kX ? xo.i2r[s] : s;
-    if (xo.bacc) {  // a_i = f_i / rho_i + b_i + g (Solver.cu:708-710)
-        const float4 b = xo.bacc[o];
-        ax += b.x;
-        ay += b.y;
-        az += b.z;
-    }
-    if (!kIntegrate) {
-        accel[o] = make_float4(ax, ay, az, 0.0f);
-        return;
-    }
-    // integrateD (Solver.cu:713-755): v += dt a; x += dt v; then the wall clamp.
-    float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
-    float x = xi.x + dt * vx, y = xi.y + dt * vy, z = xi.z + dt * vz;
-    if (p.flags & SPH_FLAG_WALL_CLAMP) {
-        const float e = p.restitution;
-        if (x < p.lox) { x = p.lox; vx = -e * vx; }
-        if (x > p.hix) { x = p.hix; vx = -e * vx; }
-        if (y < p.loy) { y = p.loy; vy = -e * vy; }
-        if (y > p.hiy) { y = p.hiy; vy = -e * vy; }
-        if (z < p.loz) { z = p.loz; vz = -e * vz; }
-        if (z > p.hiz) { z = p.hiz; vz = -e * vz; }
-    }
-    pos[o] = make_float4(x, y, z, xi.w);
-    vel[o] = make_float4(vx, vy, vz, vi.w);
-    if (accel) accel[o] = make_float4(ax, ay, az, 0.0f);
+    force_finish<kTension, kIntegrate>(S, p, xi, vi, ai, o, xo.bacc, dt, pos, vel, accel);
 }
 
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
@@ -646,7 +685,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph_integrate(
     float Sx = 0.f, Sy = 0.f, Sz = 0.f;
     auto pair = [&](const float4& xb, const float4& v) {
         const float dx = xi.x - xb.x, dy = xi.y - xb.y, dz = xi.z - xb.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r2 = dist2(dx, dy, dz);
         if (r2 < p.h2) {
             const float a = p.h2 - r2;
             const float w = a * a * a * xb.w;
