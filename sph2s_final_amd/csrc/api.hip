@@ -50,6 +50,7 @@ StepParams make_step_params(const sph_config& c, uint32_t n) {
     const double c6 = 315.0 / (64.0 * kPi * h3 * h3 * h3);
     const double c45 = 45.0 / (kPi * h3 * h3);
     p.c6m = static_cast<float>(c.mass * c6);
+    p.c6m_h6 = static_cast<float>(c.mass * 315.0 / (64.0 * kPi * h3));
     p.c45m = static_cast<float>(c.mass * c45);
     p.visc = static_cast<float>(static_cast<double>(c.viscosity) * c.mass * c45);
     p.tens = static_cast<float>(static_cast<double>(c.tension) * c.mass * c6);

@@ -37,6 +37,7 @@ struct StepParams {
     float h, h2;            // effective radius and its square
     float rest_density, stiffness;
     float c6m;              // mass * 315/(64 pi h^9): rho = c6m * sum (h^2-r^2)^3
+    float c6m_h6;           // mass * 315/(64 pi h^3): rho = c6m_h6 * sum (1 - r^2/h^2)^3
     float c45m;             // mass * 45/(pi h^6): pressure acceleration factor
     float visc;             // mu * mass * 45/(pi h^6)
     float tens;             // sigma * mass * 315/(64 pi h^9)

@@ -245,42 +245,52 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-__device__ __forceinline__ float poly6_term(const float4& xi, const float4& xj, float h2) {
+// Poly6 weight in units of h^2: a = 1 - r^2 / h^2 clamped to [0, 1] (one
+// fused multiply-add with the clamp modifier), so (h^2 - r^2)^3 = h^6 a^3 for
+// r < h and the term is 0 otherwise; a = 1 for r = 0.
+__device__ __forceinline__ float poly6_unit(const float4& xi, const float4& xj, float inv_h2) {
 #pragma clang fp contract(off)
     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     const float r2 = dist2(dx, dy, dz);
-    const float a = h2 - r2;
-    return (r2 < h2) ? a * a * a : 0.0f;
+    return __builtin_amdgcn_fmed3f(__builtin_fmaf(-r2, inv_h2, 1.0f), 0.0f, 1.0f);
 }
 
-// A run's terms are added one at a time in index order, whether the run is
-// read from global memory or staged in LDS.
+// Sum of a^3 over a run of m candidates (poly6_unit), added one at a time in
+// index order (a^2 * a + sum as one fused operation), whether the run is read
+// from global memory or staged in LDS.
 constexpr int kScanGroup = 8;  // force scan: candidates loaded together before use
 
 template <class Load>
-__device__ __forceinline__ float poly6_run(const float4& xi, float h2, uint32_t m, Load&& load) {
+__device__ __forceinline__ float poly6_run(const float4& xi, float inv_h2, uint32_t m,
+                                           Load&& load) {
+#pragma clang fp contract(off)
     float rs = 0.0f;
+    auto add = [&](const float4& q) {
+        const float a = poly6_unit(xi, q, inv_h2);
+        rs = __builtin_fmaf(a * a, a, rs);
+    };
     uint32_t k = 0;
     for (; k + 8 <= m; k += 8) {
         float4 q[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) q[u] = load(k + u);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) rs += poly6_term(xi, q[u], h2);
+        for (int u = 0; u < 8; ++u) add(q[u]);
     }
     for (; k + 4 <= m; k += 4) {
         const float4 q0 = load(k), q1 = load(k + 1), q2 = load(k + 2), q3 = load(k + 3);
-        rs += poly6_term(xi, q0, h2);
-        rs += poly6_term(xi, q1, h2);
-        rs += poly6_term(xi, q2, h2);
-        rs += poly6_term(xi, q3, h2);
+        add(q0);
+        add(q1);
+        add(q2);
+        add(q3);
     }
-    for (; k < m; ++k) rs += poly6_term(xi, load(k), h2);
+    for (; k < m; ++k) add(load(k));
     return rs;
 }
 
 // computeDensitiesPressuresD + computeDensityCell (Solver.cu:198-229,536-599):
-// rho_i = m c6 sum_{r_ij < h} (h^2 - r_ij^2)^3, self included.  Each run is
+// rho_i = m c6 sum_{r_ij < h} (h^2 - r_ij^2)^3, self included, evaluated as
+// (m c6 h^6) sum a^3 with a = 1 - r_ij^2 / h^2 (poly6_run).  Each run is
 // summed in plain FP32 and the run sums are combined with Kahan compensation
 // (with kKahan).  Also emits the per-particle factors the force pass needs for
 // every neighbour j: A_j = k (rho_j - rho0) / rho_j^2 and B_j = 1 / rho_j.
@@ -313,6 +323,7 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
     StepParams p = p_in;
     GridView g = g_in;
     scene_constants<kBatched>(scenes, tvel, s, active, p, g);
+    const float inv_h2 = 1.0f / p.h2;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
     float sum = 0.0f, comp = 0.0f;
@@ -341,11 +352,11 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
             const bool staged = b < e && b >= ub && e <= seg_e;
             if (staged) {
                 const float4* t = tile + (b - ub);
-                accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
+                accumulate(poly6_run(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
             if (b < e && !staged) {
                 const float4* t = tpos + b;
-                accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
+                accumulate(poly6_run(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
         }
     }
@@ -357,10 +368,10 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
             probe_row<kX>(g, pr, cs, ce, xo.bins, n, ry, rz, b, e);
             if (b < e) {
                 const float4* t = tpos + b;
-                accumulate(poly6_run(xi, p.h2, e - b, [&](uint32_t k) { return t[k]; }));
+                accumulate(poly6_run(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
         }
-    const float r = p.c6m * sum;
+    const float r = p.c6m_h6 * sum;
     const uint32_t o = kX ? xo.i2r[s] : s;
     rho[o] = r;
     if (kFinalize) {
