@@ -11,8 +11,11 @@
 //               start in order), ranks its keys among equal digits with wave64
 //               ballots, publishes its per-digit counts, looks back over the
 //               status words of earlier tiles for its exclusive prefix
-//               (decoupled look-back), and scatters.
-// A tile is kBlock*kItems keys: wave w owns a contiguous run of 64*kItems
+//               (decoupled look-back, kLookBack tiles per round trip), and
+//               scatters.
+// A tile is kPassBlock*kItems keys (1024 threads x 4 keys: 16 waves share a
+// 4096-key tile, so each wave ranks 4 rounds instead of 16 and the CU has 16
+// waves to hide latency with): wave w owns a contiguous run of 64*kItems
 // keys, so the in-tile order is (wave, round, lane) and the sort is stable.
 // Every look-back wait is bounded: a wait that exceeds kSpinLimit polls sets
 // an error bit and gives up instead of hanging.
@@ -24,16 +27,23 @@ namespace sph {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-#ifndef SPH_SORT_ITEMS
-#define SPH_SORT_ITEMS 16
+// Digit passes: kPassBlock threads rank a tile of kPassBlock * kItems keys.
+#ifndef SPH_SORT_BLOCK
+#define SPH_SORT_BLOCK 1024
 #endif
+#ifndef SPH_SORT_ITEMS
+#define SPH_SORT_ITEMS 4
+#endif
+constexpr int kPassBlock = SPH_SORT_BLOCK;
+constexpr int kWaves = kPassBlock / 64;
+static_assert(kPassBlock >= 256 && kPassBlock <= 1024 && kPassBlock % 64 == 0,
+              "a digit pass needs one thread per digit");
 #ifndef SPH_HIST_ITEMS
 #define SPH_HIST_ITEMS 8
 #endif
 constexpr int kItems = SPH_SORT_ITEMS;
 constexpr int kWaveKeys = 64 * kItems;
-constexpr int kTile = kBlock * kItems;
+constexpr int kTile = kPassBlock * kItems;
 // The histogram kernel reads its keys in smaller tiles (more blocks in flight).
 constexpr int kHistItems = SPH_HIST_ITEMS;
 constexpr int kHistTile = kBlock * kHistItems;
@@ -43,6 +53,10 @@ constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
+#ifndef SPH_SORT_LOOKBACK
+#define SPH_SORT_LOOKBACK 4
+#endif
+constexpr int kLookBack = SPH_SORT_LOOKBACK;  // predecessor status words per look-back round trip
 
 // Header of the temporary buffer, in 32-bit words.
 constexpr uint32_t kHistWords = kMaxPasses * kRadix;        // global histograms
@@ -165,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __rest
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pass(
+__global__ __launch_bounds__(kPassBlock) void k_pass(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
     const uint32_t* __restrict__ n_dev, int shift, int bits, const uint32_t* __restrict__ ghist,
@@ -180,17 +194,17 @@ __global__ __launch_bounds__(kBlock) void k_pass(
     const int t = threadIdx.x;
 
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int i = t; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0u;
-    // Exclusive scan of this pass's global digit histogram.
-    dbase[t] = (t < radix) ? ghist[t] : 0u;
+    for (int i = t; i < kWaves * kRadix; i += kPassBlock) (&wcnt[0][0])[i] = 0u;
+    // Exclusive scan of this pass's global digit histogram (threads t < 256).
+    if (t < kRadix) dbase[t] = (t < radix) ? ghist[t] : 0u;
     __syncthreads();
     for (int off = 1; off < kRadix; off <<= 1) {
-        const uint32_t v = (t >= off) ? dbase[t - off] : 0u;
+        const uint32_t v = (t < kRadix && t >= off) ? dbase[t - off] : 0u;
         __syncthreads();
-        dbase[t] += v;
+        if (t < kRadix) dbase[t] += v;
         __syncthreads();
     }
-    const uint32_t digit_excl = dbase[t] - ((t < radix) ? ghist[t] : 0u);
+    const uint32_t digit_excl = (t < radix) ? dbase[t] - ghist[t] : 0u;
     const uint32_t tile = s_tile;
     // The grid covers the capacity; tickets past the live tiles have no keys.
     // Tickets are taken in dispatch order, so live tiles never wait on them.
@@ -230,7 +244,11 @@ __global__ __launch_bounds__(kBlock) void k_pass(
             wcnt[w][t] = run;
             run += c;
         }
-        // Decoupled look-back over earlier tiles for digit t.
+        // Decoupled look-back over earlier tiles for digit t, kLookBack
+        // predecessors per round trip: their words are loaded together and
+        // taken nearest first -- aggregates are added until an inclusive
+        // prefix ends the walk, and a word not yet published stops the window
+        // (it is polled again).  Tile 0 always publishes a prefix.
         uint32_t* my = status + static_cast<size_t>(tile) * kRadix + t;
         uint32_t excl = 0;
         if (tile == 0) {
@@ -239,10 +257,26 @@ __global__ __launch_bounds__(kBlock) void k_pass(
             store_status(my, kFlagAggregate | run);
             int32_t pred = static_cast<int32_t>(tile) - 1;
             uint32_t spins = 0;
-            while (pred >= 0) {
-                const uint32_t v = load_status(status + static_cast<size_t>(pred) * kRadix + t);
-                const uint32_t flag = v & ~kCountMask;
-                if (flag == 0u) {
+            bool done = false;
+            while (!done) {
+                uint32_t v[kLookBack];
+#pragma unroll
+                for (int q = 0; q < kLookBack; ++q)
+                    v[q] = (pred - q >= 0)
+                               ? load_status(status + static_cast<size_t>(pred - q) * kRadix + t)
+                               : 0u;
+                int taken = 0;
+#pragma unroll
+                for (int q = 0; q < kLookBack; ++q) {
+                    if (done || taken < q || pred - q < 0) continue;
+                    const uint32_t flag = v[q] & ~kCountMask;
+                    if (flag == 0u) continue;  // not published: the window stops here
+                    excl += v[q] & kCountMask;
+                    taken = q + 1;
+                    done = flag == kFlagPrefix;
+                }
+                if (done) break;
+                if (taken == 0) {
                     if (++spins > kSpinLimit) {
                         atomicOr(error_word, 1u);
                         break;
@@ -250,9 +284,8 @@ __global__ __launch_bounds__(kBlock) void k_pass(
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += v & kCountMask;
-                if (flag == kFlagPrefix) break;
-                --pred;
+                pred -= taken;
+                if (pred < 0) break;  // unreachable: tile 0 publishes a prefix
             }
             store_status(my, kFlagPrefix | ((excl + run) & kCountMask));
         }
@@ -353,7 +386,7 @@ int run_passes(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t* kout = to_out ? keys_out : L.ktmp;
         uint32_t* vout = to_out ? vals_out : L.vtmp;
-        hipLaunchKernelGGL(k_pass, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, kout, vout,
+        hipLaunchKernelGGL(k_pass, dim3(tiles), dim3(kPassBlock), 0, stream, kin, vin, kout, vout,
                            L.header + kNWord, shift, bits, L.header + p * kRadix,
                            L.status + static_cast<size_t>(p) * tiles * kRadix,
                            L.header + kHistWords + p, L.header + kErrorWord);
