@@ -264,10 +264,10 @@ __device__ __forceinline__ void probe_row(const GridView& g, const Probe& pr,
 template <int kBlockThreads>
 __device__ __forceinline__ void block_run_union(uint32_t b, uint32_t e, uint32_t* red,
                                                 uint32_t& ub, uint32_t& ue) {
-    for (int off = 32; off > 0; off >>= 1) {
-        b = min(b, static_cast<uint32_t>(__shfl_xor(static_cast<int>(b), off)));
-        e = max(e, static_cast<uint32_t>(__shfl_xor(static_cast<int>(e), off)));
-    }
+    // Wave-wide min / max (DPP-based device-library reductions, every lane
+    // active), then across the block's waves through LDS.
+    b = __ockl_wfred_min_u32(b);
+    e = __ockl_wfred_max_u32(e);
     constexpr int kW = kBlockThreads / 64;
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
