@@ -245,6 +245,16 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
+// Keeps all four components of a candidate read from LDS live at its point of
+// use, so the compiler reads the whole 16-byte element (ds_read_b128) instead
+// of narrowing the access to the 12 bytes of x, y, z: on gfx950 the 12-byte
+// form is markedly slower (the density pass took about 20% longer in
+// compressed states).  The empty asm only pins a register; it touches no
+// memory.
+__device__ __forceinline__ void keep_lds_quad(const float4& q) {
+    asm volatile("" ::"v"(q.w));
+}
+
 // Poly6 weight in units of h^2: a = 1 - r^2 / h^2 clamped to [0, 1] (one
 // fused multiply-add with the clamp modifier), so (h^2 - r^2)^3 = h^6 a^3 for
 // r < h and the term is 0 otherwise; a = 1 for r = 0.
@@ -257,15 +267,16 @@ __device__ __forceinline__ float poly6_unit(const float4& xi, const float4& xj, 
 
 // Sum of a^3 over a run of m candidates (poly6_unit), added one at a time in
 // index order (a^2 * a + sum as one fused operation), whether the run is read
-// from global memory or staged in LDS.
+// from global memory or staged in LDS (kQuad: LDS, read in 16-byte elements).
 constexpr int kScanGroup = 8;  // force scan: candidates loaded together before use
 
-template <class Load>
+template <bool kQuad, class Load>
 __device__ __forceinline__ float poly6_run(const float4& xi, float inv_h2, uint32_t m,
                                            Load&& load) {
 #pragma clang fp contract(off)
     float rs = 0.0f;
     auto add = [&](const float4& q) {
+        if (kQuad) keep_lds_quad(q);
         const float a = poly6_unit(xi, q, inv_h2);
         rs = __builtin_fmaf(a * a, a, rs);
     };
@@ -352,11 +363,11 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
             const bool staged = b < e && b >= ub && e <= seg_e;
             if (staged) {
                 const float4* t = tile + (b - ub);
-                accumulate(poly6_run(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
+                accumulate(poly6_run<true>(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
             if (b < e && !staged) {
                 const float4* t = tpos + b;
-                accumulate(poly6_run(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
+                accumulate(poly6_run<false>(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
         }
     }
@@ -368,7 +379,7 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
             probe_row<kX>(g, pr, cs, ce, xo.bins, n, ry, rz, b, e);
             if (b < e) {
                 const float4* t = tpos + b;
-                accumulate(poly6_run(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
+                accumulate(poly6_run<false>(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
             }
         }
     const float r = p.c6m_h6 * sum;
@@ -608,6 +619,7 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
                     for (int u = 0; u < kG; ++u) q[u] = tp[t + u];
 #pragma unroll
                     for (int u = 0; u < kG; ++u) {
+                        keep_lds_quad(q[u]);
                         const bool hit = (t + u < t1) & in_range(q[u]);  // no short circuit
                         list[cnt][threadIdx.x] = static_cast<TList>(t + u);
                         cnt += hit ? 1u : 0u;
