@@ -255,10 +255,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
             auto flush = [&]() {
                 for (uint32_t k = 0; k < cnt; ++k) {
                     const uint32_t t = queue[k][threadIdx.x];
-                    const float4 xa = tx[t];
+                    const float4 xa = tx[t], va = tv[t], ba = ta[t];
+                    keep_lds_quad(va);
+                    keep_lds_quad(ba);
                     const float dx = xi.x - xa.x, dy = xi.y - xa.y, dz = xi.z - xa.z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    pair(dx, dy, dz, r2, xa, tv[t], ta[t]);
+                    pair(dx, dy, dz, r2, xa, va, ba);
                 }
                 cnt = 0;
             };
@@ -268,6 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
                 for (int u = 0; u < kAdjGroup; ++u) q[u] = tx[t + u];
 #pragma unroll
                 for (int u = 0; u < kAdjGroup; ++u) {
+                    keep_lds_quad(q[u]);
                     const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     const bool hit = (t + u < t1) & pair_in_range(r2, h2_bits_m1);
@@ -355,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
                 for (int u = 0; u < kAdjGroup; ++u) q[u] = tile[t + u];
 #pragma unroll
                 for (int u = 0; u < kAdjGroup; ++u) {
+                    keep_lds_quad(q[u]);
                     const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     if ((t + u < t1) & pair_in_range(r2, h2_bits_m1)) pair(dx, dy, dz, r2, q[u].w);

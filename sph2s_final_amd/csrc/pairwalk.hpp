@@ -15,6 +15,16 @@
 
 namespace sph {
 
+// Keeps all four components of a candidate read from LDS live at its point of
+// use, so the compiler reads the whole 16-byte element (ds_read_b128) instead
+// of narrowing the access to the 12 bytes of x, y, z: on gfx950 the 12-byte
+// form is markedly slower (the single-scale density pass took about 20%
+// longer in compressed states).  The empty asm only pins a register; it
+// touches no memory.
+__device__ __forceinline__ void keep_lds_quad(const float4& q) {
+    asm volatile("" ::"v"(q.w));
+}
+
 // stage(k, j): write candidate j (global index) into LDS slot k.
 // tile_run(t0, t1, base): scan LDS slots [t0, t1) (global index base + t).
 // global_run(b, e): scan global indices [b, e).

@@ -245,16 +245,6 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-// Keeps all four components of a candidate read from LDS live at its point of
-// use, so the compiler reads the whole 16-byte element (ds_read_b128) instead
-// of narrowing the access to the 12 bytes of x, y, z: on gfx950 the 12-byte
-// form is markedly slower (the density pass took about 20% longer in
-// compressed states).  The empty asm only pins a register; it touches no
-// memory.
-__device__ __forceinline__ void keep_lds_quad(const float4& q) {
-    asm volatile("" ::"v"(q.w));
-}
-
 // Poly6 weight in units of h^2: a = 1 - r^2 / h^2 clamped to [0, 1] (one
 // fused multiply-add with the clamp modifier), so (h^2 - r^2)^3 = h^6 a^3 for
 // r < h and the term is 0 otherwise; a = 1 for r = 0.
@@ -729,7 +719,12 @@ __global__ __launch_bounds__(kBlock) void k_xsph_integrate(
             tv[k] = vprime(j);
         },
         [&](uint32_t t0, uint32_t t1, uint32_t) {
-            for (uint32_t t = t0; t < t1; ++t) pair(tx[t], tv[t]);
+            for (uint32_t t = t0; t < t1; ++t) {
+                const float4 q = tx[t], v = tv[t];
+                keep_lds_quad(q);
+                keep_lds_quad(v);
+                pair(q, v);
+            }
         },
         [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {

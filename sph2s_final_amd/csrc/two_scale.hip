@@ -289,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
         [&](uint32_t t0, uint32_t t1, uint32_t) {
             for (uint32_t t = t0; t < t1; ++t) {
                 const float4 q = tile[t];
+                keep_lds_quad(q);
                 pair(q, q.w);
             }
         },
@@ -382,7 +383,11 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
             tl[k] = tbl_a[j].x;
         },
         [&](uint32_t t0, uint32_t t1, uint32_t) {
-            for (uint32_t t = t0; t < t1; ++t) pair(tx[t], tv[t], tl[t]);
+            for (uint32_t t = t0; t < t1; ++t) {
+                const float4 q = tx[t];
+                keep_lds_quad(q);
+                pair(q, tv[t], tl[t]);
+            }
         },
         [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
