@@ -237,8 +237,8 @@ int sph_device_errors(sph_solver* s, uint32_t* bits);
 /* ---- launch variants and the autotuner -------------------------------------
  * The reference launches fixed block shapes (Solver.cu:1082-1198).  Here the
  * density and force passes have launch variants, selected by a tune word:
- *   bits 0-3   density LDS tile: 0 = 1024, 1 = 512, 2 = 1536, 3 = 2048, 4 = 768,
- *              5 = 1280 particles
+ *   bits 0-3   density LDS tile: 0 = 1280, 1 = 512, 2 = 1536, 3 = 2048, 4 = 768,
+ *              5 = 1024 particles
  *   bits 4-7   force LDS tile / neighbour-list capacity / entry width:
  *              0 = 640/16/16-bit, 1 = 768/16/16, 2 = 768/24/16, 3 = 768/8/32,
  *              4 = 512/16/16, 5 = 1024/16/16, 6 = 512/8/32, 7 = 640/24/16,

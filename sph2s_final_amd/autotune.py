@@ -18,7 +18,7 @@ from __future__ import annotations
 import json
 import os
 
-DENSITY_TILES = (1024, 512, 1536, 2048, 768, 1280)
+DENSITY_TILES = (1280, 512, 1536, 2048, 768, 1024)
 # (LDS tile, list capacity, list entry bits)
 FORCE_VARIANTS = ((640, 16, 16), (768, 16, 16), (768, 24, 16), (768, 8, 32), (512, 16, 16),
                   (1024, 16, 16), (512, 8, 32), (640, 24, 16), (1024, 8, 32), (512, 24, 16))

@@ -5,8 +5,9 @@
 // Single-sweep scheme, 8-bit digits (fewer bits in the last pass):
 //   k_prepare   zeroes the digit histograms, tile tickets and look-back
 //               status words (and, for the solver, resets the cell ranges)
-//   k_hist      one read of the keys -> LDS digit histograms of every pass,
-//               added into the global per-pass histograms
+//   k_hist      one read of the keys -> LDS digit histograms of every pass
+//               (counted by runs of equal digits: the solver's keys arrive
+//               nearly sorted), added into the global per-pass histograms
 //   k_pass      one kernel per digit: a block takes a tile ticket (so tiles
 //               start in order), ranks its keys among equal digits with wave64
 //               ballots, publishes its per-digit counts, looks back over the
@@ -147,6 +148,11 @@ struct BatchHashKeys {
     }
 };
 
+// The keys arrive nearly sorted (the solver's particles are stored in the
+// previous step's cell order), so each digit is counted by runs: a lane whose
+// digit differs from the previous lane's starts a run and adds its length.
+// The keys of all rounds are computed (and their positions loaded) before any
+// is counted.
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __restrict__ n_dev,
                                                  int passes, uint32_t key_bits,
@@ -155,20 +161,35 @@ __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __rest
     if (blockIdx.x * kHistTile >= n) return;  // block-uniform: the grid covers the capacity
     __shared__ uint32_t cnt[kMaxPasses][kRadix];
     for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kBlock) (&cnt[0][0])[i] = 0u;
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t base = blockIdx.x * kHistTile + wave * (64 * kHistItems);
+    uint32_t key[kHistItems];
+#pragma unroll
     for (int r = 0; r < kHistItems; ++r) {
         const uint32_t idx = base + r * 64 + lane;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? src(idx) : 0u;
+        key[r] = (idx < n) ? src(idx) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kHistItems; ++r) {
+        const uint32_t first = base + r * 64;
+        // Valid lanes form a prefix of the wave.
+        const uint32_t nvalid = (first < n) ? min(64u, n - first) : 0u;
+        const bool valid = static_cast<uint32_t>(lane) < nvalid;
+        const uint32_t prev = static_cast<uint32_t>(__shfl_up(static_cast<int>(key[r]), 1));
         for (int p = 0; p < passes; ++p) {
             const int shift = 8 * p;
             const int bits = min(8, static_cast<int>(key_bits) - shift);
-            const uint32_t d = (key >> shift) & ((1u << bits) - 1u);
-            const uint64_t peers = match_digit(d, valid, bits);
-            if (valid && lane == __builtin_ctzll(peers))
-                atomicAdd(&cnt[p][d], static_cast<uint32_t>(__popcll(peers)));
+            const uint32_t m = (1u << bits) - 1u;
+            const uint32_t d = (key[r] >> shift) & m;
+            const bool start = valid && (lane == 0 || ((prev >> shift) & m) != d);
+            const uint64_t starts = __ballot(start);
+            if (start) {
+                const uint64_t later = (lane == 63) ? 0ull : (starts >> (lane + 1));
+                const uint32_t end =
+                    later ? static_cast<uint32_t>(lane + 1 + __builtin_ctzll(later)) : nvalid;
+                atomicAdd(&cnt[p][d], end - static_cast<uint32_t>(lane));
+            }
         }
     }
     __syncthreads();

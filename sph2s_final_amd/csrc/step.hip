@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_xbins(const uint32_t* __restrict__ s
 }
 
 #ifndef SPH_DENSITY_TILE
-#define SPH_DENSITY_TILE 1024
+#define SPH_DENSITY_TILE 1280
 #endif
 #ifndef SPH_FORCE_TILE
 #define SPH_FORCE_TILE 640
@@ -189,7 +189,7 @@ constexpr int kWavesPerBlock = kBlock / 64;
 // bit-identical results: the tile size only decides which runs are read from
 // LDS instead of global memory, the list capacity only when queued pairs are
 // evaluated, and the grid shape only which block processes which particles.
-//   bits 0-3   density LDS tile: 1024 (default), 512, 1536, 2048, 768, 1280
+//   bits 0-3   density LDS tile: 1280 (default), 512, 1536, 2048, 768, 1024
 //   bits 4-7   force tile / list capacity / list entry:  640/16/u16 (default),
 //              768/16/u16, 768/24/u16, 768/8/u32, 512/16/u16, 1024/16/u16,
 //              512/8/u32, 640/24/u16, 1024/8/u32, 512/24/u16
@@ -766,7 +766,7 @@ void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
             case 2: SPH_DENSITY_VARIANT(1536); return;
             case 3: SPH_DENSITY_VARIANT(2048); return;
             case 4: SPH_DENSITY_VARIANT(768); return;
-            case 5: SPH_DENSITY_VARIANT(1280); return;
+            case 5: SPH_DENSITY_VARIANT(1024); return;
             default: break;
         }
     }

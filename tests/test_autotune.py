@@ -21,7 +21,7 @@ WORDS = [0x0000, 0x0001, 0x0002, 0x0003, 0x0004, 0x0005, 0x0010, 0x0020, 0x0030,
 # ---------------------------------------------------------------- CPU
 def test_describe_tune_words():
     d = autotune.describe(0)
-    assert d == {"density_tile": 1024, "force_tile": 640, "force_list_cap": 16,
+    assert d == {"density_tile": 1280, "force_tile": 640, "force_list_cap": 16,
                  "force_list_bits": 16, "density_blocks_per_cu": None,
                  "force_blocks_per_cu": None, "block_threads": 256}
     d = autotune.describe(0x4432)
