@@ -5,8 +5,11 @@
 //                  histograms, plus the empty-cell reset of the ranges
 //                  (computeHashs :401-424 and the two cudaMemsets :1118-1127)
 //   radix passes   stable (hash, slot) sort          (thrust, :1110-1115)
-//   k_reorder      gather of x, v into sorted slots and cell start/end
+//   k_reorder      cell start/end, and the gather of x, v into sorted slots
+//                  when a pass needs the sorted copies
 //                  (reorderAndComputeCellStartEndD :478-534)
+//   k_xorder/xbins internal x-order of each cell (reads the state through
+//                  the sort's permutation) and its x-bin tables
 //   k_density      Poly6 density                    (:536-599, :198-229)
 //   k_force        spiky pressure + viscosity Laplacian (+ tension) fused with
 //                  symplectic Euler and the wall clamp (:601-711, :713-755)
@@ -27,7 +30,11 @@ constexpr int kBlock = 256;
 
 // Boundary detection compares each sorted key with its predecessor read
 // straight from global memory (no block barrier, so no divergent
-// __syncthreads as in the reference at Solver.cu:494-518).
+// __syncthreads as in the reference at Solver.cu:494-518).  With kCopy the
+// state is also gathered into the sorted copies tpos / tvel; without it only
+// the cell ranges are written (the internal x-order then reads the state
+// through perm itself).
+template <bool kCopy>
 __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ perm,
                                                     const float4* __restrict__ pos,
@@ -54,9 +61,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
         }
     }
     if (s == n - 1) cell_end[k] = n;
-    const uint32_t j = min(perm[s], n - 1);
-    tpos[s] = pos[j];
-    tvel[s] = vel[j];
+    if (kCopy) {
+        const uint32_t j = min(perm[s], n - 1);
+        tpos[s] = pos[j];
+        tvel[s] = vel[j];
+    }
 }
 
 // Internal x-order: slot s of the sorted order (cell c, run [b, e)) moves to
@@ -66,10 +75,12 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
 // a direct count over the cell.  The block stages
 // the x of every particle in its cells' runs in LDS (lanes of one cell then
 // read the same LDS word each step); runs beyond the staged range are read
-// from global memory.
+// from global memory.  The state of sorted slot j is tpos[j] / tvel[j], or,
+// with kPerm, pos[perm[j]] / vel[perm[j]] of the previous order (tpos / tvel
+// then name pos / vel).
 constexpr int kXorderTile = 2048;
 
-template <bool kBatched>
+template <bool kBatched, bool kPerm = false>
 __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ skeys,
                                                    const float4* __restrict__ tpos,
                                                    const float4* __restrict__ tvel,
@@ -81,10 +92,12 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
                                                    float4* __restrict__ ipos,
                                                    float4* __restrict__ ivel,
                                                    uint32_t* __restrict__ i2r,
-                                                   uint4* __restrict__ bins) {
+                                                   uint4* __restrict__ bins,
+                                                   const uint32_t* __restrict__ perm = nullptr) {
     __shared__ float xs_tile[kXorderTile];
     __shared__ uint32_t red[2 * (kBlock / 64)];
     const uint32_t n = *n_dev;
+    auto slot = [&](uint32_t j) { return kPerm ? min(perm[j], n - 1) : j; };
     const uint32_t block0 = blockIdx.x * kBlock;
     if (block0 >= n) return;  // block-uniform
     const uint32_t s = block0 + threadIdx.x;
@@ -99,11 +112,12 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
     uint32_t ub, ue;
     block_run_union<kBlock>(b, e, red, ub, ue);
     const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kXorderTile)) : 0u;
-    for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) xs_tile[k] = tpos[ub + k].x;
+    for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) xs_tile[k] = tpos[slot(ub + k)].x;
     __syncthreads();
     if (!active) return;
-    const float4 xs = tpos[s];
-    const float4 vs = tvel[s];
+    const uint32_t src = slot(s);
+    const float4 xs = tpos[src];
+    const float4 vs = tvel[src];
     const bool staged = e <= ub + seg_n;
     uint32_t rank = 0;
     if (staged) {
@@ -113,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
         }
     } else {
         for (uint32_t j = b; j < e; ++j) {
-            const float xj = tpos[j].x;
+            const float xj = tpos[slot(j)].x;
             rank += ((xj < xs.x) | ((xj == xs.x) & (j < s))) ? 1u : 0u;
         }
     }
@@ -840,19 +854,36 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
             if (rc != SPH_OK) return rc;
             break;
         }
-        case kPassReorder:
-            hipLaunchKernelGGL(k_reorder, blocks_for(p.n), dim3(kBlock), 0, stream, b.skeys,
-                               b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start, b.cell_end,
-                               b.n_dev, p.num_cells);
+        case kPassReorder: {
+            // The sorted copies tpos / tvel are read by the colour, wall and
+            // XSPH passes and by a step without the x-order; otherwise the
+            // x-order reads the state through perm and no copy is made.
+            const bool copy = !b.ipos || b.color || (b.walls && b.bacc) ||
+                              (p.flags & SPH_FLAG_XSPH);
+            if (copy)
+                hipLaunchKernelGGL(k_reorder<true>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.skeys, b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start,
+                                   b.cell_end, b.n_dev, p.num_cells);
+            else
+                hipLaunchKernelGGL(k_reorder<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                   b.skeys, b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start,
+                                   b.cell_end, b.n_dev, p.num_cells);
             if (b.ipos) {
-                hipLaunchKernelGGL(k_xorder<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.skeys, b.tpos, b.tvel, b.cell_start, b.cell_end, b.n_dev, p,
-                                   kNoScenes, b.ipos, b.ivel, b.i2r, b.xbins);
+                if (copy)
+                    hipLaunchKernelGGL(k_xorder<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
+                                       b.skeys, b.tpos, b.tvel, b.cell_start, b.cell_end, b.n_dev,
+                                       p, kNoScenes, b.ipos, b.ivel, b.i2r, b.xbins, nullptr);
+                else
+                    hipLaunchKernelGGL((k_xorder<false, true>), blocks_for(p.n), dim3(kBlock), 0,
+                                       stream, b.skeys, b.pos, b.vel, b.cell_start, b.cell_end,
+                                       b.n_dev, p, kNoScenes, b.ipos, b.ivel, b.i2r, b.xbins,
+                                       b.perm);
                 hipLaunchKernelGGL(k_xbins<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
                                    b.skeys, b.ipos, b.ivel, b.cell_start, b.cell_end, b.n_dev, p,
                                    kNoScenes, b.xbins);
             }
             break;
+        }
         case kPassDensity:
             if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
             if (p.flags & SPH_FLAG_KAHAN)
