@@ -314,8 +314,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
 }
 
 // Density-pair adjoint, then the scatter to the step's input order.
-// Candidates are staged in LDS as (x, dL/drho).
-__global__ __launch_bounds__(kBlock) void k_adj_density(
+// Candidates are staged in LDS as (x, dL/drho).  Five waves per SIMD fit in
+// 93 VGPRs (the compiler's own choice, 106, allowed four).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_adj_density(
     const float4* __restrict__ tpos, const float4* __restrict__ tvel,
     const float* __restrict__ grho, const uint32_t* __restrict__ cs,
     const uint32_t* __restrict__ ce, const uint4* __restrict__ bins,
@@ -361,7 +362,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(
                     keep_lds_quad(q[u]);
                     const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    if ((t + u < t1) & pair_in_range(r2, h2_bits_m1)) pair(dx, dy, dz, r2, q[u].w);
+                    // Branch-free: a candidate outside the run or out of range
+                    // adds zero.
+                    const bool hit = (t + u < t1) & pair_in_range(r2, h2_bits_m1);
+                    const float a = hit ? h2 - r2 : 0.0f;
+                    const float c = a * a * (gi + q[u].w);
+                    sx += c * dx;
+                    sy += c * dy;
+                    sz += c * dz;
                 }
             }
         },

@@ -2,7 +2,8 @@
 // for gfx950.  Replaces thrust::sort_by_key(hash, ids)
 // (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:1110-1115).
 //
-// Single-sweep scheme, 8-bit digits (fewer bits in the last pass):
+// Single-sweep scheme, 10-bit digits (fewer bits in the last pass; 19-bit
+// cell hashes take two passes), one thread per digit in the pass kernel:
 //   k_prepare   zeroes the digit histograms, tile tickets and look-back
 //               status words (and, for the solver, resets the cell ranges)
 //   k_hist      one read of the keys -> LDS digit histograms of every pass
@@ -38,7 +39,7 @@ constexpr int kBlock = 256;
 constexpr int kPassBlock = SPH_SORT_BLOCK;
 constexpr int kWaves = kPassBlock / 64;
 static_assert(kPassBlock >= 256 && kPassBlock <= 1024 && kPassBlock % 64 == 0,
-              "a digit pass needs one thread per digit");
+              "digit-pass block shape");
 #ifndef SPH_HIST_ITEMS
 #define SPH_HIST_ITEMS 8
 #endif
@@ -48,8 +49,12 @@ constexpr int kTile = kPassBlock * kItems;
 // The histogram kernel reads its keys in smaller tiles (more blocks in flight).
 constexpr int kHistItems = SPH_HIST_ITEMS;
 constexpr int kHistTile = kBlock * kHistItems;
-constexpr int kRadix = 256;
-constexpr int kMaxPasses = 4;
+#ifndef SPH_SORT_DIGIT_BITS
+#define SPH_SORT_DIGIT_BITS 10
+#endif
+constexpr int kDigitBits = SPH_SORT_DIGIT_BITS;
+constexpr int kRadix = 1 << kDigitBits;
+constexpr int kMaxPasses = (32 + kDigitBits - 1) / kDigitBits;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kCountMask = (1u << 30) - 1u;
@@ -178,8 +183,8 @@ __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __rest
         const bool valid = static_cast<uint32_t>(lane) < nvalid;
         const uint32_t prev = static_cast<uint32_t>(__shfl_up(static_cast<int>(key[r]), 1));
         for (int p = 0; p < passes; ++p) {
-            const int shift = 8 * p;
-            const int bits = min(8, static_cast<int>(key_bits) - shift);
+            const int shift = kDigitBits * p;
+            const int bits = min(kDigitBits, static_cast<int>(key_bits) - shift);
             const uint32_t m = (1u << bits) - 1u;
             const uint32_t d = (key[r] >> shift) & m;
             const bool start = valid && (lane == 0 || ((prev >> shift) & m) != d);
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __rest
     }
     __syncthreads();
     for (int p = 0; p < passes; ++p) {
-        const int bits = min(8, static_cast<int>(key_bits) - 8 * p);
+        const int bits = min(kDigitBits, static_cast<int>(key_bits) - kDigitBits * p);
         for (int d = threadIdx.x; d < (1 << bits); d += kBlock)
             if (cnt[p][d]) atomicAdd(&ghist[p * kRadix + d], cnt[p][d]);
     }
@@ -207,7 +212,11 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(
     uint32_t* __restrict__ status, uint32_t* __restrict__ ticket,
     uint32_t* __restrict__ error_word) {
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t wcnt[kWaves][kRadix];
+    // Per-wave digit counts, then their exclusive prefix over the waves: at
+    // most one tile, kTile <= 65535, so 16 bits suffice.
+    static_assert(kRadix <= kPassBlock, "a digit pass needs one thread per digit");
+    static_assert(kTile <= 65535, "16-bit digit counts");
+    __shared__ uint16_t wcnt[kWaves][kRadix];
     __shared__ uint32_t dbase[kRadix];
     const int radix = 1 << bits;
     const uint32_t mask = static_cast<uint32_t>(radix - 1);
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(
     const int t = threadIdx.x;
 
     if (t == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int i = t; i < kWaves * kRadix; i += kPassBlock) (&wcnt[0][0])[i] = 0u;
+    for (int i = t; i < kWaves * kRadix; i += kPassBlock) (&wcnt[0][0])[i] = 0;
     // Exclusive scan of this pass's global digit histogram (threads t < 256).
     if (t < kRadix) dbase[t] = (t < radix) ? ghist[t] : 0u;
     __syncthreads();
@@ -253,7 +262,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(
         // One wave instruction reads the count before the leader's write, and
         // the written value depends on the read, so all lanes see the old count.
         if (valid && lane == __builtin_ctzll(peers))
-            wcnt[wave][d] = before + static_cast<uint32_t>(__popcll(peers));
+            wcnt[wave][d] = static_cast<uint16_t>(before + static_cast<uint32_t>(__popcll(peers)));
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
@@ -262,7 +271,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) {
             const uint32_t c = wcnt[w][t];
-            wcnt[w][t] = run;
+            wcnt[w][t] = static_cast<uint16_t>(run);
             run += c;
         }
         // Decoupled look-back over earlier tiles for digit t, kLookBack
@@ -378,7 +387,7 @@ int prepare_hist(Src src, uint32_t cap, const uint32_t* n_src, uint32_t n_host, 
                  hipStream_t stream) {
     const uint32_t tiles = tiles_for(cap);
     const TempLayout L = layout(temp, cap);
-    const int passes = static_cast<int>((key_bits + 7) / 8);
+    const int passes = static_cast<int>((key_bits + kDigitBits - 1) / kDigitBits);
     const uint32_t zero_words =
         static_cast<uint32_t>(L.status - L.header) + static_cast<uint32_t>(passes) * tiles * kRadix;
     const uint32_t prep_words = zero_words > fill_words ? zero_words : fill_words;
@@ -398,12 +407,12 @@ int run_passes(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_
                hipStream_t stream) {
     const uint32_t tiles = tiles_for(cap);
     const TempLayout L = layout(temp, cap);
-    const int passes = static_cast<int>((key_bits + 7) / 8);
+    const int passes = static_cast<int>((key_bits + kDigitBits - 1) / kDigitBits);
     const uint32_t* kin = keys_in;
     const uint32_t* vin = vals_in;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        const int bits = std::min(8, static_cast<int>(key_bits) - shift);
+        const int shift = kDigitBits * p;
+        const int bits = std::min(kDigitBits, static_cast<int>(key_bits) - shift);
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t* kout = to_out ? keys_out : L.ktmp;
         uint32_t* vout = to_out ? vals_out : L.vtmp;
