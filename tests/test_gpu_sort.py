@@ -44,7 +44,7 @@ def test_random_keys_sizes(n):
     _check(rng.integers(0, 1 << 19, n, dtype=np.uint32), 19)
 
 
-@pytest.mark.parametrize("bits", [1, 3, 7, 8, 9, 15, 16, 17, 24, 31, 32])
+@pytest.mark.parametrize("bits", [1, 3, 7, 8, 9, 10, 11, 15, 16, 17, 20, 21, 24, 30, 31, 32])
 def test_key_widths(bits):
     rng = np.random.default_rng(bits)
     hi = (1 << bits) if bits < 32 else (1 << 32)
