@@ -13,8 +13,9 @@ import numpy as np
 import pytest
 
 import oracle as O
-from sph2s_final_amd import (FIELD_ACCEL, FIELD_CELL_END, FIELD_CELL_START, FIELD_SORTED_HASH,
-                             FIELD_SORTED_INDEX, SPH_DEBUG_ACCEL, ParticleData, Solver, scenes)
+from sph2s_final_amd import (FIELD_ACCEL, FIELD_CELL_END, FIELD_CELL_START, FIELD_DENSITY,
+                             FIELD_SORTED_HASH, FIELD_SORTED_INDEX, SPH_DEBUG_ACCEL,
+                             SPH_DEBUG_COLOR, ParticleData, Solver, scenes)
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +91,24 @@ def test_graph_equals_eager_bitwise():
     b = _one_step(sc, eager=True)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_sorted_copy_and_permutation_read_give_identical_steps():
+    # The colour pass reads the sorted copies, so with it on the reorder pass
+    # writes them and the internal x-order reads them; with it off the x-order
+    # reads the state through the permutation.  The steps must be identical.
+    sc = scenes.dam_break((24, 20, 22), seed=5, velocity_sigma=0.3)
+    print(f"case sorted-copy vs permutation read n={sc.n}", flush=True)
+    out = []
+    for debug in (0, SPH_DEBUG_COLOR):
+        with Solver(ParticleData(sc.n, sc.positions, sc.velocities), None, sc.config) as s:
+            s.set_debug(debug)
+            s.Advance(sc.dt, 12)
+            s.synchronize()
+            assert s.device_errors() == 0
+            out.append([s.positions(), s.velocities(), s.download(FIELD_DENSITY)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
 
 
 def test_rollout_10_steps():
