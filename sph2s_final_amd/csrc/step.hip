@@ -8,8 +8,8 @@
 //   k_reorder      cell start/end, and the gather of x, v into sorted slots
 //                  when a pass needs the sorted copies
 //                  (reorderAndComputeCellStartEndD :478-534)
-//   k_xorder/xbins internal x-order of each cell (reads the state through
-//                  the sort's permutation) and its x-bin tables
+//   k_xorder       internal x-order of each cell (reads the state through
+//                  the sort's permutation) and its x-bin table
 //   k_density      Poly6 density                    (:536-599, :198-229)
 //   k_force        spiky pressure + viscosity Laplacian (+ tension) fused with
 //                  symplectic Euler and the wall clamp (:601-711, :713-755)
@@ -119,70 +119,57 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
     const float4 xs = tpos[src];
     const float4 vs = tvel[src];
     const bool staged = e <= ub + seg_n;
+    // rank: particles before s in (x, sorted slot) order; xg: the smallest x
+    // after it, which is the x of the next particle in the cell's x-order.
     uint32_t rank = 0;
+    float xg = __builtin_inff();
+    auto count = [&](uint32_t j, float xj) {
+        const bool before = (xj < xs.x) | ((xj == xs.x) & (j < s));
+        rank += before ? 1u : 0u;
+        xg = (!before & (j != s)) ? fminf(xg, xj) : xg;
+    };
     if (staged) {
-        for (uint32_t j = b; j < e; ++j) {
-            const float xj = xs_tile[j - ub];
-            rank += ((xj < xs.x) | ((xj == xs.x) & (j < s))) ? 1u : 0u;
-        }
+        for (uint32_t j = b; j < e; ++j) count(j, xs_tile[j - ub]);
     } else {
-        for (uint32_t j = b; j < e; ++j) {
-            const float xj = tpos[slot(j)].x;
-            rank += ((xj < xs.x) | ((xj == xs.x) & (j < s))) ? 1u : 0u;
-        }
+        for (uint32_t j = b; j < e; ++j) count(j, tpos[slot(j)].x);
     }
     const uint32_t q = b + rank;
     ipos[q] = xs;
     ivel[q] = vs;
     i2r[q] = s;
-}
 
-// Per cell, the cumulative counts of its 8 x-bins (packed 16-bit), from the
-// internal order: bins are non-decreasing along a cell's x-sorted particles,
-// so the particle at internal slot q (local index l = q - start) holds the
-// count l + 1 for every bin from its own up to (not including) the next
-// particle's bin, or up to the last bin if it is the cell's last particle;
-// the cell's first particle also writes 0 for the bins below its own.  Every
-// entry is written by exactly one particle.  Cells of more than 0xFFFE
-// particles get the all-ones marker (no trimming).
-template <bool kBatched>
-__global__ __launch_bounds__(kBlock) void k_xbins(const uint32_t* __restrict__ skeys,
-                                                  const float4* __restrict__ ipos,
-                                                  const float4* __restrict__ ivel,
-                                                  const uint32_t* __restrict__ cell_start,
-                                                  const uint32_t* __restrict__ cell_end,
-                                                  const uint32_t* __restrict__ n_dev,
-                                                  StepParams p,
-                                                  const SceneParams* __restrict__ scenes,
-                                                  uint4* __restrict__ bins) {
-    const uint32_t n = *n_dev;
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= n) return;
-    const uint32_t key = min(skeys[q], p.num_cells - 1);
-    const uint32_t b = min(cell_start[key], q), e = max(min(cell_end[key], n), q + 1);
+    // The cell's x-bin table: cumulative counts of its 8 x-bins (packed
+    // 16-bit).  Bins are non-decreasing along the x-order, so the particle of
+    // rank r holds the count r + 1 for every bin from its own up to (not
+    // including) the next particle's bin, or up to the last bin if it is the
+    // cell's last particle; the first particle also writes 0 for the bins
+    // below its own.  Every entry is written by exactly one particle.  Cells
+    // of more than 0xFFFE particles get the all-ones marker (no trimming).
     uint16_t* t = reinterpret_cast<uint16_t*>(bins + key);
-    if (e - b > 0xFFFEu) {
-        if (q == b)
+    const uint32_t cnt = e - b;
+    if (cnt > 0xFFFEu) {
+        if (rank == 0)
             for (int k = 0; k < 8; ++k) t[k] = 0xFFFFu;
         return;
     }
     float ox = p.ox, inv = 1.0f / p.spacing;
     uint32_t local = key, dx = static_cast<uint32_t>(p.dx);
     if (kBatched) {
-        const GridView& g = scenes[__float_as_uint(ivel[q].w)].g;
+        const GridView& g = scenes[__float_as_uint(vs.w)].g;
         ox = g.ox;
         inv = g.inv_spacing;
         dx = static_cast<uint32_t>(g.dx);
         local = key - g.cell_offset;
     }
     const float cx = static_cast<float>(local % dx);
-    const int kq = xbin_of((ipos[q].x - ox) * inv - cx);
-    const int kn = (q + 1 < e) ? max(kq, xbin_of((ipos[q + 1].x - ox) * inv - cx)) : 8;
-    if (q == b)
+    const int kq = xbin_of((xs.x - ox) * inv - cx);
+    const int kn = (rank + 1 < cnt) ? max(kq, xbin_of((xg - ox) * inv - cx)) : 8;
+    if (rank == 0)
         for (int k = 0; k < kq; ++k) t[k] = 0u;
-    const uint16_t cnt = static_cast<uint16_t>(q - b + 1);
-    for (int k = kq; k < kn; ++k) t[k] = cnt;
+    const uint16_t c = static_cast<uint16_t>(rank + 1);
+    for (int k = kq; k < kn; ++k) t[k] = c;
 }
+
 
 #ifndef SPH_DENSITY_TILE
 #define SPH_DENSITY_TILE 1280
@@ -878,9 +865,6 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
                                        stream, b.skeys, b.pos, b.vel, b.cell_start, b.cell_end,
                                        b.n_dev, p, kNoScenes, b.ipos, b.ivel, b.i2r, b.xbins,
                                        b.perm);
-                hipLaunchKernelGGL(k_xbins<false>, blocks_for(p.n), dim3(kBlock), 0, stream,
-                                   b.skeys, b.ipos, b.ivel, b.cell_start, b.cell_end, b.n_dev, p,
-                                   kNoScenes, b.xbins);
             }
             break;
         }
@@ -1024,9 +1008,6 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
             hipLaunchKernelGGL(k_xorder<true>, blocks_for(cap.n), dim3(kBlock), 0, stream, b.skeys,
                                b.tpos, b.tvel, b.cell_start, b.cell_end, b.n_dev, cap, scenes,
                                b.ipos, b.ivel, b.i2r, b.xbins);
-            hipLaunchKernelGGL(k_xbins<true>, blocks_for(cap.n), dim3(kBlock), 0, stream, b.skeys,
-                               b.ipos, b.ivel, b.cell_start, b.cell_end, b.n_dev, cap, scenes,
-                               b.xbins);
             break;
         }
         case kPassDensity:
