@@ -333,14 +333,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     const bool active = s < n;
     const SceneParams& sp = scenes[active ? __float_as_uint(tvel[s].w) : 0u];
     const GridView g = sp.g;
-    const float h2 = sp.p.h2;
+    const float h2 = sp.p.h2, inv_h2 = 1.0f / h2;
     const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float gi = active ? grho[s] : 0.0f;
-    const uint32_t h2_bits_m1 = __float_as_uint(h2) - 1u;
     const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
+    // Sums of a^2 (g_i + g_j) x_ij with a = 1 - r^2 / h^2 clamped to [0, 1]
+    // (one clamped fused multiply-add): the pair's (h^2 - r^2)^2 is h^4 a^2,
+    // and a = 0 out of range.  A coincident pair (x_ij = 0) adds zero.
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    auto pair = [&](float dx, float dy, float dz, float r2, float gj) {
-        const float a = h2 - r2;
+    auto pair = [&](float dx, float dy, float dz, float gj, bool live) {
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float a = live ? __builtin_amdgcn_fmed3f(__builtin_fmaf(-r2, inv_h2, 1.0f), 0.0f, 1.0f)
+                             : 0.0f;
         const float c = a * a * (gi + gj);
         sx += c * dx;
         sy += c * dy;
@@ -360,30 +364,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 #pragma unroll
                 for (int u = 0; u < kAdjGroup; ++u) {
                     keep_lds_quad(q[u]);
-                    const float dx = xi.x - q[u].x, dy = xi.y - q[u].y, dz = xi.z - q[u].z;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    // Branch-free: a candidate outside the run or out of range
-                    // adds zero.
-                    const bool hit = (t + u < t1) & pair_in_range(r2, h2_bits_m1);
-                    const float a = hit ? h2 - r2 : 0.0f;
-                    const float c = a * a * (gi + q[u].w);
-                    sx += c * dx;
-                    sy += c * dy;
-                    sz += c * dz;
+                    // Branch-free: a candidate past the run adds zero.
+                    pair(xi.x - q[u].x, xi.y - q[u].y, xi.z - q[u].z, q[u].w, t + u < t1);
                 }
             }
         },
         [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
                 const float4 xj = tpos[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (pair_in_range(r2, h2_bits_m1)) pair(dx, dy, dz, r2, grho[j]);
+                pair(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z, grho[j], true);
             }
         },
         bins);
     if (!active) return;
-    const float k6 = -6.0f * sp.p.c6m;
+    const float k6 = -6.0f * sp.p.c6m * (h2 * h2);
     float4 X = gxd[s];
     X.x += k6 * sx;
     X.y += k6 * sy;
