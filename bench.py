@@ -332,6 +332,23 @@ def main() -> int:
     pass_ms = {name: per_pass[i] / prof_steps for i, name in enumerate(C.PASS_NAMES)}
     replay.close()
 
+    # Pair work for the FLOP model of the neighbour passes (about 18 flop per
+    # candidate pair over both passes and 50 per in-range pair), sampled every
+    # 50 steps along the same trajectory by a third solver.
+    counter = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
+    counter.Advance(scene.dt, args.warmup)
+    samples, done = [], 0
+    while done < args.steps:
+        k = min(50, args.steps - done)
+        counter.Advance(scene.dt, k)
+        done += k
+        samples.append(counter.count_pairs())
+    counter.close()
+    cand = sum(c for c, _ in samples) / len(samples)
+    nbr = sum(b for _, b in samples) / len(samples)
+    flop_step = 18.0 * cand + 50.0 * nbr
+    valu_tflops = flop_step / (ms_per_step / 1e3) / 1e12
+
     alg = algorithmic_bytes(n, cells)
     force_s = pass_ms["force_integrate"] / 1e3
     achieved = alg["force_integrate"] / force_s / 1e9
@@ -363,6 +380,18 @@ def main() -> int:
         "step_hbm": {"alg_bytes": alg["step"], "achieved_gbs": step_gbs,
                      "frac": step_gbs / HBM_PEAK_GBS},
         "pass_ms": pass_ms,
+        "pass_hbm": {k: {"alg_bytes": alg[k], "gbs": alg[k] / (v / 1e3) / 1e9,
+                         "frac": alg[k] / (v / 1e3) / 1e9 / HBM_PEAK_GBS}
+                     for k, v in pass_ms.items() if k in alg and v > 0},
+        "valu_model": {"candidates_per_particle": cand / n, "neighbours_per_particle": nbr / n,
+                       "flop_per_step": flop_step, "achieved_tflops": valu_tflops,
+                       "peak_tflops": FP32_VALU_PEAK_TFLOPS,
+                       "frac": valu_tflops / FP32_VALU_PEAK_TFLOPS,
+                       "frac_unpacked": valu_tflops / (FP32_VALU_PEAK_TFLOPS / 2),
+                       "samples": len(samples),
+                       "note": "18 flop per candidate pair (distance test, both passes) + 50 per "
+                               "in-range pair, pairs counted on the state every 50 steps; "
+                               "frac_unpacked against half the peak (packed FP32 is not used)"},
         "state_finite": finite,
         "device_errors": dev_errors,
         "profiled_steps": prof_steps,

@@ -234,6 +234,13 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
 #define SPH_DEVERR_CAPACITY  2u   /* two-scale: a split/merge exceeded a capacity */
 int sph_device_errors(sph_solver* s, uint32_t* bits);
 
+/* Pair work of the most recent step (no counterpart in the reference; a
+ * diagnostic for the FLOP model of the neighbour passes): counts[0] = the
+ * candidates one neighbour pass walks (every particle's stencil-row runs),
+ * counts[1] = the pairs among them with 0 < r < h.  Both are 0 when no step
+ * has run since the state was set.  Synchronises the solver's stream. */
+int sph_count_pairs(sph_solver* s, uint64_t counts[2]);
+
 /* ---- launch variants and the autotuner -------------------------------------
  * The reference launches fixed block shapes (Solver.cu:1082-1198).  Here the
  * density and force passes have launch variants, selected by a tune word:

@@ -123,6 +123,7 @@ SIGNATURES = [
     ("sph_export_positions_aos", c_int, [c_void_p, c_void_p, c_int]),
     ("sph_set_debug", c_int, [c_void_p, c_uint32]),
     ("sph_device_errors", c_int, [c_void_p, POINTER(c_uint32)]),
+    ("sph_count_pairs", c_int, [c_void_p, POINTER(c_uint64)]),
     ("sph_profile_passes", c_int, [c_void_p, c_float, c_uint32, POINTER(c_float)]),
     ("sph_autotune", c_int, [c_void_p, c_float, c_uint32, POINTER(c_uint32), POINTER(c_float)]),
     ("sph_set_tune", c_int, [c_void_p, c_uint32]),

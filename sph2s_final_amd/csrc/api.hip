@@ -97,6 +97,7 @@ struct sph_solver {
     hipGraphExec_t graph = nullptr;
     float graph_dt = 0.0f;
     bool graph_accel = false;
+    bool stepped = false;        // a step ran since the state was last set
     uint32_t debug = 0;
     uint32_t tune = 0;           // launch variant of the neighbour passes
     Walls walls{};               // boundary particles (none by default)
@@ -166,6 +167,7 @@ int upload_state(sph_solver* s, const float* pos, const float* vel, uint32_t n) 
     SPH_HIP(hipMemsetAsync(s->buf.rho, 0, sizeof(float) * s->max_n, s->stream));
     SPH_HIP(hipStreamSynchronize(s->stream));
     destroy_graph(s);
+    s->stepped = false;
     return SPH_OK;
 }
 
@@ -467,6 +469,7 @@ int sph_advance(sph_solver* s, float dt, uint32_t n_steps) {
         if (rc != SPH_OK) return rc;
     }
     for (uint32_t i = 0; i < n_steps; ++i) SPH_HIP(hipGraphLaunch(s->graph, s->stream));
+    s->stepped = true;
     return SPH_OK;
 }
 
@@ -481,6 +484,7 @@ int sph_advance_eager(sph_solver* s, float dt, uint32_t n_steps) {
         int rc = launch_step(p, s->buf, dt, s->key_bits, accel, s->stream);
         if (rc != SPH_OK) return rc;
     }
+    if (n_steps > 0 && s->n > 0) s->stepped = true;
     return SPH_OK;
 }
 
@@ -580,6 +584,29 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+int sph_count_pairs(sph_solver* s, uint64_t counts[2]) {
+    if (!s || !counts) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    counts[0] = counts[1] = 0;
+    if (!s->stepped || s->n == 0) return SPH_OK;
+    unsigned long long* d = nullptr;
+    SPH_HIP(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), s->stream);
+    int rc = (e == hipSuccess) ? SPH_OK : hip_error(e, "hipMemsetAsync", __FILE__, __LINE__);
+    if (rc == SPH_OK) rc = launch_count_pairs(make_step_params(s->cfg, s->n), s->buf, d, s->stream);
+    if (rc == SPH_OK) {
+        e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = hip_error(e, "count pairs", __FILE__, __LINE__);
+    }
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (rc != SPH_OK) return rc;
+    counts[0] = h[0];
+    counts[1] = h[1];
+    return SPH_OK;
 }
 
 int sph_device_errors(sph_solver* s, uint32_t* bits) {

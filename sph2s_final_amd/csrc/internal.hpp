@@ -156,6 +156,10 @@ int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& 
 int launch_pack_state(const float* pos_xyz_dev, const float* vel_xyz_dev, uint32_t n,
                       float4* pos, float4* vel, hipStream_t stream);
 int launch_export_aos(const float4* src, uint32_t n, float* dst_xyz, hipStream_t stream);
+// Candidates and in-range pairs (out[0], out[1], added to) of one neighbour
+// pass of the most recent step.
+int launch_count_pairs(const StepParams& p, const StepBuffers& b, unsigned long long* out,
+                       hipStream_t stream);
 
 }  // namespace sph
 

@@ -967,6 +967,58 @@ int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& 
     return SPH_OK;
 }
 
+// Pair work of one neighbour pass of the most recent step, over its internal
+// x-order: the candidates its runs walk (the nine rows and any rows beyond)
+// and the pairs with 0 < r < h among them.  Wave sums, then one 64-bit atomic
+// per wave per counter.
+__global__ __launch_bounds__(kBlock) void k_count_pairs(const float4* __restrict__ ipos,
+                                                        const uint32_t* __restrict__ cs,
+                                                        const uint32_t* __restrict__ ce,
+                                                        const uint4* __restrict__ bins,
+                                                        const uint32_t* __restrict__ n_dev,
+                                                        GridView g, float h2,
+                                                        unsigned long long* __restrict__ out) {
+    const uint32_t n = *n_dev;
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t cand = 0, nbr = 0;
+    if (s < n) {
+        const float4 xi = ipos[s];
+        const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
+        const uint32_t h2_bits_m1 = __float_as_uint(h2) - 1u;
+        auto run = [&](uint32_t b, uint32_t e) {
+            if (b >= e) return;
+            cand += e - b;
+            for (uint32_t j = b; j < e; ++j) {
+                const float4 xj = ipos[j];
+                const float r2 = dist2(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z);
+                nbr += (__float_as_uint(r2) - 1u < h2_bits_m1) ? 1u : 0u;
+            }
+        };
+        for (int rz = pr.sz; rz <= pr.ez; ++rz)
+            for (int ry = pr.sy; ry <= pr.ey; ++ry) {
+                uint32_t b, e;
+                probe_row<true>(g, pr, cs, ce, bins, n, ry, rz, b, e);
+                run(b, e);
+            }
+    }
+    cand = __ockl_wfred_add_u32(cand);
+    nbr = __ockl_wfred_add_u32(nbr);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], static_cast<unsigned long long>(cand));
+        atomicAdd(&out[1], static_cast<unsigned long long>(nbr));
+    }
+}
+
+int launch_count_pairs(const StepParams& p, const StepBuffers& b, unsigned long long* out,
+                       hipStream_t stream) {
+    if (p.n == 0 || !b.ipos) return SPH_OK;
+    const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
+    hipLaunchKernelGGL(k_count_pairs, blocks_for(p.n), dim3(kBlock), 0, stream, b.ipos,
+                       b.cell_start, b.cell_end, b.xbins, b.n_dev, g, p.h2, out);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
 int launch_pack_state(const float* pxyz, const float* vxyz, uint32_t n, float4* pos,
                       float4* vel, hipStream_t stream) {
     if (n == 0) return SPH_OK;

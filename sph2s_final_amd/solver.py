@@ -237,6 +237,13 @@ class Solver:
         C.check(C.lib().sph_device_errors(self._h, byref(bits)))
         return int(bits.value)
 
+    def count_pairs(self) -> tuple[int, int]:
+        """(candidates, pairs with 0 < r < h) that one neighbour pass of the most
+        recent step walked; (0, 0) before the first step.  Synchronises."""
+        out = (ctypes.c_uint64 * 2)()
+        C.check(C.lib().sph_count_pairs(self._h, out))
+        return int(out[0]), int(out[1])
+
     def profile_passes(self, dt: float, steps: int) -> np.ndarray:
         """Runs ``steps`` eager steps with hipEvents around each pass; returns
         the summed milliseconds per pass (PASS_NAMES order)."""

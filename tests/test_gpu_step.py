@@ -111,6 +111,28 @@ def test_sorted_copy_and_permutation_read_give_identical_steps():
         assert np.array_equal(a, b)
 
 
+def test_count_pairs_matches_a_kdtree_count():
+    from scipy.spatial import cKDTree
+    sc = scenes.dam_break((20, 16, 18), seed=7, velocity_sigma=0.2)
+    print(f"case count pairs n={sc.n}", flush=True)
+    with Solver(ParticleData(sc.n, sc.positions, sc.velocities), None, sc.config) as s:
+        assert s.count_pairs() == (0, 0)
+        s.Advance(sc.dt, 1)
+        s.synchronize()
+        cand, nbr = s.count_pairs()
+        assert s.device_errors() == 0
+    # The step's neighbour passes ran on the initial positions.
+    tree = cKDTree(np.asarray(sc.positions, np.float64))
+    h = float(np.float32(sc.config.h))
+    lo = 2 * len(tree.query_pairs(h * (1 - 1e-5)))
+    hi = 2 * len(tree.query_pairs(h * (1 + 1e-5)))
+    assert lo <= nbr <= hi, (lo, nbr, hi)
+    # Every particle's own cell is a candidate run, and the runs cover no more
+    # than the 27 cells of the stencil.
+    assert cand >= nbr + sc.n
+    assert cand < 27 * sc.n * 8
+
+
 def test_rollout_10_steps():
     sc = scenes.dam_break((16, 16, 16), seed=1)
     steps = 10
