@@ -443,6 +443,12 @@ struct sph_rollout {
     float* grho;
     double *kacc, *macc, *red;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // With more than one segment the backward keeps two sets of K records:
+    // the recomputation of one segment (on stream rs) overlaps the adjoint of
+    // the next later one (on the batch's stream).
+    uint32_t n_sets = 1;
+    hipStream_t rs = nullptr;
+    hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_adj[2] = {nullptr, nullptr}, ev_seed = nullptr;
     bool have_forward = false;
 };
 
@@ -522,11 +528,18 @@ extern "C" {
 int sph_rollout_destroy(sph_rollout* r) {
     if (!r) return SPH_OK;
     if (r->b && r->b->stream) (void)hipStreamSynchronize(r->b->stream);
+    if (r->rs) (void)hipStreamSynchronize(r->rs);
     for (auto& s : r->slot)
         if (s.graph) (void)hipGraphExecDestroy(s.graph);
     if (r->arena) (void)hipFree(r->arena);
     if (r->ev_in) (void)hipEventDestroy(r->ev_in);
     if (r->ev_out) (void)hipEventDestroy(r->ev_out);
+    for (int k = 0; k < 2; ++k) {
+        if (r->ev_rec[k]) (void)hipEventDestroy(r->ev_rec[k]);
+        if (r->ev_adj[k]) (void)hipEventDestroy(r->ev_adj[k]);
+    }
+    if (r->ev_seed) (void)hipEventDestroy(r->ev_seed);
+    if (r->rs) (void)hipStreamDestroy(r->rs);
     if (r->b) sph_batch_destroy(r->b);
     delete r;
     return SPH_OK;
@@ -555,8 +568,9 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     const size_t N = r->b->n, C = r->b->total_cells, S = r->b->n_scenes;
     const uint32_t K = n_steps ? r->every : 0u;
     const size_t n_stored = r->n_ckpt ? r->n_ckpt - 1 : 0;  // the last segment keeps records
+    r->n_sets = (r->n_ckpt > 1) ? 2u : 1u;
     std::vector<size_t> sizes = {al(16 * N * n_stored), al(16 * N * n_stored)};
-    for (uint32_t k = 0; k < K; ++k)
+    for (uint32_t k = 0; k < K * r->n_sets; ++k)
         for (size_t v : {16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 4 * N, 4 * N, 4 * C, 4 * C, 16 * C})
             sizes.push_back(al(v));
     for (size_t v : {16 * N, 16 * N, 16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 8 * N, 16 * S})
@@ -566,6 +580,14 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     hipError_t e = hipMalloc(&r->arena, total);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_out, hipEventDisableTiming);
+    if (r->n_sets > 1) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->rs, hipStreamNonBlocking);
+        for (int k = 0; k < 2; ++k) {
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_rec[k], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_adj[k], hipEventDisableTiming);
+        }
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_seed, hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         sph_rollout_destroy(r);
         return hip_error(e, "rollout allocation", __FILE__, __LINE__);
@@ -580,7 +602,7 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     };
     r->ckpt_pos = static_cast<float4*>(take());
     r->ckpt_vel = static_cast<float4*>(take());
-    r->slot.resize(K);
+    r->slot.resize(static_cast<size_t>(K) * r->n_sets);
     for (auto& s : r->slot) {
         s.ipos = static_cast<float4*>(take());
         s.ivel = static_cast<float4*>(take());
@@ -673,24 +695,50 @@ int sph_rollout_backward(sph_rollout* r, const float* grad_final, float* grad_po
     SPH_HIP(hipGetLastError());
     SPH_HIP(hipMemsetAsync(r->kacc, 0, 8 * size_t(n), b->stream));
     SPH_HIP(hipMemsetAsync(r->macc, 0, 8 * size_t(n), b->stream));
+    // Segment c's records live in set (last - c) % 2 (the forward wrote the
+    // last segment's into set 0).  Segment c - 1 is recomputed on rs while
+    // the batch's stream runs the adjoint of segment c; a set is refilled only
+    // after the adjoint that read it, and read only after its recomputation.
+    const uint32_t last = r->n_ckpt ? r->n_ckpt - 1 : 0u;
+    const uint32_t K = r->every;
+    auto set_of = [&](uint32_t c) { return (last - c) & 1u; };
+    auto slot_of = [&](uint32_t c, uint32_t t) -> RolloutSlot& {
+        return r->slot[static_cast<size_t>(set_of(c)) * K + (t - c * K)];
+    };
+    // Every graph exists before any of them is queued (captures run on the
+    // batch's stream).
+    for (auto& sl : r->slot) {
+        rc = slot_graph(r, sl);
+        if (rc != SPH_OK) return rc;
+    }
+    hipStream_t rs = r->n_sets > 1 ? r->rs : b->stream;
+    if (r->n_sets > 1) {  // the seed reads the final state the recomputation overwrites
+        SPH_HIP(hipEventRecord(r->ev_seed, b->stream));
+        SPH_HIP(hipStreamWaitEvent(rs, r->ev_seed, 0));
+    }
+    bool adj_done[2] = {false, false};
     for (uint32_t c = r->n_ckpt; c-- > 0;) {
         const uint32_t t0 = c * r->every, t1 = std::min(r->n_steps, t0 + r->every);
-        if (c + 1 < r->n_ckpt) {  // the last segment's records come from the forward
-            const size_t o = static_cast<size_t>(c) * n;
+        if (c >= 1) {  // recompute segment c - 1 on rs, overlapping the adjoint of c
+            const uint32_t p = c - 1, ps = set_of(p);
+            if (r->n_sets > 1 && adj_done[ps]) SPH_HIP(hipStreamWaitEvent(rs, r->ev_adj[ps], 0));
+            const size_t o = static_cast<size_t>(p) * n;
             SPH_HIP(hipMemcpyAsync(b->buf.pos, r->ckpt_pos + o, 16 * size_t(n),
-                                   hipMemcpyDeviceToDevice, b->stream));
+                                   hipMemcpyDeviceToDevice, rs));
             SPH_HIP(hipMemcpyAsync(b->buf.vel, r->ckpt_vel + o, 16 * size_t(n),
-                                   hipMemcpyDeviceToDevice, b->stream));
-            for (uint32_t t = t0; t < t1; ++t) {
-                RolloutSlot& s = r->slot[t - t0];
-                rc = slot_graph(r, s);
-                if (rc != SPH_OK) return rc;
-                SPH_HIP(hipGraphLaunch(s.graph, b->stream));
-            }
+                                   hipMemcpyDeviceToDevice, rs));
+            const uint32_t p1 = std::min(r->n_steps, p * K + K);
+            for (uint32_t t = p * K; t < p1; ++t) SPH_HIP(hipGraphLaunch(slot_of(p, t).graph, rs));
+            if (r->n_sets > 1) SPH_HIP(hipEventRecord(r->ev_rec[ps], rs));
         }
+        if (c < last && r->n_sets > 1) SPH_HIP(hipStreamWaitEvent(b->stream, r->ev_rec[set_of(c)], 0));
         for (uint32_t t = t1; t-- > t0;) {
-            rc = adjoint_step(r, r->slot[t - t0]);
+            rc = adjoint_step(r, slot_of(c, t));
             if (rc != SPH_OK) return rc;
+        }
+        if (r->n_sets > 1) {
+            SPH_HIP(hipEventRecord(r->ev_adj[set_of(c)], b->stream));
+            adj_done[set_of(c)] = true;
         }
     }
     // The state now holds a recomputed segment's end (or, with one segment,
