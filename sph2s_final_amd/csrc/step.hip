@@ -1052,14 +1052,15 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
         case kPassSort:
             return launch_pass(pass, cap, b, dt, key_bits, write_accel, stream);
         case kPassReorder: {
+            // Cell ranges only, then the x-order reads the state through perm
+            // (no pass of a batched step reads the sorted copies).
             if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
-            StepBuffers sorted_only = b;
-            sorted_only.ipos = nullptr;
-            const int rc = launch_pass(pass, cap, sorted_only, dt, key_bits, write_accel, stream);
-            if (rc != SPH_OK) return rc;
-            hipLaunchKernelGGL(k_xorder<true>, blocks_for(cap.n), dim3(kBlock), 0, stream, b.skeys,
-                               b.tpos, b.tvel, b.cell_start, b.cell_end, b.n_dev, cap, scenes,
-                               b.ipos, b.ivel, b.i2r, b.xbins);
+            hipLaunchKernelGGL(k_reorder<false>, blocks_for(cap.n), dim3(kBlock), 0, stream, b.skeys,
+                               b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start, b.cell_end,
+                               b.n_dev, cap.num_cells);
+            hipLaunchKernelGGL((k_xorder<true, true>), blocks_for(cap.n), dim3(kBlock), 0, stream,
+                               b.skeys, b.pos, b.vel, b.cell_start, b.cell_end, b.n_dev, cap,
+                               scenes, b.ipos, b.ivel, b.i2r, b.xbins, b.perm);
             break;
         }
         case kPassDensity:
