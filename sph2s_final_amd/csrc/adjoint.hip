@@ -350,7 +350,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         sy += c * dy;
         sz += c * dz;
     };
-    walk_stencil_staged<kBlock, kAdjDensityTile, true>(
+    // Padded: the scan below loads whole groups and multiplies the loads past
+    // its run by a zero weight, which needs them finite.
+    walk_stencil_staged<kBlock, kAdjDensityTile, true, kAdjGroup>(
         g, pr, cs, ce, n, active, red,
         [&](uint32_t k, uint32_t j) {
             const float4 x = tpos[j];
@@ -364,7 +366,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 #pragma unroll
                 for (int u = 0; u < kAdjGroup; ++u) {
                     keep_lds_quad(q[u]);
-                    // Branch-free: a candidate past the run adds zero.
+                    // Branch-free: a candidate past the run (a finite pad
+                    // or staged copy) adds zero.
                     pair(xi.x - q[u].x, xi.y - q[u].y, xi.z - q[u].z, q[u].w, t + u < t1);
                 }
             }

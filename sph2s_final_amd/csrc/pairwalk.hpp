@@ -30,9 +30,14 @@ __device__ __forceinline__ void keep_lds_quad(const float4& q) {
 // global_run(b, e): scan global indices [b, e).
 // kX: the indices are of the internal x-order (cells sorted by x inside) and
 // rows are trimmed with the cells' x-bin tables `bins` (probe_row_run_x).
+// kPad: LDS slots [seg_n, seg_n + kPad) after a staged segment also get a
+// candidate (copies of its last one), so a scan that loads whole groups past
+// the end of its run reads finite data (the tile needs kTile + kPad slots).
+// LDS is not cleared between kernels: without the pad those slots hold
+// whatever an earlier workgroup left there, NaN bit patterns included.
 // Every lane of the block must call this (it synchronises the block);
 // inactive lanes pass active = false.
-template <int kBlockThreads, int kTile, bool kX = false, class Stage, class TileRun,
+template <int kBlockThreads, int kTile, bool kX = false, int kPad = 0, class Stage, class TileRun,
           class GlobalRun>
 __device__ __forceinline__ void walk_stencil_staged(const GridView& g, const Probe& pr,
                                                     const uint32_t* __restrict__ cs,
@@ -49,7 +54,13 @@ __device__ __forceinline__ void walk_stencil_staged(const GridView& g, const Pro
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
-            for (uint32_t k = threadIdx.x; k < seg_n; k += kBlockThreads) stage(k, ub + k);
+            if constexpr (kPad == 0) {
+                for (uint32_t k = threadIdx.x; k < seg_n; k += kBlockThreads) stage(k, ub + k);
+            } else {
+                const uint32_t slots = seg_n ? seg_n + kPad : 0u;
+                for (uint32_t k = threadIdx.x; k < slots; k += kBlockThreads)
+                    stage(k, ub + min(k, seg_n - 1));
+            }
             __syncthreads();
             if (b < e) {
                 if (b >= ub && e <= seg_e)
