@@ -282,8 +282,9 @@ def main() -> int:
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not supported "
-              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+        print("bench.py: the solver runs on exactly one GPU; multi-GPU runs are not supported "
+              "(scenes of a batch would be bin-packed onto GPUs, one large scene split into "
+              "x-slabs with one-cell halos; neither is built)", file=sys.stderr)
         return 2
 
     import torch
