@@ -180,3 +180,39 @@ int main() {
                           f"-Wl,-rpath,{os.path.dirname(C.LIB_PATH)}"],
                          capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
+
+
+FACADE_DEMO = os.path.join(ROOT, "tests", "_bin", "facade_demo")
+
+
+@pytest.mark.gpu
+def test_cpp_facade_program_steps_like_the_python_path(tmp_path):
+    """A reference-style host program written against the C++ facade only
+    (tests/facade_demo.cpp: fluid box, canvas tank and pillar walls,
+    MakeConfiguration, Solver::Advance once per step) runs on the GPU, and
+    its exported positions equal, bit for bit, those of the same scene
+    (walled_tank_small, built with the Python mirror of the same classes)
+    stepped through the Python path."""
+    assert os.path.exists(FACADE_DEMO), "tests/_bin/facade_demo is not built (build())"
+    steps = 6
+    out = tmp_path / "xyz.bin"
+    print(f"case facade program steps={steps}", flush=True)
+    res = subprocess.run([FACADE_DEMO, str(steps), str(out)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:4], np.uint32)[0])
+    xyz = np.frombuffer(raw[4:], np.float32).reshape(n, 3)
+
+    sc = scenes.walled_tank_small()  # the same scene through the Python mirror classes
+    assert n == sc.n
+    with S.Solver(S.ParticleData(sc.n, sc.positions), S.ParticleData(len(sc.walls), sc.walls),
+                  sc.config) as s:
+        for _ in range(steps):
+            s.Advance(0.001)
+        s.synchronize()
+        ref = s.positions()
+        assert s.device_errors() == 0
+    assert np.all(np.isfinite(ref))
+    assert not np.array_equal(ref, sc.positions)  # the fluid moved
+    assert np.array_equal(xyz, ref)
