@@ -43,6 +43,21 @@ __host__ __device__ inline GridView make_grid_view(float ox, float oy, float oz,
     return g;
 }
 
+// Offset of a coordinate from the grid origin, clamped into the grid's extent
+// [0, dim * sp], for the pruning tests below.  A particle outside the grid is
+// hashed into a boundary cell (clamped cell coordinates) although it lies
+// beyond that cell's slab, so its distances to the slabs, measured from where
+// it is, would overstate the distance to its own cell's particles and to
+// others outside the grid (such a particle's density once lost even its self
+// term).  Measured from the nearest point of the grid they are lower bounds
+// again, so the pruning stays conservative.  Inside the grid the value is
+// unchanged, and so is every result.  A candidate outside the grid is always
+// beyond its slab on the side away from the grid, so for a target inside the
+// grid the slab distance already bounds its distance from below.
+__device__ __forceinline__ float grid_offset(float x, float o, float sp, int dim) {
+    return fminf(fmaxf(x - o, 0.0f), static_cast<float>(dim) * sp);
+}
+
 // Distance from f to the interval [c*sp, (c+1)*sp].
 __device__ __forceinline__ float slab_gap(float f, int c, float sp) {
     const float lo = static_cast<float>(c) * sp;
@@ -86,7 +101,9 @@ __device__ __forceinline__ void for_each_run(const GridView& g, const uint32_t* 
     const int ex = cell_coord_off(x, g.h, g.ox, g.spacing, g.dx);
     const int ey = cell_coord_off(y, g.h, g.oy, g.spacing, g.dy);
     const int ez = cell_coord_off(z, g.h, g.oz, g.spacing, g.dz);
-    const float fx = x - g.ox, fy = y - g.oy, fz = z - g.oz;
+    const float fx = grid_offset(x, g.ox, g.spacing, g.dx);
+    const float fy = grid_offset(y, g.oy, g.spacing, g.dy);
+    const float fz = grid_offset(z, g.oz, g.spacing, g.dz);
     for (int cz = sz; cz <= ez; ++cz) {
         const float gz = slab_gap(fz, cz, g.spacing);
         for (int cy = sy; cy <= ey; ++cy) {
@@ -137,7 +154,7 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
                           hipStream_t stream, uint32_t* tune_out, float* ms_out);
 
 // Per-particle view of the reference stencil: own cell, stencil bounds and
-// position relative to the grid origin.
+// position relative to the grid origin (clamped into the grid, grid_offset).
 struct Probe {
     float fx, fy, fz;
     int cx, cy, cz;
@@ -155,9 +172,9 @@ __device__ __forceinline__ Probe make_probe(const GridView& g, float x, float y,
     pr.cx = cell_coord(x, g.ox, g.spacing, g.dx);
     pr.cy = cell_coord(y, g.oy, g.spacing, g.dy);
     pr.cz = cell_coord(z, g.oz, g.spacing, g.dz);
-    pr.fx = x - g.ox;
-    pr.fy = y - g.oy;
-    pr.fz = z - g.oz;
+    pr.fx = grid_offset(x, g.ox, g.spacing, g.dx);
+    pr.fy = grid_offset(y, g.oy, g.spacing, g.dy);
+    pr.fz = grid_offset(z, g.oz, g.spacing, g.dz);
     return pr;
 }
 

@@ -210,3 +210,37 @@ def test_compressed_state_parity_after_300_steps():
     ref = O.step(O.params_from_config(sc.config), pos, vel, sc.dt)
     _check_step(sc2, _one_step(sc2), ref)
     del s_rho
+
+
+def test_particles_outside_the_grid_keep_their_neighbours():
+    # Particles outside the grid are hashed into boundary cells (clamped cell
+    # coordinates) but lie beyond those cells' slabs.  The stencil pruning must
+    # still find their neighbours, the particle itself included: without the
+    # clamp of the probe position their density lost even its self term (zero,
+    # then NaN forces).  Here a cluster below the grid, a pair beyond its top
+    # corner and a particle just outside one face, within h of the fluid.
+    sc = scenes.dam_break((12, 12, 12), seed=21)
+    cfg = sc.config
+    o = np.array([cfg.grid.origin[d] for d in range(3)], np.float64)
+    ext = np.array([cfg.grid.dims[d] for d in range(3)], np.float64) * cfg.grid.spacing
+    h = cfg.h
+    pos = sc.positions.astype(np.float64).copy()
+    base = pos[np.argmin(pos[:, 1])]  # a fluid particle on the floor
+    out = [base + [0.0, -(base[1] - o[1]) - 0.3 * h, 0.0],        # just below the grid
+           base + [0.2 * h, -(base[1] - o[1]) - 0.35 * h, 0.1 * h],
+           base + [0.0, -(base[1] - o[1]) - 2.5 * h, 0.0],        # far below, with a partner
+           base + [0.3 * h, -(base[1] - o[1]) - 2.6 * h, 0.2 * h],
+           o + ext + [0.2 * h, 0.1 * h, 0.3 * h],                  # beyond the top corner, a pair
+           o + ext + [0.5 * h, 0.4 * h, 0.2 * h]]
+    pos[:len(out)] = np.array(out)
+    pos = pos.astype(np.float32)
+    vel = sc.velocities.copy()
+    outside = np.any((pos < o) | (pos >= o + ext), axis=1)
+    assert outside.sum() == len(out)
+    sc2 = scenes.Scene("dam_break_12_outside", cfg, pos, vel)
+    ref = O.step(O.params_from_config(cfg), pos, vel, sc.dt)
+    got = _one_step(sc2)
+    self_term = float(np.float32(cfg.mass)) * 315.0 / (64.0 * np.pi * float(h) ** 3)
+    assert np.all(got["rho"] >= 0.999 * self_term)
+    assert np.all(np.isfinite(got["acc"]))
+    _check_step(sc2, got, ref)
