@@ -172,6 +172,35 @@ def test_two_particle_closed_form():
         assert np.allclose(out["acc"][i], a_p + a_v, rtol=1e-10, atol=1e-12)
 
 
+def test_particles_outside_the_grid_keep_their_density():
+    # The reference's stencil loops over clamped cells without pruning, so a
+    # particle outside the grid (hashed into a boundary cell) still sees
+    # itself and its neighbours: an isolated one has exactly the self density
+    # m W(0), a pair at distance r has m (W(0) + W(r)) each, wherever they are.
+    p = _params(gravity=(0.0, 0.0, 0.0), wall_clamp=0)
+    h = float(p.h)
+    lo = np.array([p.grid.origin[d] for d in range(3)], np.float64)
+    hi = lo + np.array([p.grid.dims[d] for d in range(3)], np.float64) * float(p.grid.spacing)
+    r = 0.45 * h
+    pos = np.array([lo + [-0.3, 0.2, 0.2],                    # isolated, below x
+                    hi + [0.5 * h, 0.7 * h, 0.1 * h],          # a pair beyond the top corner
+                    hi + [0.5 * h + r, 0.7 * h, 0.1 * h],
+                    [0.5 * (lo[0] + hi[0]), lo[1] - 3.0 * h, 0.5 * (lo[2] + hi[2])]],  # below y
+                   np.float32)
+    out = O.step(p, pos, np.zeros_like(pos), 1e-3)
+    m = float(p.mass)
+    c6 = 315.0 / (64.0 * math.pi * h ** 9)
+    xs = pos[out["perm"]].astype(np.float64)
+    rho_self = m * c6 * h ** 6
+    rr = np.linalg.norm(xs[:, None, :] - xs[None, :, :], axis=2)
+    expect = np.array([m * c6 * sum((h * h - rr[i, j] ** 2) ** 3 for j in range(len(xs))
+                                    if rr[i, j] < h) for i in range(len(xs))])
+    assert np.allclose(out["rho"], expect, rtol=1e-12)
+    assert sorted(np.round(out["rho"] / rho_self, 9).tolist())[:2] == [1.0, 1.0]
+    pair = m * (c6 * h ** 6 + c6 * (h * h - r * r) ** 3)
+    assert np.sum(np.isclose(out["rho"], pair, rtol=1e-5)) == 2
+
+
 def test_momentum_is_conserved_by_pair_forces():
     sc = scenes.dam_break((10, 10, 10), seed=3, velocity_sigma=0.4)
     p = O.params_from_config(sc.config)
