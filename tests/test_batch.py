@@ -173,3 +173,38 @@ def test_batch_64_scenes_graph_equals_eager_and_single_runs():
                 one.Advance(sc.dt, 5)
                 one.synchronize()
                 assert np.array_equal(pa[a.scene_slice(s)], one.positions()), s
+
+
+@pytest.mark.gpu
+def test_batch_particles_outside_their_grids_match_oracle():
+    # Each scene gets a few particles outside its own grid (below it, and a
+    # close pair beyond its top corner); they are hashed into the scene's
+    # boundary cells and must keep their neighbours and self term, as in the
+    # oracle's unpruned stencil.
+    bs = _small_batch(n_scenes=3, seed=11)
+    for sc in bs.scenes:
+        cfg = sc.config
+        o = np.array([cfg.grid.origin[d] for d in range(3)], np.float64)
+        ext = np.array([cfg.grid.dims[d] for d in range(3)], np.float64) * cfg.grid.spacing
+        h = cfg.h
+        pos = sc.positions.astype(np.float64)
+        base = pos[np.argmin(pos[:, 1])]
+        pos[:4] = [base + [0.0, -(base[1] - o[1]) - 0.4 * h, 0.0],
+                   base + [0.1 * h, -(base[1] - o[1]) - 2.2 * h, 0.0],
+                   o + ext + [0.3 * h, 0.2 * h, 0.1 * h],
+                   o + ext + [0.6 * h, 0.3 * h, 0.3 * h]]
+        sc.positions = pos.astype(np.float32)
+    print(f"case batch_outside counts={bs.counts.tolist()}", flush=True)
+    with _batch(bs) as b:
+        b.advance_eager(bs.dt)
+        b.synchronize()
+        assert b.device_errors() == 0
+        for s, sc in enumerate(bs.scenes):
+            got = _scene_fields(b, s, STEP_FIELDS)
+            ref = O.step(O.params_from_config(sc.config), sc.positions, sc.velocities, sc.dt)
+            for k in ("hash", "perm", "cell_start", "cell_end"):
+                assert np.array_equal(got[k], ref[k]), (s, k)
+            rho_err = np.max(np.abs(got["rho"] - ref["rho"]) / ref["rho"])
+            acc_err = np.max(np.abs(got["acc"] - ref["acc"])) / np.max(np.abs(ref["acc"]))
+            print(f"  scene {s} n={sc.n}: rho {rho_err:.2e} acc {acc_err:.2e}", flush=True)
+            assert rho_err <= 1e-5 and acc_err <= 1e-5
