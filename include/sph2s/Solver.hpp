@@ -179,4 +179,62 @@ private:
     sph_solver* mHandle = nullptr;
 };
 
+// The two-scale Solver (SPH2S_FINAL_TWO_SCALE/Solver.h:120-130): fluidData is
+// the low-resolution set (h0, m0), fluidDataHigh the high-resolution set (h0/2,
+// m0/8), which starts empty and is filled by splits (its MaxParticles is the
+// capacity), boundaryData the static walls.  The reference triggers a split
+// from the renderer's surface marks (left commented out upstream,
+// TWO_SCALE/Solver.cu:1077-1082); here low-resolution particles inside
+// [regionLo, regionHi] split, and setOptions / sph2_mark_split select the
+// reference's blended coupling and per-particle trigger.  Advance reads both
+// counts back into NumParticles, as the reference's per-step count copy does
+// (TWO_SCALE/Solver.cu:1633-1638), so it synchronises once per step.
+class TwoScaleSolver {
+public:
+    TwoScaleSolver(ParticleData* fluidData, ParticleData* fluidDataHigh,
+                   ParticleData* boundaryData, const SolverConfiguration* configuration,
+                   float3 regionLo, float3 regionHi)
+        : mLow(fluidData), mHigh(fluidDataHigh) {
+        const float lo[3] = {regionLo.x, regionLo.y, regionLo.z};
+        const float hi[3] = {regionHi.x, regionHi.y, regionHi.z};
+        const float* vel = fluidData->hostVelocities.empty() ? nullptr
+                                                             : fluidData->hostVelocities.data();
+        SafeCall(sph2_create(configuration, lo, hi, fluidData->hostPositions.data(), vel,
+                             fluidData->NumParticles, fluidData->MaxParticles,
+                             fluidDataHigh->MaxParticles, &mHandle));
+        if (boundaryData && boundaryData->NumParticles) {
+            const int rc = sph2_set_boundary(mHandle, boundaryData->hostPositions.data(),
+                                             boundaryData->NumParticles);
+            if (rc != SPH_OK) {
+                sph2_destroy(mHandle);
+                SafeCall(rc);
+            }
+        }
+        refresh();
+    }
+    ~TwoScaleSolver() { sph2_destroy(mHandle); }
+    TwoScaleSolver(const TwoScaleSolver&) = delete;
+    TwoScaleSolver& operator=(const TwoScaleSolver&) = delete;
+
+    void Bind() const {}
+    void Advance(float dt) {
+        SafeCall(sph2_advance(mHandle, dt, 1));
+        refresh();
+    }
+    void setOptions(const sph2_options& o) { SafeCall(sph2_set_options(mHandle, &o)); }
+    void Synchronize() { SafeCall(sph2_synchronize(mHandle)); }
+    sph_two_scale* handle() { return mHandle; }
+
+private:
+    void refresh() {
+        uint32_t c[4];
+        SafeCall(sph2_counts(mHandle, c));
+        mLow->NumParticles = c[0];
+        mHigh->NumParticles = c[1];
+    }
+    ParticleData* mLow;
+    ParticleData* mHigh;
+    sph_two_scale* mHandle = nullptr;
+};
+
 }  // namespace sph2s

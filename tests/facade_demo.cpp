@@ -4,19 +4,66 @@
 // canvas tank and a pillar as boundary particles, MakeConfiguration), then
 // Solver::Advance(dt) once per step.  Writes NumParticles and the state's
 // positions (sph_export_positions_aos, 3 floats per particle, state order) to
-// the file named by argv[2] after argv[1] steps.  Built by build() into
+// the file named by argv[2] after argv[1] steps.  "two <steps> <out>" runs
+// the two-scale form (TwoScaleSolver) instead.  Built by build() into
 // tests/_bin/; the GPU tests run it and compare with the Python path.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "sph2s/Solver.hpp"
 
 using namespace sph2s;
 
+// Two-scale mode: a lattice box of fluid, MakeConfiguration, a split region
+// over one corner of it, TwoScaleSolver::Advance once per step; writes the
+// (low, high) counts and both sets' positions.
+static int two_scale(int steps, const char* path) {
+    Grid fluidGrid = Grid::MakeGrid(make_float3(0.02f, 0.02f, 0.02f),
+                                    make_float3(0.2f, 0.2f, 0.2f), 0.01f);
+    ParticleData* low = ParticleData::CreateParticleBox(fluidGrid);
+    ParticleData* high = new ParticleData(8 * low->MaxParticles);
+    SolverConfiguration c = SolverConfiguration::MakeConfiguration(
+        make_float3(0.0f, 0.0f, 0.0f), make_float3(0.4f, 0.4f, 0.4f),
+        Grid::ComputeVolume(fluidGrid), low->MaxParticles, 35, 1000.0f, 30.0f, 5.0f, 88.1472f,
+        0.0f);
+    TwoScaleSolver* s = new TwoScaleSolver(low, high, nullptr, &c,
+                                           make_float3(0.02f, 0.02f, 0.02f),
+                                           make_float3(0.1f, 0.1f, 0.1f));
+    s->Bind();
+    for (int t = 0; t < steps; ++t) s->Advance(0.001f);
+    s->Synchronize();
+    const unsigned n[2] = {low->NumParticles, high->NumParticles};
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return 3;
+    std::fwrite(n, sizeof(unsigned), 2, f);
+    for (int r = 0; r < 2; ++r) {
+        std::vector<float> xyz(3 * static_cast<size_t>(n[r]));
+        if (n[r])
+            SafeCall(sph2_download(s->handle(), r, SPH_FIELD_POSITION, xyz.data(),
+                                   xyz.size() * sizeof(float)));
+        std::fwrite(xyz.data(), sizeof(float), xyz.size(), f);
+    }
+    std::fclose(f);
+    std::printf("facade_demo two-scale: %u low, %u high, %d steps\n", n[0], n[1], steps);
+    delete s;
+    delete low;
+    delete high;
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 4 && std::string(argv[1]) == "two") {
+        try {
+            return two_scale(std::atoi(argv[2]), argv[3]);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "facade_demo: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc != 3) {
-        std::fprintf(stderr, "usage: facade_demo <steps> <out.bin>\n");
+        std::fprintf(stderr, "usage: facade_demo <steps> <out.bin> | two <steps> <out.bin>\n");
         return 2;
     }
     const int steps = std::atoi(argv[1]);

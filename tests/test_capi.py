@@ -216,3 +216,43 @@ def test_cpp_facade_program_steps_like_the_python_path(tmp_path):
     assert np.all(np.isfinite(ref))
     assert not np.array_equal(ref, sc.positions)  # the fluid moved
     assert np.array_equal(xyz, ref)
+
+
+@pytest.mark.gpu
+def test_cpp_facade_two_scale_program_steps_like_the_python_path(tmp_path):
+    """The facade's TwoScaleSolver (the reference's two-scale Solver(fluid,
+    fluidHigh, boundary, cfg) plus a split region) in the facade program:
+    after 6 steps (splits in the region box) its (low, high) counts and both
+    sets' positions equal, bit for bit, the same scene stepped through the
+    Python TwoScaleSolver."""
+    assert os.path.exists(FACADE_DEMO), "tests/_bin/facade_demo is not built (build())"
+    steps = 6
+    out = tmp_path / "two.bin"
+    print(f"case facade two-scale program steps={steps}", flush=True)
+    res = subprocess.run([FACADE_DEMO, "two", str(steps), str(out)], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    raw = out.read_bytes()
+    n_low, n_high = (int(v) for v in np.frombuffer(raw[:8], np.uint32))
+    xyz = np.frombuffer(raw[8:], np.float32)
+    low, high = xyz[:3 * n_low].reshape(-1, 3), xyz[3 * n_low:].reshape(-1, 3)
+
+    grid = S.Grid.MakeGrid((0.02, 0.02, 0.02), (0.2, 0.2, 0.2), 0.01)
+    fluid = S.ParticleData.CreateParticleBox(grid)
+    cfg = S.SolverConfiguration.MakeConfiguration(
+        (0.0, 0.0, 0.0), (0.4, 0.4, 0.4), S.Grid.ComputeVolume(grid), fluid.MaxParticles, 35,
+        1000.0, 30.0, 5.0, 88.1472, 0.0)
+    t = S.TwoScaleSolver(cfg, (0.02, 0.02, 0.02), (0.1, 0.1, 0.1), fluid.positions, None,
+                         max_low=fluid.MaxParticles, max_high=8 * fluid.MaxParticles)
+    try:
+        for _ in range(steps):
+            t.Advance(0.001)
+        t.synchronize()
+        c = t.counts()
+        assert t.device_errors() == 0
+        assert (n_low, n_high) == c[:2]
+        assert n_high > 0 and n_low < fluid.MaxParticles  # the region split
+        assert np.array_equal(low, t.download(0, S.FIELD_POSITION))
+        assert np.array_equal(high, t.download(1, S.FIELD_POSITION))
+    finally:
+        t.close()
