@@ -76,6 +76,8 @@ def lib() -> ctypes.CDLL:
                                         vp, vp]
         L.oracle_rollout_walls.argtypes = [P, W, vp, vp, c_uint32, c_double, c_uint32, vp, vp,
                                            vp]
+        L.oracle_rollout_rho.argtypes = [P, W, vp, vp, c_uint32, c_double, c_uint32, vp, vp, vp,
+                                         vp]
         L.oracle_energy_colors.argtypes = [P, vp, vp, c_uint32, vp, vp]
         L.oracle_surface_indicator.argtypes = [P, vp, vp, c_uint32, vp, vp, c_uint32, vp]
         L.oracle_boundary_weight.argtypes = [c_double, c_double, c_double]
@@ -170,17 +172,20 @@ def step(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, wall=None
 
 
 def rollout(p: OracleParams, pos: np.ndarray, vel: np.ndarray, dt: float, steps: int,
-            wall=None):
+            wall=None, with_density: bool = False):
+    """(x, v, ids) after ``steps`` FP64 steps, in the last step's sorted order;
+    with_density adds the last step's densities in the same order."""
     pos = np.ascontiguousarray(pos, np.float32)
     vel = np.ascontiguousarray(vel, np.float32)
     n = pos.shape[0]
     x = np.empty((n, 3))
     v = np.empty((n, 3))
     ids = np.empty(n, np.uint32)
-    lib().oracle_rollout_walls(ctypes.byref(p), None if wall is None else ctypes.byref(wall),
-                               _p(pos), _p(vel), n, float(dt), int(steps), _p(x), _p(v),
-                               _p(ids))
-    return x, v, ids
+    rho = np.empty(n)
+    lib().oracle_rollout_rho(ctypes.byref(p), None if wall is None else ctypes.byref(wall),
+                             _p(pos), _p(vel), n, float(dt), int(steps), _p(x), _p(v),
+                             _p(ids), _p(rho))
+    return (x, v, ids, rho) if with_density else (x, v, ids)
 
 
 def energy_colors(p: OracleParams, pos, vel):
@@ -322,6 +327,7 @@ class OracleTwoScale:
             L.oracle_ts_set_blend.argtypes = [vp, c_int, vp, vp]
             L.oracle_ts_mark_split.argtypes = [vp, vp, c_uint32]
             L.oracle_ts_set_walls.argtypes = [vp, POINTER(OracleWalls)]
+            L.oracle_ts_plant_mistake.argtypes = [c_int]
             L._ts_bound = True
         g = OracleGrid()
         for d in range(3):
@@ -370,6 +376,14 @@ class OracleTwoScale:
     def mark_split(self, ids):
         ids = np.ascontiguousarray(ids, np.uint32)
         lib().oracle_ts_mark_split(self._h, _p(ids) if len(ids) else None, len(ids))
+
+    @staticmethod
+    def plant_mistake(which: int) -> None:
+        """Plant one deliberate mistake in the fixed coupling's cross sums
+        (oracle_ts_plant_mistake; 0 removes it).  Tests use it to show that
+        their checks catch such a mistake."""
+        lib().oracle_ts_plant_mistake.argtypes = [c_int]
+        lib().oracle_ts_plant_mistake(int(which))
 
     def step(self, dt: float) -> None:
         lib().oracle_ts_step(self._h, float(dt))

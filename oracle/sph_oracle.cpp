@@ -508,6 +508,12 @@ void oracle_rollout(const oracle_params* p, const float* pos, const float* vel,
 void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
                           const float* vel, uint32_t n, double dt, uint32_t n_steps,
                           double* pos_out, double* vel_out, uint32_t* ids_out) {
+    oracle_rollout_rho(p, w, pos, vel, n, dt, n_steps, pos_out, vel_out, ids_out, nullptr);
+}
+
+void oracle_rollout_rho(const oracle_params* p, const oracle_walls* w, const float* pos,
+                        const float* vel, uint32_t n, double dt, uint32_t n_steps,
+                        double* pos_out, double* vel_out, uint32_t* ids_out, double* rho_out) {
     State s = make_state(pos, vel, n);
     Neighbourhood nb;
     std::vector<double> r, a;
@@ -526,6 +532,7 @@ void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const f
     if (pos_out) std::copy(s.x.begin(), s.x.end(), pos_out);
     if (vel_out) std::copy(s.v.begin(), s.v.end(), vel_out);
     if (ids_out) std::copy(s.id.begin(), s.id.end(), ids_out);
+    if (rho_out) std::copy(r.begin(), r.end(), rho_out);
 }
 
 void oracle_density_bruteforce(const oracle_params* p, const double* x, uint32_t n,
@@ -872,6 +879,9 @@ namespace {
 
 inline bool ts_blended(const oracle_ts* t) { return t->opt.coupling == 1u; }
 
+// The mistake planted by oracle_ts_plant_mistake (0 = none); see sph_oracle.h.
+int g_planted = 0;
+
 // Fixed coupling: own-set density and acceleration of the single-scale step,
 // plus the complementary sums with blend weight 1.
 void ts_forces_fixed(oracle_ts* t) {
@@ -888,6 +898,9 @@ void ts_forces_fixed(oracle_ts* t) {
         densities(S.p, st, S.nb, S.rho);
     }
     // complementary densities: + m_B sum_{r < h0} 1/2 (W_h0 + W_h1)
+    const double half = (g_planted == 3) ? 1.0 : 0.5;
+    const double cut = (g_planted == 2) ? h1 : h0;
+    const float reach = (g_planted == 4) ? H.p.h_f32 : L.p.h_f32;
     std::vector<double> rho_new[2];
     for (int r = 0; r < 2; ++r) {
         TsSet& A = t->set[r];
@@ -898,22 +911,22 @@ void ts_forces_fixed(oracle_ts* t) {
             const double* xi = &A.x[3 * i];
             double sum = 0.0;
             if (!B.id.empty())
-                for_each_cell_in(B.p.grid, L.p.h_f32, xi, [&](uint32_t c) {
+                for_each_cell_in(B.p.grid, reach, xi, [&](uint32_t c) {
                     const uint32_t b = B.nb.start[c], e = B.nb.end[c];
                     if (b == kEmpty) return;
                     for (uint32_t j = b; j < e; ++j) {
                         const double dx = xi[0] - B.x[3 * j], dy = xi[1] - B.x[3 * j + 1],
                                      dz = xi[2] - B.x[3 * j + 2];
                         const double rr = std::sqrt(dx * dx + dy * dy + dz * dz);
-                        if (rr < h0) {
+                        if (rr < cut) {
                             const double a0 = h0 * h0 - rr * rr, a1 = h1 * h1 - rr * rr;
                             const double w0 = k0.c6 * a0 * a0 * a0;
                             const double w1 = (rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0;
-                            sum += 0.5 * (w0 + w1);
+                            sum += half * (w0 + w1);
                         }
                     }
                 });
-            rho_new[r][i] += B.p.mass * sum;
+            rho_new[r][i] += ((g_planted == 1) ? A.p.mass : B.p.mass) * sum;
         }
     }
     L.rho = rho_new[0];
@@ -933,8 +946,9 @@ void ts_forces_fixed(oracle_ts* t) {
             const double rhoi = A.rho[i];
             const double Ai = A.p.stiffness * (rhoi - A.p.rest_density) / (rhoi * rhoi);
             double f[3] = {0.0, 0.0, 0.0};
+            const double mB = (g_planted == 1) ? A.p.mass : B.p.mass;
             if (!B.id.empty())
-                for_each_cell_in(B.p.grid, L.p.h_f32, xi, [&](uint32_t c) {
+                for_each_cell_in(B.p.grid, reach, xi, [&](uint32_t c) {
                     const uint32_t b = B.nb.start[c], e = B.nb.end[c];
                     if (b == kEmpty) return;
                     for (uint32_t j = b; j < e; ++j) {
@@ -942,20 +956,20 @@ void ts_forces_fixed(oracle_ts* t) {
                                                xi[2] - B.x[3 * j + 2]};
                         const double rr = std::sqrt(xij[0] * xij[0] + xij[1] * xij[1] +
                                                     xij[2] * xij[2]);
-                        if (!(rr != 0.0 && rr < h0)) continue;
+                        if (!(rr != 0.0 && rr < cut)) continue;
                         const double rhoj = B.rho[j];
                         const double Aj = A.p.stiffness * (rhoj - A.p.rest_density) / (rhoj * rhoj);
                         const double hr0 = h0 - rr, hr1 = std::max(h1 - rr, 0.0);
-                        const double g = 0.5 * (k0.c45 * hr0 * hr0 + k1.c45 * hr1 * hr1) / rr;
-                        const double l = 0.5 * (k0.c45 * hr0 + k1.c45 * hr1);
+                        const double g = half * (k0.c45 * hr0 * hr0 + k1.c45 * hr1 * hr1) / rr;
+                        const double l = half * (k0.c45 * hr0 + k1.c45 * hr1);
                         const double a0 = h0 * h0 - rr * rr, a1 = h1 * h1 - rr * rr;
-                        const double w = 0.5 * (k0.c6 * a0 * a0 * a0 +
-                                                ((rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0));
+                        const double w = half * (k0.c6 * a0 * a0 * a0 +
+                                                 ((rr < h1) ? k1.c6 * a1 * a1 * a1 : 0.0));
                         for (int d = 0; d < 3; ++d) {
-                            f[d] += B.p.mass * (Ai + Aj) * g * xij[d];
-                            f[d] += A.p.viscosity * B.p.mass / (rhoi * rhoj) * l *
+                            f[d] += mB * (Ai + Aj) * g * xij[d];
+                            f[d] += A.p.viscosity * mB / (rhoi * rhoj) * l *
                                     (B.v[3 * j + d] - vi[d]);
-                            f[d] -= A.p.tension * B.p.mass / rhoi * w * xij[d];
+                            f[d] -= A.p.tension * mB / rhoi * w * xij[d];
                         }
                     }
                 });
@@ -1115,6 +1129,8 @@ oracle_ts* oracle_ts_create(const oracle_params* p_low, const oracle_grid* grid_
 }
 
 void oracle_ts_destroy(oracle_ts* t) { delete t; }
+
+void oracle_ts_plant_mistake(int which) { g_planted = which; }
 
 void oracle_ts_set_state(oracle_ts* t, const float* pos_l, const float* vel_l,
                          const uint32_t* ids_l, uint32_t n_l, const float* pos_h,

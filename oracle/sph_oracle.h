@@ -99,6 +99,11 @@ void oracle_step_walls(const oracle_params* p, const oracle_walls* w, const floa
 void oracle_rollout_walls(const oracle_params* p, const oracle_walls* w, const float* pos,
                           const float* vel, uint32_t n, double dt, uint32_t n_steps,
                           double* pos_out, double* vel_out, uint32_t* ids_out);
+/* oracle_rollout_walls that also returns the last step's densities (rho_out,
+ * n f64 in the order of pos_out / ids_out: the last step's sorted order). */
+void oracle_rollout_rho(const oracle_params* p, const oracle_walls* w, const float* pos,
+                        const float* vel, uint32_t n, double dt, uint32_t n_steps,
+                        double* pos_out, double* vel_out, uint32_t* ids_out, double* rho_out);
 
 /* Colour value of every particle of a state (FP32 AoS pos/vel, n), by input
  * index: the Mexican-hat energy of SPH2S_FINAL_TWO_SCALE/Solver.cu:206-213,
@@ -194,6 +199,14 @@ void oracle_ts_mark_split(oracle_ts* t, const uint32_t* ids, uint32_t n);
  * sets get the boundary acceleration with h0 and m0
  * (SPH2S_FINAL_TWO_SCALE/Solver.cu:454-499, 1007-1044).  w may be NULL. */
 void oracle_ts_set_walls(oracle_ts* t, const oracle_walls* w);
+
+/* Plants one deliberate mistake in the complementary (cross-resolution) sums
+ * of the fixed coupling, so that tests can show their checks catch it:
+ * 0 none; 1 the own set's mass in place of the other set's; 2 cutoff h1 in
+ * place of h0; 3 the averaged kernels without the factor 1/2; 4 a stencil
+ * reaching only +/- h1 (not +/- h0) into the other set's grid.  Process-wide;
+ * tests reset it to 0. */
+void oracle_ts_plant_mistake(int which);
 
 #ifdef __cplusplus
 }
