@@ -204,3 +204,62 @@ def test_two_scale_rollout_conserves_mass_and_matches_eager():
                 assert np.array_equal(a.download(r, f), b.download(r, f))
             x = a.download(r, FIELD_POSITION)
             assert np.all(np.isfinite(x))
+
+
+@pytest.mark.gpu
+def test_two_scale_250k_full_size():
+    """The two-scale workload at its own size (63^3 = 250,047 low particles,
+    about 460k active after the first split): 100 graph-replayed steps equal
+    100 eager steps bit for bit, particle count is conserved (8 n_low + n_high
+    = 8 N), the state stays finite with no device error; then one step from
+    that GPU state matches the oracle step from the same FP32 state (per set:
+    sorted hashes, permutation and cell ranges bit-exact, density within 1e-5
+    relative, acceleration within 1e-5 of max |a|, and the split/merge
+    outcome: counts, ids and tags)."""
+    sc = scenes.two_scale_250k()
+    steps = 100
+    print(f"case two_scale_250k n={sc.n} steps={steps}", flush=True)
+    with _gpu(sc, debug=False) as a, _gpu(sc, debug=False) as b:
+        a.Advance(sc.dt, steps)
+        b.advance_eager(sc.dt, steps)
+        a.synchronize()
+        b.synchronize()
+        assert a.device_errors() == 0 and b.device_errors() == 0
+        ca, cb = a.counts(), b.counts()
+        print(f"  counts after {steps} steps: {ca[:2]}", flush=True)
+        assert ca == cb
+        assert 8 * ca[0] + ca[1] == 8 * sc.n and ca[1] > 100_000
+        for r in (0, 1):
+            for f in (FIELD_POSITION, FIELD_VELOCITY, FIELD_ID):
+                assert np.array_equal(a.download(r, f), b.download(r, f)), (r, f)
+            assert np.array_equal(a.download(1, FIELD_TAG), b.download(1, FIELD_TAG))
+            assert np.all(np.isfinite(a.download(r, FIELD_POSITION)))
+            assert np.all(np.isfinite(a.download(r, FIELD_VELOCITY)))
+        # One more step from the GPU state, on the GPU and on the oracle.
+        o = _oracle(sc)
+        o.set_state(a.download(0, FIELD_POSITION), a.download(0, FIELD_VELOCITY),
+                    a.download(0, FIELD_ID), a.download(1, FIELD_POSITION),
+                    a.download(1, FIELD_VELOCITY), a.download(1, FIELD_ID),
+                    a.download(1, FIELD_TAG))
+        a.set_debug(SPH_DEBUG_ACCEL)
+        a.Advance(sc.dt)
+        a.synchronize()
+        o.step(sc.dt)
+        assert a.device_errors() == 0
+        assert a.counts() == o.counts()
+        for r in (0, 1):
+            assert np.array_equal(a.download(r, FIELD_SORTED_HASH), o.get(r, "hash"))
+            assert np.array_equal(a.download(r, FIELD_SORTED_INDEX), o.get(r, "perm"))
+            assert np.array_equal(a.download(r, FIELD_CELL_START), o.get(r, "start"))
+            assert np.array_equal(a.download(r, FIELD_CELL_END), o.get(r, "end"))
+            rho, rho_ref = a.download(r, FIELD_DENSITY), o.get(r, "rho")
+            acc, acc_ref = a.download(r, FIELD_ACCEL), o.get(r, "acc")
+            rho_err = np.max(np.abs(rho - rho_ref) / rho_ref)
+            acc_err = np.max(np.abs(acc - acc_ref)) / np.max(np.abs(acc_ref))
+            print(f"  res {r}: rho {rho_err:.2e} acc {acc_err:.2e} "
+                  f"(max |a*| {np.max(np.abs(acc_ref)):.1f})", flush=True)
+            assert rho_err <= 1e-5 and acc_err <= 1e-5
+        # Velocities take dt * (the acceleration error); positions one more dt.
+        amax = max(np.max(np.abs(o.get(r, "acc"))) for r in (0, 1))
+        _compare_state(a, o, atol_x=1e-6 + sc.dt ** 2 * 1e-5 * amax,
+                       atol_v=1e-6 + sc.dt * 1e-5 * amax)
