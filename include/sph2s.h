@@ -35,6 +35,8 @@ extern "C" {
 #define SPH_ERR_NONFINITE         4   /* NaN/Inf found in the particle state   */
 #define SPH_ERR_UNSUPPORTED       5
 #define SPH_ERR_NO_DEVICE         6
+#define SPH_ERR_DEVICE            7   /* a device-side check failed (see the
+                                         *_device_errors bits)              */
 
 /* Empty cell marker of cell_start/cell_end
  * (SPH2S_FINAL_SINGLE_SCALE/Solver.cu:862,1118-1127). */
@@ -136,6 +138,11 @@ int sph_destroy(sph_solver* s);
 int sph_advance(sph_solver* s, float dt, uint32_t n_steps);
 /* Same, launching each kernel directly instead of replaying the graph. */
 int sph_advance_eager(sph_solver* s, float dt, uint32_t n_steps);
+/* Waits for the solver's stream, then reports device-side failures of the
+ * steps since the bits were last cleared (sph_device_errors):
+ * SPH_ERR_NONFINITE (SPH_DEVERR_NONFINITE), SPH_ERR_CAPACITY (two-scale,
+ * SPH_DEVERR_CAPACITY) or SPH_ERR_DEVICE (SPH_DEVERR_SORT_WAIT).  The bits
+ * stay set, so every later synchronize reports them until they are cleared. */
 int sph_synchronize(sph_solver* s);
 
 /* Replace the particle state from host arrays (ids restart at 0..n-1). */
@@ -207,6 +214,29 @@ int sph_export_positions_aos(sph_solver* s, float* dst, int dst_is_device);
  * Stream-ordered on the solver's stream (sph_synchronize before reading). */
 int sph_export_render(sph_solver* s, float* xyz, float* color);
 
+/* Renderer buffers the solver keeps current: the device buffers of
+ * ParticleData that the reference's renderer binds and its solver rewrites
+ * every step -- dPositions (AoS, 3 floats per particle), dColorValues (1 float)
+ * and, two-scale, dStates (1 int) (SPH2S_FINAL_TWO_SCALE/ParticleData.h:39-45,
+ * Renderer.cpp:38-46,71, SSFRenderer.cu:438-439; single scale
+ * SPH2S_FINAL_SINGLE_SCALE/ParticleData.h:39-41).  Once enabled, the solver
+ * owns them (capacity max_n each) and every step rewrites them in the state's
+ * order at its end, inside the captured step graph; sph_set_state refreshes
+ * them too.  With SPH_RENDER_COLORS the colour pass (SPH_DEBUG_COLOR) runs
+ * every step and fills color_values; without it they stay 0, as the
+ * single-scale reference never writes them.  Single-scale states are 0.
+ * The pointers stay valid until the handle is destroyed. */
+#define SPH_RENDER_COLORS 1u
+typedef struct sph_render_buffers {
+    float*   positions;     /* float[3 max_n]: x, y, z per particle (dPositions) */
+    float*   color_values;  /* float[max_n]   (dColorValues)                     */
+    int32_t* states;        /* int[max_n]     (dStates)                          */
+    uint32_t n;             /* NumParticles: entries written                     */
+    uint32_t max_n;         /* MaxParticles                                      */
+} sph_render_buffers;
+int sph_enable_render_buffers(sph_solver* s, uint32_t flags);
+int sph_get_render_buffers(sph_solver* s, sph_render_buffers* out);
+
 #define SPH_DEBUG_ACCEL 1u   /* keep per-particle accelerations in accel4 */
 /* Compute each particle's colour value every step, as the two-scale
  * reference's force pass does for dColorValues (SPH2S_FINAL_TWO_SCALE/
@@ -232,6 +262,9 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
  * gave up, so the step that raised it has an invalid ordering. */
 #define SPH_DEVERR_SORT_WAIT 1u
 #define SPH_DEVERR_CAPACITY  2u   /* two-scale: a split/merge exceeded a capacity */
+/* SPH_DEVERR_NONFINITE: a step's integration wrote a NaN or infinite position
+ * or velocity (the force pass checks every particle it integrates). */
+#define SPH_DEVERR_NONFINITE 4u
 int sph_device_errors(sph_solver* s, uint32_t* bits);
 
 /* Pair work of the most recent step (no counterpart in the reference; a
@@ -298,6 +331,8 @@ int sph2_apply_split_merge(sph_two_scale* t);
  * sph_set_boundary with h0 and m0: the reference passes LOW_RES for both
  * (SPH2S_FINAL_TWO_SCALE/Solver.cu:454-499, :1007-1044). */
 int sph2_set_boundary(sph_two_scale* t, const float* pos_xyz, uint32_t n);
+/* Waits and reports device-side failures like sph_synchronize (both sets'
+ * bits and the capacity bit, SPH_ERR_CAPACITY). */
 int sph2_synchronize(sph_two_scale* t);
 int sph2_set_debug(sph_two_scale* t, uint32_t flags);
 /* counts[0..1] = current (low, high) counts; counts[2..3] = the counts the
@@ -384,6 +419,15 @@ int sph2_mark_split(sph_two_scale* t, const uint32_t* ids, uint32_t n);
  * stream (sph2_synchronize before reading). */
 int sph2_export_render(sph_two_scale* t, int resolution, float* xyz, float* color,
                        int32_t* state);
+/* Solver-owned renderer buffers of both sets, rewritten at the end of every
+ * step (inside the step graph), by sph2_set_state and by
+ * sph2_apply_split_merge, in the compacted order that sph2_export_render
+ * copies (see sph_enable_render_buffers; the two-scale reference writes
+ * dColorValues and dStates every step, TWO_SCALE/Solver.cu:1050-1083,
+ * 1240-1289).  sph2_get_render_buffers reads the set's count back
+ * (synchronises); capacity max_low or max_high. */
+int sph2_enable_render_buffers(sph_two_scale* t, uint32_t flags);
+int sph2_get_render_buffers(sph_two_scale* t, int resolution, sph_render_buffers* out);
 
 /* Surface marks (blended coupling).  The reference's screen-space pass sets
  * state bit 8 on the low particles its surface index map shows
@@ -419,6 +463,7 @@ int sph_batch_create(const sph_config* cfgs, uint32_t n_scenes,
 int sph_batch_destroy(sph_batch* b);
 int sph_batch_advance(sph_batch* b, float dt, uint32_t n_steps);
 int sph_batch_advance_eager(sph_batch* b, float dt, uint32_t n_steps);
+/* Waits and reports device-side failures like sph_synchronize. */
 int sph_batch_synchronize(sph_batch* b);
 int sph_batch_set_debug(sph_batch* b, uint32_t flags);
 /* part_offsets and cell_offsets (n_scenes + 1 entries each) may be NULL. */

@@ -11,7 +11,8 @@ from ._capi import (LibraryNotBuilt, SPHError, lib, FIELD_POSITION, FIELD_VELOCI
                     FIELD_STATE, FIELD_COLOR, SPH_DEBUG_ACCEL, SPH_DEBUG_COLOR, SPH_FLAG_XSPH,
                     SPH_EMPTY_CELL, SPH_FLAG_KAHAN,
                     SPH_FLAG_WALL_CLAMP, SPH2_COUPLING_FIXED, SPH2_COUPLING_BLENDED,
-                    SPH2_SPLIT_LATTICE, SPH2_SPLIT_REFERENCE)
+                    SPH2_SPLIT_LATTICE, SPH2_SPLIT_REFERENCE, SPH_RENDER_COLORS,
+                    SPH_DEVERR_SORT_WAIT, SPH_DEVERR_CAPACITY, SPH_DEVERR_NONFINITE)
 from . import autotune  # noqa: F401
 from .solver import (BatchSolver, Grid, ParticleData, Solver, SolverConfiguration,  # noqa: F401
                      TwoScaleSolver)

@@ -22,6 +22,7 @@ SPH_ERR_CAPACITY = 3
 SPH_ERR_NONFINITE = 4
 SPH_ERR_UNSUPPORTED = 5
 SPH_ERR_NO_DEVICE = 6
+SPH_ERR_DEVICE = 7
 
 SPH_EMPTY_CELL = 0xFFFFFFFF
 SPH_FLAG_WALL_CLAMP = 1
@@ -51,6 +52,9 @@ SPH2_SPLIT_REFERENCE = 1
 
 SPH_DEVERR_SORT_WAIT = 1
 SPH_DEVERR_CAPACITY = 2
+SPH_DEVERR_NONFINITE = 4
+
+SPH_RENDER_COLORS = 1
 
 PASS_NAMES = ["hash_reset", "radix_sort", "reorder_cells", "density", "force_integrate"]
 
@@ -89,6 +93,11 @@ class SphViews(Structure):
         ("cell_start", c_void_p), ("cell_end", c_void_p), ("accel4", c_void_p),
         ("n", c_uint32), ("num_cells", c_uint32), ("stream", c_void_p),
     ]
+
+
+class SphRenderBuffers(Structure):
+    _fields_ = [("positions", c_void_p), ("color_values", c_void_p), ("states", c_void_p),
+                ("n", c_uint32), ("max_n", c_uint32)]
 
 
 class Sph2Options(Structure):
@@ -150,6 +159,10 @@ SIGNATURES = [
     ("sph2_mark_surface_map", c_int, [c_void_p, c_void_p, c_uint32, c_int]),
     ("sph2_detect_surface", c_int, [c_void_p, c_float, c_int]),
     ("sph_export_render", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("sph_enable_render_buffers", c_int, [c_void_p, c_uint32]),
+    ("sph_get_render_buffers", c_int, [c_void_p, POINTER(SphRenderBuffers)]),
+    ("sph2_enable_render_buffers", c_int, [c_void_p, c_uint32]),
+    ("sph2_get_render_buffers", c_int, [c_void_p, c_int, POINTER(SphRenderBuffers)]),
     ("sph_config_set_xsph", c_int, [POINTER(SphConfig), c_float]),
     ("sph_batch_create", c_int, [POINTER(SphConfig), c_uint32, c_void_p, c_void_p, c_void_p,
                                  POINTER(c_void_p)]),

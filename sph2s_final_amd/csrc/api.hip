@@ -32,6 +32,25 @@ int hip_error(hipError_t e, const char* what, const char* file, int line) {
     return set_error(SPH_ERR_HIP, buf);
 }
 
+int device_error_status(uint32_t bits) {
+    if (bits & SPH_DEVERR_NONFINITE)
+        return set_error(SPH_ERR_NONFINITE,
+                         "a step wrote a NaN or infinite position or velocity (device error bits)");
+    if (bits & SPH_DEVERR_CAPACITY)
+        return set_error(SPH_ERR_CAPACITY,
+                         "a split or merge exceeded a set's capacity; particles were dropped");
+    if (bits & SPH_DEVERR_SORT_WAIT)
+        return set_error(SPH_ERR_DEVICE,
+                         "a radix-sort wait gave up: that step's particle order is invalid");
+    return SPH_OK;
+}
+
+int read_error_word(const uint32_t* word, hipStream_t stream, uint32_t* out) {
+    SPH_HIP(hipMemcpyAsync(out, word, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    SPH_HIP(hipStreamSynchronize(stream));
+    return SPH_OK;
+}
+
 StepParams make_step_params(const sph_config& c, uint32_t n) {
     StepParams p{};
     p.ox = c.grid.origin[0];
@@ -103,6 +122,11 @@ struct sph_solver {
     Walls walls{};               // boundary particles (none by default)
     float4* bacc = nullptr;      // boundary acceleration per sorted slot
     float* color = nullptr;      // colour values (SPH_DEBUG_COLOR)
+    // Renderer buffers (sph_enable_render_buffers): AoS xyz, colour, state.
+    float* render_xyz = nullptr;
+    float* render_color = nullptr;
+    int32_t* render_state = nullptr;
+    uint32_t render_flags = 0;
 };
 
 namespace {
@@ -171,13 +195,29 @@ int upload_state(sph_solver* s, const float* pos, const float* vel, uint32_t n) 
     return SPH_OK;
 }
 
+// The renderer buffers after a step (or a state upload): AoS positions and,
+// with SPH_RENDER_COLORS, the colour values, in the state's order.
+int export_render_buffers(sph_solver* s) {
+    if (!s->render_xyz) return SPH_OK;
+    const bool colors = (s->render_flags & SPH_RENDER_COLORS) && s->buf.color;
+    return launch_export_render(s->buf.pos, nullptr, colors ? s->buf.color : nullptr,
+                                s->buf.n_dev, s->max_n, 0u, s->render_xyz,
+                                colors ? s->render_color : nullptr, nullptr, s->stream);
+}
+
+// One step: the passes, then the renderer buffers when they are enabled.
+int launch_full_step(sph_solver* s, const StepParams& p, float dt, bool accel) {
+    const int rc = launch_step(p, s->buf, dt, s->key_bits, accel, s->stream);
+    return rc == SPH_OK ? export_render_buffers(s) : rc;
+}
+
 int capture_graph(sph_solver* s, float dt, bool accel) {
     destroy_graph(s);
     StepParams p = make_step_params(s->cfg, s->n);
     p.tune = s->tune;
     hipGraph_t g = nullptr;
     SPH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_step(p, s->buf, dt, s->key_bits, accel, s->stream);
+    int rc = launch_full_step(s, p, dt, accel);
     hipError_t e = hipStreamEndCapture(s->stream, &g);
     if (rc != SPH_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -432,6 +472,7 @@ int sph_destroy(sph_solver* s) {
     walls_destroy(&s->walls);
     if (s->bacc) (void)hipFree(s->bacc);
     if (s->color) (void)hipFree(s->color);
+    if (s->render_xyz) (void)hipFree(s->render_xyz);
     if (s->arena) (void)hipFree(s->arena);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -455,7 +496,12 @@ int sph_set_boundary(sph_solver* s, const float* pos, uint32_t n) {
 
 int sph_set_state(sph_solver* s, const float* pos, const float* vel, uint32_t n) {
     if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
-    return upload_state(s, pos, vel, n);
+    int rc = upload_state(s, pos, vel, n);
+    if (rc == SPH_OK && s->render_xyz) {
+        rc = export_render_buffers(s);
+        if (rc == SPH_OK) SPH_HIP(hipStreamSynchronize(s->stream));
+    }
+    return rc;
 }
 
 int sph_advance(sph_solver* s, float dt, uint32_t n_steps) {
@@ -481,7 +527,7 @@ int sph_advance_eager(sph_solver* s, float dt, uint32_t n_steps) {
     p.tune = s->tune;
     const bool accel = (s->debug & SPH_DEBUG_ACCEL) != 0u;
     for (uint32_t i = 0; i < n_steps; ++i) {
-        int rc = launch_step(p, s->buf, dt, s->key_bits, accel, s->stream);
+        int rc = launch_full_step(s, p, dt, accel);
         if (rc != SPH_OK) return rc;
     }
     if (n_steps > 0 && s->n > 0) s->stepped = true;
@@ -522,8 +568,9 @@ int sph_get_tune(const sph_solver* s, uint32_t* tune) {
 
 int sph_synchronize(sph_solver* s) {
     if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
-    SPH_HIP(hipStreamSynchronize(s->stream));
-    return SPH_OK;
+    uint32_t v = 0;
+    const int rc = read_error_word(radix_sort_error_word(s->buf.sort_temp), s->stream, &v);
+    return rc != SPH_OK ? rc : device_error_status(public_error_bits(v));
 }
 
 int sph_num_particles(const sph_solver* s, uint32_t* n) {
@@ -619,12 +666,14 @@ int sph_device_errors(sph_solver* s, uint32_t* bits) {
         SPH_HIP(hipMemsetAsync(word, 0, sizeof(v), s->stream));
         SPH_HIP(hipStreamSynchronize(s->stream));
     }
-    *bits = v ? SPH_DEVERR_SORT_WAIT : 0u;
+    *bits = public_error_bits(v);
     return SPH_OK;
 }
 
 int sph_set_debug(sph_solver* s, uint32_t flags) {
     if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    // Renderer colour values need the colour pass.
+    if (s->render_flags & SPH_RENDER_COLORS) flags |= SPH_DEBUG_COLOR;
     if ((flags & SPH_DEBUG_COLOR) && !s->color) {
         SPH_HIP(hipMalloc(&s->color, 4ull * s->max_n));
         SPH_HIP(hipMemsetAsync(s->color, 0, 4ull * s->max_n, s->stream));
@@ -697,6 +746,45 @@ int sph_export_render(sph_solver* s, float* xyz, float* color) {
         return set_error(SPH_ERR_INVALID_ARGUMENT, "colour values need SPH_DEBUG_COLOR");
     return launch_export_render(s->buf.pos, nullptr, s->buf.color, s->buf.n_dev, s->max_n, 0u, xyz,
                                 color, nullptr, s->stream);
+}
+
+int sph_enable_render_buffers(sph_solver* s, uint32_t flags) {
+    if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
+    if (flags & ~SPH_RENDER_COLORS) return set_error(SPH_ERR_INVALID_ARGUMENT, "unknown render flag");
+    SPH_HIP(hipStreamSynchronize(s->stream));
+    if (!s->render_xyz) {
+        // One allocation: xyz (12 B), colour (4 B) and state (4 B) per particle.
+        const size_t n = s->max_n;
+        char* base = nullptr;
+        SPH_HIP(hipMalloc(&base, align_up(12 * n) + align_up(4 * n) + align_up(4 * n)));
+        s->render_xyz = reinterpret_cast<float*>(base);
+        s->render_color = reinterpret_cast<float*>(base + align_up(12 * n));
+        s->render_state = reinterpret_cast<int32_t*>(base + align_up(12 * n) + align_up(4 * n));
+        SPH_HIP(hipMemsetAsync(base, 0, align_up(12 * n) + align_up(4 * n) + align_up(4 * n),
+                               s->stream));
+    }
+    s->render_flags = flags;
+    if (flags & SPH_RENDER_COLORS) {
+        const int rc = sph_set_debug(s, s->debug | SPH_DEBUG_COLOR);
+        if (rc != SPH_OK) return rc;
+    }
+    destroy_graph(s);
+    const int rc = export_render_buffers(s);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(hipStreamSynchronize(s->stream));
+    return SPH_OK;
+}
+
+int sph_get_render_buffers(sph_solver* s, sph_render_buffers* out) {
+    if (!s || !out) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    if (!s->render_xyz)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "render buffers are not enabled");
+    out->positions = s->render_xyz;
+    out->color_values = s->render_color;
+    out->states = s->render_state;
+    out->n = s->n;
+    out->max_n = s->max_n;
+    return SPH_OK;
 }
 
 size_t sph_radix_sort_temp_bytes(uint32_t n) { return radix_sort_temp_bytes(n); }

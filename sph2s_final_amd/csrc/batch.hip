@@ -317,8 +317,9 @@ int sph_batch_get_tune(const sph_batch* b, uint32_t* tune) {
 
 int sph_batch_synchronize(sph_batch* b) {
     if (!b) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch is null");
-    SPH_HIP(hipStreamSynchronize(b->stream));
-    return SPH_OK;
+    uint32_t v = 0;
+    const int rc = read_error_word(radix_sort_error_word(b->buf.sort_temp), b->stream, &v);
+    return rc != SPH_OK ? rc : device_error_status(public_error_bits(v));
 }
 
 int sph_batch_set_debug(sph_batch* b, uint32_t flags) {
@@ -348,7 +349,7 @@ int sph_batch_device_errors(sph_batch* b, uint32_t* bits) {
         SPH_HIP(hipMemsetAsync(word, 0, 4, b->stream));
         SPH_HIP(hipStreamSynchronize(b->stream));
     }
-    *bits = v ? SPH_DEVERR_SORT_WAIT : 0u;
+    *bits = public_error_bits(v);
     return SPH_OK;
 }
 

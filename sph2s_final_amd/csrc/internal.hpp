@@ -87,6 +87,20 @@ __device__ __forceinline__ uint32_t cell_hash(float x, float y, float z, float o
 size_t radix_sort_temp_bytes(uint32_t n);
 // Device word set to non-zero if a look-back wait of the sort gave up.
 uint32_t* radix_sort_error_word(void* temp);
+// Bits of that word: the sort's look-back gave up (set by the sort), and the
+// force pass's integration wrote a NaN or infinite position or velocity.
+constexpr uint32_t kDevErrSortWait = 1u;
+constexpr uint32_t kDevErrNonFinite = 2u;
+// The SPH_DEVERR_* bits of an error word.
+inline uint32_t public_error_bits(uint32_t word) {
+    return ((word & kDevErrSortWait) ? SPH_DEVERR_SORT_WAIT : 0u) |
+           ((word & kDevErrNonFinite) ? SPH_DEVERR_NONFINITE : 0u);
+}
+// The status a synchronize reports for SPH_DEVERR_* bits (the bits stay set
+// until the handle's device-errors query clears them).
+int device_error_status(uint32_t public_bits);
+// Reads an error word after the stream's work (synchronises the stream).
+int read_error_word(const uint32_t* word, hipStream_t stream, uint32_t* out);
 // Stable LSD sort of (key, value) pairs by the low key_bits of key.
 // vals_in == nullptr means values 0..n-1.  Launches on stream.
 int radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,

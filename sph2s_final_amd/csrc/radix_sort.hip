@@ -308,7 +308,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(
                 if (done) break;
                 if (taken == 0) {
                     if (++spins > kSpinLimit) {
-                        atomicOr(error_word, 1u);
+                        atomicOr(error_word, kDevErrSortWait);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);

@@ -207,6 +207,9 @@ struct XOrder {
     const uint4* bins;
     float2* iaux;  // density: internal-order copy of aux for the force pass
     const float4* bacc;  // force: boundary acceleration per sorted slot (or NULL)
+    // force: device error word; the integration ORs kDevErrNonFinite into it
+    // when it writes a NaN or infinite position or velocity (or NULL).
+    uint32_t* err = nullptr;
 };
 constexpr uint32_t kNumCus = 256;
 
@@ -461,7 +464,8 @@ __device__ __forceinline__ void force_finish(const PairSums<kTension>& S, const 
                                              const float2& ai, uint32_t o,
                                              const float4* __restrict__ bacc, float dt,
                                              float4* __restrict__ pos, float4* __restrict__ vel,
-                                             float4* __restrict__ accel) {
+                                             float4* __restrict__ accel,
+                                             uint32_t* __restrict__ err) {
 #pragma clang fp contract(off)
     const float vc = p.visc * ai.y;
     float ax = __builtin_fmaf(vc, S.Vx, p.c45m * S.Px) + p.gx;
@@ -499,6 +503,8 @@ __device__ __forceinline__ void force_finish(const PairSums<kTension>& S, const 
     pos[o] = make_float4(x, y, z, xi.w);
     vel[o] = make_float4(vx, vy, vz, vi.w);
     if (accel) accel[o] = make_float4(ax, ay, az, 0.0f);
+    // A NaN or infinity in any component makes the sum non-finite.
+    if (err && !__builtin_isfinite(((x + y) + (z + vx)) + (vy + vz))) atomicOr(err, kDevErrNonFinite);
 }
 
 // a_i = m c45 sum_j (A_i + A_j)(h-r)^2/r x_ij            (pressure)
@@ -632,7 +638,7 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
         }
 
     const uint32_t o = kX ? xo.i2r[s] : s;
-    force_finish<kTension, kIntegrate>(S, p, xi, vi, ai, o, xo.bacc, dt, pos, vel, accel);
+    force_finish<kTension, kIntegrate>(S, p, xi, vi, ai, o, xo.bacc, dt, pos, vel, accel, xo.err);
 }
 
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
@@ -911,7 +917,8 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
                                    b.n_dev, dt, b.pos, b.vel);
                 break;
             }
-            const XOrder xo{b.i2r, b.xbins, nullptr, walls ? b.bacc : nullptr};
+            const XOrder xo{b.i2r, b.xbins, nullptr, walls ? b.bacc : nullptr,
+                            radix_sort_error_word(b.sort_temp)};
             if (p.tens != 0.0f)
                 launch_force<true, true, false>(p.tune, p.n, stream, b.ipos, b.ivel, b.iaux,
                                                 b.cell_start, b.cell_end, p, g, b.n_dev, dt, b.pos,
@@ -1077,7 +1084,7 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
         case kPassForce: {
             if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
             float4* acc = write_accel ? b.accel : nullptr;
-            const XOrder xo{b.i2r, b.xbins, nullptr};
+            const XOrder xo{b.i2r, b.xbins, nullptr, nullptr, radix_sort_error_word(b.sort_temp)};
             if (tension)
                 launch_force<true, true, true>(cap.tune, cap.n, stream, b.ipos, b.ivel, b.iaux,
                                                b.cell_start, b.cell_end, cap, g0, b.n_dev, dt,

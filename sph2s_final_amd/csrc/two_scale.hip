@@ -59,6 +59,7 @@ struct CrossParams {
     float lox, loy, loz, hix, hiy, hiz;
     uint32_t flags;
     float vmax;            // velocity cap c h_A / dt of the target set; 0 = off
+    uint32_t* err;         // target set's device error word (non-finite state), or NULL
 };
 
 // The velocity cap of integrateD (TWO_SCALE/Solver.cu:1212-1223):
@@ -240,6 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
     }
     pos_a[s] = make_float4(x, y, z, xi.w);
     vel_a[s] = make_float4(vx, vy, vz, vi.w);
+    if (c.err && !__builtin_isfinite(((x + y) + (z + vx)) + (vy + vz)))
+        atomicOr(c.err, kDevErrNonFinite);
     if (acc_out) acc_out[s] = make_float4(ax, ay, az, 0.0f);
 }
 
@@ -478,6 +481,8 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     }
     pos_a[s] = make_float4(x, y, z, xi.w);
     vel_a[s] = make_float4(vx, vy, vz, vi.w);
+    if (c.err && !__builtin_isfinite(((x + y) + (z + vx)) + (vy + vz)))
+        atomicOr(c.err, kDevErrNonFinite);
     if (acc_out) acc_out[s] = make_float4(ax, ay, az, 0.0f);
     // Blend update (Solver.cu:1242-1257): lambda += c[mode] inc with
     // c = {0, -1, +1, 0}; a fading-in particle reaching 1 settles.  Removal
@@ -977,6 +982,12 @@ struct sph_two_scale {
     uint32_t graph_debug = 0;
     uint32_t debug = 0;
     uint32_t* surf = nullptr;    // per low id: at the free surface (surface detector)
+    // Renderer buffers per set (sph2_enable_render_buffers): AoS xyz, colour, state.
+    float* render_xyz[2] = {nullptr, nullptr};
+    float* render_color[2] = {nullptr, nullptr};
+    int32_t* render_state[2] = {nullptr, nullptr};
+    void* render_arena = nullptr;
+    uint32_t render_flags = 0;
 };
 
 namespace {
@@ -1018,6 +1029,7 @@ CrossParams cross_params(const sph_two_scale* t, int target) {
     c.hiz = lo.domain_hi[2];
     c.flags = lo.flags;
     c.vmax = 0.0f;
+    c.err = radix_sort_error_word(t->set[target].buf.sort_temp);
     return c;
 }
 
@@ -1214,11 +1226,35 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
     return launch_split_merge(t);
 }
 
+// Both sets' renderer buffers from the compacted state: AoS positions, the
+// state words (blended coupling: the blend state; fixed: 0 low, 4 high, the
+// reference's initial states) and, with SPH_RENDER_COLORS, the colour values.
+int export_render_buffers2(sph_two_scale* t) {
+    if (!t->render_arena) return SPH_OK;
+    const bool colors = (t->render_flags & SPH_RENDER_COLORS) && (t->debug & SPH_DEBUG_COLOR);
+    for (int r = 0; r < 2; ++r) {
+        TwoScaleSet& S = t->set[r];
+        const int rc = launch_export_render(S.state_pos, blended(t) ? S.state_blend : nullptr,
+                                            S.color, S.buf.n_dev, S.cap, r == 0 ? 0u : 4u,
+                                            t->render_xyz[r],
+                                            colors ? t->render_color[r] : nullptr,
+                                            t->render_state[r], t->stream);
+        if (rc != SPH_OK) return rc;
+    }
+    return SPH_OK;
+}
+
+// One step, then the renderer buffers when they are enabled.
+int launch_full_two_scale_step(sph_two_scale* t, float dt, bool accel) {
+    const int rc = launch_two_scale_step(t, dt, accel);
+    return rc == SPH_OK ? export_render_buffers2(t) : rc;
+}
+
 int capture_two_scale(sph_two_scale* t, float dt, bool accel) {
     destroy_graph2(t);
     hipGraph_t g = nullptr;
     SPH_HIP(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_two_scale_step(t, dt, accel);
+    int rc = launch_full_two_scale_step(t, dt, accel);
     hipError_t e = hipStreamEndCapture(t->stream, &g);
     if (rc != SPH_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -1443,6 +1479,7 @@ int sph2_set_state(sph_two_scale* t, const float* pos_l, const float* vel_l,
         return set_error(SPH_ERR_INVALID_ARGUMENT, "high-resolution particles need ids and tags");
     int rc = upload_set(t, 0, pos_l, vel_l, ids_l, nullptr, n_l);
     if (rc == SPH_OK) rc = upload_set(t, 1, pos_h, vel_h, ids_h, tags_h, n_h);
+    if (rc == SPH_OK) rc = export_render_buffers2(t);
     if (rc != SPH_OK) return rc;
     SPH_HIP(hipStreamSynchronize(t->stream));
     destroy_graph2(t);
@@ -1467,7 +1504,7 @@ int sph2_advance_eager(sph_two_scale* t, float dt, uint32_t n_steps) {
     if (!(dt > 0.0f)) return set_error(SPH_ERR_INVALID_ARGUMENT, "dt must be > 0");
     const bool accel = (t->debug & SPH_DEBUG_ACCEL) != 0u;
     for (uint32_t i = 0; i < n_steps; ++i) {
-        const int rc = launch_two_scale_step(t, dt, accel);
+        const int rc = launch_full_two_scale_step(t, dt, accel);
         if (rc != SPH_OK) return rc;
     }
     return SPH_OK;
@@ -1485,7 +1522,8 @@ int sph2_apply_split_merge(sph_two_scale* t) {
     hipLaunchKernelGGL(k_fill_float, blocks_for(t->set[0].cap), dim3(kBlock), 0, t->stream,
                        t->set[0].buf.n_dev, t->set[0].cfg.rest_density, t->set[0].buf.rho);
     SPH_HIP(hipGetLastError());
-    return launch_split_merge(t);
+    const int rc = launch_split_merge(t);
+    return rc == SPH_OK ? export_render_buffers2(t) : rc;
 }
 
 int sph2_set_options(sph_two_scale* t, const sph2_options* o) {
@@ -1619,13 +1657,65 @@ int sph2_mark_surface_map(sph_two_scale* t, const uint32_t* map, uint32_t count,
 
 int sph2_synchronize(sph_two_scale* t) {
     if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
-    SPH_HIP(hipStreamSynchronize(t->stream));
-    return SPH_OK;
+    uint32_t v[3] = {};
+    int rc = read_error_word(t->error_word, t->stream, &v[0]);
+    for (int r = 0; r < 2 && rc == SPH_OK; ++r)
+        rc = read_error_word(radix_sort_error_word(t->set[r].buf.sort_temp), t->stream, &v[1 + r]);
+    if (rc != SPH_OK) return rc;
+    return device_error_status(public_error_bits(v[1] | v[2]) |
+                               ((v[0] & 2u) ? SPH_DEVERR_CAPACITY : 0u));
 }
 
 int sph2_set_debug(sph_two_scale* t, uint32_t flags) {
     if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    // Renderer colour values need the colour pass.
+    if (t->render_flags & SPH_RENDER_COLORS) flags |= SPH_DEBUG_COLOR;
     t->debug = flags;
+    return SPH_OK;
+}
+
+int sph2_enable_render_buffers(sph_two_scale* t, uint32_t flags) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    if (flags & ~SPH_RENDER_COLORS) return set_error(SPH_ERR_INVALID_ARGUMENT, "unknown render flag");
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    if (!t->render_arena) {
+        size_t total = 0;
+        for (int r = 0; r < 2; ++r) total += al(12ull * t->set[r].cap) + 2 * al(4ull * t->set[r].cap);
+        SPH_HIP(hipMalloc(&t->render_arena, total));
+        SPH_HIP(hipMemsetAsync(t->render_arena, 0, total, t->stream));
+        char* a = static_cast<char*>(t->render_arena);
+        for (int r = 0; r < 2; ++r) {
+            const size_t n = t->set[r].cap;
+            t->render_xyz[r] = reinterpret_cast<float*>(a);
+            a += al(12 * n);
+            t->render_color[r] = reinterpret_cast<float*>(a);
+            a += al(4 * n);
+            t->render_state[r] = reinterpret_cast<int32_t*>(a);
+            a += al(4 * n);
+        }
+    }
+    t->render_flags = flags;
+    if (flags & SPH_RENDER_COLORS) t->debug |= SPH_DEBUG_COLOR;
+    destroy_graph2(t);
+    const int rc = export_render_buffers2(t);
+    if (rc != SPH_OK) return rc;
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
+}
+
+int sph2_get_render_buffers(sph_two_scale* t, int resolution, sph_render_buffers* out) {
+    if (!t || !out || resolution < 0 || resolution > 1)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "bad argument");
+    if (!t->render_arena)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "render buffers are not enabled");
+    uint32_t c[4];
+    const int rc = sph2_counts(t, c);
+    if (rc != SPH_OK) return rc;
+    out->positions = t->render_xyz[resolution];
+    out->color_values = t->render_color[resolution];
+    out->states = t->render_state[resolution];
+    out->n = c[resolution];
+    out->max_n = t->set[resolution].cap;
     return SPH_OK;
 }
 
@@ -1650,7 +1740,7 @@ int sph2_device_errors(sph_two_scale* t, uint32_t* bits) {
     SPH_HIP(hipMemcpyAsync(&v[2], radix_sort_error_word(t->set[1].buf.sort_temp), 4,
                            hipMemcpyDeviceToHost, t->stream));
     SPH_HIP(hipStreamSynchronize(t->stream));
-    *bits = ((v[1] | v[2]) ? SPH_DEVERR_SORT_WAIT : 0u) | ((v[0] & 2u) ? SPH_DEVERR_CAPACITY : 0u);
+    *bits = public_error_bits(v[1] | v[2]) | ((v[0] & 2u) ? SPH_DEVERR_CAPACITY : 0u);
     if (*bits) {
         SPH_HIP(hipMemsetAsync(t->error_word, 0, 4, t->stream));
         SPH_HIP(hipMemsetAsync(radix_sort_error_word(t->set[0].buf.sort_temp), 0, 4, t->stream));
@@ -1746,6 +1836,7 @@ int sph2_destroy(sph_two_scale* t) {
     walls_destroy(&t->walls);
     for (int r = 0; r < 2; ++r)
         if (t->bacc[r]) (void)hipFree(t->bacc[r]);
+    if (t->render_arena) (void)hipFree(t->render_arena);
     if (t->arena) (void)hipFree(t->arena);
     if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;

@@ -88,8 +88,11 @@ class SolverConfiguration:
 class ParticleData:
     """ParticleData (SPH2S_FINAL_SINGLE_SCALE/ParticleData.h:9-53).
 
-    Holds host arrays until a Solver takes it; from then on ``dPositions`` /
-    ``dDensities`` are the solver's device buffers (float4 x,y,z,id and float).
+    Holds host arrays until a Solver takes it; from then on ``dPositions``,
+    ``dColorValues`` and ``dStates`` are the device addresses of the renderer's
+    buffers that the solver rewrites every step (3 floats, 1 float and 1 int
+    per particle; sph_enable_render_buffers) and ``dDensities`` that of the
+    last step's densities.
     """
 
     def __init__(self, maxParticles: int, positions=None, velocities=None):
@@ -153,11 +156,22 @@ class ParticleData:
             return np.zeros((self.NumParticles, 3), np.float32)
         return self.velocities
 
-    @property
-    def dPositions(self) -> int:
+    def _render(self) -> C.SphRenderBuffers:
         if self._solver is None:
             raise RuntimeError("ParticleData is not bound to a Solver")
-        return self._solver.views().pos4
+        return self._solver.render_buffers()
+
+    @property
+    def dPositions(self) -> int:
+        return self._render().positions
+
+    @property
+    def dColorValues(self) -> int:
+        return self._render().color_values
+
+    @property
+    def dStates(self) -> int:
+        return self._render().states
 
     @property
     def dDensities(self) -> int:
@@ -191,6 +205,7 @@ class Solver:
         fluidData._solver = self
         if boundaryData is not None and getattr(boundaryData, "NumParticles", 0):
             self.set_boundary(boundaryData.host_positions())
+        self._render_on = False
 
     # -- reference API --
     def Bind(self) -> None:
@@ -230,6 +245,20 @@ class Solver:
                                           c_void_p(col.data_ptr()) if (color and n) else None))
         self.synchronize()
         return (xyz, col) if color else xyz
+
+    def enable_render_buffers(self, colors: bool = False) -> None:
+        """Solver-owned renderer buffers rewritten by every step
+        (sph_enable_render_buffers); ``colors`` runs the colour pass too."""
+        C.check(C.lib().sph_enable_render_buffers(self._h, C.SPH_RENDER_COLORS if colors else 0))
+        self._render_on = True
+
+    def render_buffers(self) -> C.SphRenderBuffers:
+        """Device addresses of the renderer buffers (enabled on first use)."""
+        if not self._render_on:
+            self.enable_render_buffers()
+        r = C.SphRenderBuffers()
+        C.check(C.lib().sph_get_render_buffers(self._h, byref(r)))
+        return r
 
     def device_errors(self) -> int:
         """Device-side error bits since the last query (SPH_DEVERR_*); synchronises."""
@@ -436,6 +465,17 @@ class TwoScaleSolver:
                                                c_void_p(st.data_ptr())))
         self.synchronize()
         return xyz, col, st
+
+    def enable_render_buffers(self, colors: bool = False) -> None:
+        """Solver-owned renderer buffers of both sets, rewritten by every step
+        (sph2_enable_render_buffers)."""
+        C.check(C.lib().sph2_enable_render_buffers(self._h,
+                                                   C.SPH_RENDER_COLORS if colors else 0))
+
+    def render_buffers(self, resolution: int) -> C.SphRenderBuffers:
+        r = C.SphRenderBuffers()
+        C.check(C.lib().sph2_get_render_buffers(self._h, int(resolution), byref(r)))
+        return r
 
     def mark_surface_map(self, index_map, split: bool = True) -> None:
         """getSurfaceIDs: ``index_map`` is a CUDA uint32/int32 tensor of

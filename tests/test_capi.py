@@ -182,27 +182,92 @@ int main() {
     assert res.returncode == 0, res.stderr
 
 
+def test_cpp_two_scale_facade_compiles_and_links(tmp_path):
+    """The two-scale reference's program shape compiles against the facade's
+    sph2s::two_scale names: MakeConfiguration with blendIncrement as its 11th
+    argument and the reference's configuration members, Solver(fluid,
+    fluidHigh, boundary, &config), and ParticleData's renderer pointers."""
+    src = tmp_path / "facade2.cpp"
+    src.write_text(r'''
+#include "sph2s/Solver.hpp"
+using namespace sph2s::two_scale;
+int main() {
+    Grid particleGrid = Grid::MakeGrid(make_float3(0.02f, 0.04f, -0.05f),
+                                       make_float3(0.35f, 0.75f, 0.45f), 0.0075f);
+    ParticleData* fluid = ParticleData::CreateParticleBox(particleGrid);
+    ParticleData* fluidHigh = new ParticleData(fluid->MaxParticles * 8);
+    fluidHigh->NumParticles = 0;
+    SolverConfiguration config = SolverConfiguration::MakeConfiguration(
+        make_float3(-0.1f, -0.1f, -0.2f), make_float3(1.7f, 1.6f, 0.6f),
+        Grid::ComputeVolume(particleGrid), fluid->MaxParticles, 35, 1000.0f, 30.0f, 5.0f,
+        88.1472f, 0.8f, 0.0125f);
+    if (!(config.EffectiveRadius[1] == 0.5f * config.EffectiveRadius[0])) return 2;
+    if (!(config.FluidParticleMass[1] == 1.0f / 8.0f * config.FluidParticleMass[0])) return 3;
+    if (config.BlendIncrement != 0.0125f || config.Grid[1].spacing != config.EffectiveRadius[1])
+        return 4;
+    if (sph_device_count() == 0) return 0;
+    Solver s(fluid, fluidHigh, nullptr, &config);
+    s.Advance(0.001f);
+    float* xyz = fluid->dPositions;
+    float* col = fluid->dColorValues;
+    int* st = fluidHigh->dStates;
+    return (xyz && col && st) ? 0 : 1;
+}
+''')
+    exe = tmp_path / "facade2"
+    res = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{os.path.join(ROOT, 'include')}",
+                          str(src), "-o", str(exe), f"-L{os.path.dirname(C.LIB_PATH)}", "-lsph2s",
+                          f"-Wl,-rpath,{os.path.dirname(C.LIB_PATH)}"],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert run.returncode == 0, (run.returncode, run.stderr)
+
+
 FACADE_DEMO = os.path.join(ROOT, "tests", "_bin", "facade_demo")
+
+
+def _read_sets(path, n_sets):
+    """facade_demo's output: NumParticles of each set, then per set the
+    renderer buffers read through ParticleData's device pointers: dPositions
+    (3n float32), dColorValues (n float32), dStates (n int32)."""
+    raw = path.read_bytes()
+    counts = [int(v) for v in np.frombuffer(raw[:4 * n_sets], np.uint32)]
+    off, sets = 4 * n_sets, []
+    for n in counts:
+        xyz = np.frombuffer(raw[off:off + 12 * n], np.float32).reshape(n, 3)
+        off += 12 * n
+        col = np.frombuffer(raw[off:off + 4 * n], np.float32)
+        off += 4 * n
+        st = np.frombuffer(raw[off:off + 4 * n], np.int32)
+        off += 4 * n
+        sets.append((xyz, col, st))
+    assert off == len(raw)
+    return counts, sets
+
+
+def _run_demo(*args):
+    assert os.path.exists(FACADE_DEMO), "tests/_bin/facade_demo is not built (build())"
+    return subprocess.run([FACADE_DEMO, *[str(a) for a in args]], capture_output=True, text=True,
+                          timeout=120)
 
 
 @pytest.mark.gpu
 def test_cpp_facade_program_steps_like_the_python_path(tmp_path):
     """A reference-style host program written against the C++ facade only
     (tests/facade_demo.cpp: fluid box, canvas tank and pillar walls,
-    MakeConfiguration, Solver::Advance once per step) runs on the GPU, and
-    its exported positions equal, bit for bit, those of the same scene
+    MakeConfiguration, Solver::Advance once per step) runs on the GPU.  The
+    positions it copies from fluid->dPositions (3 floats per particle, the
+    renderer's layout) equal, bit for bit, those of the same scene
     (walled_tank_small, built with the Python mirror of the same classes)
-    stepped through the Python path."""
-    assert os.path.exists(FACADE_DEMO), "tests/_bin/facade_demo is not built (build())"
+    stepped through the Python path; dColorValues and dStates are 0, as the
+    single-scale reference never writes them."""
     steps = 6
     out = tmp_path / "xyz.bin"
     print(f"case facade program steps={steps}", flush=True)
-    res = subprocess.run([FACADE_DEMO, str(steps), str(out)], capture_output=True, text=True,
-                         timeout=120)
+    res = _run_demo(steps, out)
     assert res.returncode == 0, res.stderr
-    raw = out.read_bytes()
-    n = int(np.frombuffer(raw[:4], np.uint32)[0])
-    xyz = np.frombuffer(raw[4:], np.float32).reshape(n, 3)
+    (n,), [(xyz, col, st)] = _read_sets(out, 1)
 
     sc = scenes.walled_tank_small()  # the same scene through the Python mirror classes
     assert n == sc.n
@@ -216,6 +281,7 @@ def test_cpp_facade_program_steps_like_the_python_path(tmp_path):
     assert np.all(np.isfinite(ref))
     assert not np.array_equal(ref, sc.positions)  # the fluid moved
     assert np.array_equal(xyz, ref)
+    assert not col.any() and not st.any()
 
 
 @pytest.mark.gpu
@@ -223,19 +289,15 @@ def test_cpp_facade_two_scale_program_steps_like_the_python_path(tmp_path):
     """The facade's TwoScaleSolver (the reference's two-scale Solver(fluid,
     fluidHigh, boundary, cfg) plus a split region) in the facade program:
     after 6 steps (splits in the region box) its (low, high) counts and both
-    sets' positions equal, bit for bit, the same scene stepped through the
-    Python TwoScaleSolver."""
-    assert os.path.exists(FACADE_DEMO), "tests/_bin/facade_demo is not built (build())"
+    sets' dPositions equal, bit for bit, the same scene stepped through the
+    Python TwoScaleSolver; dStates hold the fixed coupling's 0 (low) and 4
+    (high)."""
     steps = 6
     out = tmp_path / "two.bin"
     print(f"case facade two-scale program steps={steps}", flush=True)
-    res = subprocess.run([FACADE_DEMO, "two", str(steps), str(out)], capture_output=True,
-                         text=True, timeout=120)
+    res = _run_demo("two", steps, out)
     assert res.returncode == 0, res.stderr
-    raw = out.read_bytes()
-    n_low, n_high = (int(v) for v in np.frombuffer(raw[:8], np.uint32))
-    xyz = np.frombuffer(raw[8:], np.float32)
-    low, high = xyz[:3 * n_low].reshape(-1, 3), xyz[3 * n_low:].reshape(-1, 3)
+    (n_low, n_high), [(low, _, st_l), (high, _, st_h)] = _read_sets(out, 2)
 
     grid = S.Grid.MakeGrid((0.02, 0.02, 0.02), (0.2, 0.2, 0.2), 0.01)
     fluid = S.ParticleData.CreateParticleBox(grid)
@@ -254,5 +316,100 @@ def test_cpp_facade_two_scale_program_steps_like_the_python_path(tmp_path):
         assert n_high > 0 and n_low < fluid.MaxParticles  # the region split
         assert np.array_equal(low, t.download(0, S.FIELD_POSITION))
         assert np.array_equal(high, t.download(1, S.FIELD_POSITION))
+        assert not st_l.any() and np.all(st_h == 4)
     finally:
         t.close()
+
+
+@pytest.mark.gpu
+def test_cpp_facade_reference_two_scale_program(tmp_path):
+    """The reference's own two-scale program shape (TWO_SCALE/main.cpp:150-254)
+    in sph2s::two_scale names: MakeConfiguration with the blend increment as
+    its 11th argument, Solver(fluid, fluidHigh, boundary, &config), splits
+    requested by id after the first step.  Counts and both sets' renderer
+    buffers (dPositions, dColorValues, dStates) equal, bit for bit, the same
+    run through the Python TwoScaleSolver with the blended coupling,
+    reference split offsets, no merging, the 0.8 h / dt velocity cap and no
+    split region."""
+    steps = 8
+    out = tmp_path / "ref2.bin"
+    print(f"case facade reference two-scale program steps={steps}", flush=True)
+    res = _run_demo("ref2", steps, out)
+    assert res.returncode == 0, res.stderr
+    (n_low, n_high), sets = _read_sets(out, 2)
+
+    grid = S.Grid.MakeGrid((0.02, 0.02, 0.02), (0.14, 0.2, 0.14), 0.01)
+    fluid = S.ParticleData.CreateParticleBox(grid)
+    walls = S.ParticleData.CreateParticleBoxCanvas(
+        S.Grid.MakeGrid((0.0, 0.0, 0.0), (0.3, 0.3, 0.3), 0.01), 2)
+    cfg = S.SolverConfiguration.MakeConfiguration(
+        (-0.05, -0.05, -0.05), (0.35, 0.35, 0.35), S.Grid.ComputeVolume(grid), fluid.MaxParticles,
+        35, 1000.0, 30.0, 5.0, 88.1472, 0.8)
+    t = S.TwoScaleSolver(cfg, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), fluid.positions, None,
+                         max_low=fluid.MaxParticles, max_high=8 * fluid.MaxParticles)
+    try:
+        t.set_boundary(walls.host_positions())
+        t.set_options(coupling=S.SPH2_COUPLING_BLENDED, split_offsets=S.SPH2_SPLIT_REFERENCE,
+                      merge=False, blend_increment=0.0125, velocity_cap=0.8)
+        t.set_debug(S.SPH_DEBUG_COLOR)
+        for k in range(steps):
+            t.Advance(0.001)
+            if k == 0:
+                t.mark_split(np.arange(0, fluid.MaxParticles, 7))
+        t.synchronize()
+        assert t.device_errors() == 0
+        c = t.counts()
+        assert (n_low, n_high) == c[:2]
+        assert n_high > 8 * 300  # the marked particles split
+        for r, (xyz, col, st) in enumerate(sets):
+            assert np.array_equal(xyz, t.download(r, S.FIELD_POSITION)), r
+            assert np.array_equal(col, t.download(r, S.FIELD_COLOR)), r
+            assert np.array_equal(st.view(np.uint32), t.download(r, S.FIELD_STATE)), r
+        assert np.any(sets[0][1] != 0)  # colour values are written
+        assert np.all((sets[1][2] & 4) == 4)  # high-set state words
+    finally:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_split_beyond_capacity_is_reported(tmp_path):
+    """A split that exceeds max_high drops the excess children (writes stay
+    within the capacity) and raises SPH_DEVERR_CAPACITY: synchronize reports
+    SPH_ERR_CAPACITY until the bits are read, the count stays at the capacity."""
+    sc = scenes.two_scale_dam_break(12, seed=3)
+    print(f"case split overflow n={sc.n} max_high=64", flush=True)
+    t = S.TwoScaleSolver(sc.config, sc.region_lo, sc.region_hi, sc.positions, sc.velocities,
+                         max_low=sc.n, max_high=64)
+    try:
+        t.Advance(sc.dt)
+        with pytest.raises(C.SPHError) as e:
+            t.synchronize()
+        assert e.value.code == C.SPH_ERR_CAPACITY
+        assert t.counts()[1] == 64
+        assert t.device_errors() & C.SPH_DEVERR_CAPACITY
+        t.synchronize()  # the bits were cleared
+    finally:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_nonfinite_state_is_reported():
+    """A NaN velocity (nothing in a step indexes by velocity) turns the
+    integrated state of that particle and its neighbours non-finite: the
+    force pass raises SPH_DEVERR_NONFINITE and synchronize reports
+    SPH_ERR_NONFINITE.  A finite run reports nothing."""
+    sc = scenes.dam_break((10, 10, 10), seed=2)
+    vel = sc.velocities.copy()
+    vel[17, 0] = np.nan
+    print(f"case non-finite velocity n={sc.n}", flush=True)
+    with S.Solver(S.ParticleData(sc.n, sc.positions, vel), None, sc.config) as s:
+        s.Advance(sc.dt)
+        with pytest.raises(C.SPHError) as e:
+            s.synchronize()
+        assert e.value.code == C.SPH_ERR_NONFINITE
+        assert s.device_errors() == C.SPH_DEVERR_NONFINITE
+        assert not np.all(np.isfinite(s.positions()))
+    with S.Solver(S.ParticleData(sc.n, sc.positions, sc.velocities), None, sc.config) as s:
+        s.Advance(sc.dt, 3)
+        s.synchronize()
+        assert s.device_errors() == 0
