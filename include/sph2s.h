@@ -527,6 +527,21 @@ int sph_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in,
                          uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
                          uint32_t key_bits, void* temp, size_t temp_bytes,
                          void* stream);
+/* The same sort without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  The temporary buffer
+ * is the library's own static device array (24 MiB, one per GPU, held until
+ * the process ends), so n is limited to SPH_RADIX_SORT_NOALLOC_MAX_KEYS pairs;
+ * a larger n returns SPH_ERR_CAPACITY before any device work.  The first call
+ * on a device takes the array's address there; no call allocates or frees.
+ * The array keeps the last call's contents; each call resets the sort state
+ * it needs with a kernel (no memset), so it is safe to capture.  Calls that
+ * overlap on one GPU (other streams, other threads) share the array and
+ * race: order them on one stream.  Otherwise as sph_radix_sort_pairs. */
+#define SPH_RADIX_SORT_NOALLOC_MAX_KEYS (1u << 21)
+int sph_radix_sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in,
+                                 uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                 uint32_t key_bits, void* stream);
 
 /* ---- diagnostics ----------------------------------------------------------- */
 const char* sph_last_error(void);

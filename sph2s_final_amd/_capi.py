@@ -194,6 +194,8 @@ SIGNATURES = [
     ("sph_radix_sort_temp_bytes", c_size_t, [c_uint32]),
     ("sph_radix_sort_pairs", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32,
                                      c_void_p, c_size_t, c_void_p]),
+    ("sph_radix_sort_pairs_noalloc", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_uint32,
+                                             c_uint32, c_void_p]),
     ("sph_last_error", c_char_p, []),
     ("sph_device_count", c_int, []),
     ("sph_version", c_char_p, []),

@@ -796,4 +796,11 @@ int sph_radix_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint3
                             static_cast<hipStream_t>(stream));
 }
 
+int sph_radix_sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in,
+                                 uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                                 uint32_t key_bits, void* stream) {
+    return radix_sort_pairs_static(keys_in, vals_in, keys_out, vals_out, n, key_bits,
+                                   static_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

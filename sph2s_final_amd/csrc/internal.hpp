@@ -87,6 +87,12 @@ __device__ __forceinline__ uint32_t cell_hash(float x, float y, float z, float o
 size_t radix_sort_temp_bytes(uint32_t n);
 // Device word set to non-zero if a look-back wait of the sort gave up.
 uint32_t* radix_sort_error_word(void* temp);
+// radix_sort_pairs with the library's static scratch array as the temporary
+// buffer (no allocation); n <= kStaticSortMaxKeys.
+constexpr uint32_t kStaticSortMaxKeys = 1u << 21;
+int radix_sort_pairs_static(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                            uint32_t* vals_out, uint32_t n, uint32_t key_bits,
+                            hipStream_t stream);
 // Bits of that word: the sort's look-back gave up (set by the sort), and the
 // force pass's integration wrote a NaN or infinite position or velocity.
 constexpr uint32_t kDevErrSortWait = 1u;

@@ -22,6 +22,7 @@
 // Every look-back wait is bounded: a wait that exceeds kSpinLimit polls sets
 // an error bit and gives up instead of hanging.
 #include <algorithm>
+#include <mutex>
 
 #include "neighbors.hpp"
 
@@ -336,8 +337,17 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(
     }
 }
 
-inline uint32_t tiles_for(uint32_t n) { return (n + kTile - 1) / kTile; }
-inline size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+constexpr uint32_t tiles_for(uint32_t n) { return (n + kTile - 1) / kTile; }
+constexpr size_t align_up(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Temporary bytes to sort n pairs: header, look-back status words, and the
+// ping-pong keys and values of the intermediate passes.
+constexpr size_t temp_bytes_for(uint32_t n) {
+    return align_up(kHeaderWords * sizeof(uint32_t)) +
+           align_up(static_cast<size_t>(kMaxPasses) * tiles_for(n > 0 ? n : 1) * kRadix *
+                    sizeof(uint32_t)) +
+           2 * align_up(static_cast<size_t>(n) * sizeof(uint32_t));
+}
 
 struct TempLayout {
     uint32_t* header;   // ghist[kMaxPasses][256], tickets, error word
@@ -429,12 +439,7 @@ int run_passes(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_
 
 }  // namespace
 
-size_t radix_sort_temp_bytes(uint32_t n) {
-    const size_t tiles = tiles_for(n > 0 ? n : 1);
-    return align_up(kHeaderWords * sizeof(uint32_t)) +
-           align_up(static_cast<size_t>(kMaxPasses) * tiles * kRadix * sizeof(uint32_t)) +
-           2 * align_up(static_cast<size_t>(n) * sizeof(uint32_t));
-}
+size_t radix_sort_temp_bytes(uint32_t n) { return temp_bytes_for(n); }
 
 uint32_t* radix_sort_error_word(void* temp) {
     return static_cast<uint32_t*>(temp) + kErrorWord;
@@ -476,6 +481,59 @@ int hash_hist_cells_batched(const StepParams& cap, const uint32_t* n_dev, const 
 int sort_cells(const StepParams& p, const uint32_t* keys, uint32_t* skeys, uint32_t* perm,
                uint32_t key_bits, void* temp, hipStream_t stream) {
     return run_passes(keys, nullptr, skeys, perm, p.n, key_bits, temp, stream);
+}
+
+// ---------------------------------------------------------------------------
+// The sort without any allocation: the temporary buffer is a file-scope
+// device array of fixed size, one copy per GPU for the life of the process.
+// Host code never passes the array's name to a kernel (on the host it only
+// stands in for the symbol): the first call on a device takes the array's
+// device address with hipGetSymbolAddress, with that device current, and
+// every later call on that device uses the kept address.
+constexpr size_t kStaticSortBytes = temp_bytes_for(kStaticSortMaxKeys);  // 24.0 MiB
+static __device__ __align__(256) unsigned char scratch[kStaticSortBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch[kMaxDevices] = {};
+
+// The device address of this GPU's scratch array (taken once per device).
+int scratch_address(void** out) {
+    int dev = 0;
+    SPH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(SPH_ERR_UNSUPPORTED, "radix sort: device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress",
+                             __FILE__, __LINE__);
+        g_scratch[dev] = ptr;
+    }
+    *out = g_scratch[dev];
+    return SPH_OK;
+}
+
+}  // namespace
+
+int radix_sort_pairs_static(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                            uint32_t* vals_out, uint32_t n, uint32_t key_bits,
+                            hipStream_t stream) {
+    if (n > kStaticSortMaxKeys)
+        return set_error(SPH_ERR_CAPACITY,
+                         "radix sort without allocation: n exceeds its maximum of 2^21 pairs "
+                         "(use sph_radix_sort_pairs with a caller-allocated buffer)");
+    if (n == 0)
+        return check_sort_args(keys_in, vals_in, keys_out, vals_out, n, key_bits, nullptr, 0);
+    void* temp = nullptr;
+    const int rc = scratch_address(&temp);
+    if (rc != SPH_OK) return rc;
+    return radix_sort_pairs(keys_in, vals_in, keys_out, vals_out, n, key_bits, temp,
+                            kStaticSortBytes, stream);
 }
 
 }  // namespace sph
