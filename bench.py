@@ -2,31 +2,46 @@
 """Benchmark of the SPH2S solver step on one MI355X.
 
 Workload: the 1,000,000-particle single-scale dam break (100^3 lattice,
-domain [0, 3.125]^3, h = 0.0457, FP32 SoA float4 state), stepped with the
-captured hipGraph.  One "step" = hash -> radix sort -> reorder + cell ranges
--> density -> force + symplectic Euler + wall clamp over all particles.
+domain [0, 3.125]^3, h = 0.0457, FP32 SoA float4 state), whose defined run is
+1000 steps.  One "step" = hash -> radix sort -> reorder + cell ranges ->
+density -> force + symplectic Euler + wall clamp over all particles, one
+captured hipGraph launch.
+
+The cost of a step grows several-fold as the column collapses and compresses
+(28 neighbours per particle at rest, 200 at step 600), so a window of K
+consecutive steps measures its own phase of the run.  The K timed steps are
+therefore spread evenly over the 1000-step trajectory: step floor(k * 1000 / K)
+for k = 0..K-1, each launched from the trajectory's own state, bracketed by a
+synchronize and hipEvents on the solver's stream; the steps in between advance
+the state untimed.  value = N * K / (sum of the K step times), an estimate of
+the whole run's rate; trajectory_1000 times the whole run in one bracket.
 
 Prints ONE JSON line with:
-  value       particle-updates/s = N * K / t over K timed graph launches
-              (state resident in HBM; hipEvents on the solver's stream)
+  value       particle-updates/s over the K sampled steps (state resident in HBM)
+  window      which steps of the trajectory were timed
   roofline    dominant kernel (force+integrate): algorithmic bytes per launch
-              (52 N + 8 C) / its mean duration over the timed steps, measured
-              with hipEvents around each pass on the solver's stream while a
-              second solver replays the same W + K steps eagerly (graph and
-              eager launches give bitwise-identical states)
+              (52 N + 8 C) / its mean duration over the same K steps, measured
+              with hipEvents around each pass on the solver's stream by a
+              second solver that replays the same trajectory (graph and eager
+              launches give bitwise-identical states)
+  trajectory_1000  the full 1000-step run timed in one bracket, and the force
+              kernel's mean over all its steps (what rocprofv3 averages)
   cpu_baseline the FP32 OpenMP CPU step (a port of the same algorithm) on the
-              same scene, timed for a bounded number of steps
+              same scene, timed for a bounded number of steps, with the FP64
+              single-threaded oracle step beside it
   adjoint     the differentiable rollout (16 scenes x 65,536 particles, 200
               steps, checkpoint every 20): forward and forward+adjoint times and
               particle-updates/s, and their ratio
-  batched     64 uneven scenes (log-uniform 4,096..262,144 particles) in one
-              batched step vs the same scenes as 64 single-scene solvers (each on
-              its own stream, and serialised)
+  batched     64 uneven scenes (log-uniform 4,096..262,144 particles, 500
+              steps) in one batched step vs the same scenes as 64 single-scene
+              solvers (each on its own stream, and serialised)
   walls       the reference's own single-scale scene: 215,460 fluid particles
               in a tank of 860,390 boundary particles (boundary-particle walls)
   two_scale   the two-scale dam break (63^3 low-resolution particles, 1->8
               split / 8->1 merge region over 20% of the domain), under the
               fixed coupling and under the reference's blended coupling
+Each of adjoint, batched and two_scale carries a "cpu" figure: the same work
+on the host, timed on a bounded sample.
 """
 from __future__ import annotations
 
@@ -42,6 +57,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_VALU_PEAK_TFLOPS = 157.3
 METRIC = "particle-updates/s at 1M particles, fwd and fwd+adjoint, % HBM roofline"
+TRAJECTORY_STEPS = 1000  # the defined run of the 1M-particle dam break
 
 
 def algorithmic_bytes(n: int, cells: int) -> dict:
@@ -65,7 +81,17 @@ def load_traffic(workload: str):
         return json.load(f).get("bytes_per_launch")
 
 
-def cpu_baseline(scene, budget_s: float, max_steps: int):
+def host_cores() -> dict:
+    """Host CPU counts: what the OpenMP CPU step uses, the process's affinity
+    (nproc) and the machine's logical CPUs."""
+    import oracle as O
+    return {"openmp_threads": O.CpuSolver.threads(), "nproc": len(os.sched_getaffinity(0)),
+            "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(scene, budget_s: float, max_steps: int, fp64_budget_s: float):
+    """FP32 OpenMP step of the oracle library (a port of the same algorithm)
+    on the same scene and state, then the FP64 single-threaded oracle step."""
     import oracle as O  # the CPU port of the step; test/bench infrastructure only
     cs = O.CpuSolver(O.params_from_config(scene.config), scene.positions, scene.velocities)
     cs.step(scene.dt, 1)  # warm-up
@@ -76,14 +102,121 @@ def cpu_baseline(scene, budget_s: float, max_steps: int):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": scene.n * done / dt,
         "unit": "particle-updates/s",
         "cores": O.CpuSolver.threads(),
         "kind": "port",
-        "sample": f"{scene.name}: {done} FP32 OpenMP steps of {scene.n} particles after 1 warm-up "
-                  f"step ({dt:.1f} s)",
+        "sample": f"{scene.name}: steps 2..{done + 1} as {done} FP32 OpenMP steps of {scene.n} "
+                  f"particles after 1 warm-up step ({dt:.1f} s)",
+        "host": host_cores(),
     }
+    out["fp64_single_thread"] = cpu_fp64_step(scene, fp64_budget_s)
+    return out
+
+
+def cpu_fp64_step(scene, budget_s: float) -> dict:
+    """The FP64 oracle step, single-threaded, from the scene's initial state:
+    one step to size the sample, then as many steps as fit the budget."""
+    import oracle as O
+    p = O.params_from_config(scene.config)
+    t0 = time.perf_counter()
+    O.rollout(p, scene.positions, scene.velocities, scene.dt, 1)
+    t1 = time.perf_counter() - t0
+    k = max(1, int(budget_s / max(t1, 1e-3)))
+    t0 = time.perf_counter()
+    O.rollout(p, scene.positions, scene.velocities, scene.dt, k)
+    dt = time.perf_counter() - t0
+    return {"value": scene.n * k / dt, "unit": "particle-updates/s", "cores": 1,
+            "sample": f"{scene.name}: steps 1..{k} of the FP64 oracle, one thread ({dt:.1f} s)"}
+
+
+def cpu_rollout(budget_s: float) -> dict:
+    """FP64 oracle forward + adjoint of the rollout, one thread, on scene 0 of
+    the 16-scene rollout (65,536 particles), for as many steps as fit."""
+    import numpy as np
+    import oracle as O
+    from sph2s_final_amd import scenes
+    bs = scenes.rollout_16x64k()
+    s0 = bs.scenes[0]
+    p = O.params_from_config(s0.config)
+    x0, v0 = s0.positions.astype(np.float64), s0.velocities.astype(np.float64)
+    t0 = time.perf_counter()
+    O.rollout_loss_grad(p, x0, v0, bs.dt, 1, x0)
+    t1 = time.perf_counter() - t0
+    k = max(1, min(200, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    O.rollout_loss_grad(p, x0, v0, bs.dt, k, x0)
+    dt = time.perf_counter() - t0
+    return {"fwd_adjoint_pu_s": s0.n * k / dt, "cores": 1, "kind": "port",
+            "sample": f"scene 0 ({s0.n} particles), {k} steps forward + adjoint, FP64 oracle, "
+                      f"one thread ({dt:.1f} s)"}
+
+
+def cpu_batched(bs, steps: int) -> dict:
+    """The FP32 OpenMP CPU step over every scene of the batch, a few steps."""
+    import oracle as O
+    solvers = [O.CpuSolver(O.params_from_config(s.config), s.positions, s.velocities)
+               for s in bs.scenes]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in solvers:
+            c.step(bs.dt, 1)
+    dt = time.perf_counter() - t0
+    return {"pu_s": bs.n * steps / dt, "cores": O.CpuSolver.threads(), "kind": "port",
+            "sample": f"all {len(bs.scenes)} scenes ({bs.n} particles), steps 1..{steps}, FP32 "
+                      f"OpenMP ({dt:.1f} s)"}
+
+
+def cpu_two_scale(sc, steps: int) -> dict:
+    """The FP64 two-scale oracle step, one thread, from the scene's split state."""
+    import oracle as O
+    from sph2s_final_amd import Grid
+    gh = Grid.MakeGrid((0.0, 0.0, 0.0), [sc.config.domain_hi[d] for d in range(3)],
+                       0.5 * sc.config.h)
+    o = O.OracleTwoScale(O.params_from_config(sc.config), gh, sc.region_lo, sc.region_hi, sc.n)
+    o.set_state(sc.positions, sc.velocities)
+    o.split_merge()
+    active, t0 = 0, time.perf_counter()
+    for _ in range(steps):
+        c = o.counts()
+        active += c[0] + c[1]
+        o.step(sc.dt)
+    dt = time.perf_counter() - t0
+    return {"pu_s": active / dt, "cores": 1, "kind": "port",
+            "sample": f"{sc.name}: steps 1..{steps} from the split state, FP64 oracle, one "
+                      f"thread ({dt:.1f} s, {active // steps} active particles per step)"}
+
+
+def bench_trajectory(torch, scene, alg) -> dict:
+    """The whole 1000-step run: graph-replayed steps timed in one bracket,
+    then the same trajectory replayed eagerly with per-pass hipEvents (the
+    force kernel's mean over every step, as rocprofv3 averages it)."""
+    from sph2s_final_amd import ParticleData, Solver
+    from sph2s_final_amd import _capi as C
+    n = scene.n
+    s = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
+    stream = torch.cuda.ExternalStream(s.views().stream)
+    s.synchronize()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    s.Advance(scene.dt, TRAJECTORY_STEPS)
+    b.record(stream)
+    s.synchronize()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / TRAJECTORY_STEPS
+    s.close()
+    r = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
+    per_pass = (C.ctypes.c_float * len(C.PASS_NAMES))()
+    C.check(C.lib().sph_profile_passes(r._h, scene.dt, TRAJECTORY_STEPS, per_pass))
+    r.close()
+    pass_ms = {name: per_pass[i] / TRAJECTORY_STEPS for i, name in enumerate(C.PASS_NAMES)}
+    f_gbs = alg["force_integrate"] / (pass_ms["force_integrate"] / 1e3) / 1e9
+    return {"steps": TRAJECTORY_STEPS, "ms_per_step": ms, "pu_s": n / (ms / 1e3),
+            "pass_ms": pass_ms, "force_integrate_gbs": f_gbs,
+            "force_integrate_frac": f_gbs / HBM_PEAK_GBS,
+            "step_hbm_frac": alg["step"] / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
 
 
 def _timed(fn, torch):
@@ -94,7 +227,8 @@ def _timed(fn, torch):
     return time.perf_counter() - t0
 
 
-def bench_adjoint(torch, steps: int, every: int, record_all: bool = True) -> dict:
+def bench_adjoint(torch, steps: int, every: int, record_all: bool = True,
+                  cpu_budget_s: float = 0.0) -> dict:
     """Forward vs forward+adjoint of the differentiable batched rollout."""
     import numpy as np
     from sph2s_final_amd import scenes
@@ -138,10 +272,12 @@ def bench_adjoint(torch, steps: int, every: int, record_all: bool = True) -> dic
         # The same rollout keeping every step's record (no recomputation):
         # about 76 MB per step at this size, 15 GB for 200 steps.
         res["record_every_step"] = bench_adjoint(torch, steps, steps, record_all=False)
+    if record_all and cpu_budget_s > 0:
+        res["cpu"] = cpu_rollout(cpu_budget_s)
     return res
 
 
-def bench_batched(torch, steps: int) -> dict:
+def bench_batched(torch, steps: int, cpu_steps: int = 0) -> dict:
     """64 uneven scenes: one batched step vs 64 single-scene solvers."""
     from sph2s_final_amd import BatchSolver, ParticleData, Solver, scenes
     bs = scenes.batched_64()
@@ -190,7 +326,8 @@ def bench_batched(torch, steps: int) -> dict:
                      "density_ms_default_tuned": tune_ms[:2],
                      "force_ms_default_tuned": tune_ms[2:]},
             "note": "one-per-launch: each scene its own solver and graph; 'streams' "
-                    "launches all 64 per step before waiting, 'serial' waits after each"}
+                    "launches all 64 per step before waiting, 'serial' waits after each",
+            **({"cpu": cpu_batched(bs, cpu_steps)} if cpu_steps > 0 else {})}
 
 
 def bench_walls(torch, steps: int) -> dict:
@@ -216,7 +353,7 @@ def bench_walls(torch, steps: int) -> dict:
             "state_finite": bool(np.all(np.isfinite(x))), "device_errors": errs}
 
 
-def bench_two_scale(torch, steps: int, blended: bool = False) -> dict:
+def bench_two_scale(torch, steps: int, blended: bool = False, cpu_steps: int = 0) -> dict:
     """Two-scale dam break; blended = the reference's lambda-blended coupling
     (fade timeline with increment 0.0125, velocity cap 0.8 h / dt, r/sqrt(3)
     split offsets) instead of the fixed coupling."""
@@ -247,6 +384,8 @@ def bench_two_scale(torch, steps: int, blended: bool = False) -> dict:
             m0 / 8 * t.download(1, FIELD_BLEND).astype(np.float64).sum()
         out["effective_mass_ratio"] = eff / (m0 * sc.n)
     t.close()
+    if cpu_steps > 0:
+        out["cpu"] = cpu_two_scale(sc, cpu_steps)
     return out
 
 
@@ -255,14 +394,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--profile-steps", type=int, default=0,
-                    help="timed steps replayed for the per-pass profile (0 = all)")
+    ap.add_argument("--no-trajectory", action="store_true",
+                    help="skip the full 1000-step run timed in one bracket")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
+    ap.add_argument("--cpu-fp64-budget", type=float, default=8.0,
+                    help="seconds of the FP64 single-threaded oracle step")
     ap.add_argument("--cpu-max-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--checkpoint-every", type=int, default=20)
-    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--two-scale-steps", type=int, default=500)
     ap.add_argument("--walls-steps", type=int, default=500)
     ap.add_argument("--only-main", action="store_true",
@@ -297,56 +438,63 @@ def main() -> int:
 
     scene = scenes.dam_break_1m()
     n = scene.n
+    run_len = max(TRAJECTORY_STEPS, args.steps)
+    sample = [k * run_len // args.steps for k in range(args.steps)]
+
+    # Warm-up: W graph steps on a solver of its own (capture, clocks, caches).
+    warm = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
+    warm.Advance(scene.dt, max(args.warmup, 1))
+    warm.synchronize()
+    warm.close()
+
+    # The K timed steps, spread over the trajectory.
     solver = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
     views = solver.views()
     cells = int(views.num_cells)
     stream = torch.cuda.ExternalStream(views.stream)
-
-    solver.Advance(scene.dt, args.warmup)
-    solver.synchronize()
-    torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    solver.Advance(scene.dt, args.steps)
-    stop.record(stream)
-    solver.synchronize()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop)
+    step_ms, at = [], 0
+    for s_k in sample:
+        if s_k > at:
+            solver.Advance(scene.dt, s_k - at)   # untimed: the trajectory up to step s_k
+        solver.synchronize()
+        torch.cuda.synchronize()
+        start.record(stream)
+        solver.Advance(scene.dt, 1)
+        stop.record(stream)
+        solver.synchronize()
+        torch.cuda.synchronize()
+        step_ms.append(start.elapsed_time(stop))
+        at = s_k + 1
+    ms = sum(step_ms)
     ms_per_step = ms / args.steps
     value = n * args.steps / (ms / 1e3)
-
     pos = solver.positions()
     finite = bool(__import__("numpy").isfinite(pos).all())
     dev_errors = solver.device_errors()
     solver.close()
 
-    # Per-pass durations over the same steps: a second solver replays the
-    # warm-up with the graph, then the timed steps eagerly with hipEvents
-    # bracketing every pass on its stream.
+    # Per-pass durations and pair counts at the same K steps: a second solver
+    # follows the same trajectory and runs each sampled step eagerly with
+    # hipEvents bracketing every pass on its stream (graph and eager steps are
+    # bitwise identical), then counts that step's candidate and in-range pairs.
     replay = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
-    replay.Advance(scene.dt, args.warmup)
-    replay.synchronize()
-    prof_steps = args.steps if args.profile_steps <= 0 else min(args.profile_steps, args.steps)
     per_pass = (C.ctypes.c_float * len(C.PASS_NAMES))()
-    C.check(C.lib().sph_profile_passes(replay._h, scene.dt, prof_steps, per_pass))
-    pass_ms = {name: per_pass[i] / prof_steps for i, name in enumerate(C.PASS_NAMES)}
+    cand = nbr = 0
+    at = 0
+    for s_k in sample:
+        if s_k > at:
+            replay.Advance(scene.dt, s_k - at)
+        C.check(C.lib().sph_profile_passes(replay._h, scene.dt, 1, per_pass))
+        c, b = replay.count_pairs()
+        cand += c
+        nbr += b
+        at = s_k + 1
     replay.close()
-
-    # Pair work for the FLOP model of the neighbour passes (about 18 flop per
-    # candidate pair over both passes and 50 per in-range pair), sampled every
-    # 50 steps along the same trajectory by a third solver.
-    counter = Solver(ParticleData(n, scene.positions, scene.velocities), None, scene.config)
-    counter.Advance(scene.dt, args.warmup)
-    samples, done = [], 0
-    while done < args.steps:
-        k = min(50, args.steps - done)
-        counter.Advance(scene.dt, k)
-        done += k
-        samples.append(counter.count_pairs())
-    counter.close()
-    cand = sum(c for c, _ in samples) / len(samples)
-    nbr = sum(b for _, b in samples) / len(samples)
+    pass_ms = {name: per_pass[i] / args.steps for i, name in enumerate(C.PASS_NAMES)}
+    cand /= args.steps
+    nbr /= args.steps
     flop_step = 18.0 * cand + 50.0 * nbr
     valu_tflops = flop_step / (ms_per_step / 1e3) / 1e12
 
@@ -373,6 +521,12 @@ def main() -> int:
                    "domain": [scene.config.domain_hi[d] for d in range(3)],
                    "h": round(float(scene.config.h), 6), "dt": scene.dt, "graph": True,
                    "path": "forward"},
+        "window": {"trajectory_steps": run_len, "timed_steps": args.steps,
+                   "spacing": run_len / args.steps, "first": sample[0], "last": sample[-1],
+                   "t_first_s": sample[0] * scene.dt, "t_last_s": sample[-1] * scene.dt,
+                   "step_ms_min": min(step_ms), "step_ms_max": max(step_ms),
+                   "note": "step floor(k * trajectory_steps / timed_steps), k = 0..timed_steps-1, "
+                           "each timed alone; the steps between advance the state untimed"},
         "roofline": {"bound": "hbm", "kernel": "force_integrate", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
@@ -389,23 +543,28 @@ def main() -> int:
                        "peak_tflops": FP32_VALU_PEAK_TFLOPS,
                        "frac": valu_tflops / FP32_VALU_PEAK_TFLOPS,
                        "frac_unpacked": valu_tflops / (FP32_VALU_PEAK_TFLOPS / 2),
-                       "samples": len(samples),
+                       "samples": args.steps,
                        "note": "18 flop per candidate pair (distance test, both passes) + 50 per "
-                               "in-range pair, pairs counted on the state every 50 steps; "
+                               "in-range pair, pairs counted at every timed step; "
                                "frac_unpacked against half the peak (packed FP32 is not used)"},
         "state_finite": finite,
         "device_errors": dev_errors,
-        "profiled_steps": prof_steps,
     }
+    if not args.no_trajectory:
+        out["trajectory_1000"] = bench_trajectory(torch, scene, alg)
+    cpu = not args.no_cpu_baseline
     if not args.only_main:
-        out["adjoint"] = bench_adjoint(torch, args.adjoint_steps, args.checkpoint_every)
+        out["adjoint"] = bench_adjoint(torch, args.adjoint_steps, args.checkpoint_every,
+                                       cpu_budget_s=4.0 if cpu else 0.0)
         out["value_fwd_adjoint"] = out["adjoint"]["fwd_adjoint_pu_s"]
-        out["batched"] = bench_batched(torch, args.batched_steps)
+        out["batched"] = bench_batched(torch, args.batched_steps, cpu_steps=3 if cpu else 0)
         out["walls"] = bench_walls(torch, args.walls_steps)
-        out["two_scale"] = bench_two_scale(torch, args.two_scale_steps)
+        out["two_scale"] = bench_two_scale(torch, args.two_scale_steps,
+                                           cpu_steps=3 if cpu else 0)
         out["two_scale_blended"] = bench_two_scale(torch, args.two_scale_steps, blended=True)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps,
+                                           args.cpu_fp64_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0

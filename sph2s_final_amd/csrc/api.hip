@@ -630,6 +630,7 @@ int sph_profile_passes(sph_solver* s, float dt, uint32_t n_steps, float* ms_out)
         }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc == SPH_OK && n_steps > 0 && s->n > 0) s->stepped = true;
     return rc;
 }
 
