@@ -58,6 +58,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_VALU_PEAK_TFLOPS = 157.3
 METRIC = "particle-updates/s at 1M particles, fwd and fwd+adjoint, % HBM roofline"
 TRAJECTORY_STEPS = 1000  # the defined run of the 1M-particle dam break
+LEGS = ("main", "trajectory", "adjoint", "batched", "walls", "two_scale", "two_scale_blended",
+        "cpu")
 
 
 def algorithmic_bytes(n: int, cells: int) -> dict:
@@ -389,53 +391,10 @@ def bench_two_scale(torch, steps: int, blended: bool = False, cpu_steps: int = 0
     return out
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--no-trajectory", action="store_true",
-                    help="skip the full 1000-step run timed in one bracket")
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
-    ap.add_argument("--cpu-fp64-budget", type=float, default=8.0,
-                    help="seconds of the FP64 single-threaded oracle step")
-    ap.add_argument("--cpu-max-steps", type=int, default=400)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--adjoint-steps", type=int, default=200)
-    ap.add_argument("--checkpoint-every", type=int, default=20)
-    ap.add_argument("--batched-steps", type=int, default=500)
-    ap.add_argument("--two-scale-steps", type=int, default=500)
-    ap.add_argument("--walls-steps", type=int, default=500)
-    ap.add_argument("--only-main", action="store_true",
-                    help="skip the adjoint, batched and two-scale measurements")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run plain 256 MiB device copies (scales rocprofv3 byte counters)")
-    args = ap.parse_args()
-
-    if args.copy_calibration:
-        import torch
-        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda").uniform_()
-        y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 5}), flush=True)
-        return 0
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: the solver runs on exactly one GPU; multi-GPU runs are not supported "
-              "(scenes of a batch would be bin-packed onto GPUs, one large scene split into "
-              "x-slabs with one-cell halos; neither is built)", file=sys.stderr)
-        return 2
-
-    import torch
-    from sph2s_final_amd import Solver, ParticleData, scenes
+def bench_main(torch, args) -> dict:
+    """The headline: K steps of the 1M dam break spread over its 1000-step run."""
+    from sph2s_final_amd import ParticleData, Solver, scenes
     from sph2s_final_amd import _capi as C
-
-    if not torch.cuda.is_available():
-        print("bench.py: no GPU visible", file=sys.stderr)
-        return 1
-
     scene = scenes.dam_break_1m()
     n = scene.n
     run_len = max(TRAJECTORY_STEPS, args.steps)
@@ -550,22 +509,97 @@ def main() -> int:
         "state_finite": finite,
         "device_errors": dev_errors,
     }
-    if not args.no_trajectory:
-        out["trajectory_1000"] = bench_trajectory(torch, scene, alg)
-    cpu = not args.no_cpu_baseline
-    if not args.only_main:
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-trajectory", action="store_true",
+                    help="skip the full 1000-step run timed in one bracket")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline")
+    ap.add_argument("--cpu-fp64-budget", type=float, default=8.0,
+                    help="seconds of the FP64 single-threaded oracle step")
+    ap.add_argument("--cpu-max-steps", type=int, default=400)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--checkpoint-every", type=int, default=20)
+    ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--two-scale-steps", type=int, default=500)
+    ap.add_argument("--walls-steps", type=int, default=500)
+    ap.add_argument("--only-main", action="store_true",
+                    help="skip the adjoint, batched and two-scale measurements")
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma-separated measurements to run, of " + ",".join(LEGS))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run plain 256 MiB device copies (scales rocprofv3 byte counters)")
+    args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        x = torch.empty(64 << 20, dtype=torch.float32, device="cuda").uniform_()
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 5}), flush=True)
+        return 0
+
+    legs = set(v for v in args.legs.split(",") if v)
+    unknown = legs - set(LEGS)
+    if unknown:
+        print(f"bench.py: unknown legs {sorted(unknown)}", file=sys.stderr)
+        return 2
+    if args.only_main:
+        legs &= {"main", "trajectory", "cpu"}
+    if args.no_trajectory:
+        legs.discard("trajectory")
+    if args.no_cpu_baseline:
+        legs.discard("cpu")
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: the solver runs on exactly one GPU; multi-GPU runs are not supported "
+              "(scenes of a batch would be bin-packed onto GPUs, one large scene split into "
+              "x-slabs with one-cell halos; neither is built)", file=sys.stderr)
+        return 2
+
+    import torch
+    from sph2s_final_amd import scenes
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+
+    scene = scenes.dam_break_1m()
+    if "main" in legs:
+        out = bench_main(torch, args)
+    else:  # a partial run (profiling one leg): not the benchmark line
+        out = {"metric": METRIC, "value": None, "legs": sorted(legs)}
+    if "trajectory" in legs:
+        cells = int(scene.config.grid.dims[0]) * int(scene.config.grid.dims[1]) * \
+            int(scene.config.grid.dims[2])
+        out["trajectory_1000"] = bench_trajectory(torch, scene, algorithmic_bytes(scene.n, cells))
+    cpu = "cpu" in legs
+    if "adjoint" in legs:
         out["adjoint"] = bench_adjoint(torch, args.adjoint_steps, args.checkpoint_every,
                                        cpu_budget_s=4.0 if cpu else 0.0)
         out["value_fwd_adjoint"] = out["adjoint"]["fwd_adjoint_pu_s"]
+    if "batched" in legs:
         out["batched"] = bench_batched(torch, args.batched_steps, cpu_steps=3 if cpu else 0)
+    if "walls" in legs:
         out["walls"] = bench_walls(torch, args.walls_steps)
+    if "two_scale" in legs:
         out["two_scale"] = bench_two_scale(torch, args.two_scale_steps,
                                            cpu_steps=3 if cpu else 0)
+    if "two_scale_blended" in legs:
         out["two_scale_blended"] = bench_two_scale(torch, args.two_scale_steps, blended=True)
     if cpu:
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps,
                                            args.cpu_fp64_budget)
-        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        if out.get("value"):
+            out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0
 

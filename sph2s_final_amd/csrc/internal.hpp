@@ -159,6 +159,9 @@ struct StepBuffers {
     // Colour value per sorted slot (Mexican-hat energy), computed with the
     // force pass when non-NULL.
     float* color;
+    // The reorder pass writes the sorted copies tpos/tvel even with the
+    // x-order (the two-scale step's other passes read them).
+    bool keep_sorted;
 };
 
 // Passes of one step, in launch order.
@@ -173,6 +176,12 @@ struct GridView;
 int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t stream);
 int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& b,
                      float4* accel_own, hipStream_t stream);
+// The same two passes in the internal x-order (b.ipos set; results at the
+// sorted slots), and the internal-order copy of the finalised A/B between them.
+int launch_density_own_x(const StepParams& p, const StepBuffers& b, hipStream_t stream);
+int launch_gather_aux(const StepParams& p, const StepBuffers& b, hipStream_t stream);
+int launch_force_own_x(const StepParams& p, const GridView& g, const StepBuffers& b,
+                       float4* accel_own, hipStream_t stream);
 int launch_pack_state(const float* pos_xyz_dev, const float* vel_xyz_dev, uint32_t n,
                       float4* pos, float4* vel, hipStream_t stream);
 int launch_export_aos(const float4* src, uint32_t n, float* dst_xyz, hipStream_t stream);

@@ -760,12 +760,12 @@ __global__ __launch_bounds__(kBlock) void k_xsph_integrate(
 // other instantiations use the defaults.  The single-scale and batched passes
 // (density with aux, force with integration) run in the internal x-order; the
 // two-scale own-set passes (neither) in the sorted order.
-template <bool kKahan, bool kFinalize, bool kBatched, class... A>
+template <bool kKahan, bool kFinalize, bool kBatched, bool kX = kFinalize, class... A>
 void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
     const LaunchShape sh = launch_shape(cap_n, kFinalize ? ((tune >> 8) & 0xFu) : 0u);
 #define SPH_DENSITY_VARIANT(TILE)                                                               \
-    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kFinalize>), sh.grid,     \
+    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kX>), sh.grid,            \
                        dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kFinalize) {
         switch (v) {
@@ -781,13 +781,13 @@ void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
 #undef SPH_DENSITY_VARIANT
 }
 
-template <bool kTension, bool kIntegrate, bool kBatched, class... A>
+template <bool kTension, bool kIntegrate, bool kBatched, bool kX = kIntegrate, class... A>
 void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     constexpr bool kTunable = kIntegrate && !kTension;
     const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
     const LaunchShape sh = launch_shape(cap_n, kTunable ? ((tune >> 12) & 0xFu) : 0u);
 #define SPH_FORCE_VARIANT(TILE, CAP, TLIST)                                                     \
-    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kIntegrate>), \
+    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kX>),         \
                        sh.grid, dim3(kBlock), 0, st, args..., sh.nvb)
     if constexpr (kTunable) {
         switch (v) {
@@ -852,7 +852,7 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt,
             // XSPH passes and by a step without the x-order; otherwise the
             // x-order reads the state through perm and no copy is made.
             const bool copy = !b.ipos || b.color || (b.walls && b.bacc) ||
-                              (p.flags & SPH_FLAG_XSPH);
+                              (p.flags & SPH_FLAG_XSPH) || b.keep_sorted;
             if (copy)
                 hipLaunchKernelGGL(k_reorder<true>, blocks_for(p.n), dim3(kBlock), 0, stream,
                                    b.skeys, b.perm, b.pos, b.vel, b.tpos, b.tvel, b.cell_start,
@@ -970,6 +970,61 @@ int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& 
         launch_force<false, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
                                           b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
                                           accel_own, kNoScenes, XOrder{});
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+// The two-scale own-set passes in the internal x-order: density without the
+// A/B finalisation (the complementary sums complete it), the internal-order
+// copy of the finalised A/B, then the force without integration.  Results
+// land at the sorted slots, as in the sorted-order passes above.
+int launch_density_own_x(const StepParams& p, const StepBuffers& b, hipStream_t stream) {
+    if (p.n == 0) return SPH_OK;
+    if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
+    const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
+    const XOrder xo{b.i2r, b.xbins, nullptr};
+    if (p.flags & SPH_FLAG_KAHAN)
+        launch_density<true, false, false, true>(p.tune, p.n, stream, b.ipos, b.cell_start,
+                                                 b.cell_end, p, g, b.n_dev, b.rho, b.aux, b.ivel,
+                                                 kNoScenes, xo);
+    else
+        launch_density<false, false, false, true>(p.tune, p.n, stream, b.ipos, b.cell_start,
+                                                  b.cell_end, p, g, b.n_dev, b.rho, b.aux,
+                                                  b.ivel, kNoScenes, xo);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_aux(const float2* __restrict__ aux,
+                                                       const uint32_t* __restrict__ i2r,
+                                                       const uint32_t* __restrict__ n_dev,
+                                                       float2* __restrict__ iaux) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < *n_dev) iaux[s] = aux[i2r[s]];
+}
+
+int launch_gather_aux(const StepParams& p, const StepBuffers& b, hipStream_t stream) {
+    if (p.n == 0) return SPH_OK;
+    hipLaunchKernelGGL(k_gather_aux, blocks_for(p.n), dim3(kBlock), 0, stream, b.aux, b.i2r,
+                       b.n_dev, b.iaux);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_force_own_x(const StepParams& p, const GridView& g, const StepBuffers& b,
+                       float4* accel_own, hipStream_t stream) {
+    if (p.n == 0) return SPH_OK;
+    if (!b.ipos) return set_error(SPH_ERR_INVALID_ARGUMENT, "step buffers lack the x-order");
+    float4* const no_state = nullptr;
+    const XOrder xo{b.i2r, b.xbins, nullptr, nullptr};
+    if (p.tens != 0.0f)
+        launch_force<true, false, false, true>(p.tune, p.n, stream, b.ipos, b.ivel, b.iaux,
+                                               b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f,
+                                               no_state, no_state, accel_own, kNoScenes, xo);
+    else
+        launch_force<false, false, false, true>(p.tune, p.n, stream, b.ipos, b.ivel, b.iaux,
+                                                b.cell_start, b.cell_end, p, g, b.n_dev, 0.0f,
+                                                no_state, no_state, accel_own, kNoScenes, xo);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }

@@ -83,9 +83,12 @@ __device__ __forceinline__ float child_offset(int k, int axis, float d) {
 }
 
 // ---------------------------------------------------------------------------
-// Occupancy of the low-resolution grid by high-resolution particles, so that a
-// low particle whose neighbourhood holds none skips the walk over the high
-// grid (most low particles lie away from the refinement region).
+// Occupancy of the low-resolution grid by each set, so that a particle with no
+// particle of the other set within reach skips the walk over that set's grid
+// (most low particles lie away from the refinement region, and most high
+// particles deep inside it).  Per low-grid cell: occ_h = 1 when it holds a high
+// particle, occ_l = 1 when it holds a low particle; near = the OR of both over
+// the cell's 3 x 3 x 3 block (bit 0 high, bit 1 low).
 __global__ void k_clear_words(uint32_t* __restrict__ w, uint32_t count) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
         w[i] = 0u;
@@ -99,10 +102,68 @@ __global__ void k_mark_occupancy(const float4* __restrict__ pos, const uint32_t*
     occ[cell_hash(x.x, x.y, x.z, g.ox, g.oy, g.oz, g.spacing, g.dx, g.dy, g.dz)] = 1u;
 }
 
+__global__ void k_dilate_occupancy(const uint32_t* __restrict__ occ_h,
+                                   const uint32_t* __restrict__ occ_l, int dx, int dy, int dz,
+                                   uint32_t* __restrict__ near) {
+    const uint32_t cells = static_cast<uint32_t>(dx) * static_cast<uint32_t>(dy) *
+                           static_cast<uint32_t>(dz);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells;
+         c += gridDim.x * blockDim.x) {
+        const int cx = static_cast<int>(c % static_cast<uint32_t>(dx));
+        const int cy = static_cast<int>((c / static_cast<uint32_t>(dx)) % static_cast<uint32_t>(dy));
+        const int cz = static_cast<int>(c / (static_cast<uint32_t>(dx) * static_cast<uint32_t>(dy)));
+        uint32_t h = 0u, l = 0u;
+        for (int z = max(cz - 1, 0); z <= min(cz + 1, dz - 1); ++z)
+            for (int y = max(cy - 1, 0); y <= min(cy + 1, dy - 1); ++y) {
+                const uint32_t row = static_cast<uint32_t>(dx) * (y + dy * z);
+                for (int x = max(cx - 1, 0); x <= min(cx + 1, dx - 1); ++x) {
+                    h |= occ_h[row + x];
+                    l |= occ_l[row + x];
+                }
+            }
+        near[c] = (h ? 1u : 0u) | (l ? 2u : 0u);
+    }
+}
+
+// The skip test of one cross walk: occ marks the low-grid cells holding a
+// particle of the other set (NULL: no test, always walk), near the dilated
+// marks (bit `bit`).
+struct Nearby {
+    const uint32_t* occ;
+    const uint32_t* near;
+    uint32_t bit;
+};
+
 // True when a low-grid cell within reach of (x, y, z) -- the cells overlapping
-// the box x +/- 1.001 h0 (a margin far above rounding) -- holds a high particle.
-__device__ __forceinline__ bool high_nearby(const uint32_t* __restrict__ occ, const GridView& ga,
-                                            float x, float y, float z) {
+// the box x +/- 1.001 h0 (a margin far above rounding) -- holds a particle of
+// the other set.  When those cells lie within the 3 x 3 x 3 block around the
+// particle's own cell (all but particles within 0.001 h0 of a cell face), the
+// block's dilated mark answers with one load; otherwise the cells are read.
+__device__ __forceinline__ bool other_nearby(const Nearby& nb, const GridView& ga, float x,
+                                             float y, float z) {
+    if (!nb.occ) return true;
+    // Fast path: the particle's position in cell units (one multiply; off by
+    // far less than the 0.001 margins from the exact division).  The box
+    // x +/- 1.001 h0 stays within cells c - 1 .. c + 1 when the fraction f of
+    // the particle's cell coordinate satisfies q - 1 < f < 2 - q with
+    // q = 1.001 h0 / spacing; the margins cover the rounding of q and f.
+    {
+        const float q = ga.h * 1.001f * ga.inv_spacing;
+        const float lo = q - 1.0f + 0.001f, hi = 2.0f - q - 0.001f;
+        const float tx = (x - ga.ox) * ga.inv_spacing, ty = (y - ga.oy) * ga.inv_spacing,
+                    tz = (z - ga.oz) * ga.inv_spacing;
+        const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+        if (tx - fx > lo && tx - fx < hi && ty - fy > lo && ty - fy < hi && tz - fz > lo &&
+            tz - fz < hi && fx >= 0.0f && fy >= 0.0f && fz >= 0.0f &&
+            fx < static_cast<float>(ga.dx) && fy < static_cast<float>(ga.dy) &&
+            fz < static_cast<float>(ga.dz)) {
+            const uint32_t c = static_cast<uint32_t>(fx) +
+                               static_cast<uint32_t>(ga.dx) *
+                                   (static_cast<uint32_t>(fy) +
+                                    static_cast<uint32_t>(ga.dy) * static_cast<uint32_t>(fz));
+            return (nb.near[c] & nb.bit) != 0u;
+        }
+    }
     const float r = ga.h * 1.001f;
     const int sx = cell_coord_off(x, -r, ga.ox, ga.spacing, ga.dx);
     const int ex = cell_coord_off(x, r, ga.ox, ga.spacing, ga.dx);
@@ -110,117 +171,329 @@ __device__ __forceinline__ bool high_nearby(const uint32_t* __restrict__ occ, co
     const int ey = cell_coord_off(y, r, ga.oy, ga.spacing, ga.dy);
     const int sz = cell_coord_off(z, -r, ga.oz, ga.spacing, ga.dz);
     const int ez = cell_coord_off(z, r, ga.oz, ga.spacing, ga.dz);
+    const int cx = cell_coord(x, ga.ox, ga.spacing, ga.dx);
+    const int cy = cell_coord(y, ga.oy, ga.spacing, ga.dy);
+    const int cz = cell_coord(z, ga.oz, ga.spacing, ga.dz);
+    if (sx >= cx - 1 && ex <= cx + 1 && sy >= cy - 1 && ey <= cy + 1 && sz >= cz - 1 &&
+        ez <= cz + 1)
+        return (nb.near[static_cast<uint32_t>(cx) +
+                        static_cast<uint32_t>(ga.dx) * (cy + ga.dy * cz)] & nb.bit) != 0u;
     uint32_t any = 0u;
-    for (int cz = sz; cz <= ez; ++cz)
-        for (int cy = sy; cy <= ey; ++cy) {
-            const uint32_t row = static_cast<uint32_t>(ga.dx) * (cy + ga.dy * cz);
-            for (int cx = sx; cx <= ex; ++cx) any |= occ[row + cx];
+    for (int cz2 = sz; cz2 <= ez; ++cz2)
+        for (int cy2 = sy; cy2 <= ey; ++cy2) {
+            const uint32_t row = static_cast<uint32_t>(ga.dx) * (cy2 + ga.dy * cz2);
+            for (int cx2 = sx; cx2 <= ex; ++cx2) any |= nb.occ[row + cx2];
         }
     return any != 0u;
 }
 
 // ---------------------------------------------------------------------------
-// complementary density: rho_i += m_B sum_{j in B, r < h0} 1/2 (W_h0 + W_h1),
-// then A_i, B_i for the force passes.
-__global__ __launch_bounds__(kBlock) void k_cross_density(
-    const float4* __restrict__ tpos_a, const uint32_t* __restrict__ n_a,
-    const float4* __restrict__ tpos_b, const uint32_t* __restrict__ cs_b,
-    const uint32_t* __restrict__ ce_b, const uint32_t* __restrict__ n_b, GridView gb,
-    CrossParams c, const uint32_t* __restrict__ occ, GridView ga, float* __restrict__ rho_a,
-    float2* __restrict__ aux_a) {
-    const uint32_t na = *n_a, nb = *n_b;
-    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= na) return;
-    const float4 xi = tpos_a[s];
-    float sum = 0.0f;
-    if (nb > 0 && (!occ || high_nearby(occ, ga, xi.x, xi.y, xi.z))) {
-        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-            for (uint32_t j = b; j < e; ++j) {
-                const float4 xj = tpos_b[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < c.h02) {
-                    const float a0 = c.h02 - r2;
-                    const float a1 = c.h12 - r2;
-                    const float w0 = c.c6_0 * a0 * a0 * a0;
-                    const float w1 = (r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f;
-                    sum += 0.5f * (w0 + w1);
-                }
-            }
-        });
-    }
-    const float r = rho_a[s] + c.m_other * sum;
-    rho_a[s] = r;
-    const float inv = __builtin_amdgcn_rcpf(r);
-    aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+// The complementary sums by wave.  Few particles have a particle of the other
+// set within h0 (those near the interface), but a low particle there sees
+// hundreds of high ones (8 per low-resolution volume), and one lane walking
+// them alone holds its whole wave -- and the kernel -- for the length of that
+// serial walk.  So the walkers are listed first (k_cross_list, which also
+// finalises the others), and each listed particle gets a whole wave: the
+// lanes probe its stencil rows in parallel, the rows' runs are laid end to
+// end, lane l takes candidates l, l + 64, ... of that sequence, and the
+// lanes' partial sums are reduced in a fixed order (so the result does not
+// depend on which wave takes which particle or on the list's order).
+constexpr int kWalkMaxRows = 64;
+constexpr uint32_t kWalkBlocks = 1024;  // 4096 waves share a step's walk list
+// The walk list is kept in 64 segments, each with its own counter 64 bytes
+// from the next (appends from one block go to segment blockIdx % 64), so the
+// appends do not all contend for one address.
+constexpr uint32_t kWalkSegments = 64;
+constexpr uint32_t kWalkCounterStride = 16;  // words
+
+inline uint32_t walk_segment_cap(uint32_t cap) {
+    const uint32_t blocks = (cap + kBlock - 1) / kBlock;
+    return ((blocks + kWalkSegments - 1) / kWalkSegments) * kBlock;
 }
 
-// complementary force (TWO_SCALE/Solver.cu:394-499 with lambda = 1) added to
-// the own-set acceleration, then symplectic Euler and the wall clamp.
+// Entry w of the segmented walk list: seg_incl holds the wave's inclusive
+// prefix of the 64 segment counts (seg_incl[63] = total > w).
+__device__ __forceinline__ uint32_t walk_entry(const uint32_t* __restrict__ list,
+                                               uint32_t seg_cap, const uint32_t* seg_incl,
+                                               uint32_t w) {
+    uint32_t k = 0u;  // the number of segments whose inclusive count is <= w
+    for (uint32_t step = 32u; step >= 1u; step >>= 1)
+        if (seg_incl[k + step - 1u] <= w) k += step;
+    const uint32_t begin = (k > 0u) ? seg_incl[k - 1u] : 0u;
+    return list[k * seg_cap + (w - begin)];
+}
+
+// The wave's inclusive prefix of the segment counts, into its LDS row;
+// returns the list's length.
+__device__ __forceinline__ uint32_t walk_segments(const uint32_t* __restrict__ counts,
+                                                  uint32_t* seg_incl) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t incl = counts[lane * kWalkCounterStride];
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off));
+        if (lane >= static_cast<uint32_t>(off)) incl += v;
+    }
+    seg_incl[lane] = incl;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63));
+}
+
+__device__ __forceinline__ uint64_t lanes_below(uint32_t lane) {
+    return (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Appends `value` of every lane with `flag` to list[*count ...] (one atomic
+// per wave).  Every lane of the wave must call it.
+__device__ __forceinline__ void append_flagged(bool flag, uint32_t value,
+                                               uint32_t* __restrict__ count,
+                                               uint32_t* __restrict__ list) {
+    const uint64_t m = __ballot(flag);
+    if (m == 0ull) return;  // wave-uniform
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0u;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(count, __popcll(m));
+    base = static_cast<uint32_t>(__shfl(static_cast<int>(base), leader));
+    if (flag) list[base + __popcll(m & lanes_below(lane))] = value;
+}
+
+// Sum over the wave's 64 lanes in a fixed butterfly order (every lane active).
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Calls body(j) for the candidates j of one particle's stencil rows over grid
+// g, spread over the wave's lanes: row r is probed by lane r, its run
+// [rb[r], rb[r] + len_r) takes positions [incl[r] - len_r, incl[r]) of the
+// rows laid end to end, and lane l visits positions l, l + 64, ...  rb_s and
+// incl_s are the wave's own 64-entry LDS rows.  Every lane of the wave calls
+// it with the same particle.
+template <class Body>
+__device__ __forceinline__ void wave_walk(const GridView& g, const uint32_t* __restrict__ cs,
+                                          const uint32_t* __restrict__ ce, uint32_t n, float x,
+                                          float y, float z, uint32_t* rb_s, uint32_t* incl_s,
+                                          Body&& body) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const int sx = cell_coord_off(x, -g.h, g.ox, g.spacing, g.dx);
+    const int sy = cell_coord_off(y, -g.h, g.oy, g.spacing, g.dy);
+    const int sz = cell_coord_off(z, -g.h, g.oz, g.spacing, g.dz);
+    const int ex = cell_coord_off(x, g.h, g.ox, g.spacing, g.dx);
+    const int ey = cell_coord_off(y, g.h, g.oy, g.spacing, g.dy);
+    const int ez = cell_coord_off(z, g.h, g.oz, g.spacing, g.dz);
+    const float fx = grid_offset(x, g.ox, g.spacing, g.dx);
+    const float fy = grid_offset(y, g.oy, g.spacing, g.dy);
+    const float fz = grid_offset(z, g.oz, g.spacing, g.dz);
+    const int ny = ey - sy + 1;
+    const int nrows = ny * (ez - sz + 1);
+    // Rows in chunks of 64 (a search radius of one cell pitch or two gives 9
+    // or 25 rows; the chunk loop only guards larger ratios).
+    for (int r0 = 0; r0 < nrows; r0 += kWalkMaxRows) {
+        const int r = r0 + static_cast<int>(lane);
+        uint32_t b = kEmptyCell, e = 0u;
+        if (r < nrows) {
+            const int cy = sy + r % ny, cz = sz + r / ny;
+            const float gy = slab_gap(fy, cy, g.spacing);
+            const float gz = slab_gap(fz, cz, g.spacing);
+            const float d2 = gy * gy + gz * gz;
+            if (d2 < g.prune_h2) {
+                const float w = __builtin_sqrtf(g.prune_h2 - d2);
+                const int lo = max(sx, static_cast<int>(floorf((fx - w) * g.inv_spacing)));
+                const int hi = min(ex, static_cast<int>(floorf((fx + w) * g.inv_spacing)));
+                if (lo <= hi)
+                    row_run(cs, ce, g.cell_offset + static_cast<uint32_t>(g.dx) * (cy + g.dy * cz),
+                            lo, hi, n, b, e);
+            }
+        }
+        const uint32_t len = (b < e) ? e - b : 0u;
+        // Inclusive prefix of the run lengths over the lanes.
+        uint32_t incl = len;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off));
+            if (lane >= static_cast<uint32_t>(off)) incl += v;
+        }
+        const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63));
+        rb_s[lane] = b;
+        incl_s[lane] = incl;
+        // (the wave's own LDS rows: no barrier beyond the wave is needed)
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // incl_s[63] = total > f, so the row search stops within the 64 rows.
+        uint32_t row = 0u;
+        for (uint32_t f = lane; f < total; f += 64u) {
+            while (incl_s[row] <= f) ++row;
+            const uint32_t begin = (row > 0u) ? incl_s[row - 1u] : 0u;
+            body(rb_s[row] + (f - begin));
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Lists set A's particles whose complementary walk is not empty and
+// finalises the others (density unchanged, A and B from it).
+__global__ __launch_bounds__(kBlock) void k_cross_list(
+    const float4* __restrict__ tpos_a, const uint32_t* __restrict__ n_a,
+    const uint32_t* __restrict__ n_b, Nearby occ, GridView ga, CrossParams c,
+    float* __restrict__ rho_a, float2* __restrict__ aux_a, uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ list, uint32_t seg_cap) {
+    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t seg = blockIdx.x % kWalkSegments;
+    const bool live = s < na;
+    bool walk = false;
+    if (live) {
+        const float4 xi = tpos_a[s];
+        walk = nb > 0 && other_nearby(occ, ga, xi.x, xi.y, xi.z);
+        if (!walk) {
+            const float r = rho_a[s];
+            const float inv = __builtin_amdgcn_rcpf(r);
+            aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+        }
+    }
+    append_flagged(walk, s, counts + seg * kWalkCounterStride, list + seg * seg_cap);
+}
+
+// complementary density of the listed particles: rho_i += m_B sum_{j in B,
+// r < h0} 1/2 (W_h0 + W_h1), then A_i, B_i.  One wave per listed particle.
+__global__ __launch_bounds__(kBlock) void k_cross_density_walk(
+    const float4* __restrict__ tpos_a, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ list, uint32_t seg_cap, const float4* __restrict__ tpos_b,
+    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
+    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, float* __restrict__ rho_a,
+    float2* __restrict__ aux_a) {
+    __shared__ uint32_t rb_s[kBlock / 64][64];
+    __shared__ uint32_t incl_s[kBlock / 64][64];
+    __shared__ uint32_t seg_s[kBlock / 64][64];
+    const uint32_t nb = *n_b;
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t nw = walk_segments(counts, seg_s[wv]);
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t w = blockIdx.x * (kBlock / 64) + wv; w < nw; w += waves) {
+        const uint32_t s = walk_entry(list, seg_cap, seg_s[wv], w);
+        const float4 xi = tpos_a[s];
+        float sum = 0.0f;
+        wave_walk(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv], [&](uint32_t j) {
+            const float4 xj = tpos_b[j];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < c.h02) {
+                const float a0 = c.h02 - r2;
+                const float a1 = c.h12 - r2;
+                const float w0 = c.c6_0 * a0 * a0 * a0;
+                const float w1 = (r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f;
+                sum += 0.5f * (w0 + w1);
+            }
+        });
+        sum = wave_sum(sum);
+        if (lane == 0u) {
+            const float r = rho_a[s] + c.m_other * sum;
+            rho_a[s] = r;
+            const float inv = __builtin_amdgcn_rcpf(r);
+            aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+        }
+    }
+}
+
+// complementary force (TWO_SCALE/Solver.cu:394-499 with lambda = 1) of the
+// listed particles, added to their own-set acceleration.  One wave each.
 template <bool kTension>
-__global__ __launch_bounds__(kBlock) void k_cross_force_integrate(
+__global__ __launch_bounds__(kBlock) void k_cross_force_walk(
     const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
-    const float2* __restrict__ aux_a, const float4* __restrict__ acc_a,
-    const uint32_t* __restrict__ n_a, const float4* __restrict__ tpos_b,
+    const float2* __restrict__ aux_a, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ list, uint32_t seg_cap, const float4* __restrict__ tpos_b,
     const float4* __restrict__ tvel_b, const float2* __restrict__ aux_b,
     const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
-    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, const uint32_t* __restrict__ occ,
-    GridView ga, const float4* __restrict__ bacc, float dt, float4* __restrict__ pos_a,
-    float4* __restrict__ vel_a, float4* __restrict__ acc_out) {
-    const uint32_t na = *n_a, nb = *n_b;
-    const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= na) return;
-    const float4 xi = tpos_a[s];
-    const float4 vi = tvel_a[s];
-    const float2 ai = aux_a[s];
-    float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f, Tz = 0.f;
-    if (nb > 0 && (!occ || high_nearby(occ, ga, xi.x, xi.y, xi.z))) {
-        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-            for (uint32_t j = b; j < e; ++j) {
-                const float4 xj = tpos_b[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < c.h02 && r2 > 0.0f) {
-                    const float4 vj = tvel_b[j];
-                    const float2 aj = aux_b[j];
-                    const float inv_r = __builtin_amdgcn_rsqf(r2);
-                    const float r = r2 * inv_r;
-                    const float hr0 = c.h0 - r;
-                    const float hr1 = fmaxf(c.h1 - r, 0.0f);
-                    // 1/2 (|grad W_h0| + |grad W_h1|) / r and 1/2 (lap W_h0 + lap W_h1)
-                    const float g = 0.5f * (c.c45_0 * hr0 * hr0 + c.c45_1 * hr1 * hr1) * inv_r;
-                    const float l = 0.5f * (c.c45_0 * hr0 + c.c45_1 * hr1);
-                    const float tp = (ai.x + aj.x) * g;
-                    Px += tp * dx;
-                    Py += tp * dy;
-                    Pz += tp * dz;
-                    const float tv = l * aj.y;
-                    Vx += tv * (vj.x - vi.x);
-                    Vy += tv * (vj.y - vi.y);
-                    Vz += tv * (vj.z - vi.z);
-                    if (kTension) {
-                        const float a0 = c.h02 - r2, a1 = c.h12 - r2;
-                        const float w = 0.5f * (c.c6_0 * a0 * a0 * a0 +
-                                                ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
-                        Tx += w * dx;
-                        Ty += w * dy;
-                        Tz += w * dz;
-                    }
+    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, float4* __restrict__ acc_a) {
+    __shared__ uint32_t rb_s[kBlock / 64][64];
+    __shared__ uint32_t incl_s[kBlock / 64][64];
+    __shared__ uint32_t seg_s[kBlock / 64][64];
+    const uint32_t nb = *n_b;
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t nw = walk_segments(counts, seg_s[wv]);
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t w = blockIdx.x * (kBlock / 64) + wv; w < nw; w += waves) {
+        const uint32_t s = walk_entry(list, seg_cap, seg_s[wv], w);
+        const float4 xi = tpos_a[s];
+        const float4 vi = tvel_a[s];
+        const float2 ai = aux_a[s];
+        float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f,
+              Tz = 0.f;
+        wave_walk(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv], [&](uint32_t j) {
+            const float4 xj = tpos_b[j];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < c.h02 && r2 > 0.0f) {
+                const float4 vj = tvel_b[j];
+                const float2 aj = aux_b[j];
+                const float inv_r = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * inv_r;
+                const float hr0 = c.h0 - r;
+                const float hr1 = fmaxf(c.h1 - r, 0.0f);
+                // 1/2 (|grad W_h0| + |grad W_h1|) / r and 1/2 (lap W_h0 + lap W_h1)
+                const float gg = 0.5f * (c.c45_0 * hr0 * hr0 + c.c45_1 * hr1 * hr1) * inv_r;
+                const float l = 0.5f * (c.c45_0 * hr0 + c.c45_1 * hr1);
+                const float tp = (ai.x + aj.x) * gg;
+                Px += tp * dx;
+                Py += tp * dy;
+                Pz += tp * dz;
+                const float tv = l * aj.y;
+                Vx += tv * (vj.x - vi.x);
+                Vy += tv * (vj.y - vi.y);
+                Vz += tv * (vj.z - vi.z);
+                if (kTension) {
+                    const float a0 = c.h02 - r2, a1 = c.h12 - r2;
+                    const float wt = 0.5f * (c.c6_0 * a0 * a0 * a0 +
+                                             ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
+                    Tx += wt * dx;
+                    Ty += wt * dy;
+                    Tz += wt * dz;
                 }
             }
         });
+        Px = wave_sum(Px);
+        Py = wave_sum(Py);
+        Pz = wave_sum(Pz);
+        Vx = wave_sum(Vx);
+        Vy = wave_sum(Vy);
+        Vz = wave_sum(Vz);
+        if (kTension) {
+            Tx = wave_sum(Tx);
+            Ty = wave_sum(Ty);
+            Tz = wave_sum(Tz);
+        }
+        if (lane == 0u) {
+            const float mv = c.m_other * c.visc * ai.y;
+            float ax = c.m_other * Px + mv * Vx, ay = c.m_other * Py + mv * Vy,
+                  az = c.m_other * Pz + mv * Vz;
+            if (kTension) {
+                const float tc = c.tension * c.m_other * ai.y;
+                ax -= tc * Tx;
+                ay -= tc * Ty;
+                az -= tc * Tz;
+            }
+            const float4 a = acc_a[s];
+            acc_a[s] = make_float4(a.x + ax, a.y + ay, a.z + az, a.w);
+        }
     }
-    const float4 a_own = acc_a[s];
-    const float mv = c.m_other * c.visc * ai.y;
-    float ax = a_own.x + c.m_other * Px + mv * Vx;
-    float ay = a_own.y + c.m_other * Py + mv * Vy;
-    float az = a_own.z + c.m_other * Pz + mv * Vz;
-    if (kTension) {
-        const float tc = c.tension * c.m_other * ai.y;
-        ax -= tc * Tx;
-        ay -= tc * Ty;
-        az -= tc * Tz;
-    }
+}
+
+// integrateD of one set (TWO_SCALE/Solver.cu:1163-1223 with lambda = 1):
+// a = own-set + complementary acceleration (+ boundary), v += dt a, the
+// velocity cap, x += dt v and the wall clamp.
+__global__ __launch_bounds__(kBlock) void k_cross_integrate(
+    const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
+    const float4* __restrict__ acc_a, const uint32_t* __restrict__ n_a, CrossParams c,
+    const float4* __restrict__ bacc, float dt, float4* __restrict__ pos_a,
+    float4* __restrict__ vel_a, float4* __restrict__ acc_out) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= *n_a) return;
+    const float4 xi = tpos_a[s];
+    const float4 vi = tvel_a[s];
+    const float4 a = acc_a[s];
+    float ax = a.x, ay = a.y, az = a.z;
     if (bacc) {  // boundary particles (Solver.cu:1046-1048)
         const float4 b = bacc[s];
         ax += b.x;
@@ -265,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
     const float4* __restrict__ tpos_b, const float2* __restrict__ tbl_b,
     const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
     const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
-    const uint32_t* __restrict__ occ, GridView g_occ, float* __restrict__ rho_a,
+    Nearby occ, GridView g_occ, float* __restrict__ rho_a,
     float2* __restrict__ aux_a) {
     __shared__ float4 tile[kBlendDensityTile];
     __shared__ uint32_t red[2 * (kBlock / 64)];
@@ -301,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
         });
     if (!active) return;
     float cross = 0.0f;
-    if (nb > 0 && (!occ || high_nearby(occ, g_occ, xi.x, xi.y, xi.z))) {
+    if (nb > 0 && other_nearby(occ, g_occ, xi.x, xi.y, xi.z)) {
         for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
                 const float4 xj = tpos_b[j];
@@ -336,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     const float2* __restrict__ aux_b, const float2* __restrict__ tbl_b,
     const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
     const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
-    const uint32_t* __restrict__ occ, GridView g_occ, const float4* __restrict__ bacc, float dt,
+    Nearby occ, GridView g_occ, const float4* __restrict__ bacc, float dt,
     float inc, float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
     float4* __restrict__ acc_out) {
     __shared__ float4 tx[kBlendForceTile];
@@ -414,7 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     // complementary sums (Solver.cu:394-499) weighted by lambda_j
     float Qx = 0.f, Qy = 0.f, Qz = 0.f, Ux = 0.f, Uy = 0.f, Uz = 0.f, Sx = 0.f, Sy = 0.f,
           Sz = 0.f;
-    if (nb > 0 && (!occ || high_nearby(occ, g_occ, xi.x, xi.y, xi.z))) {
+    if (nb > 0 && other_nearby(occ, g_occ, xi.x, xi.y, xi.z)) {
         for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
                 const float4 xj = tpos_b[j];
@@ -703,51 +976,35 @@ __global__ __launch_bounds__(kBlock) void k_block_counts(const uint32_t* __restr
 // newL, newH}.
 __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cl, uint32_t nbl,
                                                       uint32_t* __restrict__ ch, uint32_t nbh,
-                                                      uint32_t cap_l, uint32_t cap_h,
-                                                      uint32_t* __restrict__ tot,
-                                                      uint32_t* __restrict__ error_word) {
-    __shared__ uint32_t part[1024];
-    __shared__ uint32_t totals[4];
-    uint32_t* seqs[4] = {cl, cl + nbl, ch, ch + nbh};
-    const uint32_t lens[4] = {nbl, nbl, nbh, nbh};
-    const uint32_t t = threadIdx.x;
-    for (int q = 0; q < 4; ++q) {
-        uint32_t* d = seqs[q];
-        const uint32_t len = lens[q];
-        const uint32_t chunk = (len + 1023) / 1024;
-        const uint32_t b = min(t * chunk, len), e = min(b + chunk, len);
-        uint32_t sum = 0;
-        for (uint32_t i = b; i < e; ++i) sum += d[i];
-        part[t] = sum;
-        __syncthreads();
-        for (uint32_t off = 1; off < 1024; off <<= 1) {
-            const uint32_t v = (t >= off) ? part[t - off] : 0u;
-            __syncthreads();
-            part[t] += v;
-            __syncthreads();
-        }
-        uint32_t run = (t > 0) ? part[t - 1] : 0u;
-        for (uint32_t i = b; i < e; ++i) {
-            const uint32_t v = d[i];
-            d[i] = run;
-            run += v;
-        }
-        if (t == 1023) totals[q] = part[1023];
-        __syncthreads();
+                                                      uint32_t* __restrict__ tot) {
+    // Block q scans count sequence q (kept low, split low, kept high, merge
+    // leaders): a thread sums a contiguous chunk of at most a few counts, the
+    // chunk sums are scanned with wave prefix sums and the 16 wave totals,
+    // the chunk is rewritten as exclusive offsets, and tot[q] gets the total.
+    __shared__ uint32_t wtot[16];
+    const uint32_t q = blockIdx.x, t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    uint32_t* d = (q == 0) ? cl : (q == 1) ? cl + nbl : (q == 2) ? ch : ch + nbh;
+    const uint32_t len = (q < 2) ? nbl : nbh;
+    const uint32_t chunk = (len + 1023) / 1024;
+    const uint32_t b = min(t * chunk, len), e = min(b + chunk, len);
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += d[i];
+    uint32_t x = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(x), off));
+        if (lane >= static_cast<uint32_t>(off)) x += y;
     }
-    if (t == 0) {
-        const uint32_t keep_l = totals[0], split_l = totals[1], keep_h = totals[2],
-                       lead_h = totals[3];
-        const uint64_t new_l = static_cast<uint64_t>(keep_l) + lead_h;
-        const uint64_t new_h = static_cast<uint64_t>(keep_h) + 8ull * split_l;
-        if (new_l > cap_l || new_h > cap_h) atomicOr(error_word, 2u);
-        tot[0] = keep_l;
-        tot[1] = split_l;
-        tot[2] = keep_h;
-        tot[3] = lead_h;
-        tot[4] = static_cast<uint32_t>(new_l < cap_l ? new_l : cap_l);
-        tot[5] = static_cast<uint32_t>(new_h < cap_h ? new_h : cap_h);
+    if (lane == 63) wtot[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += wtot[w];
+    uint32_t run = before + x - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = d[i];
+        d[i] = run;
+        run += v;
     }
+    if (t == 1023) tot[q] = before + x;
 }
 
 // In-block exclusive rank of a predicate (block of 256, in index order).
@@ -896,8 +1153,16 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
 
 // tot[6], tot[7] keep the counts the step ran with (for inspection).
 __global__ void k_commit_counts(uint32_t* __restrict__ tot, uint32_t* __restrict__ n_l,
-                                uint32_t* __restrict__ n_h) {
+                                uint32_t* __restrict__ n_h, uint32_t cap_l, uint32_t cap_h,
+                                uint32_t* __restrict__ error_word) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        // New counts from the totals (kept + merged low; kept + 8 x split high),
+        // clamped to the capacities; a split or merge past them was dropped.
+        const uint64_t new_l = static_cast<uint64_t>(tot[0]) + tot[3];
+        const uint64_t new_h = static_cast<uint64_t>(tot[2]) + 8ull * tot[1];
+        if (new_l > cap_l || new_h > cap_h) atomicOr(error_word, 2u);
+        tot[4] = static_cast<uint32_t>(new_l < cap_l ? new_l : cap_l);
+        tot[5] = static_cast<uint32_t>(new_h < cap_h ? new_h : cap_h);
         tot[6] = *n_l;
         tot[7] = *n_h;
         *n_l = tot[4];
@@ -956,6 +1221,7 @@ struct TwoScaleSet {
     // compacted order, carried through split/merge.
     float* tcolor = nullptr;
     float* color = nullptr;
+    uint32_t* walk = nullptr;     // the step's particles with a complementary walk (sorted slots)
 };
 
 struct sph_two_scale {
@@ -982,6 +1248,7 @@ struct sph_two_scale {
     uint32_t graph_debug = 0;
     uint32_t debug = 0;
     uint32_t* surf = nullptr;    // per low id: at the free surface (surface detector)
+    uint32_t* walk_count = nullptr;  // per set: the counters of its walk list's segments
     // Renderer buffers per set (sph2_enable_render_buffers): AoS xyz, colour, state.
     float* render_xyz[2] = {nullptr, nullptr};
     float* render_color[2] = {nullptr, nullptr};
@@ -997,6 +1264,21 @@ void destroy_graph2(sph_two_scale* t) {
         (void)hipGraphExecDestroy(t->graph);
         t->graph = nullptr;
     }
+}
+
+uint32_t* walk_counts(const sph_two_scale* t, int target) {
+    return t->walk_count + static_cast<uint32_t>(target) * kWalkSegments * kWalkCounterStride;
+}
+
+// The skip test of set `target`'s walk over the other set: the low-grid cells
+// holding the other set's particles, and their dilated marks.
+Nearby nearby(const sph_two_scale* t, int target) {
+    const uint32_t C0 = t->set[0].num_cells;
+    Nearby nb;
+    nb.occ = (target == 0) ? t->occ : t->occ + C0;
+    nb.near = t->occ + 2 * C0;
+    nb.bit = (target == 0) ? 1u : 2u;
+    return nb;
 }
 
 CrossParams cross_params(const sph_two_scale* t, int target) {
@@ -1069,8 +1351,8 @@ int launch_split_merge(sph_two_scale* t) {
                        L.counts, nbl);
     hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, st, H.flag, H.buf.n_dev,
                        H.counts, nbh);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
-                       L.cap, H.cap, t->tot, t->error_word);
+    hipLaunchKernelGGL(k_scan_counts, dim3(4), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
+                       t->tot);
     const bool col = (t->debug & SPH_DEBUG_COLOR) != 0u;
     hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel,
                        L.buf.rho, bl_l, L.flag, L.buf.n_dev, L.counts, nbl, t->tot, t->region, sp,
@@ -1083,7 +1365,7 @@ int launch_split_merge(sph_two_scale* t) {
                        L.state_pos, L.state_vel, L.state_blend, L.color, H.state_pos, H.state_vel,
                        H.state_blend, H.color);
     hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(64), 0, st, t->tot, L.buf.n_dev,
-                       H.buf.n_dev);
+                       H.buf.n_dev, L.cap, H.cap, t->error_word);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -1097,7 +1379,7 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
-        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
+        const Nearby occ = nearby(t, r);
         hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
                            A.tblend, A.buf.cell_start, A.buf.cell_end, A.buf.n_dev, G[r], P[r],
                            B.buf.tpos, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
@@ -1109,7 +1391,7 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         CrossParams c = cross_params(t, r);
         c.vmax = velocity_limit(t, r, dt);
         float4* acc = accel_out ? A.buf.accel : nullptr;
-        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
+        const Nearby occ = nearby(t, r);
         const float inc = t->opt.blend_increment;
         if (P[r].tens != 0.0f || c.tension != 0.0f)
             hipLaunchKernelGGL(k_blend_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0,
@@ -1158,13 +1440,21 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
             hipLaunchKernelGGL(k_gather_blend, blocks_for(S.cap), dim3(kBlock), 0, st, S.buf.perm,
                                S.buf.n_dev, S.state_blend, S.tblend);
     }
-    // Occupancy of the low grid by the (sorted) high particles.
+    // Occupancy of the low grid by both sets (sorted positions), and its dilation.
     {
         const uint32_t C0 = t->set[0].num_cells;
-        hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 1024u)),
-                           dim3(kBlock), 0, st, t->occ, C0);
+        // occ_h and occ_l are adjacent: one clear covers both.
+        hipLaunchKernelGGL(k_clear_words,
+                           dim3(std::min<uint32_t>((2 * C0 + kBlock - 1) / kBlock, 1024u)),
+                           dim3(kBlock), 0, st, t->occ, 2 * C0);
         hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[1].cap), dim3(kBlock), 0, st,
                            t->set[1].buf.tpos, t->set[1].buf.n_dev, G[0], t->occ);
+        hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[0].cap), dim3(kBlock), 0, st,
+                           t->set[0].buf.tpos, t->set[0].buf.n_dev, G[0], t->occ + C0);
+        hipLaunchKernelGGL(k_dilate_occupancy,
+                           dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 2048u)),
+                           dim3(kBlock), 0, st, t->occ, t->occ + C0, G[0].dx, G[0].dy, G[0].dz,
+                           t->occ + 2 * C0);
     }
     // Colour values of both sets from the step's sorted state.
     if (t->debug & SPH_DEBUG_COLOR)
@@ -1183,44 +1473,61 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
             if (rc != SPH_OK) return rc;
         }
     if (blended(t)) return launch_blended_passes(t, P, G, X, dt, accel_out);
-    // Own-set densities, then the complementary sums (which also finalise A, B).
+    // Own-set densities, then the complementary sums (which also finalise A, B):
+    // the particles with a complementary walk are listed, and walked one per
+    // wave.
     for (int r = 1; r >= 0; --r) {
-        int rc = launch_density_own(P[r], t->set[r].buf, st);
+        int rc = launch_density_own_x(P[r], t->set[r].buf, st);
+        if (rc != SPH_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_clear_words, dim3(8), dim3(kBlock), 0, st, t->walk_count,
+                       2u * kWalkSegments * kWalkCounterStride);
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& A = t->set[r];
+        TwoScaleSet& B = t->set[1 - r];
+        hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+                           A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], cross_params(t, r),
+                           A.buf.rho, A.buf.aux, walk_counts(t, r), A.walk,
+                           walk_segment_cap(A.cap));
+    }
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& A = t->set[r];
+        TwoScaleSet& B = t->set[1 - r];
+        hipLaunchKernelGGL(k_cross_density_walk, dim3(kWalkBlocks), dim3(kBlock), 0, st,
+                           A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
+                           B.buf.tpos, B.buf.cell_start,
+                           B.buf.cell_end, B.buf.n_dev, X[1 - r], cross_params(t, r), A.buf.rho,
+                           A.buf.aux);
+    }
+    // Own-set forces (no integration), the complementary forces of the listed
+    // particles, then the integration.
+    for (int r = 1; r >= 0; --r) {
+        int rc = launch_gather_aux(P[r], t->set[r].buf, st);
+        if (rc == SPH_OK) rc = launch_force_own_x(P[r], G[r], t->set[r].buf, t->set[r].accel, st);
         if (rc != SPH_OK) return rc;
     }
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
-        // Low particles (r = 0) skip the walk when no high particle is near.
-        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
-        hipLaunchKernelGGL(k_cross_density, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
-                           A.buf.n_dev, B.buf.tpos, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                           X[1 - r], cross_params(t, r), occ, X[0], A.buf.rho, A.buf.aux);
-    }
-    // Own-set forces (no integration), then complementary forces + integration.
-    for (int r = 1; r >= 0; --r) {
-        int rc = launch_force_own(P[r], G[r], t->set[r].buf, t->set[r].accel, st);
-        if (rc != SPH_OK) return rc;
+        const CrossParams c = cross_params(t, r);
+        if (c.tension != 0.0f)
+            hipLaunchKernelGGL(k_cross_force_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, st,
+                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
+                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, A.accel);
+        else
+            hipLaunchKernelGGL(k_cross_force_walk<false>, dim3(kWalkBlocks), dim3(kBlock), 0, st,
+                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
+                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, A.accel);
     }
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
-        TwoScaleSet& B = t->set[1 - r];
         CrossParams c = cross_params(t, r);
         c.vmax = velocity_limit(t, r, dt);
-        float4* acc = accel_out ? A.buf.accel : nullptr;
-        const uint32_t* occ = (r == 0) ? t->occ : nullptr;
-        if (c.tension != 0.0f)
-            hipLaunchKernelGGL(k_cross_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0, st,
-                               A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev, B.buf.tpos,
-                               B.buf.tvel, B.buf.aux, B.buf.cell_start, B.buf.cell_end,
-                               B.buf.n_dev, X[1 - r], c, occ, X[0], wall_acc(t, r), dt, A.buf.pos,
-                               A.buf.vel, acc);
-        else
-            hipLaunchKernelGGL(k_cross_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
-                               st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.accel, A.buf.n_dev,
-                               B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
-                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, occ, X[0],
-                               wall_acc(t, r), dt, A.buf.pos, A.buf.vel, acc);
+        hipLaunchKernelGGL(k_cross_integrate, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+                           A.buf.tvel, A.accel, A.buf.n_dev, c, wall_acc(t, r), dt, A.buf.pos,
+                           A.buf.vel, accel_out ? A.buf.accel : nullptr);
     }
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
@@ -1352,14 +1659,21 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         for (int k = 0; k < 3; ++k) add(8 * N);    // state_blend tblend nblend
         add(4 * N);                                // tcolor
         add(4 * N);                                // color
+        add(4ull * kWalkSegments * walk_segment_cap(S.cap));  // walk list segments
+        add(16 * N);                               // ipos
+        add(16 * N);                               // ivel
+        add(8 * N);                                // iaux
+        add(4 * N);                                // i2r
+        add(16 * C);                               // xbins
     }
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
     add(4 * static_cast<size_t>(t->num_groups));    // grp_cnt
     add(64);                                        // tot[8] + error word
-    add(4 * static_cast<size_t>(t->set[0].num_cells));  // occupancy of the low grid
+    add(12 * static_cast<size_t>(t->set[0].num_cells));  // low-grid occupancy: occ_h, occ_l, near
     add(4 * static_cast<size_t>(t->num_groups));    // grp_live
     add(4 * static_cast<size_t>(t->num_groups));    // surf
+    add(4ull * 2 * kWalkSegments * kWalkCounterStride);  // walk list counters
 
     hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1406,6 +1720,13 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         S.nblend = static_cast<float2*>(take());
         S.tcolor = static_cast<float*>(take());
         S.color = static_cast<float*>(take());
+        S.walk = static_cast<uint32_t*>(take());
+        S.buf.ipos = static_cast<float4*>(take());
+        S.buf.ivel = static_cast<float4*>(take());
+        S.buf.iaux = static_cast<float2*>(take());
+        S.buf.i2r = static_cast<uint32_t*>(take());
+        S.buf.xbins = static_cast<uint4*>(take());
+        S.buf.keep_sorted = true;
     }
     t->grp_pos = static_cast<float4*>(take());
     t->grp_vel = static_cast<float4*>(take());
@@ -1415,6 +1736,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     t->occ = static_cast<uint32_t*>(take());
     t->grp_live = static_cast<uint32_t*>(take());
     t->surf = static_cast<uint32_t*>(take());
+    t->walk_count = static_cast<uint32_t*>(take());
 
     e = hipMemsetAsync(t->arena, 0, total, t->stream);
     if (e != hipSuccess) {
