@@ -170,14 +170,11 @@ int launch_pass(int pass, const StepParams& p, const StepBuffers& b, float dt, u
                 bool write_accel, hipStream_t stream);
 int launch_step(const StepParams& p, const StepBuffers& b, float dt, uint32_t key_bits,
                 bool write_accel, hipStream_t stream);
-// Two-scale building blocks: own-set density without the A/B finalisation and
-// own-set acceleration (gravity included) without integration.
+// Two-scale building blocks, in the internal x-order (b.ipos set; results at
+// the sorted slots): own-set density without the A/B finalisation, the
+// internal-order copy of the finalised A/B, and the own-set acceleration
+// (gravity included) without integration.
 struct GridView;
-int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t stream);
-int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& b,
-                     float4* accel_own, hipStream_t stream);
-// The same two passes in the internal x-order (b.ipos set; results at the
-// sorted slots), and the internal-order copy of the finalised A/B between them.
 int launch_density_own_x(const StepParams& p, const StepBuffers& b, hipStream_t stream);
 int launch_gather_aux(const StepParams& p, const StepBuffers& b, hipStream_t stream);
 int launch_force_own_x(const StepParams& p, const GridView& g, const StepBuffers& b,

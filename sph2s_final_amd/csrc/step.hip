@@ -945,35 +945,6 @@ int launch_step(const StepParams& p, const StepBuffers& b, float dt, uint32_t ke
     return SPH_OK;
 }
 
-int launch_density_own(const StepParams& p, const StepBuffers& b, hipStream_t stream) {
-    if (p.n == 0) return SPH_OK;
-    const GridView g = make_grid_view(p.ox, p.oy, p.oz, p.spacing, p.dx, p.dy, p.dz, p.h);
-    if (p.flags & SPH_FLAG_KAHAN)
-        launch_density<true, false, false>(0u, p.n, stream, b.tpos, b.cell_start, b.cell_end, p, g,
-                                           b.n_dev, b.rho, b.aux, b.tvel, kNoScenes, XOrder{});
-    else
-        launch_density<false, false, false>(0u, p.n, stream, b.tpos, b.cell_start, b.cell_end, p,
-                                            g, b.n_dev, b.rho, b.aux, b.tvel, kNoScenes, XOrder{});
-    SPH_HIP(hipGetLastError());
-    return SPH_OK;
-}
-
-int launch_force_own(const StepParams& p, const GridView& g, const StepBuffers& b,
-                     float4* accel_own, hipStream_t stream) {
-    if (p.n == 0) return SPH_OK;
-    float4* const no_state = nullptr;
-    if (p.tens != 0.0f)
-        launch_force<true, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
-                                         b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
-                                         accel_own, kNoScenes, XOrder{});
-    else
-        launch_force<false, false, false>(0u, p.n, stream, b.tpos, b.tvel, b.aux, b.cell_start,
-                                          b.cell_end, p, g, b.n_dev, 0.0f, no_state, no_state,
-                                          accel_own, kNoScenes, XOrder{});
-    SPH_HIP(hipGetLastError());
-    return SPH_OK;
-}
-
 // The two-scale own-set passes in the internal x-order: density without the
 // A/B finalisation (the complementary sums complete it), the internal-order
 // copy of the finalised A/B, then the force without integration.  Results
