@@ -283,9 +283,13 @@ int sph_count_pairs(sph_solver* s, uint64_t counts[2]);
  *              0 = 640/16/16-bit, 1 = 768/16/16, 2 = 768/24/16, 3 = 768/8/32,
  *              4 = 512/16/16, 5 = 1024/16/16, 6 = 512/8/32, 7 = 640/24/16,
  *              8 = 1024/8/32, 9 = 512/24/16
- *   bits 8-11  density grid, bits 12-15 force grid: 0 = one 256-thread block
- *              per 256 particles; 1..4 = persistent grid of 1, 2, 4, 8 blocks
- *              per CU looping over those blocks
+ *   bits 8-11  density grid, bits 12-15 force grid: 0 = one block per
+ *              block-size particles; 1..4 = persistent grid of 1, 2, 4, 8
+ *              blocks per CU looping over those blocks
+ *   bits 16-17 density block size, bits 18-19 force block size: 0 = 256,
+ *              1 = 128, 2 = 512 threads (one particle per lane); a non-zero
+ *              block size requires that pass's tile/list field to be 0
+ *   bits 20-31 must be zero; other words are refused with SPH_ERR_ARG
  * All variants give bit-identical results.  sph_autotune times the variants
  * on the solver's current state (reps launches each), keeps the fastest and
  * returns its tune word; ms[4] (may be NULL) receives the default and tuned

@@ -11,7 +11,8 @@ workload's shape (kind, particle count, cell count, scene count), so a later
 run of the same workload reuses it without timing.
 
 Tune word fields (include/sph2s.h): bits 0-3 density tile, 4-7 force
-tile/list, 8-11 density grid, 12-15 force grid.
+tile/list, 8-11 density grid, 12-15 force grid, 16-17 density block size,
+18-19 force block size.
 """
 from __future__ import annotations
 
@@ -22,17 +23,20 @@ DENSITY_TILES = (1280, 512, 1536, 2048, 768, 1024)
 # (LDS tile, list capacity, list entry bits)
 FORCE_VARIANTS = ((640, 16, 16), (768, 16, 16), (768, 24, 16), (768, 8, 32), (512, 16, 16),
                   (1024, 16, 16), (512, 8, 32), (640, 24, 16), (1024, 8, 32), (512, 24, 16))
-GRID_BLOCKS_PER_CU = (None, 1, 2, 4, 8)  # None: one block per 256 particles
+GRID_BLOCKS_PER_CU = (None, 1, 2, 4, 8)  # None: one block per block-size particles
+BLOCK_THREADS = (256, 128, 512)
 
 
 def describe(tune: int) -> dict:
     """The launch shape a tune word selects."""
     d, f = tune & 0xF, (tune >> 4) & 0xF
     dg, fg = (tune >> 8) & 0xF, (tune >> 12) & 0xF
+    db, fb = (tune >> 16) & 0x3, (tune >> 18) & 0x3
     return {"density_tile": DENSITY_TILES[d], "force_tile": FORCE_VARIANTS[f][0],
             "force_list_cap": FORCE_VARIANTS[f][1], "force_list_bits": FORCE_VARIANTS[f][2],
             "density_blocks_per_cu": GRID_BLOCKS_PER_CU[dg],
-            "force_blocks_per_cu": GRID_BLOCKS_PER_CU[fg], "block_threads": 256}
+            "force_blocks_per_cu": GRID_BLOCKS_PER_CU[fg],
+            "density_block_threads": BLOCK_THREADS[db], "force_block_threads": BLOCK_THREADS[fb]}
 
 
 def workload_key(kind: str, particles: int, cells: int, scenes: int = 1, device: str = "") -> str:

@@ -194,9 +194,13 @@ constexpr int kWavesPerBlock = kBlock / 64;
 //   bits 4-7   force tile / list capacity / list entry:  640/16/u16 (default),
 //              768/16/u16, 768/24/u16, 768/8/u32, 512/16/u16, 1024/16/u16,
 //              512/8/u32, 640/24/u16, 1024/8/u32, 512/24/u16
-//   bits 8-11  density grid:       one block per 256 particles, or a
+//   bits 8-11  density grid:       one block per block-size particles, or a
 //   bits 12-15 force grid:         persistent grid of 1, 2, 4, 8 blocks per CU
-//                                  looping over the 256-particle blocks
+//                                  looping over those blocks
+//   bits 16-17 density block size, bits 18-19 force block size: 256 (0), 128
+//              (1), 512 (2) threads, one particle per lane; a non-default block
+//              size runs the pass's default tile and queue
+// The block size also decides how many particles share one staged union.
 constexpr uint32_t kPersistPerCu[] = {0, 1, 2, 4, 8};
 const SceneParams* const kNoScenes = nullptr;
 
@@ -215,11 +219,11 @@ constexpr uint32_t kNumCus = 256;
 
 struct LaunchShape {
     dim3 grid;
-    uint32_t nvb;  // 256-particle blocks to cover
+    uint32_t nvb;  // blocks of `block` particles to cover
 };
 
-inline LaunchShape launch_shape(uint32_t cap_n, uint32_t persist_idx) {
-    const uint32_t nvb = cap_n ? (cap_n + kBlock - 1) / kBlock : 1u;
+inline LaunchShape launch_shape(uint32_t cap_n, uint32_t persist_idx, uint32_t block = kBlock) {
+    const uint32_t nvb = cap_n ? (cap_n + block - 1) / block : 1u;
     uint32_t g = nvb;
     if (persist_idx > 0 && persist_idx < 5) {
         g = std::min(nvb, kPersistPerCu[persist_idx] * kNumCus);
@@ -313,7 +317,7 @@ __device__ __forceinline__ void scene_constants(const SceneParams* __restrict__ 
     }
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX>
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB>
 __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float4* tile,
                                               uint32_t* red, const float4* __restrict__ tpos,
                                               const uint32_t* __restrict__ cs,
@@ -347,12 +351,12 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
             uint32_t b = kEmptyCell, e = 0u;
             if (active) probe_row<kX>(g, pr, cs, ce, xo.bins, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
-            block_run_union<kBlock>(b, e, red, ub, ue);
+            block_run_union<kB>(b, e, red, ub, ue);
             // Loaded range [ub, seg_e); empty when every lane's run is empty.
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
-            for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) tile[k] = tpos[ub + k];
+            for (uint32_t k = threadIdx.x; k < seg_n; k += kB) tile[k] = tpos[ub + k];
             __syncthreads();
             const bool staged = b < e && b >= ub && e <= seg_e;
             if (staged) {
@@ -397,8 +401,8 @@ constexpr int force_waves(bool batched, bool tension, bool integrate) {
     return (!batched && !tension && integrate) ? 5 : 1;
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(density_waves(kBatched))))
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB>
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(density_waves(kBatched))))
 void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
@@ -410,12 +414,12 @@ void k_density(const float4* __restrict__ tpos,
                                                     const SceneParams* __restrict__ scenes,
                                                     XOrder xo, uint32_t nvb) {
     __shared__ float4 tile[kTile];
-    __shared__ uint32_t red[2 * kWavesPerBlock];
+    __shared__ uint32_t red[2 * (kB / 64)];
     const uint32_t n = *n_dev;
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
+        const uint32_t block0 = xcd_local_block(vb, nvb) * kB;
         if (block0 < n)  // block-uniform
-            density_block<kKahan, kFinalize, kBatched, kTile, kX>(
+            density_block<kKahan, kFinalize, kBatched, kTile, kX, kB>(
                 block0, n, tile, red, tpos, cs, ce, p_in, g_in, rho, aux, tvel, scenes, xo);
     }
 }
@@ -522,9 +526,9 @@ __device__ __forceinline__ void force_finish(const PairSums<kTension>& S, const 
 // With kIntegrate = false (two-scale own-set pass) the kernel only writes the
 // acceleration, gravity included, to accel.
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
-          bool kX>
+          bool kX, int kB>
 __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4* tp, float4* tv,
-                                            TList (*list)[kBlock], uint32_t* red,
+                                            TList (*list)[kB], uint32_t* red,
                                             const float4* __restrict__ tpos,
                                             const float4* __restrict__ tvel,
                                             const float2* __restrict__ aux,
@@ -591,11 +595,11 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
             uint32_t b = kEmptyCell, e = 0u;
             if (active) probe_row<kX>(g, pr, cs, ce, xo.bins, n, pr.cy + dy, pr.cz + dz, b, e);
             uint32_t ub, ue;
-            block_run_union<kBlock>(b, e, red, ub, ue);
+            block_run_union<kB>(b, e, red, ub, ue);
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
-            for (uint32_t k = threadIdx.x; k < seg_n; k += kBlock) {
+            for (uint32_t k = threadIdx.x; k < seg_n; k += kB) {
                 const float4 x = tpos[ub + k];
                 const float4 v = tvel[ub + k];
                 const float2 a = aux[ub + k];
@@ -642,8 +646,8 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
 }
 
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
-          bool kX>
-__global__ __launch_bounds__(kBlock)
+          bool kX, int kB>
+__global__ __launch_bounds__(kB)
 __attribute__((amdgpu_waves_per_eu(force_waves(kBatched, kTension, kIntegrate))))
 void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
@@ -661,13 +665,13 @@ void k_force(const float4* __restrict__ tpos,
     static_assert(kTile + kScanGroup <= 65536 || sizeof(TList) == 4, "list entries index the tile");
     __shared__ float4 tp[kTile + kScanGroup];  // padded for whole-group loads
     __shared__ float4 tv[kTile];
-    __shared__ TList list[kCap][kBlock];
-    __shared__ uint32_t red[2 * kWavesPerBlock];
+    __shared__ TList list[kCap][kB];
+    __shared__ uint32_t red[2 * (kB / 64)];
     const uint32_t n = *n_dev;
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        const uint32_t block0 = xcd_local_block(vb, nvb) * kBlock;
+        const uint32_t block0 = xcd_local_block(vb, nvb) * kB;
         if (block0 < n)  // block-uniform
-            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList, kX>(
+            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList, kX, kB>(
                 block0, n, tp, tv, list, red, tpos, tvel, aux, cs, ce, p_in, g_in, dt, pos, vel,
                 accel, scenes, xo);
     }
@@ -760,24 +764,33 @@ __global__ __launch_bounds__(kBlock) void k_xsph_integrate(
 // other instantiations use the defaults.  The single-scale and batched passes
 // (density with aux, force with integration) run in the internal x-order; the
 // two-scale own-set passes (neither) in the sorted order.
+// Block size of a tunable pass from its tune-word bits: 0 = 256, 1 = 128,
+// 2 = 512 threads; 3 is invalid, refused by tune_word_valid.
+
 template <bool kKahan, bool kFinalize, bool kBatched, bool kX = kFinalize, class... A>
 void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
-    const LaunchShape sh = launch_shape(cap_n, kFinalize ? ((tune >> 8) & 0xFu) : 0u);
-#define SPH_DENSITY_VARIANT(TILE)                                                               \
-    hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kX>), sh.grid,            \
-                       dim3(kBlock), 0, st, args..., sh.nvb)
+    const uint32_t bv = kFinalize ? ((tune >> 16) & 0x3u) : 0u;
+    const uint32_t persist = kFinalize ? ((tune >> 8) & 0xFu) : 0u;
+#define SPH_DENSITY_VARIANT(TILE, B)                                                            \
+    do {                                                                                        \
+        const LaunchShape sh = launch_shape(cap_n, persist, B);                                 \
+        hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kX, B>), sh.grid,     \
+                           dim3(B), 0, st, args..., sh.nvb);                                    \
+    } while (0)
     if constexpr (kFinalize) {
+        if (bv == 1) { SPH_DENSITY_VARIANT(kDensityTile, 128); return; }
+        if (bv == 2) { SPH_DENSITY_VARIANT(kDensityTile, 512); return; }
         switch (v) {
-            case 1: SPH_DENSITY_VARIANT(512); return;
-            case 2: SPH_DENSITY_VARIANT(1536); return;
-            case 3: SPH_DENSITY_VARIANT(2048); return;
-            case 4: SPH_DENSITY_VARIANT(768); return;
-            case 5: SPH_DENSITY_VARIANT(1024); return;
+            case 1: SPH_DENSITY_VARIANT(512, kBlock); return;
+            case 2: SPH_DENSITY_VARIANT(1536, kBlock); return;
+            case 3: SPH_DENSITY_VARIANT(2048, kBlock); return;
+            case 4: SPH_DENSITY_VARIANT(768, kBlock); return;
+            case 5: SPH_DENSITY_VARIANT(1024, kBlock); return;
             default: break;
         }
     }
-    SPH_DENSITY_VARIANT(kDensityTile);
+    SPH_DENSITY_VARIANT(kDensityTile, kBlock);
 #undef SPH_DENSITY_VARIANT
 }
 
@@ -785,25 +798,31 @@ template <bool kTension, bool kIntegrate, bool kBatched, bool kX = kIntegrate, c
 void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     constexpr bool kTunable = kIntegrate && !kTension;
     const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
-    const LaunchShape sh = launch_shape(cap_n, kTunable ? ((tune >> 12) & 0xFu) : 0u);
-#define SPH_FORCE_VARIANT(TILE, CAP, TLIST)                                                     \
-    hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kX>),         \
-                       sh.grid, dim3(kBlock), 0, st, args..., sh.nvb)
+    const uint32_t bv = kTunable ? ((tune >> 18) & 0x3u) : 0u;
+    const uint32_t persist = kTunable ? ((tune >> 12) & 0xFu) : 0u;
+#define SPH_FORCE_VARIANT(TILE, CAP, TLIST, B)                                                  \
+    do {                                                                                        \
+        const LaunchShape sh = launch_shape(cap_n, persist, B);                                 \
+        hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kX, B>),  \
+                           sh.grid, dim3(B), 0, st, args..., sh.nvb);                           \
+    } while (0)
     if constexpr (kTunable) {
+        if (bv == 1) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, 128); return; }
+        if (bv == 2) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, 512); return; }
         switch (v) {
-            case 1: SPH_FORCE_VARIANT(768, 16, uint16_t); return;
-            case 2: SPH_FORCE_VARIANT(768, 24, uint16_t); return;
-            case 3: SPH_FORCE_VARIANT(768, 8, uint32_t); return;
-            case 4: SPH_FORCE_VARIANT(512, 16, uint16_t); return;
-            case 5: SPH_FORCE_VARIANT(1024, 16, uint16_t); return;
-            case 6: SPH_FORCE_VARIANT(512, 8, uint32_t); return;
-            case 7: SPH_FORCE_VARIANT(640, 24, uint16_t); return;
-            case 8: SPH_FORCE_VARIANT(1024, 8, uint32_t); return;
-            case 9: SPH_FORCE_VARIANT(512, 24, uint16_t); return;
+            case 1: SPH_FORCE_VARIANT(768, 16, uint16_t, kBlock); return;
+            case 2: SPH_FORCE_VARIANT(768, 24, uint16_t, kBlock); return;
+            case 3: SPH_FORCE_VARIANT(768, 8, uint32_t, kBlock); return;
+            case 4: SPH_FORCE_VARIANT(512, 16, uint16_t, kBlock); return;
+            case 5: SPH_FORCE_VARIANT(1024, 16, uint16_t, kBlock); return;
+            case 6: SPH_FORCE_VARIANT(512, 8, uint32_t, kBlock); return;
+            case 7: SPH_FORCE_VARIANT(640, 24, uint16_t, kBlock); return;
+            case 8: SPH_FORCE_VARIANT(1024, 8, uint32_t, kBlock); return;
+            case 9: SPH_FORCE_VARIANT(512, 24, uint16_t, kBlock); return;
             default: break;
         }
     }
-    SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t);
+    SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, kBlock);
 #undef SPH_FORCE_VARIANT
 }
 
@@ -1133,8 +1152,11 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
 namespace sph {
 
 bool tune_word_valid(uint32_t tune) {
-    return (tune >> 16) == 0u && (tune & 0xFu) <= 5u && ((tune >> 4) & 0xFu) <= 9u &&
-           ((tune >> 8) & 0xFu) <= 4u && ((tune >> 12) & 0xFu) <= 4u;
+    const uint32_t dtile = tune & 0xFu, fvar = (tune >> 4) & 0xFu;
+    const uint32_t dblock = (tune >> 16) & 0x3u, fblock = (tune >> 18) & 0x3u;
+    return (tune >> 20) == 0u && dtile <= 5u && fvar <= 9u && ((tune >> 8) & 0xFu) <= 4u &&
+           ((tune >> 12) & 0xFu) <= 4u && dblock <= 2u && fblock <= 2u &&
+           (dblock == 0u || dtile == 0u) && (fblock == 0u || fvar == 0u);
 }
 
 int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams* scenes,
@@ -1184,31 +1206,41 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
         ms /= static_cast<float>(reps);
         return SPH_OK;
     };
-    // Density: tile first, then the grid shape for the best tile.
+    // Density: tile first (256-thread blocks), then the 128- and 512-thread
+    // blocks with the default tile, then the grid shape for the best of them.
     float d_def = 0.f, d_best = 0.f, ms = 0.f;
-    uint32_t d_tile = 0, d_grid = 0;
+    uint32_t d_shape = 0, d_grid = 0;  // d_shape: tile bits, or block bits << 16
     for (uint32_t v = 0; v <= 5 && rc == SPH_OK; ++v) {
         rc = time_pass(kPassDensity, v, ms);
         if (v == 0) d_def = d_best = ms;
-        else if (ms < d_best) { d_best = ms; d_tile = v; }
+        else if (ms < d_best) { d_best = ms; d_shape = v; }
+    }
+    for (uint32_t bv = 1; bv <= 2 && rc == SPH_OK; ++bv) {
+        rc = time_pass(kPassDensity, bv << 16, ms);
+        if (ms < d_best) { d_best = ms; d_shape = bv << 16; }
     }
     for (uint32_t gsh = 1; gsh <= 4 && rc == SPH_OK; ++gsh) {
-        rc = time_pass(kPassDensity, d_tile | (gsh << 8), ms);
+        rc = time_pass(kPassDensity, d_shape | (gsh << 8), ms);
         if (ms < d_best) { d_best = ms; d_grid = gsh; }
     }
     if (rc != SPH_OK) return fail(rc);
-    const uint32_t dword = d_tile | (d_grid << 8);
+    const uint32_t dword = d_shape | (d_grid << 8);
     rc = run(kPassDensity, dword);  // aux for the force pass
-    // Force: variant first, then the grid shape (tension runs the default).
+    // Force: variant first, then the 128- and 512-thread blocks with the
+    // default variant, then the grid shape (tension runs the default).
     float f_def = 0.f, f_best = 0.f;
-    uint32_t f_var = 0, f_grid = 0;
+    uint32_t f_shape = 0, f_grid = 0;  // variant bits << 4, or block bits << 18
     for (uint32_t v = 0; v <= (tension ? 0u : 9u) && rc == SPH_OK; ++v) {
         rc = time_pass(kPassForce, dword | (v << 4), ms);
         if (v == 0) f_def = f_best = ms;
-        else if (ms < f_best) { f_best = ms; f_var = v; }
+        else if (ms < f_best) { f_best = ms; f_shape = v << 4; }
+    }
+    for (uint32_t bv = 1; bv <= (tension ? 0u : 2u) && rc == SPH_OK; ++bv) {
+        rc = time_pass(kPassForce, dword | (bv << 18), ms);
+        if (ms < f_best) { f_best = ms; f_shape = bv << 18; }
     }
     for (uint32_t gsh = 1; gsh <= (tension ? 0u : 4u) && rc == SPH_OK; ++gsh) {
-        rc = time_pass(kPassForce, dword | (f_var << 4) | (gsh << 12), ms);
+        rc = time_pass(kPassForce, dword | f_shape | (gsh << 12), ms);
         if (ms < f_best) { f_best = ms; f_grid = gsh; }
     }
     if (rc != SPH_OK) return fail(rc);
@@ -1217,7 +1249,7 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
         e = hipMemcpyAsync(b.vel, static_cast<char*>(saved) + bytes, bytes,
                            hipMemcpyDeviceToDevice, stream);
     if (e != hipSuccess) return fail(hip_error(e, "autotune restore", __FILE__, __LINE__));
-    *tune_out = dword | (f_var << 4) | (f_grid << 12);
+    *tune_out = dword | f_shape | (f_grid << 12);
     if (ms_out) {
         ms_out[0] = d_def;
         ms_out[1] = d_best;

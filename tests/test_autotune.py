@@ -15,7 +15,8 @@ from sph2s_final_amd import (FIELD_DENSITY, FIELD_POSITION, FIELD_VELOCITY, Batc
 
 WORDS = [0x0000, 0x0001, 0x0002, 0x0003, 0x0004, 0x0005, 0x0010, 0x0020, 0x0030, 0x0040,
          0x0050, 0x0060, 0x0070, 0x0080, 0x0090, 0x1100, 0x2200, 0x3300, 0x4400, 0x4452,
-         0x1231, 0x2393, 0x4485]
+         0x1231, 0x2393, 0x4485, 0x10000, 0x20000, 0x40000, 0x80000, 0x50000, 0xA0000,
+         0x91100, 0x64400]
 
 
 # ---------------------------------------------------------------- CPU
@@ -23,11 +24,15 @@ def test_describe_tune_words():
     d = autotune.describe(0)
     assert d == {"density_tile": 1280, "force_tile": 640, "force_list_cap": 16,
                  "force_list_bits": 16, "density_blocks_per_cu": None,
-                 "force_blocks_per_cu": None, "block_threads": 256}
+                 "force_blocks_per_cu": None, "density_block_threads": 256,
+                 "force_block_threads": 256}
     d = autotune.describe(0x4432)
     assert d["density_tile"] == 1536
     assert (d["force_tile"], d["force_list_cap"], d["force_list_bits"]) == (768, 8, 32)
     assert d["density_blocks_per_cu"] == 8 and d["force_blocks_per_cu"] == 8
+    d = autotune.describe(0x91100)
+    assert (d["density_block_threads"], d["force_block_threads"]) == (128, 512)
+    assert d["density_blocks_per_cu"] == 1 and d["force_blocks_per_cu"] == 1
 
 
 class _FakeSolver:
@@ -82,7 +87,7 @@ def test_every_variant_is_bit_identical_batched():
     bs = scenes.batched_dam_breaks(5, seed=13, n_min=2000, n_max=40000)
     print(f"case variants batched counts={bs.counts.tolist()}", flush=True)
     outs = []
-    for w in (0, 0x0032, 0x3341, 0x4450, 0x0073, 0x2295):
+    for w in (0, 0x0032, 0x3341, 0x4450, 0x0073, 0x2295, 0x50000, 0xA3300):
         with BatchSolver([s.config for s in bs.scenes], [s.positions for s in bs.scenes]) as b:
             b.set_tune(w)
             b.Advance(bs.dt, 6)
@@ -122,7 +127,8 @@ def test_batch_autotune_and_invalid_words():
     with BatchSolver([s.config for s in bs.scenes], [s.positions for s in bs.scenes]) as b:
         word, ms = b.autotune(bs.dt, reps=2)
         assert b.get_tune() == word and len(ms) == 4
-        for bad in (0x6, 0xA0, 0x500, 0x5000, 0x10000):
+        for bad in (0x6, 0xA0, 0x500, 0x5000, 0x30000, 0xC0000, 0x10001, 0x40010,
+                    0x100000):
             with pytest.raises(SPHError):
                 b.set_tune(bad)
         assert b.get_tune() == word
