@@ -1241,6 +1241,11 @@ struct sph_two_scale {
     float4* bacc[2] = {nullptr, nullptr};  // per set: boundary acceleration per sorted slot
     sph2_options opt{SPH2_COUPLING_FIXED, SPH2_SPLIT_LATTICE, 1u, 0.0125f, 0.0f};
     hipStream_t stream = nullptr;
+    // The low set's per-set passes run on a second stream forked from and
+    // joined back to `stream` (branches of the step's graph when captured);
+    // fork/join k records ev[2k] on `stream` / ev[2k + 1] on `stream2`.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev[6] = {};
     void* arena = nullptr;
     hipGraphExec_t graph = nullptr;
     float graph_dt = 0.0f;
@@ -1258,6 +1263,27 @@ struct sph_two_scale {
 };
 
 namespace {
+
+void destroy_streams2(sph_two_scale* t) {
+    for (hipEvent_t& ev : t->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (t->stream2) (void)hipStreamDestroy(t->stream2);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+}
+
+// Fork k: `stream2` waits for the work issued on `stream` so far.  Join k:
+// `stream` waits for the work issued on `stream2` so far.
+int fork2(sph_two_scale* t, int k) {
+    SPH_HIP(hipEventRecord(t->ev[2 * k], t->stream));
+    SPH_HIP(hipStreamWaitEvent(t->stream2, t->ev[2 * k], 0));
+    return SPH_OK;
+}
+
+int join2(sph_two_scale* t, int k) {
+    SPH_HIP(hipEventRecord(t->ev[2 * k + 1], t->stream2));
+    SPH_HIP(hipStreamWaitEvent(t->stream, t->ev[2 * k + 1], 0));
+    return SPH_OK;
+}
 
 void destroy_graph2(sph_two_scale* t) {
     if (t->graph) {
@@ -1425,110 +1451,110 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         X[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
                               t->set[0].cfg.h);
     }
-    // Neighbourhoods: the reference order is high then low (Solver.cu:1576-1577).
-    // The state the step reads is the compacted state; the reorder gathers from it.
+    // The two sets' own passes are independent until the complementary sums:
+    // the high set's run on t->stream, the low set's on t->stream2, joined
+    // before every pass that reads the other set's results of this step.
+    const hipStream_t sst[2] = {t->stream2, st};
+    const uint32_t C0 = t->set[0].num_cells;
+    int rc = fork2(t, 0);
+    if (rc != SPH_OK) return rc;
+    // Neighbourhoods (the reference runs high then low, Solver.cu:1576-1577;
+    // the sets do not interact here).  The state the step reads is the
+    // compacted state; the reorder gathers from it.  Then each set marks the
+    // low-grid cells it occupies (occ_h for the high set, occ_l for the low),
+    // its colour values and boundary accelerations, and, with the fixed
+    // coupling, its own-set densities.
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& S = t->set[r];
+        const hipStream_t s = sst[r];
         StepBuffers b = S.buf;
         b.pos = S.state_pos;
         b.vel = S.state_vel;
-        int rc = launch_pass(kPassHash, P[r], b, dt, S.key_bits, false, st);
-        if (rc == SPH_OK) rc = launch_pass(kPassSort, P[r], b, dt, S.key_bits, false, st);
-        if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, dt, S.key_bits, false, st);
+        rc = launch_pass(kPassHash, P[r], b, dt, S.key_bits, false, s);
+        if (rc == SPH_OK) rc = launch_pass(kPassSort, P[r], b, dt, S.key_bits, false, s);
+        if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, dt, S.key_bits, false, s);
         if (rc != SPH_OK) return rc;
         if (blended(t))
-            hipLaunchKernelGGL(k_gather_blend, blocks_for(S.cap), dim3(kBlock), 0, st, S.buf.perm,
+            hipLaunchKernelGGL(k_gather_blend, blocks_for(S.cap), dim3(kBlock), 0, s, S.buf.perm,
                                S.buf.n_dev, S.state_blend, S.tblend);
-    }
-    // Occupancy of the low grid by both sets (sorted positions), and its dilation.
-    {
-        const uint32_t C0 = t->set[0].num_cells;
-        // occ_h and occ_l are adjacent: one clear covers both.
-        hipLaunchKernelGGL(k_clear_words,
-                           dim3(std::min<uint32_t>((2 * C0 + kBlock - 1) / kBlock, 1024u)),
-                           dim3(kBlock), 0, st, t->occ, 2 * C0);
-        hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[1].cap), dim3(kBlock), 0, st,
-                           t->set[1].buf.tpos, t->set[1].buf.n_dev, G[0], t->occ);
-        hipLaunchKernelGGL(k_mark_occupancy, blocks_for(t->set[0].cap), dim3(kBlock), 0, st,
-                           t->set[0].buf.tpos, t->set[0].buf.n_dev, G[0], t->occ + C0);
-        hipLaunchKernelGGL(k_dilate_occupancy,
-                           dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 2048u)),
-                           dim3(kBlock), 0, st, t->occ, t->occ + C0, G[0].dx, G[0].dy, G[0].dz,
-                           t->occ + 2 * C0);
-    }
-    // Colour values of both sets from the step's sorted state.
-    if (t->debug & SPH_DEBUG_COLOR)
-        for (int r = 1; r >= 0; --r) {
-            TwoScaleSet& S = t->set[r];
-            const int rc = launch_energy_color(S.buf.tpos, S.buf.tvel, S.buf.cell_start,
-                                               S.buf.cell_end, S.buf.n_dev, S.cap, G[r],
-                                               S.cfg.mass, S.tcolor, st);
+        uint32_t* occ = (r == 1) ? t->occ : t->occ + C0;
+        hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 1024u)),
+                           dim3(kBlock), 0, s, occ, C0);
+        hipLaunchKernelGGL(k_mark_occupancy, blocks_for(S.cap), dim3(kBlock), 0, s, S.buf.tpos,
+                           S.buf.n_dev, G[0], occ);
+        if (t->debug & SPH_DEBUG_COLOR) {
+            rc = launch_energy_color(S.buf.tpos, S.buf.tvel, S.buf.cell_start, S.buf.cell_end,
+                                     S.buf.n_dev, S.cap, G[r], S.cfg.mass, S.tcolor, s);
             if (rc != SPH_OK) return rc;
         }
-    // Boundary accelerations of both sets (walls on the low grid).
-    if (t->walls.n)
-        for (int r = 1; r >= 0; --r) {
-            const int rc = launch_wall_accel(t->walls, t->set[r].buf.tpos, t->set[r].buf.n_dev,
-                                             t->set[r].cap, t->bacc[r], st);
+        if (t->walls.n) {  // walls on the low grid
+            rc = launch_wall_accel(t->walls, S.buf.tpos, S.buf.n_dev, S.cap, t->bacc[r], s);
             if (rc != SPH_OK) return rc;
         }
+        if (!blended(t)) {
+            rc = launch_density_own_x(P[r], S.buf, s);
+            if (rc != SPH_OK) return rc;
+        }
+    }
+    rc = join2(t, 0);
+    if (rc != SPH_OK) return rc;
+    // Dilation of both sets' occupancy.
+    hipLaunchKernelGGL(k_dilate_occupancy, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 2048u)),
+                       dim3(kBlock), 0, st, t->occ, t->occ + C0, G[0].dx, G[0].dy, G[0].dz,
+                       t->occ + 2 * C0);
     if (blended(t)) return launch_blended_passes(t, P, G, X, dt, accel_out);
-    // Own-set densities, then the complementary sums (which also finalise A, B):
-    // the particles with a complementary walk are listed, and walked one per
-    // wave.
-    for (int r = 1; r >= 0; --r) {
-        int rc = launch_density_own_x(P[r], t->set[r].buf, st);
-        if (rc != SPH_OK) return rc;
-    }
+    // The complementary densities (which also finalise A, B): the particles
+    // with a complementary walk are listed and walked one per wave; then each
+    // set's own-set forces (no integration).
     hipLaunchKernelGGL(k_clear_words, dim3(8), dim3(kBlock), 0, st, t->walk_count,
                        2u * kWalkSegments * kWalkCounterStride);
+    rc = fork2(t, 1);
+    if (rc != SPH_OK) return rc;
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
-        hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+        const hipStream_t s = sst[r];
+        hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], cross_params(t, r),
                            A.buf.rho, A.buf.aux, walk_counts(t, r), A.walk,
                            walk_segment_cap(A.cap));
-    }
-    for (int r = 1; r >= 0; --r) {
-        TwoScaleSet& A = t->set[r];
-        TwoScaleSet& B = t->set[1 - r];
-        hipLaunchKernelGGL(k_cross_density_walk, dim3(kWalkBlocks), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL(k_cross_density_walk, dim3(kWalkBlocks), dim3(kBlock), 0, s,
                            A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
-                           B.buf.tpos, B.buf.cell_start,
-                           B.buf.cell_end, B.buf.n_dev, X[1 - r], cross_params(t, r), A.buf.rho,
-                           A.buf.aux);
-    }
-    // Own-set forces (no integration), the complementary forces of the listed
-    // particles, then the integration.
-    for (int r = 1; r >= 0; --r) {
-        int rc = launch_gather_aux(P[r], t->set[r].buf, st);
-        if (rc == SPH_OK) rc = launch_force_own_x(P[r], G[r], t->set[r].buf, t->set[r].accel, st);
+                           B.buf.tpos, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
+                           cross_params(t, r), A.buf.rho, A.buf.aux);
+        rc = launch_gather_aux(P[r], A.buf, s);
+        if (rc == SPH_OK) rc = launch_force_own_x(P[r], G[r], A.buf, A.accel, s);
         if (rc != SPH_OK) return rc;
     }
+    rc = join2(t, 1);
+    if (rc == SPH_OK) rc = fork2(t, 2);
+    if (rc != SPH_OK) return rc;
+    // The complementary forces of the listed particles (they read the other
+    // set's final A, B), then the integration.
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
-        const CrossParams c = cross_params(t, r);
-        if (c.tension != 0.0f)
-            hipLaunchKernelGGL(k_cross_force_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, st,
-                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
-                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
-                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, A.accel);
-        else
-            hipLaunchKernelGGL(k_cross_force_walk<false>, dim3(kWalkBlocks), dim3(kBlock), 0, st,
-                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
-                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux, B.buf.cell_start,
-                               B.buf.cell_end, B.buf.n_dev, X[1 - r], c, A.accel);
-    }
-    for (int r = 1; r >= 0; --r) {
-        TwoScaleSet& A = t->set[r];
+        const hipStream_t s = sst[r];
         CrossParams c = cross_params(t, r);
+        if (c.tension != 0.0f)
+            hipLaunchKernelGGL(k_cross_force_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
+                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
+                               B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r], c,
+                               A.accel);
+        else
+            hipLaunchKernelGGL(k_cross_force_walk<false>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
+                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
+                               B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r], c,
+                               A.accel);
         c.vmax = velocity_limit(t, r, dt);
-        hipLaunchKernelGGL(k_cross_integrate, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+        hipLaunchKernelGGL(k_cross_integrate, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.tvel, A.accel, A.buf.n_dev, c, wall_acc(t, r), dt, A.buf.pos,
                            A.buf.vel, accel_out ? A.buf.accel : nullptr);
     }
+    rc = join2(t, 2);
+    if (rc != SPH_OK) return rc;
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
 }
@@ -1676,13 +1702,17 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     add(4ull * 2 * kWalkSegments * kWalkCounterStride);  // walk list counters
 
     hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->stream2, hipStreamNonBlocking);
+    for (hipEvent_t& ev : t->ev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     if (e != hipSuccess) {
+        destroy_streams2(t);
         delete t;
         return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
     }
     e = hipMalloc(&t->arena, total);
     if (e != hipSuccess) {
-        (void)hipStreamDestroy(t->stream);
+        destroy_streams2(t);
         delete t;
         return hip_error(e, "hipMalloc", __FILE__, __LINE__);
     }
@@ -2160,7 +2190,7 @@ int sph2_destroy(sph_two_scale* t) {
         if (t->bacc[r]) (void)hipFree(t->bacc[r]);
     if (t->render_arena) (void)hipFree(t->render_arena);
     if (t->arena) (void)hipFree(t->arena);
-    if (t->stream) (void)hipStreamDestroy(t->stream);
+    destroy_streams2(t);
     delete t;
     return SPH_OK;
 }
