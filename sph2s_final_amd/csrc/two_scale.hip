@@ -1241,9 +1241,9 @@ struct sph_two_scale {
     float4* bacc[2] = {nullptr, nullptr};  // per set: boundary acceleration per sorted slot
     sph2_options opt{SPH2_COUPLING_FIXED, SPH2_SPLIT_LATTICE, 1u, 0.0125f, 0.0f};
     hipStream_t stream = nullptr;
-    // The low set's per-set passes run on a second stream forked from and
-    // joined back to `stream` (branches of the step's graph when captured);
-    // fork/join k records ev[2k] on `stream` / ev[2k + 1] on `stream2`.
+    // The low set's passes run on a second stream forked from and joined back
+    // to `stream` (branches of the step's graph when captured), with events
+    // ev[] at the points where one set's passes read the other's results.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev[6] = {};
     void* arena = nullptr;
@@ -1271,17 +1271,10 @@ void destroy_streams2(sph_two_scale* t) {
     if (t->stream) (void)hipStreamDestroy(t->stream);
 }
 
-// Fork k: `stream2` waits for the work issued on `stream` so far.  Join k:
-// `stream` waits for the work issued on `stream2` so far.
-int fork2(sph_two_scale* t, int k) {
-    SPH_HIP(hipEventRecord(t->ev[2 * k], t->stream));
-    SPH_HIP(hipStreamWaitEvent(t->stream2, t->ev[2 * k], 0));
-    return SPH_OK;
-}
-
-int join2(sph_two_scale* t, int k) {
-    SPH_HIP(hipEventRecord(t->ev[2 * k + 1], t->stream2));
-    SPH_HIP(hipStreamWaitEvent(t->stream, t->ev[2 * k + 1], 0));
+// `to` waits for the work issued on `from` so far (event k).
+int wait_for(sph_two_scale* t, int k, hipStream_t from, hipStream_t to) {
+    SPH_HIP(hipEventRecord(t->ev[k], from));
+    SPH_HIP(hipStreamWaitEvent(to, t->ev[k], 0));
     return SPH_OK;
 }
 
@@ -1451,19 +1444,22 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         X[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
                               t->set[0].cfg.h);
     }
-    // The two sets' own passes are independent until the complementary sums:
-    // the high set's run on t->stream, the low set's on t->stream2, joined
-    // before every pass that reads the other set's results of this step.
+    // The two sets' passes run side by side: the high set's on t->stream, the
+    // low set's on t->stream2.  They meet where a pass reads the other set's
+    // results of this step: the occupancy dilation (both sets' marks), the
+    // complementary walks (the other set's sorted state and cell ranges, and
+    // the dilation), the complementary forces (the other set's final A, B)
+    // and the split/merge stage (everything).
     const hipStream_t sst[2] = {t->stream2, st};
+    const hipStream_t sl = t->stream2;
     const uint32_t C0 = t->set[0].num_cells;
-    int rc = fork2(t, 0);
+    int rc = wait_for(t, 0, st, sl);  // fork
     if (rc != SPH_OK) return rc;
     // Neighbourhoods (the reference runs high then low, Solver.cu:1576-1577;
     // the sets do not interact here).  The state the step reads is the
     // compacted state; the reorder gathers from it.  Then each set marks the
-    // low-grid cells it occupies (occ_h for the high set, occ_l for the low),
-    // its colour values and boundary accelerations, and, with the fixed
-    // coupling, its own-set densities.
+    // low-grid cells it occupies (occ_h for the high set, occ_l for the low)
+    // and computes its colour values and boundary accelerations.
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& S = t->set[r];
         const hipStream_t s = sst[r];
@@ -1491,24 +1487,24 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
             rc = launch_wall_accel(t->walls, S.buf.tpos, S.buf.n_dev, S.cap, t->bacc[r], s);
             if (rc != SPH_OK) return rc;
         }
-        if (!blended(t)) {
-            rc = launch_density_own_x(P[r], S.buf, s);
-            if (rc != SPH_OK) return rc;
-        }
+        hipLaunchKernelGGL(k_clear_words, dim3(4), dim3(kBlock), 0, s, walk_counts(t, r),
+                           kWalkSegments * kWalkCounterStride);
     }
-    rc = join2(t, 0);
+    rc = wait_for(t, 1, sl, st);  // the low set's sorted state and marks
     if (rc != SPH_OK) return rc;
-    // Dilation of both sets' occupancy.
     hipLaunchKernelGGL(k_dilate_occupancy, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 2048u)),
                        dim3(kBlock), 0, st, t->occ, t->occ + C0, G[0].dx, G[0].dy, G[0].dz,
                        t->occ + 2 * C0);
+    // The blended coupling's passes read both sets throughout: they run on
+    // t->stream after the join above.
     if (blended(t)) return launch_blended_passes(t, P, G, X, dt, accel_out);
-    // The complementary densities (which also finalise A, B): the particles
-    // with a complementary walk are listed and walked one per wave; then each
-    // set's own-set forces (no integration).
-    hipLaunchKernelGGL(k_clear_words, dim3(8), dim3(kBlock), 0, st, t->walk_count,
-                       2u * kWalkSegments * kWalkCounterStride);
-    rc = fork2(t, 1);
+    // Own-set densities; the complementary densities (which also finalise A,
+    // B): the particles with a complementary walk are listed and walked one
+    // per wave; the own-set forces (no integration); the complementary forces
+    // of the listed particles; the integration.
+    rc = launch_density_own_x(P[0], t->set[0].buf, sl);
+    if (rc == SPH_OK) rc = wait_for(t, 2, st, sl);  // the dilation and the high set's sorted state
+    if (rc == SPH_OK) rc = launch_density_own_x(P[1], t->set[1].buf, st);
     if (rc != SPH_OK) return rc;
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
@@ -1522,19 +1518,16 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
                            A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
                            B.buf.tpos, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
                            cross_params(t, r), A.buf.rho, A.buf.aux);
+        SPH_HIP(hipEventRecord(t->ev[4 - r], s));  // event 3: high A, B final; 4: low
         rc = launch_gather_aux(P[r], A.buf, s);
         if (rc == SPH_OK) rc = launch_force_own_x(P[r], G[r], A.buf, A.accel, s);
         if (rc != SPH_OK) return rc;
     }
-    rc = join2(t, 1);
-    if (rc == SPH_OK) rc = fork2(t, 2);
-    if (rc != SPH_OK) return rc;
-    // The complementary forces of the listed particles (they read the other
-    // set's final A, B), then the integration.
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
         const hipStream_t s = sst[r];
+        SPH_HIP(hipStreamWaitEvent(s, t->ev[3 + r], 0));  // the other set's final A, B
         CrossParams c = cross_params(t, r);
         if (c.tension != 0.0f)
             hipLaunchKernelGGL(k_cross_force_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
@@ -1553,7 +1546,7 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
                            A.buf.tvel, A.accel, A.buf.n_dev, c, wall_acc(t, r), dt, A.buf.pos,
                            A.buf.vel, accel_out ? A.buf.accel : nullptr);
     }
-    rc = join2(t, 2);
+    rc = wait_for(t, 5, sl, st);  // join
     if (rc != SPH_OK) return rc;
     SPH_HIP(hipGetLastError());
     return launch_split_merge(t);
