@@ -241,6 +241,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(
         gWy += wB * ay;
         gWz += wB * az;
     };
+    // The scan loads whole groups of kAdjGroup slots: tx must hold them past
+    // a full tile (the loads past a run are masked out by the range test).
+    static_assert(sizeof(tx) / sizeof(tx[0]) >= kAdjForceTile + kAdjGroup,
+                  "the force-adjoint tile must be padded by one group");
     walk_stencil_staged<kBlock, kAdjForceTile, true>(
         g, pr, cs, ce, n, active, red,
         [&](uint32_t k, uint32_t j) {
@@ -352,6 +356,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     };
     // Padded: the scan below loads whole groups and multiplies the loads past
     // its run by a zero weight, which needs them finite.
+    static_assert(sizeof(tile) / sizeof(tile[0]) >= kAdjDensityTile + kAdjGroup,
+                  "the staged walk fills kTile + kPad slots");
     walk_stencil_staged<kBlock, kAdjDensityTile, true, kAdjGroup>(
         g, pr, cs, ce, n, active, red,
         [&](uint32_t k, uint32_t j) {
