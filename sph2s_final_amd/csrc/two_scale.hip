@@ -1392,21 +1392,33 @@ int launch_split_merge(sph_two_scale* t) {
 // Blended coupling: lambda-weighted densities of both sets, then the
 // lambda-weighted forces with integration and the blend update, then the
 // split/merge stage.
+// The two sets' densities run side by side (each reads only the other set's
+// sorted positions and blend words), as do the two force kernels once both
+// densities are done (each reads the other set's A, B): the high set's on
+// t->stream, the low set's on t->stream2.
 int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView* G,
                           const GridView* X, float dt, bool accel_out) {
-    hipStream_t st = t->stream;
+    const hipStream_t st = t->stream, sl = t->stream2;
+    const hipStream_t sst[2] = {sl, st};
+    int rc = wait_for(t, 2, st, sl);  // the dilation and the high set's sorted state
+    if (rc != SPH_OK) return rc;
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
         const Nearby occ = nearby(t, r);
-        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, st, A.buf.tpos,
+        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, sst[r], A.buf.tpos,
                            A.tblend, A.buf.cell_start, A.buf.cell_end, A.buf.n_dev, G[r], P[r],
                            B.buf.tpos, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
                            X[1 - r], cross_params(t, r), occ, X[0], A.buf.rho, A.buf.aux);
     }
+    SPH_HIP(hipEventRecord(t->ev[3], sl));  // the low set's A, B
+    SPH_HIP(hipEventRecord(t->ev[4], st));  // the high set's A, B
+    SPH_HIP(hipStreamWaitEvent(st, t->ev[3], 0));
+    SPH_HIP(hipStreamWaitEvent(sl, t->ev[4], 0));
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
+        const hipStream_t s = sst[r];
         CrossParams c = cross_params(t, r);
         c.vmax = velocity_limit(t, r, dt);
         float4* acc = accel_out ? A.buf.accel : nullptr;
@@ -1414,20 +1426,22 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         const float inc = t->opt.blend_increment;
         if (P[r].tens != 0.0f || c.tension != 0.0f)
             hipLaunchKernelGGL(k_blend_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0,
-                               st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
+                               s, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
                                A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
                                B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
                                X[1 - r], c, occ, X[0], wall_acc(t, r), dt, inc, A.buf.pos,
                                A.buf.vel, A.nblend, acc);
         else
             hipLaunchKernelGGL(k_blend_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
-                               st, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
+                               s, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
                                A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
                                B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
                                X[1 - r], c, occ, X[0], wall_acc(t, r), dt, inc, A.buf.pos,
                                A.buf.vel, A.nblend, acc);
     }
     SPH_HIP(hipGetLastError());
+    rc = wait_for(t, 5, sl, st);  // join
+    if (rc != SPH_OK) return rc;
     return launch_split_merge(t);
 }
 
