@@ -206,6 +206,11 @@ constexpr uint32_t kWalkBlocks = 1024;  // 4096 waves share a step's walk list
 constexpr uint32_t kWalkSegments = 64;
 constexpr uint32_t kWalkCounterStride = 16;  // words
 
+// Null arguments of the fixed coupling's walks (no blend weights, no separate
+// complementary acceleration).
+float4* const kNoXacc = nullptr;
+const float2* const kNoBlend = nullptr;
+
 inline uint32_t walk_segment_cap(uint32_t cap) {
     const uint32_t blocks = (cap + kBlock - 1) / kBlock;
     return ((blocks + kWalkSegments - 1) / kWalkSegments) * kBlock;
@@ -333,12 +338,14 @@ __device__ __forceinline__ void wave_walk(const GridView& g, const uint32_t* __r
 }
 
 // Lists set A's particles whose complementary walk is not empty and
-// finalises the others (density unchanged, A and B from it).
+// finalises the others: density unchanged, A and B from it, and (with xacc_a,
+// the blended coupling) a complementary acceleration of -0, which the
+// integration adds without changing the own-set sum (x + -0 = x for every x).
 __global__ __launch_bounds__(kBlock) void k_cross_list(
     const float4* __restrict__ tpos_a, const uint32_t* __restrict__ n_a,
     const uint32_t* __restrict__ n_b, Nearby occ, GridView ga, CrossParams c,
-    float* __restrict__ rho_a, float2* __restrict__ aux_a, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ list, uint32_t seg_cap) {
+    float* __restrict__ rho_a, float2* __restrict__ aux_a, float4* __restrict__ xacc_a,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ list, uint32_t seg_cap) {
     const uint32_t na = *n_a, nb = *n_b;
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t seg = blockIdx.x % kWalkSegments;
@@ -351,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_list(
             const float r = rho_a[s];
             const float inv = __builtin_amdgcn_rcpf(r);
             aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+            if (xacc_a) xacc_a[s] = make_float4(-0.0f, -0.0f, -0.0f, 0.0f);
         }
     }
     append_flagged(walk, s, counts + seg * kWalkCounterStride, list + seg * seg_cap);
@@ -358,12 +366,14 @@ __global__ __launch_bounds__(kBlock) void k_cross_list(
 
 // complementary density of the listed particles: rho_i += m_B sum_{j in B,
 // r < h0} 1/2 (W_h0 + W_h1), then A_i, B_i.  One wave per listed particle.
+// kBlend: every term weighted by lambda_j (tbl_b[j].x; Solver.cu:252-290).
+template <bool kBlend>
 __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
     const float4* __restrict__ tpos_a, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ list, uint32_t seg_cap, const float4* __restrict__ tpos_b,
-    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
-    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, float* __restrict__ rho_a,
-    float2* __restrict__ aux_a) {
+    const float2* __restrict__ tbl_b, const uint32_t* __restrict__ cs_b,
+    const uint32_t* __restrict__ ce_b, const uint32_t* __restrict__ n_b, GridView gb,
+    CrossParams c, float* __restrict__ rho_a, float2* __restrict__ aux_a) {
     __shared__ uint32_t rb_s[kBlock / 64][64];
     __shared__ uint32_t incl_s[kBlock / 64][64];
     __shared__ uint32_t seg_s[kBlock / 64][64];
@@ -384,7 +394,10 @@ __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
                 const float a1 = c.h12 - r2;
                 const float w0 = c.c6_0 * a0 * a0 * a0;
                 const float w1 = (r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f;
-                sum += 0.5f * (w0 + w1);
+                if (kBlend)
+                    sum += tbl_b[j].x * (0.5f * (w0 + w1));
+                else
+                    sum += 0.5f * (w0 + w1);
             }
         });
         sum = wave_sum(sum);
@@ -397,16 +410,19 @@ __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
     }
 }
 
-// complementary force (TWO_SCALE/Solver.cu:394-499 with lambda = 1) of the
-// listed particles, added to their own-set acceleration.  One wave each.
-template <bool kTension>
+// complementary force (TWO_SCALE/Solver.cu:394-499) of the listed particles.
+// One wave each.  Fixed coupling (lambda = 1): added to their own-set
+// acceleration in acc_a.  kBlend: every term weighted by lambda_j (tbl_b[j].x)
+// and the sum written to acc_a, which the blended force kernel adds.
+template <bool kTension, bool kBlend>
 __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
     const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
     const float2* __restrict__ aux_a, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ list, uint32_t seg_cap, const float4* __restrict__ tpos_b,
     const float4* __restrict__ tvel_b, const float2* __restrict__ aux_b,
-    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
-    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c, float4* __restrict__ acc_a) {
+    const float2* __restrict__ tbl_b, const uint32_t* __restrict__ cs_b,
+    const uint32_t* __restrict__ ce_b, const uint32_t* __restrict__ n_b, GridView gb,
+    CrossParams c, float4* __restrict__ acc_a) {
     __shared__ uint32_t rb_s[kBlock / 64][64];
     __shared__ uint32_t incl_s[kBlock / 64][64];
     __shared__ uint32_t seg_s[kBlock / 64][64];
@@ -435,18 +451,20 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
                 // 1/2 (|grad W_h0| + |grad W_h1|) / r and 1/2 (lap W_h0 + lap W_h1)
                 const float gg = 0.5f * (c.c45_0 * hr0 * hr0 + c.c45_1 * hr1 * hr1) * inv_r;
                 const float l = 0.5f * (c.c45_0 * hr0 + c.c45_1 * hr1);
-                const float tp = (ai.x + aj.x) * gg;
+                const float lj = kBlend ? tbl_b[j].x : 1.0f;
+                const float tp = kBlend ? lj * (ai.x + aj.x) * gg : (ai.x + aj.x) * gg;
                 Px += tp * dx;
                 Py += tp * dy;
                 Pz += tp * dz;
-                const float tv = l * aj.y;
+                const float tv = kBlend ? lj * l * aj.y : l * aj.y;
                 Vx += tv * (vj.x - vi.x);
                 Vy += tv * (vj.y - vi.y);
                 Vz += tv * (vj.z - vi.z);
                 if (kTension) {
                     const float a0 = c.h02 - r2, a1 = c.h12 - r2;
-                    const float wt = 0.5f * (c.c6_0 * a0 * a0 * a0 +
-                                             ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
+                    const float wt = (kBlend ? lj * 0.5f : 0.5f) *
+                                     (c.c6_0 * a0 * a0 * a0 +
+                                      ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
                     Tx += wt * dx;
                     Ty += wt * dy;
                     Tz += wt * dz;
@@ -474,8 +492,12 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
                 ay -= tc * Ty;
                 az -= tc * Tz;
             }
-            const float4 a = acc_a[s];
-            acc_a[s] = make_float4(a.x + ax, a.y + ay, a.z + az, a.w);
+            if (kBlend) {
+                acc_a[s] = make_float4(ax, ay, az, 0.0f);
+            } else {
+                const float4 a = acc_a[s];
+                acc_a[s] = make_float4(a.x + ax, a.y + ay, a.z + az, a.w);
+            }
         }
     }
 }
@@ -534,15 +556,11 @@ constexpr int kBlendForceTile = 512;
 __global__ __launch_bounds__(kBlock) void k_blend_density(
     const float4* __restrict__ tpos_a, const float2* __restrict__ tbl_a,
     const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
-    const uint32_t* __restrict__ n_a, GridView ga, StepParams pa,
-    const float4* __restrict__ tpos_b, const float2* __restrict__ tbl_b,
-    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
-    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
-    Nearby occ, GridView g_occ, float* __restrict__ rho_a,
-    float2* __restrict__ aux_a) {
+    const uint32_t* __restrict__ n_a, GridView ga, StepParams pa, CrossParams c,
+    float* __restrict__ rho_a, float2* __restrict__ aux_a) {
     __shared__ float4 tile[kBlendDensityTile];
     __shared__ uint32_t red[2 * (kBlock / 64)];
-    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t na = *n_a;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool active = s < na;
     const float4 xi = active ? tpos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -573,24 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
             for (uint32_t j = b; j < e; ++j) pair(tpos_a[j], tbl_a[j].x);
         });
     if (!active) return;
-    float cross = 0.0f;
-    if (nb > 0 && other_nearby(occ, g_occ, xi.x, xi.y, xi.z)) {
-        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-            for (uint32_t j = b; j < e; ++j) {
-                const float4 xj = tpos_b[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < c.h02) {
-                    const float a0 = c.h02 - r2;
-                    const float a1 = c.h12 - r2;
-                    const float w0 = c.c6_0 * a0 * a0 * a0;
-                    const float w1 = (r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f;
-                    cross += tbl_b[j].x * (0.5f * (w0 + w1));
-                }
-            }
-        });
-    }
-    const float r = pa.c6m * own + c.m_other * cross;
+    const float r = pa.c6m * own;
     rho_a[s] = r;
     const float inv = __builtin_amdgcn_rcpf(r);
     aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
@@ -604,19 +605,15 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
     const float2* __restrict__ aux_a, const float2* __restrict__ tbl_a,
     const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
-    const uint32_t* __restrict__ n_a, GridView ga, StepParams pa,
-    const float4* __restrict__ tpos_b, const float4* __restrict__ tvel_b,
-    const float2* __restrict__ aux_b, const float2* __restrict__ tbl_b,
-    const uint32_t* __restrict__ cs_b, const uint32_t* __restrict__ ce_b,
-    const uint32_t* __restrict__ n_b, GridView gb, CrossParams c,
-    Nearby occ, GridView g_occ, const float4* __restrict__ bacc, float dt,
-    float inc, float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
+    const uint32_t* __restrict__ n_a, GridView ga, StepParams pa, CrossParams c,
+    const float4* __restrict__ xacc_a, const float4* __restrict__ bacc, float dt, float inc,
+    float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
     float4* __restrict__ acc_out) {
     __shared__ float4 tx[kBlendForceTile];
     __shared__ float4 tv[kBlendForceTile];
     __shared__ float tl[kBlendForceTile];
     __shared__ uint32_t red[2 * (kBlock / 64)];
-    const uint32_t na = *n_a, nb = *n_b;
+    const uint32_t na = *n_a;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool active = s < na;
     const float4 xi = active ? tpos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -684,56 +681,12 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
         ay -= tc * Ty;
         az -= tc * Tz;
     }
-    // complementary sums (Solver.cu:394-499) weighted by lambda_j
-    float Qx = 0.f, Qy = 0.f, Qz = 0.f, Ux = 0.f, Uy = 0.f, Uz = 0.f, Sx = 0.f, Sy = 0.f,
-          Sz = 0.f;
-    if (nb > 0 && other_nearby(occ, g_occ, xi.x, xi.y, xi.z)) {
-        for_each_run(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, [&](uint32_t b, uint32_t e) {
-            for (uint32_t j = b; j < e; ++j) {
-                const float4 xj = tpos_b[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 < c.h02 && r2 > 0.0f) {
-                    const float4 vj = tvel_b[j];
-                    const float2 aj = aux_b[j];
-                    const float lj = tbl_b[j].x;
-                    const float inv_r = __builtin_amdgcn_rsqf(r2);
-                    const float r = r2 * inv_r;
-                    const float hr0 = c.h0 - r;
-                    const float hr1 = fmaxf(c.h1 - r, 0.0f);
-                    const float g = 0.5f * (c.c45_0 * hr0 * hr0 + c.c45_1 * hr1 * hr1) * inv_r;
-                    const float l = 0.5f * (c.c45_0 * hr0 + c.c45_1 * hr1);
-                    const float tp = lj * (ai.x + aj.x) * g;
-                    Qx += tp * dx;
-                    Qy += tp * dy;
-                    Qz += tp * dz;
-                    const float tvv = lj * l * aj.y;
-                    Ux += tvv * (vj.x - vi.x);
-                    Uy += tvv * (vj.y - vi.y);
-                    Uz += tvv * (vj.z - vi.z);
-                    if (kTension) {
-                        const float a0 = c.h02 - r2, a1 = c.h12 - r2;
-                        const float w = lj * 0.5f *
-                                        (c.c6_0 * a0 * a0 * a0 +
-                                         ((r2 < c.h12) ? c.c6_1 * a1 * a1 * a1 : 0.0f));
-                        Sx += w * dx;
-                        Sy += w * dy;
-                        Sz += w * dz;
-                    }
-                }
-            }
-        });
-    }
-    const float mv = c.m_other * c.visc * ai.y;
-    ax += c.m_other * Qx + mv * Ux;
-    ay += c.m_other * Qy + mv * Uy;
-    az += c.m_other * Qz + mv * Uz;
-    if (kTension) {
-        const float tc = c.tension * c.m_other * ai.y;
-        ax -= tc * Sx;
-        ay -= tc * Sy;
-        az -= tc * Sz;
-    }
+    // The complementary sums (Solver.cu:394-499, weighted by lambda_j) of
+    // the step's walk; -0 for the particles it does not list.
+    const float4 xa = xacc_a[s];
+    ax += xa.x;
+    ay += xa.y;
+    az += xa.z;
     if (bacc) {  // boundary particles (Solver.cu:1046-1048)
         const float4 b = bacc[s];
         ax += b.x;
@@ -1392,10 +1345,15 @@ int launch_split_merge(sph_two_scale* t) {
 // Blended coupling: lambda-weighted densities of both sets, then the
 // lambda-weighted forces with integration and the blend update, then the
 // split/merge stage.
-// The two sets' densities run side by side (each reads only the other set's
-// sorted positions and blend words), as do the two force kernels once both
-// densities are done (each reads the other set's A, B): the high set's on
-// t->stream, the low set's on t->stream2.
+// The blended coupling's passes (Solver.cu:215-499, 732-1083) in the fixed
+// coupling's shape: each set's lambda-weighted own-set density; the
+// complementary sums of the particles with a non-empty walk, one wave each
+// (k_cross_list lists them and finalises the others), weighted by lambda_j;
+// then each set's own-set force with the integration and blend update, which
+// adds the walk's complementary acceleration (kept in the set's own-set
+// acceleration buffer, unused by this coupling).  The high set's passes run
+// on t->stream, the low set's on t->stream2; the complementary forces wait
+// for both sets' final A, B.
 int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView* G,
                           const GridView* X, float dt, bool accel_out) {
     const hipStream_t st = t->stream, sl = t->stream2;
@@ -1405,11 +1363,18 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
     for (int r = 1; r >= 0; --r) {
         TwoScaleSet& A = t->set[r];
         TwoScaleSet& B = t->set[1 - r];
-        const Nearby occ = nearby(t, r);
-        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, sst[r], A.buf.tpos,
+        const hipStream_t s = sst[r];
+        const CrossParams c = cross_params(t, r);
+        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.tblend, A.buf.cell_start, A.buf.cell_end, A.buf.n_dev, G[r], P[r],
+                           c, A.buf.rho, A.buf.aux);
+        hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
+                           A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], c, A.buf.rho, A.buf.aux,
+                           A.accel, walk_counts(t, r), A.walk, walk_segment_cap(A.cap));
+        hipLaunchKernelGGL(k_cross_density_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
+                           A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
                            B.buf.tpos, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                           X[1 - r], cross_params(t, r), occ, X[0], A.buf.rho, A.buf.aux);
+                           X[1 - r], c, A.buf.rho, A.buf.aux);
     }
     SPH_HIP(hipEventRecord(t->ev[3], sl));  // the low set's A, B
     SPH_HIP(hipEventRecord(t->ev[4], st));  // the high set's A, B
@@ -1420,24 +1385,31 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         TwoScaleSet& B = t->set[1 - r];
         const hipStream_t s = sst[r];
         CrossParams c = cross_params(t, r);
+        if (c.tension != 0.0f)
+            hipLaunchKernelGGL((k_cross_force_walk<true, true>), dim3(kWalkBlocks), dim3(kBlock),
+                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
+                               B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
+                               c, A.accel);
+        else
+            hipLaunchKernelGGL((k_cross_force_walk<false, true>), dim3(kWalkBlocks), dim3(kBlock),
+                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
+                               B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
+                               c, A.accel);
         c.vmax = velocity_limit(t, r, dt);
         float4* acc = accel_out ? A.buf.accel : nullptr;
-        const Nearby occ = nearby(t, r);
         const float inc = t->opt.blend_increment;
-        if (P[r].tens != 0.0f || c.tension != 0.0f)
+        if (P[r].tens != 0.0f)
             hipLaunchKernelGGL(k_blend_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0,
                                s, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
-                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
-                               B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                               X[1 - r], c, occ, X[0], wall_acc(t, r), dt, inc, A.buf.pos,
-                               A.buf.vel, A.nblend, acc);
+                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], c, A.accel,
+                               wall_acc(t, r), dt, inc, A.buf.pos, A.buf.vel, A.nblend, acc);
         else
             hipLaunchKernelGGL(k_blend_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
                                s, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
-                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], B.buf.tpos, B.buf.tvel,
-                               B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                               X[1 - r], c, occ, X[0], wall_acc(t, r), dt, inc, A.buf.pos,
-                               A.buf.vel, A.nblend, acc);
+                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], c, A.accel,
+                               wall_acc(t, r), dt, inc, A.buf.pos, A.buf.vel, A.nblend, acc);
     }
     SPH_HIP(hipGetLastError());
     rc = wait_for(t, 5, sl, st);  // join
@@ -1526,12 +1498,12 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         const hipStream_t s = sst[r];
         hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], cross_params(t, r),
-                           A.buf.rho, A.buf.aux, walk_counts(t, r), A.walk,
+                           A.buf.rho, A.buf.aux, kNoXacc, walk_counts(t, r), A.walk,
                            walk_segment_cap(A.cap));
-        hipLaunchKernelGGL(k_cross_density_walk, dim3(kWalkBlocks), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL(k_cross_density_walk<false>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
                            A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
-                           B.buf.tpos, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
-                           cross_params(t, r), A.buf.rho, A.buf.aux);
+                           B.buf.tpos, kNoBlend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
+                           X[1 - r], cross_params(t, r), A.buf.rho, A.buf.aux);
         SPH_HIP(hipEventRecord(t->ev[4 - r], s));  // event 3: high A, B final; 4: low
         rc = launch_gather_aux(P[r], A.buf, s);
         if (rc == SPH_OK) rc = launch_force_own_x(P[r], G[r], A.buf, A.accel, s);
@@ -1544,17 +1516,17 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         SPH_HIP(hipStreamWaitEvent(s, t->ev[3 + r], 0));  // the other set's final A, B
         CrossParams c = cross_params(t, r);
         if (c.tension != 0.0f)
-            hipLaunchKernelGGL(k_cross_force_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
-                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+            hipLaunchKernelGGL((k_cross_force_walk<true, false>), dim3(kWalkBlocks), dim3(kBlock),
+                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
                                walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
-                               B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r], c,
-                               A.accel);
+                               kNoBlend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
+                               c, A.accel);
         else
-            hipLaunchKernelGGL(k_cross_force_walk<false>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
-                               A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
+            hipLaunchKernelGGL((k_cross_force_walk<false, false>), dim3(kWalkBlocks), dim3(kBlock),
+                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
                                walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
-                               B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r], c,
-                               A.accel);
+                               kNoBlend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
+                               c, A.accel);
         c.vmax = velocity_limit(t, r, dt);
         hipLaunchKernelGGL(k_cross_integrate, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.tvel, A.accel, A.buf.n_dev, c, wall_acc(t, r), dt, A.buf.pos,
