@@ -287,9 +287,13 @@ int sph_count_pairs(sph_solver* s, uint64_t counts[2]);
  *              block-size particles; 1..4 = persistent grid of 1, 2, 4, 8
  *              blocks per CU looping over those blocks
  *   bits 16-17 density block size, bits 18-19 force block size: 0 = 256,
- *              1 = 128, 2 = 512 threads (one particle per lane); a non-zero
- *              block size requires that pass's tile/list field to be 0
- *   bits 20-31 must be zero; other words are refused with SPH_ERR_ARG
+ *              1 = 128, 2 = 512 threads; a non-zero block size requires that
+ *              pass's tile/list field to be 0
+ *   bits 20-21 density, bits 22-23 force: particles per lane, 0 = 1, 1 = 2
+ *              (particles per wavefront 64 or 128; a block's targets share
+ *              one staged union per row); 2 requires that pass's tile/list
+ *              and block-size fields to be 0, and a batch refuses it
+ *   bits 24-31 must be zero; other words are refused with SPH_ERR_ARG
  * All variants give bit-identical results.  sph_autotune times the variants
  * on the solver's current state (reps launches each), keeps the fastest and
  * returns its tune word; ms[4] (may be NULL) receives the default and tuned

@@ -12,7 +12,8 @@ run of the same workload reuses it without timing.
 
 Tune word fields (include/sph2s.h): bits 0-3 density tile, 4-7 force
 tile/list, 8-11 density grid, 12-15 force grid, 16-17 density block size,
-18-19 force block size.
+18-19 force block size, 20-21 density and 22-23 force particles per lane
+(particles per wavefront = 64 x particles per lane; single scenes only).
 """
 from __future__ import annotations
 
@@ -32,11 +33,13 @@ def describe(tune: int) -> dict:
     d, f = tune & 0xF, (tune >> 4) & 0xF
     dg, fg = (tune >> 8) & 0xF, (tune >> 12) & 0xF
     db, fb = (tune >> 16) & 0x3, (tune >> 18) & 0x3
+    dp, fp = (tune >> 20) & 0x3, (tune >> 22) & 0x3
     return {"density_tile": DENSITY_TILES[d], "force_tile": FORCE_VARIANTS[f][0],
             "force_list_cap": FORCE_VARIANTS[f][1], "force_list_bits": FORCE_VARIANTS[f][2],
             "density_blocks_per_cu": GRID_BLOCKS_PER_CU[dg],
             "force_blocks_per_cu": GRID_BLOCKS_PER_CU[fg],
-            "density_block_threads": BLOCK_THREADS[db], "force_block_threads": BLOCK_THREADS[fb]}
+            "density_block_threads": BLOCK_THREADS[db], "force_block_threads": BLOCK_THREADS[fb],
+            "density_particles_per_lane": 1 + dp, "force_particles_per_lane": 1 + fp}
 
 
 def workload_key(kind: str, particles: int, cells: int, scenes: int = 1, device: str = "") -> str:

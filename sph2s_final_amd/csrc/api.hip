@@ -552,7 +552,7 @@ int sph_autotune(sph_solver* s, float dt, uint32_t reps, uint32_t* tune, float* 
 
 int sph_set_tune(sph_solver* s, uint32_t tune) {
     if (!s) return set_error(SPH_ERR_INVALID_ARGUMENT, "solver is null");
-    if (!tune_word_valid(tune)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
+    if (!tune_word_valid(tune, false)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
     if (tune != s->tune) {
         s->tune = tune;
         destroy_graph(s);

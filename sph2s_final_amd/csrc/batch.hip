@@ -301,7 +301,7 @@ int sph_batch_autotune(sph_batch* b, float dt, uint32_t reps, uint32_t* tune, fl
 
 int sph_batch_set_tune(sph_batch* b, uint32_t tune) {
     if (!b) return set_error(SPH_ERR_INVALID_ARGUMENT, "batch is null");
-    if (!tune_word_valid(tune)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
+    if (!tune_word_valid(tune, true)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
     if (tune != b->cap.tune) {
         b->cap.tune = tune;
         destroy_graph(b);

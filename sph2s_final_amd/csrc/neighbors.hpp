@@ -148,7 +148,7 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
 // passes on the current state (scenes == NULL: a single scene with p's
 // constants) and returns the fastest tune word; ms_out (may be NULL) gets the
 // default and tuned milliseconds of density and force.  pos/vel are restored.
-bool tune_word_valid(uint32_t tune);
+bool tune_word_valid(uint32_t tune, bool batched);
 int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams* scenes,
                           bool kahan, bool tension, float dt, uint32_t key_bits, uint32_t reps,
                           hipStream_t stream, uint32_t* tune_out, float* ms_out);

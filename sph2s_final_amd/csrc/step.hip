@@ -198,9 +198,12 @@ constexpr int kWavesPerBlock = kBlock / 64;
 //   bits 12-15 force grid:         persistent grid of 1, 2, 4, 8 blocks per CU
 //                                  looping over those blocks
 //   bits 16-17 density block size, bits 18-19 force block size: 256 (0), 128
-//              (1), 512 (2) threads, one particle per lane; a non-default block
-//              size runs the pass's default tile and queue
-// The block size also decides how many particles share one staged union.
+//              (1), 512 (2) threads; a non-default block size runs the pass's
+//              default tile and queue
+//   bits 20-21 density, bits 22-23 force: particles per lane, 1 (0) or 2 (1),
+//              with the default tile, queue and block size (single scenes)
+// The block size times the particles per lane decides how many targets share
+// one staged union.
 constexpr uint32_t kPersistPerCu[] = {0, 1, 2, 4, 8};
 const SceneParams* const kNoScenes = nullptr;
 
@@ -317,7 +320,11 @@ __device__ __forceinline__ void scene_constants(const SceneParams* __restrict__ 
     }
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB>
+// kP targets per lane (particles per wavefront 64 kP): the block covers
+// the kP kB consecutive targets block0 + q kB + lane, q < kP, stages one union
+// of all their runs per row and scans each lane's targets in turn against it.
+// Each target's sums are taken exactly as with one target per lane.
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB, int kP>
 __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float4* tile,
                                               uint32_t* red, const float4* __restrict__ tpos,
                                               const uint32_t* __restrict__ cs,
@@ -327,67 +334,96 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
                                               const float4* __restrict__ tvel,
                                               const SceneParams* __restrict__ scenes,
                                               const XOrder& xo) {
-    const uint32_t s = block0 + threadIdx.x;
-    const bool active = s < n;
+    static_assert(kP == 1 || !kBatched, "batched scenes run one target per lane");
     StepParams p = p_in;
     GridView g = g_in;
-    scene_constants<kBatched>(scenes, tvel, s, active, p, g);
+    uint32_t s[kP];
+    bool active[kP];
+    float4 xi[kP];
+    Probe pr[kP];
+    float sum[kP], comp[kP];
+#pragma unroll
+    for (int q = 0; q < kP; ++q) {
+        s[q] = block0 + q * kB + threadIdx.x;
+        active[q] = s[q] < n;
+        if (kBatched) scene_constants<kBatched>(scenes, tvel, s[q], active[q], p, g);
+        xi[q] = active[q] ? tpos[s[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        pr[q] = make_probe(g, xi[q].x, xi[q].y, xi[q].z);
+        sum[q] = 0.0f;
+        comp[q] = 0.0f;
+    }
     const float inv_h2 = 1.0f / p.h2;
-    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
-    float sum = 0.0f, comp = 0.0f;
-    auto accumulate = [&](float rs) {
+    auto accumulate = [&](int q, float rs) {
         if (kKahan) {
-            const float y = rs - comp;
-            const float t = sum + y;
-            comp = (t - sum) - y;
-            sum = t;
+            const float y = rs - comp[q];
+            const float t = sum[q] + y;
+            comp[q] = (t - sum[q]) - y;
+            sum[q] = t;
         } else {
-            sum += rs;
+            sum[q] += rs;
         }
     };
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
-            uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row<kX>(g, pr, cs, ce, xo.bins, n, pr.cy + dy, pr.cz + dz, b, e);
+            uint32_t b[kP], e[kP];
+            uint32_t lb = kEmptyCell, le = 0u;
+#pragma unroll
+            for (int q = 0; q < kP; ++q) {
+                b[q] = kEmptyCell;
+                e[q] = 0u;
+                if (active[q])
+                    probe_row<kX>(g, pr[q], cs, ce, xo.bins, n, pr[q].cy + dy, pr[q].cz + dz,
+                                  b[q], e[q]);
+                lb = min(lb, b[q]);
+                le = max(le, e[q]);
+            }
             uint32_t ub, ue;
-            block_run_union<kB>(b, e, red, ub, ue);
+            block_run_union<kB>(lb, le, red, ub, ue);
             // Loaded range [ub, seg_e); empty when every lane's run is empty.
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
             for (uint32_t k = threadIdx.x; k < seg_n; k += kB) tile[k] = tpos[ub + k];
             __syncthreads();
-            const bool staged = b < e && b >= ub && e <= seg_e;
-            if (staged) {
-                const float4* t = tile + (b - ub);
-                accumulate(poly6_run<true>(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
-            }
-            if (b < e && !staged) {
-                const float4* t = tpos + b;
-                accumulate(poly6_run<false>(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
+#pragma unroll
+            for (int q = 0; q < kP; ++q) {
+                const bool staged = b[q] < e[q] && b[q] >= ub && e[q] <= seg_e;
+                if (staged) {
+                    const float4* t = tile + (b[q] - ub);
+                    accumulate(q, poly6_run<true>(xi[q], inv_h2, e[q] - b[q],
+                                                  [&](uint32_t k) { return t[k]; }));
+                }
+                if (b[q] < e[q] && !staged) {
+                    const float4* t = tpos + b[q];
+                    accumulate(q, poly6_run<false>(xi[q], inv_h2, e[q] - b[q],
+                                                   [&](uint32_t k) { return t[k]; }));
+                }
             }
         }
     }
-    if (!active) return;
-    for (int rz = pr.sz; rz <= pr.ez; ++rz)
-        for (int ry = pr.sy; ry <= pr.ey; ++ry) {
-            if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
-            uint32_t b, e;
-            probe_row<kX>(g, pr, cs, ce, xo.bins, n, ry, rz, b, e);
-            if (b < e) {
-                const float4* t = tpos + b;
-                accumulate(poly6_run<false>(xi, inv_h2, e - b, [&](uint32_t k) { return t[k]; }));
+#pragma unroll
+    for (int q = 0; q < kP; ++q) {
+        if (!active[q]) continue;
+        for (int rz = pr[q].sz; rz <= pr[q].ez; ++rz)
+            for (int ry = pr[q].sy; ry <= pr[q].ey; ++ry) {
+                if (abs(ry - pr[q].cy) <= 1 && abs(rz - pr[q].cz) <= 1) continue;
+                uint32_t b, e;
+                probe_row<kX>(g, pr[q], cs, ce, xo.bins, n, ry, rz, b, e);
+                if (b < e) {
+                    const float4* t = tpos + b;
+                    accumulate(q, poly6_run<false>(xi[q], inv_h2, e - b,
+                                                   [&](uint32_t k) { return t[k]; }));
+                }
             }
+        const float r = p.c6m_h6 * sum[q];
+        const uint32_t o = kX ? xo.i2r[s[q]] : s[q];
+        rho[o] = r;
+        if (kFinalize) {
+            const float inv = __builtin_amdgcn_rcpf(r);
+            const float2 ab = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
+            aux[o] = ab;
+            if (kX) xo.iaux[s[q]] = ab;
         }
-    const float r = p.c6m_h6 * sum;
-    const uint32_t o = kX ? xo.i2r[s] : s;
-    rho[o] = r;
-    if (kFinalize) {
-        const float inv = __builtin_amdgcn_rcpf(r);
-        const float2 ab = make_float2(p.stiffness * (r - p.rest_density) * inv * inv, inv);
-        aux[o] = ab;
-        if (kX) xo.iaux[s] = ab;
     }
 }
 
@@ -396,13 +432,13 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
 // the 640-candidate tile (LDS 28.8 KB per block); the hints make the
 // compiler allocate within those budgets.  Other instantiations (batched
 // scenes, tension, two-scale own-set passes) keep the compiler's choice.
-constexpr int density_waves(bool batched) { return batched ? 1 : 6; }
-constexpr int force_waves(bool batched, bool tension, bool integrate) {
-    return (!batched && !tension && integrate) ? 5 : 1;
+constexpr int density_waves(bool batched, int ppl = 1) { return batched ? 1 : (ppl == 1 ? 6 : 4); }
+constexpr int force_waves(bool batched, bool tension, bool integrate, int ppl = 1) {
+    return (!batched && !tension && integrate) ? (ppl == 1 ? 5 : 4) : 1;
 }
 
-template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB>
-__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(density_waves(kBatched))))
+template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB, int kP>
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(density_waves(kBatched, kP))))
 void k_density(const float4* __restrict__ tpos,
                                                     const uint32_t* __restrict__ cs,
                                                     const uint32_t* __restrict__ ce,
@@ -417,9 +453,9 @@ void k_density(const float4* __restrict__ tpos,
     __shared__ uint32_t red[2 * (kB / 64)];
     const uint32_t n = *n_dev;
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        const uint32_t block0 = xcd_local_block(vb, nvb) * kB;
+        const uint32_t block0 = xcd_local_block(vb, nvb) * (kB * kP);
         if (block0 < n)  // block-uniform
-            density_block<kKahan, kFinalize, kBatched, kTile, kX, kB>(
+            density_block<kKahan, kFinalize, kBatched, kTile, kX, kB, kP>(
                 block0, n, tile, red, tpos, cs, ce, p_in, g_in, rho, aux, tvel, scenes, xo);
     }
 }
@@ -526,7 +562,7 @@ __device__ __forceinline__ void force_finish(const PairSums<kTension>& S, const 
 // With kIntegrate = false (two-scale own-set pass) the kernel only writes the
 // acceleration, gravity included, to accel.
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
-          bool kX, int kB>
+          bool kX, int kB, int kP>
 __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4* tp, float4* tv,
                                             TList (*list)[kB], uint32_t* red,
                                             const float4* __restrict__ tpos,
@@ -540,62 +576,81 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
                                             float4* __restrict__ accel,
                                             const SceneParams* __restrict__ scenes,
                                             const XOrder& xo) {
-    const uint32_t s = block0 + threadIdx.x;
-    const bool active = s < n;
+    static_assert(kP == 1 || !kBatched, "batched scenes run one target per lane");
     StepParams p = p_in;
     GridView g = g_in;
-    scene_constants<kBatched>(scenes, tvel, s, active, p, g);
-    const float4 xi = active ? tpos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 vi = active ? tvel[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float2 ai = active ? aux[s] : make_float2(0.f, 1.f);
-    const Probe pr = make_probe(g, xi.x, xi.y, xi.z);
-    PairSums<kTension> S;
+    uint32_t s[kP];
+    bool active[kP];
+    float4 xi[kP], vi[kP];
+    float2 ai[kP];
+    Probe pr[kP];
+    PairSums<kTension> S[kP];
+#pragma unroll
+    for (int q = 0; q < kP; ++q) {
+        s[q] = block0 + q * kB + threadIdx.x;
+        active[q] = s[q] < n;
+        if (kBatched) scene_constants<kBatched>(scenes, tvel, s[q], active[q], p, g);
+        xi[q] = active[q] ? tpos[s[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        vi[q] = active[q] ? tvel[s[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        ai[q] = active[q] ? aux[s[q]] : make_float2(0.f, 1.f);
+        pr[q] = make_probe(g, xi[q].x, xi[q].y, xi[q].z);
+    }
     uint32_t cnt = 0;
 
-    auto pair = [&](const float4& xa, const float4& vb) {
-        S.add(xi, vi, ai.x, p.h, p.h2, xa, vb);
+    auto pair = [&](int q, const float4& xa, const float4& vb) {
+        S[q].add(xi[q], vi[q], ai[q].x, p.h, p.h2, xa, vb);
     };
     // 0 < r2 < h2 as one unsigned compare: r2 >= +0 is a sum of squares, and
     // for non-negative floats the bit patterns order like the values.
     const uint32_t h2_bits_m1 = __float_as_uint(p.h2) - 1u;
-    auto in_range = [&](const float4& xj) {
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    auto in_range = [&](int q, const float4& xj) {
+        const float dx = xi[q].x - xj.x, dy = xi[q].y - xj.y, dz = xi[q].z - xj.z;
         const float r2 = dist2(dx, dy, dz);
         return __float_as_uint(r2) - 1u < h2_bits_m1;
     };
     // Queued pairs in queue order; two list entries are fetched per step so
     // their tile reads overlap.
-    auto flush = [&]() {
+    auto flush = [&](int q) {
         uint32_t k = 0;
         for (; k + 2 <= cnt; k += 2) {
             const uint32_t t0 = list[k][threadIdx.x], t1 = list[k + 1][threadIdx.x];
             const float4 a0 = tp[t0], b0 = tv[t0], a1 = tp[t1], b1 = tv[t1];
-            pair(a0, b0);
-            pair(a1, b1);
+            pair(q, a0, b0);
+            pair(q, a1, b1);
         }
         if (k < cnt) {
             const uint32_t t0 = list[k][threadIdx.x];
-            pair(tp[t0], tv[t0]);
+            pair(q, tp[t0], tv[t0]);
         }
         cnt = 0;
     };
-    auto global_run = [&](uint32_t b, uint32_t e) {
+    auto global_run = [&](int q, uint32_t b, uint32_t e) {
         for (uint32_t j = b; j < e; ++j) {
             const float4 xj = tpos[j];
-            if (in_range(xj)) {
+            if (in_range(q, xj)) {
                 const float4 vj = tvel[j];
                 const float2 aj = aux[j];
-                pair(make_float4(xj.x, xj.y, xj.z, aj.x), make_float4(vj.x, vj.y, vj.z, aj.y));
+                pair(q, make_float4(xj.x, xj.y, xj.z, aj.x), make_float4(vj.x, vj.y, vj.z, aj.y));
             }
         }
     };
 
     for (int dz = -1; dz <= 1; ++dz) {
         for (int dy = -1; dy <= 1; ++dy) {
-            uint32_t b = kEmptyCell, e = 0u;
-            if (active) probe_row<kX>(g, pr, cs, ce, xo.bins, n, pr.cy + dy, pr.cz + dz, b, e);
+            uint32_t b[kP], e[kP];
+            uint32_t lb = kEmptyCell, le = 0u;
+#pragma unroll
+            for (int q = 0; q < kP; ++q) {
+                b[q] = kEmptyCell;
+                e[q] = 0u;
+                if (active[q])
+                    probe_row<kX>(g, pr[q], cs, ce, xo.bins, n, pr[q].cy + dy, pr[q].cz + dz,
+                                  b[q], e[q]);
+                lb = min(lb, b[q]);
+                le = max(le, e[q]);
+            }
             uint32_t ub, ue;
-            block_run_union<kB>(b, e, red, ub, ue);
+            block_run_union<kB>(lb, le, red, ub, ue);
             const uint32_t seg_n = (ue > ub) ? min(ue - ub, static_cast<uint32_t>(kTile)) : 0u;
             const uint32_t seg_e = ub + seg_n;
             __syncthreads();
@@ -607,48 +662,54 @@ __device__ __forceinline__ void force_block(uint32_t block0, uint32_t n, float4*
                 tv[k] = make_float4(v.x, v.y, v.z, a.y);
             }
             __syncthreads();
-            const bool staged = b < e && b >= ub && e <= seg_e;
             constexpr int kG = kCap >= 16 ? kScanGroup : kScanGroup / 2;
-            if (staged) {
-                // Whole groups of candidates are loaded before any test (the
-                // tile is padded); every candidate's index is written at the
-                // queue tail and the tail advances only for in-range ones.
-                const uint32_t t0 = b - ub, t1 = e - ub;
-                for (uint32_t t = t0; t < t1; t += kG) {
-                    float4 q[kG];
 #pragma unroll
-                    for (int u = 0; u < kG; ++u) q[u] = tp[t + u];
+            for (int q = 0; q < kP; ++q) {
+                const bool staged = b[q] < e[q] && b[q] >= ub && e[q] <= seg_e;
+                if (staged) {
+                    // Whole groups of candidates are loaded before any test (the
+                    // tile is padded); every candidate's index is written at the
+                    // queue tail and the tail advances only for in-range ones.
+                    const uint32_t t0 = b[q] - ub, t1 = e[q] - ub;
+                    for (uint32_t t = t0; t < t1; t += kG) {
+                        float4 c[kG];
 #pragma unroll
-                    for (int u = 0; u < kG; ++u) {
-                        keep_lds_quad(q[u]);
-                        const bool hit = (t + u < t1) & in_range(q[u]);  // no short circuit
-                        list[cnt][threadIdx.x] = static_cast<TList>(t + u);
-                        cnt += hit ? 1u : 0u;
+                        for (int u = 0; u < kG; ++u) c[u] = tp[t + u];
+#pragma unroll
+                        for (int u = 0; u < kG; ++u) {
+                            keep_lds_quad(c[u]);
+                            const bool hit = (t + u < t1) & in_range(q, c[u]);  // no short circuit
+                            list[cnt][threadIdx.x] = static_cast<TList>(t + u);
+                            cnt += hit ? 1u : 0u;
+                        }
+                        if (__ballot(cnt > kCap - kG) != 0ull) flush(q);
                     }
-                    if (__ballot(cnt > kCap - kG) != 0ull) flush();
                 }
+                if (b[q] < e[q] && !staged) global_run(q, b[q], e[q]);
+                flush(q);
             }
-            if (b < e && !staged) global_run(b, e);
-            flush();
         }
     }
-    if (!active) return;
-    for (int rz = pr.sz; rz <= pr.ez; ++rz)
-        for (int ry = pr.sy; ry <= pr.ey; ++ry) {
-            if (abs(ry - pr.cy) <= 1 && abs(rz - pr.cz) <= 1) continue;
-            uint32_t b, e;
-            probe_row<kX>(g, pr, cs, ce, xo.bins, n, ry, rz, b, e);
-            global_run(b, e);
-        }
-
-    const uint32_t o = kX ? xo.i2r[s] : s;
-    force_finish<kTension, kIntegrate>(S, p, xi, vi, ai, o, xo.bacc, dt, pos, vel, accel, xo.err);
+#pragma unroll
+    for (int q = 0; q < kP; ++q) {
+        if (!active[q]) continue;
+        for (int rz = pr[q].sz; rz <= pr[q].ez; ++rz)
+            for (int ry = pr[q].sy; ry <= pr[q].ey; ++ry) {
+                if (abs(ry - pr[q].cy) <= 1 && abs(rz - pr[q].cz) <= 1) continue;
+                uint32_t b, e;
+                probe_row<kX>(g, pr[q], cs, ce, xo.bins, n, ry, rz, b, e);
+                global_run(q, b, e);
+            }
+        const uint32_t o = kX ? xo.i2r[s[q]] : s[q];
+        force_finish<kTension, kIntegrate>(S[q], p, xi[q], vi[q], ai[q], o, xo.bacc, dt, pos,
+                                           vel, accel, xo.err);
+    }
 }
 
 template <bool kTension, bool kIntegrate, bool kBatched, int kTile, int kCap, class TList,
-          bool kX, int kB>
+          bool kX, int kB, int kP>
 __global__ __launch_bounds__(kB)
-__attribute__((amdgpu_waves_per_eu(force_waves(kBatched, kTension, kIntegrate))))
+__attribute__((amdgpu_waves_per_eu(force_waves(kBatched, kTension, kIntegrate, kP))))
 void k_force(const float4* __restrict__ tpos,
                                                   const float4* __restrict__ tvel,
                                                   const float2* __restrict__ aux,
@@ -669,9 +730,9 @@ void k_force(const float4* __restrict__ tpos,
     __shared__ uint32_t red[2 * (kB / 64)];
     const uint32_t n = *n_dev;
     for (uint32_t vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        const uint32_t block0 = xcd_local_block(vb, nvb) * kB;
+        const uint32_t block0 = xcd_local_block(vb, nvb) * (kB * kP);
         if (block0 < n)  // block-uniform
-            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList, kX, kB>(
+            force_block<kTension, kIntegrate, kBatched, kTile, kCap, TList, kX, kB, kP>(
                 block0, n, tp, tv, list, red, tpos, tvel, aux, cs, ce, p_in, g_in, dt, pos, vel,
                 accel, scenes, xo);
     }
@@ -771,26 +832,29 @@ template <bool kKahan, bool kFinalize, bool kBatched, bool kX = kFinalize, class
 void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
     const uint32_t bv = kFinalize ? ((tune >> 16) & 0x3u) : 0u;
+    const uint32_t pv = (kFinalize && !kBatched) ? ((tune >> 20) & 0x3u) : 0u;
     const uint32_t persist = kFinalize ? ((tune >> 8) & 0xFu) : 0u;
-#define SPH_DENSITY_VARIANT(TILE, B)                                                            \
+#define SPH_DENSITY_VARIANT(TILE, B, P)                                                         \
     do {                                                                                        \
-        const LaunchShape sh = launch_shape(cap_n, persist, B);                                 \
-        hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kX, B>), sh.grid,     \
+        const LaunchShape sh = launch_shape(cap_n, persist, (B) * (P));                         \
+        hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kX, B, P>), sh.grid,  \
                            dim3(B), 0, st, args..., sh.nvb);                                    \
     } while (0)
     if constexpr (kFinalize) {
-        if (bv == 1) { SPH_DENSITY_VARIANT(kDensityTile, 128); return; }
-        if (bv == 2) { SPH_DENSITY_VARIANT(kDensityTile, 512); return; }
+        if constexpr (!kBatched)
+            if (pv == 1) { SPH_DENSITY_VARIANT(kDensityTile, kBlock, 2); return; }
+        if (bv == 1) { SPH_DENSITY_VARIANT(kDensityTile, 128, 1); return; }
+        if (bv == 2) { SPH_DENSITY_VARIANT(kDensityTile, 512, 1); return; }
         switch (v) {
-            case 1: SPH_DENSITY_VARIANT(512, kBlock); return;
-            case 2: SPH_DENSITY_VARIANT(1536, kBlock); return;
-            case 3: SPH_DENSITY_VARIANT(2048, kBlock); return;
-            case 4: SPH_DENSITY_VARIANT(768, kBlock); return;
-            case 5: SPH_DENSITY_VARIANT(1024, kBlock); return;
+            case 1: SPH_DENSITY_VARIANT(512, kBlock, 1); return;
+            case 2: SPH_DENSITY_VARIANT(1536, kBlock, 1); return;
+            case 3: SPH_DENSITY_VARIANT(2048, kBlock, 1); return;
+            case 4: SPH_DENSITY_VARIANT(768, kBlock, 1); return;
+            case 5: SPH_DENSITY_VARIANT(1024, kBlock, 1); return;
             default: break;
         }
     }
-    SPH_DENSITY_VARIANT(kDensityTile, kBlock);
+    SPH_DENSITY_VARIANT(kDensityTile, kBlock, 1);
 #undef SPH_DENSITY_VARIANT
 }
 
@@ -799,30 +863,33 @@ void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
     constexpr bool kTunable = kIntegrate && !kTension;
     const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
     const uint32_t bv = kTunable ? ((tune >> 18) & 0x3u) : 0u;
+    const uint32_t pv = (kTunable && !kBatched) ? ((tune >> 22) & 0x3u) : 0u;
     const uint32_t persist = kTunable ? ((tune >> 12) & 0xFu) : 0u;
-#define SPH_FORCE_VARIANT(TILE, CAP, TLIST, B)                                                  \
+#define SPH_FORCE_VARIANT(TILE, CAP, TLIST, B, P)                                               \
     do {                                                                                        \
-        const LaunchShape sh = launch_shape(cap_n, persist, B);                                 \
-        hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kX, B>),  \
+        const LaunchShape sh = launch_shape(cap_n, persist, (B) * (P));                         \
+        hipLaunchKernelGGL((k_force<kTension, kIntegrate, kBatched, TILE, CAP, TLIST, kX, B, P>), \
                            sh.grid, dim3(B), 0, st, args..., sh.nvb);                           \
     } while (0)
     if constexpr (kTunable) {
-        if (bv == 1) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, 128); return; }
-        if (bv == 2) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, 512); return; }
+        if constexpr (!kBatched)
+            if (pv == 1) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, kBlock, 2); return; }
+        if (bv == 1) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, 128, 1); return; }
+        if (bv == 2) { SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, 512, 1); return; }
         switch (v) {
-            case 1: SPH_FORCE_VARIANT(768, 16, uint16_t, kBlock); return;
-            case 2: SPH_FORCE_VARIANT(768, 24, uint16_t, kBlock); return;
-            case 3: SPH_FORCE_VARIANT(768, 8, uint32_t, kBlock); return;
-            case 4: SPH_FORCE_VARIANT(512, 16, uint16_t, kBlock); return;
-            case 5: SPH_FORCE_VARIANT(1024, 16, uint16_t, kBlock); return;
-            case 6: SPH_FORCE_VARIANT(512, 8, uint32_t, kBlock); return;
-            case 7: SPH_FORCE_VARIANT(640, 24, uint16_t, kBlock); return;
-            case 8: SPH_FORCE_VARIANT(1024, 8, uint32_t, kBlock); return;
-            case 9: SPH_FORCE_VARIANT(512, 24, uint16_t, kBlock); return;
+            case 1: SPH_FORCE_VARIANT(768, 16, uint16_t, kBlock, 1); return;
+            case 2: SPH_FORCE_VARIANT(768, 24, uint16_t, kBlock, 1); return;
+            case 3: SPH_FORCE_VARIANT(768, 8, uint32_t, kBlock, 1); return;
+            case 4: SPH_FORCE_VARIANT(512, 16, uint16_t, kBlock, 1); return;
+            case 5: SPH_FORCE_VARIANT(1024, 16, uint16_t, kBlock, 1); return;
+            case 6: SPH_FORCE_VARIANT(512, 8, uint32_t, kBlock, 1); return;
+            case 7: SPH_FORCE_VARIANT(640, 24, uint16_t, kBlock, 1); return;
+            case 8: SPH_FORCE_VARIANT(1024, 8, uint32_t, kBlock, 1); return;
+            case 9: SPH_FORCE_VARIANT(512, 24, uint16_t, kBlock, 1); return;
             default: break;
         }
     }
-    SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, kBlock);
+    SPH_FORCE_VARIANT(kForceTile, kListCap, uint16_t, kBlock, 1);
 #undef SPH_FORCE_VARIANT
 }
 
@@ -1151,12 +1218,15 @@ int launch_batch_pass(int pass, const StepParams& cap, const StepBuffers& b,
 
 namespace sph {
 
-bool tune_word_valid(uint32_t tune) {
+bool tune_word_valid(uint32_t tune, bool batched) {
     const uint32_t dtile = tune & 0xFu, fvar = (tune >> 4) & 0xFu;
     const uint32_t dblock = (tune >> 16) & 0x3u, fblock = (tune >> 18) & 0x3u;
-    return (tune >> 20) == 0u && dtile <= 5u && fvar <= 9u && ((tune >> 8) & 0xFu) <= 4u &&
-           ((tune >> 12) & 0xFu) <= 4u && dblock <= 2u && fblock <= 2u &&
-           (dblock == 0u || dtile == 0u) && (fblock == 0u || fvar == 0u);
+    const uint32_t dppl = (tune >> 20) & 0x3u, fppl = (tune >> 22) & 0x3u;
+    return (tune >> 24) == 0u && dtile <= 5u && fvar <= 9u && ((tune >> 8) & 0xFu) <= 4u &&
+           ((tune >> 12) & 0xFu) <= 4u && dblock <= 2u && fblock <= 2u && dppl <= 1u &&
+           fppl <= 1u && (dblock == 0u || dtile == 0u) && (fblock == 0u || fvar == 0u) &&
+           (dppl == 0u || (dtile == 0u && dblock == 0u)) &&
+           (fppl == 0u || (fvar == 0u && fblock == 0u)) && (!batched || (dppl | fppl) == 0u);
 }
 
 int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams* scenes,
@@ -1207,7 +1277,8 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
         return SPH_OK;
     };
     // Density: tile first (256-thread blocks), then the 128- and 512-thread
-    // blocks with the default tile, then the grid shape for the best of them.
+    // blocks and two particles per lane with the default tile, then the grid
+    // shape for the best of them.
     float d_def = 0.f, d_best = 0.f, ms = 0.f;
     uint32_t d_shape = 0, d_grid = 0;  // d_shape: tile bits, or block bits << 16
     for (uint32_t v = 0; v <= 5 && rc == SPH_OK; ++v) {
@@ -1219,6 +1290,10 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
         rc = time_pass(kPassDensity, bv << 16, ms);
         if (ms < d_best) { d_best = ms; d_shape = bv << 16; }
     }
+    if (!scenes && rc == SPH_OK) {  // two particles per lane (single scene)
+        rc = time_pass(kPassDensity, 1u << 20, ms);
+        if (ms < d_best) { d_best = ms; d_shape = 1u << 20; }
+    }
     for (uint32_t gsh = 1; gsh <= 4 && rc == SPH_OK; ++gsh) {
         rc = time_pass(kPassDensity, d_shape | (gsh << 8), ms);
         if (ms < d_best) { d_best = ms; d_grid = gsh; }
@@ -1226,8 +1301,9 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
     if (rc != SPH_OK) return fail(rc);
     const uint32_t dword = d_shape | (d_grid << 8);
     rc = run(kPassDensity, dword);  // aux for the force pass
-    // Force: variant first, then the 128- and 512-thread blocks with the
-    // default variant, then the grid shape (tension runs the default).
+    // Force: variant first, then the 128- and 512-thread blocks and two
+    // particles per lane with the default variant, then the grid shape
+    // (tension runs the default).
     float f_def = 0.f, f_best = 0.f;
     uint32_t f_shape = 0, f_grid = 0;  // variant bits << 4, or block bits << 18
     for (uint32_t v = 0; v <= (tension ? 0u : 9u) && rc == SPH_OK; ++v) {
@@ -1238,6 +1314,10 @@ int tune_neighbour_passes(StepParams p, const StepBuffers& b, const SceneParams*
     for (uint32_t bv = 1; bv <= (tension ? 0u : 2u) && rc == SPH_OK; ++bv) {
         rc = time_pass(kPassForce, dword | (bv << 18), ms);
         if (ms < f_best) { f_best = ms; f_shape = bv << 18; }
+    }
+    if (!scenes && !tension && rc == SPH_OK) {  // two particles per lane (single scene)
+        rc = time_pass(kPassForce, dword | (1u << 22), ms);
+        if (ms < f_best) { f_best = ms; f_shape = 1u << 22; }
     }
     for (uint32_t gsh = 1; gsh <= (tension ? 0u : 4u) && rc == SPH_OK; ++gsh) {
         rc = time_pass(kPassForce, dword | f_shape | (gsh << 12), ms);

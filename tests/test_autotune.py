@@ -16,7 +16,7 @@ from sph2s_final_amd import (FIELD_DENSITY, FIELD_POSITION, FIELD_VELOCITY, Batc
 WORDS = [0x0000, 0x0001, 0x0002, 0x0003, 0x0004, 0x0005, 0x0010, 0x0020, 0x0030, 0x0040,
          0x0050, 0x0060, 0x0070, 0x0080, 0x0090, 0x1100, 0x2200, 0x3300, 0x4400, 0x4452,
          0x1231, 0x2393, 0x4485, 0x10000, 0x20000, 0x40000, 0x80000, 0x50000, 0xA0000,
-         0x91100, 0x64400]
+         0x91100, 0x64400, 0x100000, 0x400000, 0x500000, 0x501100, 0x424400]
 
 
 # ---------------------------------------------------------------- CPU
@@ -25,7 +25,8 @@ def test_describe_tune_words():
     assert d == {"density_tile": 1280, "force_tile": 640, "force_list_cap": 16,
                  "force_list_bits": 16, "density_blocks_per_cu": None,
                  "force_blocks_per_cu": None, "density_block_threads": 256,
-                 "force_block_threads": 256}
+                 "force_block_threads": 256, "density_particles_per_lane": 1,
+                 "force_particles_per_lane": 1}
     d = autotune.describe(0x4432)
     assert d["density_tile"] == 1536
     assert (d["force_tile"], d["force_list_cap"], d["force_list_bits"]) == (768, 8, 32)
@@ -33,6 +34,9 @@ def test_describe_tune_words():
     d = autotune.describe(0x91100)
     assert (d["density_block_threads"], d["force_block_threads"]) == (128, 512)
     assert d["density_blocks_per_cu"] == 1 and d["force_blocks_per_cu"] == 1
+    d = autotune.describe(0x501100)
+    assert (d["density_particles_per_lane"], d["force_particles_per_lane"]) == (2, 2)
+    assert d["density_block_threads"] == 256 and d["density_blocks_per_cu"] == 1
 
 
 class _FakeSolver:
@@ -128,7 +132,18 @@ def test_batch_autotune_and_invalid_words():
         word, ms = b.autotune(bs.dt, reps=2)
         assert b.get_tune() == word and len(ms) == 4
         for bad in (0x6, 0xA0, 0x500, 0x5000, 0x30000, 0xC0000, 0x10001, 0x40010,
-                    0x100000):
+                    0x100000, 0x400000, 0x1000000):
             with pytest.raises(SPHError):
                 b.set_tune(bad)
         assert b.get_tune() == word
+
+
+@pytest.mark.gpu
+def test_single_solver_refuses_invalid_words():
+    sc = scenes.dam_break((12, 12, 12), seed=6)
+    with Solver(ParticleData(sc.n, sc.positions, sc.velocities), None, sc.config) as s:
+        s.set_tune(0x500000)  # two particles per lane in both passes
+        for bad in (0x200000, 0x800000, 0x100001, 0x110000, 0x400010, 0x440000, 0x1000000):
+            with pytest.raises(SPHError):
+                s.set_tune(bad)
+        assert s.get_tune() == 0x500000
