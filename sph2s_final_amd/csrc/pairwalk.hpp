@@ -1,5 +1,6 @@
 // pairwalk.hpp -- block-cooperative walk over a particle's reference stencil
-// with each grid row's candidates staged in LDS (used by the adjoint passes).
+// with each grid row's candidates staged in LDS (used by the adjoint passes,
+// the XSPH integrator and the blended two-scale passes).
 //
 // The nine grid rows (cy + dy, cz + dz), dy, dz in {-1, 0, 1}, are block-wide
 // phases: each lane finds its candidate run in the row, the block takes the
