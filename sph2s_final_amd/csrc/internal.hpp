@@ -177,6 +177,9 @@ int launch_step(const StepParams& p, const StepBuffers& b, float dt, uint32_t ke
 struct GridView;
 int launch_density_own_x(const StepParams& p, const StepBuffers& b, hipStream_t stream);
 int launch_gather_aux(const StepParams& p, const StepBuffers& b, hipStream_t stream);
+// dst[s] = src[i2r[s]] for the live slots s < *n_dev of a capacity of cap.
+int launch_gather_x(const float2* src, const uint32_t* i2r, const uint32_t* n_dev, uint32_t cap,
+                    float2* dst, hipStream_t stream);
 int launch_force_own_x(const StepParams& p, const GridView& g, const StepBuffers& b,
                        float4* accel_own, hipStream_t stream);
 int launch_pack_state(const float* pos_xyz_dev, const float* vel_xyz_dev, uint32_t n,

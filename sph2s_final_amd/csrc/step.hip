@@ -1061,9 +1061,14 @@ __global__ __launch_bounds__(kBlock) void k_gather_aux(const float2* __restrict_
 }
 
 int launch_gather_aux(const StepParams& p, const StepBuffers& b, hipStream_t stream) {
-    if (p.n == 0) return SPH_OK;
-    hipLaunchKernelGGL(k_gather_aux, blocks_for(p.n), dim3(kBlock), 0, stream, b.aux, b.i2r,
-                       b.n_dev, b.iaux);
+    return launch_gather_x(b.aux, b.i2r, b.n_dev, p.n, b.iaux, stream);
+}
+
+int launch_gather_x(const float2* src, const uint32_t* i2r, const uint32_t* n_dev, uint32_t cap,
+                    float2* dst, hipStream_t stream) {
+    if (cap == 0) return SPH_OK;
+    hipLaunchKernelGGL(k_gather_aux, blocks_for(cap), dim3(kBlock), 0, stream, src, i2r, n_dev,
+                       dst);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }

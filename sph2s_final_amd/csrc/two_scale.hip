@@ -554,8 +554,9 @@ constexpr int kBlendForceTile = 512;
 // rho_i = m_A sum_{A, r < h_A} lambda_j W_hA + m_B sum_{B, r < h0} lambda_j
 // 1/2 (W_h0 + W_h1), then A_i, B_i.
 __global__ __launch_bounds__(kBlock) void k_blend_density(
-    const float4* __restrict__ tpos_a, const float2* __restrict__ tbl_a,
+    const float4* __restrict__ ipos_a, const float2* __restrict__ ibl_a,
     const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
+    const uint4* __restrict__ bins_a, const uint32_t* __restrict__ i2r_a,
     const uint32_t* __restrict__ n_a, GridView ga, StepParams pa, CrossParams c,
     float* __restrict__ rho_a, float2* __restrict__ aux_a) {
     __shared__ float4 tile[kBlendDensityTile];
@@ -563,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
     const uint32_t na = *n_a;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool active = s < na;
-    const float4 xi = active ? tpos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 xi = active ? ipos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     const Probe pr = make_probe(ga, xi.x, xi.y, xi.z);
     float own = 0.0f;
     auto pair = [&](const float4& xj, float lj) {
@@ -574,11 +575,11 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
             own += lj * (a * a * a);
         }
     };
-    walk_stencil_staged<kBlock, kBlendDensityTile>(
+    walk_stencil_staged<kBlock, kBlendDensityTile, true>(
         ga, pr, cs_a, ce_a, na, active, red,
         [&](uint32_t k, uint32_t j) {
-            const float4 x = tpos_a[j];
-            tile[k] = make_float4(x.x, x.y, x.z, tbl_a[j].x);
+            const float4 x = ipos_a[j];
+            tile[k] = make_float4(x.x, x.y, x.z, ibl_a[j].x);
         },
         [&](uint32_t t0, uint32_t t1, uint32_t) {
             for (uint32_t t = t0; t < t1; ++t) {
@@ -588,13 +589,15 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
             }
         },
         [&](uint32_t b, uint32_t e) {
-            for (uint32_t j = b; j < e; ++j) pair(tpos_a[j], tbl_a[j].x);
-        });
+            for (uint32_t j = b; j < e; ++j) pair(ipos_a[j], ibl_a[j].x);
+        },
+        bins_a);
     if (!active) return;
     const float r = pa.c6m * own;
-    rho_a[s] = r;
+    const uint32_t o = i2r_a[s];
+    rho_a[o] = r;
     const float inv = __builtin_amdgcn_rcpf(r);
-    aux_a[s] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
+    aux_a[o] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
 }
 
 // Blended acceleration (own + complementary, lambda-weighted; gravity), the
@@ -602,9 +605,10 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
 // integrateD's blend update; results at sorted slot s.
 template <bool kTension>
 __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
-    const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
-    const float2* __restrict__ aux_a, const float2* __restrict__ tbl_a,
+    const float4* __restrict__ ipos_a, const float4* __restrict__ ivel_a,
+    const float2* __restrict__ iaux_a, const float2* __restrict__ ibl_a,
     const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
+    const uint4* __restrict__ bins_a, const uint32_t* __restrict__ i2r_a,
     const uint32_t* __restrict__ n_a, GridView ga, StepParams pa, CrossParams c,
     const float4* __restrict__ xacc_a, const float4* __restrict__ bacc, float dt, float inc,
     float4* __restrict__ pos_a, float4* __restrict__ vel_a, float2* __restrict__ nbl_a,
@@ -616,9 +620,9 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     const uint32_t na = *n_a;
     const uint32_t s = xcd_local_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool active = s < na;
-    const float4 xi = active ? tpos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 vi = active ? tvel_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float2 ai = active ? aux_a[s] : make_float2(0.f, 1.f);
+    const float4 xi = active ? ipos_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi = active ? ivel_a[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float2 ai = active ? iaux_a[s] : make_float2(0.f, 1.f);
     const Probe pr = make_probe(ga, xi.x, xi.y, xi.z);
     float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f,
           Tz = 0.f;
@@ -646,14 +650,14 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
             Tz += w * dz;
         }
     };
-    walk_stencil_staged<kBlock, kBlendForceTile>(
+    walk_stencil_staged<kBlock, kBlendForceTile, true>(
         ga, pr, cs_a, ce_a, na, active, red,
         [&](uint32_t k, uint32_t j) {
-            const float4 x = tpos_a[j], v = tvel_a[j];
-            const float2 a = aux_a[j];
+            const float4 x = ipos_a[j], v = ivel_a[j];
+            const float2 a = iaux_a[j];
             tx[k] = make_float4(x.x, x.y, x.z, a.x);
             tv[k] = make_float4(v.x, v.y, v.z, a.y);
-            tl[k] = tbl_a[j].x;
+            tl[k] = ibl_a[j].x;
         },
         [&](uint32_t t0, uint32_t t1, uint32_t) {
             for (uint32_t t = t0; t < t1; ++t) {
@@ -664,13 +668,15 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
         },
         [&](uint32_t b, uint32_t e) {
             for (uint32_t j = b; j < e; ++j) {
-                const float4 x = tpos_a[j], v = tvel_a[j];
-                const float2 a = aux_a[j];
+                const float4 x = ipos_a[j], v = ivel_a[j];
+                const float2 a = iaux_a[j];
                 pair(make_float4(x.x, x.y, x.z, a.x), make_float4(v.x, v.y, v.z, a.y),
-                     tbl_a[j].x);
+                     ibl_a[j].x);
             }
-        });
+        },
+        bins_a);
     if (!active) return;
+    const uint32_t o = i2r_a[s];  // the target's sorted slot
     const float vc = pa.visc * ai.y;
     float ax = pa.c45m * Px + vc * Vx + pa.gx;
     float ay = pa.c45m * Py + vc * Vy + pa.gy;
@@ -683,12 +689,12 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     }
     // The complementary sums (Solver.cu:394-499, weighted by lambda_j) of
     // the step's walk; -0 for the particles it does not list.
-    const float4 xa = xacc_a[s];
+    const float4 xa = xacc_a[o];
     ax += xa.x;
     ay += xa.y;
     az += xa.z;
     if (bacc) {  // boundary particles (Solver.cu:1046-1048)
-        const float4 b = bacc[s];
+        const float4 b = bacc[o];
         ax += b.x;
         ay += b.y;
         az += b.z;
@@ -705,15 +711,15 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
         if (z < c.loz) { z = c.loz; vz = -e * vz; }
         if (z > c.hiz) { z = c.hiz; vz = -e * vz; }
     }
-    pos_a[s] = make_float4(x, y, z, xi.w);
-    vel_a[s] = make_float4(vx, vy, vz, vi.w);
+    pos_a[o] = make_float4(x, y, z, xi.w);
+    vel_a[o] = make_float4(vx, vy, vz, vi.w);
     if (c.err && !__builtin_isfinite(((x + y) + (z + vx)) + (vy + vz)))
         atomicOr(c.err, kDevErrNonFinite);
-    if (acc_out) acc_out[s] = make_float4(ax, ay, az, 0.0f);
+    if (acc_out) acc_out[o] = make_float4(ax, ay, az, 0.0f);
     // Blend update (Solver.cu:1242-1257): lambda += c[mode] inc with
     // c = {0, -1, +1, 0}; a fading-in particle reaching 1 settles.  Removal
     // (fading out, lambda <= 0) happens in the compaction.
-    const float2 bl = tbl_a[s];
+    const float2 bl = ibl_a[s];
     uint32_t st = __float_as_uint(bl.y);
     const uint32_t mode = st & 3u;
     float lam = bl.x;
@@ -723,7 +729,7 @@ __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
         st &= 4u;
         lam = 1.0f;
     }
-    nbl_a[s] = make_float2(lam, __uint_as_float(st));
+    nbl_a[o] = make_float2(lam, __uint_as_float(st));
 }
 
 __global__ void k_gather_blend(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ n_dev,
@@ -1175,6 +1181,7 @@ struct TwoScaleSet {
     float* tcolor = nullptr;
     float* color = nullptr;
     uint32_t* walk = nullptr;     // the step's particles with a complementary walk (sorted slots)
+    float2* iblend = nullptr;     // blended coupling: the step's blend words in the internal order
 };
 
 struct sph_two_scale {
@@ -1365,9 +1372,9 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         TwoScaleSet& B = t->set[1 - r];
         const hipStream_t s = sst[r];
         const CrossParams c = cross_params(t, r);
-        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
-                           A.tblend, A.buf.cell_start, A.buf.cell_end, A.buf.n_dev, G[r], P[r],
-                           c, A.buf.rho, A.buf.aux);
+        hipLaunchKernelGGL(k_blend_density, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.ipos,
+                           A.iblend, A.buf.cell_start, A.buf.cell_end, A.buf.xbins, A.buf.i2r,
+                           A.buf.n_dev, G[r], P[r], c, A.buf.rho, A.buf.aux);
         hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], c, A.buf.rho, A.buf.aux,
                            A.accel, walk_counts(t, r), A.walk, walk_segment_cap(A.cap));
@@ -1400,16 +1407,20 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         c.vmax = velocity_limit(t, r, dt);
         float4* acc = accel_out ? A.buf.accel : nullptr;
         const float inc = t->opt.blend_increment;
+        rc = launch_gather_aux(P[r], A.buf, s);  // the final A, B in the internal order
+        if (rc != SPH_OK) return rc;
         if (P[r].tens != 0.0f)
             hipLaunchKernelGGL(k_blend_force_integrate<true>, blocks_for(A.cap), dim3(kBlock), 0,
-                               s, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
-                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], c, A.accel,
-                               wall_acc(t, r), dt, inc, A.buf.pos, A.buf.vel, A.nblend, acc);
+                               s, A.buf.ipos, A.buf.ivel, A.buf.iaux, A.iblend, A.buf.cell_start,
+                               A.buf.cell_end, A.buf.xbins, A.buf.i2r, A.buf.n_dev, G[r], P[r], c,
+                               A.accel, wall_acc(t, r), dt, inc, A.buf.pos, A.buf.vel, A.nblend,
+                               acc);
         else
             hipLaunchKernelGGL(k_blend_force_integrate<false>, blocks_for(A.cap), dim3(kBlock), 0,
-                               s, A.buf.tpos, A.buf.tvel, A.buf.aux, A.tblend, A.buf.cell_start,
-                               A.buf.cell_end, A.buf.n_dev, G[r], P[r], c, A.accel,
-                               wall_acc(t, r), dt, inc, A.buf.pos, A.buf.vel, A.nblend, acc);
+                               s, A.buf.ipos, A.buf.ivel, A.buf.iaux, A.iblend, A.buf.cell_start,
+                               A.buf.cell_end, A.buf.xbins, A.buf.i2r, A.buf.n_dev, G[r], P[r], c,
+                               A.accel, wall_acc(t, r), dt, inc, A.buf.pos, A.buf.vel, A.nblend,
+                               acc);
     }
     SPH_HIP(hipGetLastError());
     rc = wait_for(t, 5, sl, st);  // join
@@ -1456,9 +1467,12 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         if (rc == SPH_OK) rc = launch_pass(kPassSort, P[r], b, dt, S.key_bits, false, s);
         if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, dt, S.key_bits, false, s);
         if (rc != SPH_OK) return rc;
-        if (blended(t))
+        if (blended(t)) {
             hipLaunchKernelGGL(k_gather_blend, blocks_for(S.cap), dim3(kBlock), 0, s, S.buf.perm,
                                S.buf.n_dev, S.state_blend, S.tblend);
+            rc = launch_gather_x(S.tblend, S.buf.i2r, S.buf.n_dev, S.cap, S.iblend, s);
+            if (rc != SPH_OK) return rc;
+        }
         uint32_t* occ = (r == 1) ? t->occ : t->occ + C0;
         hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 1024u)),
                            dim3(kBlock), 0, s, occ, C0);
@@ -1670,6 +1684,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         add(8 * N);                                // iaux
         add(4 * N);                                // i2r
         add(16 * C);                               // xbins
+        add(8 * N);                                // iblend
     }
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
@@ -1735,6 +1750,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         S.buf.iaux = static_cast<float2*>(take());
         S.buf.i2r = static_cast<uint32_t*>(take());
         S.buf.xbins = static_cast<uint4*>(take());
+        S.iblend = static_cast<float2*>(take());
         S.buf.keep_sorted = true;
     }
     t->grp_pos = static_cast<float4*>(take());
