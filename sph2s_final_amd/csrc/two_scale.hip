@@ -199,7 +199,7 @@ __device__ __forceinline__ bool other_nearby(const Nearby& nb, const GridView& g
 // lanes' partial sums are reduced in a fixed order (so the result does not
 // depend on which wave takes which particle or on the list's order).
 constexpr int kWalkMaxRows = 64;
-constexpr uint32_t kWalkBlocks = 1024;  // 4096 waves share a step's walk list
+constexpr uint32_t kWalkBlocks = 4096;  // 16384 waves share a step's walk list
 // The walk list is kept in 64 segments, each with its own counter 64 bytes
 // from the next (appends from one block go to segment blockIdx % 64), so the
 // appends do not all contend for one address.
