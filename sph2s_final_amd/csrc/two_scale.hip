@@ -1304,10 +1304,16 @@ float velocity_limit(const sph_two_scale* t, int target, float dt) {
 }
 
 // The integrated states -> split/merge -> compacted states and new counts.
-int launch_split_merge(sph_two_scale* t) {
+// The split/merge stage in two parts.  The marks: the high set's merge marks
+// and flags and per-block counts on hs, the low set's split flags and
+// per-block counts on ls.  Under the fixed coupling the two sets' marks are
+// independent (hs and ls may be the two branches of a step); the blended
+// coupling's split marks read the merge marks' grp_live bits, so there both
+// must be one stream.  The commit (on t->stream, after both): the scan of the
+// counts, the scatters and the new counts.
+int launch_split_merge_marks(sph_two_scale* t, hipStream_t hs, hipStream_t ls) {
     TwoScaleSet& L = t->set[0];
     TwoScaleSet& H = t->set[1];
-    hipStream_t st = t->stream;
     const uint32_t nbl = blocks_for(L.cap).x, nbh = blocks_for(H.cap).x;
     // Blended coupling: the step's updated blend words; fixed: none (NULL).
     const bool bl = blended(t);
@@ -1318,18 +1324,36 @@ int launch_split_merge(sph_two_scale* t) {
     sp.merge = t->opt.merge;
     sp.ref_offsets = t->opt.split_offsets == SPH2_SPLIT_REFERENCE ? 1u : 0u;
     sp.m0 = L.cfg.mass;
+    if (bl && hs != ls) return set_error(SPH_ERR_INVALID_ARGUMENT, "blended marks need one stream");
     // High marks first: a split waits for its group's live children.
-    hipLaunchKernelGGL(k_merge_mark, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel, bl_h,
+    hipLaunchKernelGGL(k_merge_mark, dim3(nbh), dim3(kBlock), 0, hs, H.buf.pos, H.buf.vel, bl_h,
                        H.buf.n_dev, t->region, sp.merge, t->grp_pos, t->grp_vel, t->grp_cnt,
                        live, t->num_groups);
-    hipLaunchKernelGGL(k_split_mark, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, bl_l, L.buf.n_dev,
+    hipLaunchKernelGGL(k_split_mark, dim3(nbl), dim3(kBlock), 0, ls, L.buf.pos, bl_l, L.buf.n_dev,
                        t->region, live, t->num_groups, L.flag);
-    hipLaunchKernelGGL(k_merge_flags, dim3(nbh), dim3(kBlock), 0, st, H.buf.vel, bl_h,
+    hipLaunchKernelGGL(k_merge_flags, dim3(nbh), dim3(kBlock), 0, hs, H.buf.vel, bl_h,
                        H.buf.n_dev, t->grp_cnt, live, t->num_groups, H.flag);
-    hipLaunchKernelGGL(k_block_counts, dim3(nbl), dim3(kBlock), 0, st, L.flag, L.buf.n_dev,
+    hipLaunchKernelGGL(k_block_counts, dim3(nbl), dim3(kBlock), 0, ls, L.flag, L.buf.n_dev,
                        L.counts, nbl);
-    hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, st, H.flag, H.buf.n_dev,
+    hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, hs, H.flag, H.buf.n_dev,
                        H.counts, nbh);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+}
+
+int launch_split_merge_commit(sph_two_scale* t) {
+    TwoScaleSet& L = t->set[0];
+    TwoScaleSet& H = t->set[1];
+    hipStream_t st = t->stream;
+    const uint32_t nbl = blocks_for(L.cap).x, nbh = blocks_for(H.cap).x;
+    const bool bl = blended(t);
+    const float2* bl_l = bl ? L.nblend : nullptr;
+    const float2* bl_h = bl ? H.nblend : nullptr;
+    uint32_t* live = bl ? t->grp_live : nullptr;
+    SplitParams sp{};
+    sp.merge = t->opt.merge;
+    sp.ref_offsets = t->opt.split_offsets == SPH2_SPLIT_REFERENCE ? 1u : 0u;
+    sp.m0 = L.cfg.mass;
     hipLaunchKernelGGL(k_scan_counts, dim3(4), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
                        t->tot);
     const bool col = (t->debug & SPH_DEBUG_COLOR) != 0u;
@@ -1347,6 +1371,12 @@ int launch_split_merge(sph_two_scale* t) {
                        H.buf.n_dev, L.cap, H.cap, t->error_word);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
+}
+
+// The whole split/merge stage on t->stream.
+int launch_split_merge(sph_two_scale* t) {
+    const int rc = launch_split_merge_marks(t, t->stream, t->stream);
+    return rc == SPH_OK ? launch_split_merge_commit(t) : rc;
 }
 
 // Blended coupling: lambda-weighted densities of both sets, then the
@@ -1546,10 +1576,12 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
                            A.buf.tvel, A.accel, A.buf.n_dev, c, wall_acc(t, r), dt, A.buf.pos,
                            A.buf.vel, accel_out ? A.buf.accel : nullptr);
     }
-    rc = wait_for(t, 5, sl, st);  // join
+    // Each set's split/merge marks at the end of its own branch, then the join
+    // and the commit.
+    rc = launch_split_merge_marks(t, st, sl);
+    if (rc == SPH_OK) rc = wait_for(t, 5, sl, st);  // join
     if (rc != SPH_OK) return rc;
-    SPH_HIP(hipGetLastError());
-    return launch_split_merge(t);
+    return launch_split_merge_commit(t);
 }
 
 // Both sets' renderer buffers from the compacted state: AoS positions, the
