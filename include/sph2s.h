@@ -406,6 +406,19 @@ typedef struct sph2_options {
 } sph2_options;
 int sph2_set_options(sph_two_scale* t, const sph2_options* options);
 int sph2_get_options(const sph_two_scale* t, sph2_options* options);
+/* Launch variants of the two-scale own-set passes (no counterpart in the
+ * reference, like sph_autotune): each resolution (0 = low, 1 = high) has a
+ * tune word with the fields of sph_set_tune's, applied to its own-set density
+ * and force passes (the tension form of the force runs the default); every
+ * word gives bit-identical results.  sph2_autotune times the variants of both
+ * sets' own-set passes on the current state (reps launches each), keeps the
+ * fastest per set and returns the two words; ms[8] (may be NULL) receives,
+ * per set (low first), the default and tuned milliseconds of density and
+ * force.  The state is unchanged; intermediate fields hold the tuning pass's
+ * values until the next step. */
+int sph2_set_tune(sph_two_scale* t, int resolution, uint32_t tune);
+int sph2_get_tune(const sph_two_scale* t, int resolution, uint32_t* tune);
+int sph2_autotune(sph_two_scale* t, uint32_t reps, uint32_t tune[2], float* ms);
 /* Replace one set's blend weights and state words (blended coupling; n must
  * equal the set's current count; either array may be NULL to keep it). */
 int sph2_set_blend(sph_two_scale* t, int resolution, const float* lambda,

@@ -153,6 +153,9 @@ SIGNATURES = [
     ("sph2_set_options", c_int, [c_void_p, POINTER(Sph2Options)]),
     ("sph2_set_boundary", c_int, [c_void_p, c_void_p, c_uint32]),
     ("sph2_get_options", c_int, [c_void_p, POINTER(Sph2Options)]),
+    ("sph2_set_tune", c_int, [c_void_p, c_int, c_uint32]),
+    ("sph2_get_tune", c_int, [c_void_p, c_int, POINTER(c_uint32)]),
+    ("sph2_autotune", c_int, [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_float)]),
     ("sph2_set_blend", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_uint32]),
     ("sph2_mark_split", c_int, [c_void_p, c_void_p, c_uint32]),
     ("sph2_export_render", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

@@ -830,17 +830,20 @@ __global__ __launch_bounds__(kBlock) void k_xsph_integrate(
 
 template <bool kKahan, bool kFinalize, bool kBatched, bool kX = kFinalize, class... A>
 void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
-    const uint32_t v = kFinalize ? (tune & 0xFu) : 0u;
-    const uint32_t bv = kFinalize ? ((tune >> 16) & 0x3u) : 0u;
-    const uint32_t pv = (kFinalize && !kBatched) ? ((tune >> 20) & 0x3u) : 0u;
-    const uint32_t persist = kFinalize ? ((tune >> 8) & 0xFu) : 0u;
+    // The step's density pass and the two-scale own-set pass (x-order, no
+    // finalisation) take the tune word.
+    constexpr bool kTunable = kFinalize || kX;
+    const uint32_t v = kTunable ? (tune & 0xFu) : 0u;
+    const uint32_t bv = kTunable ? ((tune >> 16) & 0x3u) : 0u;
+    const uint32_t pv = (kTunable && !kBatched) ? ((tune >> 20) & 0x3u) : 0u;
+    const uint32_t persist = kTunable ? ((tune >> 8) & 0xFu) : 0u;
 #define SPH_DENSITY_VARIANT(TILE, B, P)                                                         \
     do {                                                                                        \
         const LaunchShape sh = launch_shape(cap_n, persist, (B) * (P));                         \
         hipLaunchKernelGGL((k_density<kKahan, kFinalize, kBatched, TILE, kX, B, P>), sh.grid,  \
                            dim3(B), 0, st, args..., sh.nvb);                                    \
     } while (0)
-    if constexpr (kFinalize) {
+    if constexpr (kTunable) {
         if constexpr (!kBatched)
             if (pv == 1) { SPH_DENSITY_VARIANT(kDensityTile, kBlock, 2); return; }
         if (bv == 1) { SPH_DENSITY_VARIANT(kDensityTile, 128, 1); return; }
@@ -860,7 +863,9 @@ void launch_density(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
 
 template <bool kTension, bool kIntegrate, bool kBatched, bool kX = kIntegrate, class... A>
 void launch_force(uint32_t tune, uint32_t cap_n, hipStream_t st, A... args) {
-    constexpr bool kTunable = kIntegrate && !kTension;
+    // The step's force pass and the two-scale own-set pass (x-order, no
+    // integration) take the tune word; the tension forms run the default.
+    constexpr bool kTunable = !kTension && (kIntegrate || kX);
     const uint32_t v = kTunable ? ((tune >> 4) & 0xFu) : 0u;
     const uint32_t bv = kTunable ? ((tune >> 18) & 0x3u) : 0u;
     const uint32_t pv = (kTunable && !kBatched) ? ((tune >> 22) & 0x3u) : 0u;

@@ -1206,6 +1206,9 @@ struct sph_two_scale {
     // ev[] at the points where one set's passes read the other's results.
     hipStream_t stream2 = nullptr;
     hipEvent_t ev[6] = {};
+    // Per set: the tune word of its own-set density and force passes (the
+    // fields of the single-scale tune word; sph2_set_tune, sph2_autotune).
+    uint32_t tune[2] = {0u, 0u};
     void* arena = nullptr;
     hipGraphExec_t graph = nullptr;
     float graph_dt = 0.0f;
@@ -1458,19 +1461,25 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
     return launch_split_merge(t);
 }
 
+// Each set's step parameters (with its own-set tune word), grid view and the
+// cross-search view of its grid (searched to h0, the coupling cutoff).
+void set_views(const sph_two_scale* t, StepParams* P, GridView* G, GridView* X) {
+    for (int r = 0; r < 2; ++r) {
+        const TwoScaleSet& S = t->set[r];
+        P[r] = make_step_params(S.cfg, S.cap);
+        P[r].tune = t->tune[r];
+        G[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
+                              P[r].h);
+        X[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
+                              t->set[0].cfg.h);
+    }
+}
+
 int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
     hipStream_t st = t->stream;
     StepParams P[2];
     GridView G[2], X[2];
-    for (int r = 0; r < 2; ++r) {
-        TwoScaleSet& S = t->set[r];
-        P[r] = make_step_params(S.cfg, S.cap);
-        G[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
-                              P[r].h);
-        // Cross search: the other set's grid, searched to h0 (the coupling cutoff).
-        X[r] = make_grid_view(P[r].ox, P[r].oy, P[r].oz, P[r].spacing, P[r].dx, P[r].dy, P[r].dz,
-                              t->set[0].cfg.h);
-    }
+    set_views(t, P, G, X);
     // The two sets' passes run side by side: the high set's on t->stream, the
     // low set's on t->stream2.  They meet where a pass reads the other set's
     // results of this step: the occupancy dilation (both sets' marks), the
@@ -1633,9 +1642,153 @@ int capture_two_scale(sph_two_scale* t, float dt, bool accel) {
 
 size_t al(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
+// Times the launch variants of each set's own-set density and force passes
+// (tile and queue, block size, particles per lane, then the grid shape of the
+// best) on the current state and returns the fastest word per set.  It runs
+// the neighbourhood passes first; the passes it times write only the step's
+// intermediate buffers (densities, the internal-order A/B copy, the own-set
+// acceleration), which the next step recomputes, so the state is unchanged.
+// The force is timed with the A, B those buffers hold.
+int tune_own_passes(sph_two_scale* t, uint32_t reps, uint32_t* tune_out, float* ms_out) {
+    hipStream_t st = t->stream;
+    reps = std::max(reps, 1u);
+    StepParams P[2];
+    GridView G[2], X[2];
+    set_views(t, P, G, X);
+    SPH_HIP(hipStreamSynchronize(st));
+    for (int r = 1; r >= 0; --r) {
+        TwoScaleSet& S = t->set[r];
+        StepBuffers b = S.buf;
+        b.pos = S.state_pos;
+        b.vel = S.state_vel;
+        int rc = launch_pass(kPassHash, P[r], b, 0.0f, S.key_bits, false, st);
+        if (rc == SPH_OK) rc = launch_pass(kPassSort, P[r], b, 0.0f, S.key_bits, false, st);
+        if (rc == SPH_OK) rc = launch_pass(kPassReorder, P[r], b, 0.0f, S.key_bits, false, st);
+        if (rc != SPH_OK) return rc;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto done = [&](int code) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        return code;
+    };
+    if (e != hipSuccess) return done(hip_error(e, "autotune events", __FILE__, __LINE__));
+    for (int r = 0; r < 2; ++r) {
+        TwoScaleSet& S = t->set[r];
+        // Mean milliseconds of one launch with tune word w (after one untimed).
+        auto time = [&](bool force, uint32_t w, float& ms) -> int {
+            StepParams p = P[r];
+            p.tune = w;
+            auto run = [&]() {
+                return force ? launch_force_own_x(p, G[r], S.buf, S.accel, st)
+                             : launch_density_own_x(p, S.buf, st);
+            };
+            int rc = run();
+            if (rc != SPH_OK) return rc;
+            SPH_HIP(hipEventRecord(e0, st));
+            for (uint32_t i = 0; i < reps && rc == SPH_OK; ++i) rc = run();
+            if (rc != SPH_OK) return rc;
+            SPH_HIP(hipEventRecord(e1, st));
+            SPH_HIP(hipEventSynchronize(e1));
+            SPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            ms /= static_cast<float>(reps);
+            return SPH_OK;
+        };
+        uint32_t word[2] = {0u, 0u};
+        float def[2] = {0.f, 0.f}, best[2] = {0.f, 0.f};
+        const bool tension = P[r].tens != 0.0f;
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool force = pass == 1;
+            if (force && tension) {  // the tension form runs the default
+                int rc = time(true, 0u, def[1]);
+                if (rc != SPH_OK) return done(rc);
+                best[1] = def[1];
+                break;
+            }
+            // tile/queue field, block-size field, particles-per-lane field, grid field
+            const uint32_t nvar = force ? 9u : 5u;
+            const int vshift = force ? 4 : 0, bshift = force ? 18 : 16, pshift = force ? 22 : 20,
+                      gshift = force ? 12 : 8;
+            float ms = 0.f;
+            uint32_t shape = 0u, grid = 0u;
+            int rc = time(force, 0u, def[pass]);
+            best[pass] = def[pass];
+            for (uint32_t v = 1; v <= nvar && rc == SPH_OK; ++v) {
+                rc = time(force, v << vshift, ms);
+                if (ms < best[pass]) { best[pass] = ms; shape = v << vshift; }
+            }
+            for (uint32_t bv = 1; bv <= 2 && rc == SPH_OK; ++bv) {
+                rc = time(force, bv << bshift, ms);
+                if (ms < best[pass]) { best[pass] = ms; shape = bv << bshift; }
+            }
+            if (rc == SPH_OK) {
+                rc = time(force, 1u << pshift, ms);
+                if (ms < best[pass]) { best[pass] = ms; shape = 1u << pshift; }
+            }
+            for (uint32_t gsh = 1; gsh <= 4 && rc == SPH_OK; ++gsh) {
+                rc = time(force, shape | (gsh << gshift), ms);
+                if (ms < best[pass]) { best[pass] = ms; grid = gsh; }
+            }
+            if (rc != SPH_OK) return done(rc);
+            word[pass] = shape | (grid << gshift);
+            if (!force) {  // densities and the A, B copy for the force timings
+                StepParams p = P[r];
+                p.tune = word[0];
+                rc = launch_density_own_x(p, S.buf, st);
+                if (rc == SPH_OK) rc = launch_gather_aux(p, S.buf, st);
+                if (rc != SPH_OK) return done(rc);
+            }
+        }
+        tune_out[r] = word[0] | word[1];
+        if (ms_out) {
+            ms_out[4 * r + 0] = def[0];
+            ms_out[4 * r + 1] = best[0];
+            ms_out[4 * r + 2] = def[1];
+            ms_out[4 * r + 3] = best[1];
+        }
+    }
+    return done(SPH_OK);
+}
+
 }  // namespace
 
 extern "C" {
+
+int sph2_set_tune(sph_two_scale* t, int resolution, uint32_t tune) {
+    if (!t) return set_error(SPH_ERR_INVALID_ARGUMENT, "handle is null");
+    if (resolution != 0 && resolution != 1)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "resolution must be 0 (low) or 1 (high)");
+    if (!tune_word_valid(tune, false)) return set_error(SPH_ERR_INVALID_ARGUMENT, "invalid tune word");
+    if (t->tune[resolution] == tune) return SPH_OK;
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    t->tune[resolution] = tune;
+    destroy_graph2(t);
+    return SPH_OK;
+}
+
+int sph2_get_tune(const sph_two_scale* t, int resolution, uint32_t* tune) {
+    if (!t || !tune) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    if (resolution != 0 && resolution != 1)
+        return set_error(SPH_ERR_INVALID_ARGUMENT, "resolution must be 0 (low) or 1 (high)");
+    *tune = t->tune[resolution];
+    return SPH_OK;
+}
+
+int sph2_autotune(sph_two_scale* t, uint32_t reps, uint32_t tune[2], float* ms) {
+    if (!t || !tune) return set_error(SPH_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t words[2] = {0u, 0u};
+    const int rc = tune_own_passes(t, reps, words, ms);
+    if (rc != SPH_OK) return rc;
+    for (int r = 0; r < 2; ++r) {
+        const int rc2 = sph2_set_tune(t, r, words[r]);
+        if (rc2 != SPH_OK) return rc2;
+        tune[r] = words[r];
+    }
+    SPH_HIP(hipStreamSynchronize(t->stream));
+    return SPH_OK;
+}
 
 int sph2_create(const sph_config* cfg, const float region_lo[3], const float region_hi[3],
                 const float* pos, const float* vel, uint32_t n_low, uint32_t max_low,

@@ -10,7 +10,7 @@ Grid.h): ``Grid.MakeGrid``, ``SolverConfiguration.MakeConfiguration``,
 from __future__ import annotations
 
 import ctypes
-from ctypes import byref, c_uint32, c_void_p
+from ctypes import byref, c_float, c_uint32, c_void_p
 
 import numpy as np
 
@@ -439,6 +439,24 @@ class TwoScaleSolver:
         return {"coupling": int(o.coupling), "split_offsets": int(o.split_offsets),
                 "merge": bool(o.merge), "blend_increment": float(o.blend_increment),
                 "velocity_cap": float(o.velocity_cap)}
+
+    def set_tune(self, resolution: int, word: int) -> None:
+        """Tune word of one set's own-set passes (sph2_set_tune)."""
+        C.check(C.lib().sph2_set_tune(self._h, int(resolution), int(word)))
+
+    def get_tune(self, resolution: int) -> int:
+        w = c_uint32()
+        C.check(C.lib().sph2_get_tune(self._h, int(resolution), byref(w)))
+        return int(w.value)
+
+    def autotune(self, reps: int = 5):
+        """Times both sets' own-set pass variants on the current state and
+        applies the fastest (sph2_autotune).  Returns ([low word, high word],
+        [per set: density default, tuned, force default, tuned ms])."""
+        words = (c_uint32 * 2)()
+        ms = (c_float * 8)()
+        C.check(C.lib().sph2_autotune(self._h, int(reps), words, ms))
+        return [int(words[0]), int(words[1])], [float(x) for x in ms]
 
     def set_blend(self, resolution: int, lam=None, state=None) -> None:
         """Replace one set's blend weights and/or state words (blended coupling)."""

@@ -263,3 +263,64 @@ def test_two_scale_250k_full_size():
         amax = max(np.max(np.abs(o.get(r, "acc"))) for r in (0, 1))
         _compare_state(a, o, atol_x=1e-6 + sc.dt ** 2 * 1e-5 * amax,
                        atol_v=1e-6 + sc.dt * 1e-5 * amax)
+
+
+# ---------------------------------------------------------------- GPU: tune words
+TS_WORDS = [(0x0000, 0x0000), (0x0032, 0x0041), (0x10000, 0x80000), (0x100000, 0x400000),
+            (0x501100, 0x0000), (0x0000, 0x424400), (0x4485, 0x2393)]
+
+
+def _ts_run(sc, words, steps):
+    with _gpu(sc, debug=False) as s:
+        s.set_tune(0, words[0])
+        s.set_tune(1, words[1])
+        assert (s.get_tune(0), s.get_tune(1)) == tuple(words)
+        s.Advance(sc.dt, steps)
+        s.synchronize()
+        assert s.device_errors() == 0
+        return s.counts(), [s.download(r, f) for r in (0, 1)
+                            for f in (FIELD_POSITION, FIELD_VELOCITY, FIELD_ID)]
+
+
+@pytest.mark.gpu
+def test_two_scale_tune_words_are_bit_identical():
+    # The own-set passes' launch variants only change which runs are staged in
+    # LDS, when queued pairs are evaluated and which block handles which
+    # particles, so every word steps exactly like the default.
+    sc = scenes.two_scale_dam_break(16, seed=5)
+    print(f"case two_scale tune words n={sc.n}", flush=True)
+    ref_counts, ref = _ts_run(sc, TS_WORDS[0], 12)
+    for words in TS_WORDS[1:]:
+        counts, got = _ts_run(sc, words, 12)
+        assert counts == ref_counts, words
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b), words
+
+
+@pytest.mark.gpu
+def test_two_scale_autotune_keeps_state_and_results():
+    sc = scenes.two_scale_dam_break(16, seed=6)
+    print(f"case two_scale autotune n={sc.n}", flush=True)
+    with _gpu(sc, debug=False) as s, _gpu(sc, debug=False) as ref:
+        s.Advance(sc.dt, 4)
+        ref.Advance(sc.dt, 4)
+        s.synchronize()
+        before = [s.download(r, FIELD_POSITION) for r in (0, 1)]
+        words, ms = s.autotune(reps=2)
+        print(f"  words {[hex(w) for w in words]} ms {ms}", flush=True)
+        assert [s.get_tune(0), s.get_tune(1)] == words and len(ms) == 8
+        for r in (0, 1):
+            assert np.array_equal(s.download(r, FIELD_POSITION), before[r])
+        assert ms[1] <= ms[0] and ms[5] <= ms[4]
+        s.Advance(sc.dt, 6)
+        ref.Advance(sc.dt, 6)
+        s.synchronize()
+        ref.synchronize()
+        assert s.counts() == ref.counts()
+        for r in (0, 1):
+            for f in (FIELD_POSITION, FIELD_VELOCITY, FIELD_ID):
+                assert np.array_equal(s.download(r, f), ref.download(r, f))
+        for bad_res, bad_word in ((2, 0), (-1, 0), (0, 0x6), (1, 0x200000), (0, 0x1000000)):
+            with pytest.raises(Exception):
+                s.set_tune(bad_res, bad_word)
+        assert [s.get_tune(0), s.get_tune(1)] == words
