@@ -15,7 +15,11 @@
 //   * blended: the reference's own coupling -- every neighbour term weighted
 //     by the neighbour's lambda, integrateD's fade-in/fade-out state machine
 //     and velocity cap, splits that fade the parent out and the children in,
-//     and a faded merge (k_blend_density, k_blend_force_integrate).
+//     and a faded merge (own-set sums in k_blend_density and
+//     k_blend_force_integrate, complementary sums in the fixed coupling's
+//     walks with lambda weights).
+// Both couplings run the two sets' passes on two streams (graph branches),
+// joined where one set's passes read the other's results of the step.
 // Departures of the fixed coupling, all deliberate:
 //   * split: children at x +/- h1/4 on each axis (a uniform child lattice of
 //     spacing h1/2, the low-resolution lattice's h-relative spacing), mass m1,
@@ -544,15 +548,19 @@ __global__ __launch_bounds__(kBlock) void k_cross_integrate(
 // ---------------------------------------------------------------------------
 // Blended coupling (TWO_SCALE/Solver.cu:215-499, 732-1083, 1163-1289): every
 // neighbour term carries the neighbour's blend weight lambda_j.  Each set's
-// own sums walk its own grid with the block-staged stencil walk; the
-// complementary sums walk the other set's grid per particle, as in the fixed
-// coupling.  Blend state words: float2 (lambda, state bits).
+// own sums walk its own grid in the internal x-order with the block-staged
+// stencil walk (the kernels below); the complementary sums are the fixed
+// coupling's walks (one wave per listed particle) with lambda weights.  Blend
+// state words: float2 (lambda, state bits).
 
 constexpr int kBlendDensityTile = 1024;
 constexpr int kBlendForceTile = 512;
 
-// rho_i = m_A sum_{A, r < h_A} lambda_j W_hA + m_B sum_{B, r < h0} lambda_j
-// 1/2 (W_h0 + W_h1), then A_i, B_i.
+// The own-set part of rho_i = m_A sum_{A, r < h_A} lambda_j W_hA + m_B
+// sum_{B, r < h0} lambda_j 1/2 (W_h0 + W_h1), and A_i, B_i from it (the
+// complementary walk adds the second sum and recomputes A_i, B_i for the
+// particles it lists).  Targets in the internal order; results at the sorted
+// slot i2r[s].
 __global__ __launch_bounds__(kBlock) void k_blend_density(
     const float4* __restrict__ ipos_a, const float2* __restrict__ ibl_a,
     const uint32_t* __restrict__ cs_a, const uint32_t* __restrict__ ce_a,
@@ -600,9 +608,12 @@ __global__ __launch_bounds__(kBlock) void k_blend_density(
     aux_a[o] = make_float2(c.stiffness * (r - c.rest_density) * inv * inv, inv);
 }
 
-// Blended acceleration (own + complementary, lambda-weighted; gravity), the
-// velocity update with the cap, the position update and the wall clamp, then
-// integrateD's blend update; results at sorted slot s.
+// Blended acceleration: the lambda-weighted own-set sums (targets in the
+// internal order) plus gravity, plus the complementary acceleration of the
+// step's walk (xacc_a, -0 for the particles it does not list) and the
+// boundary term; then the velocity update with the cap, the position update
+// and the wall clamp, and integrateD's blend update; results at the sorted
+// slot i2r[s].
 template <bool kTension>
 __global__ __launch_bounds__(kBlock) void k_blend_force_integrate(
     const float4* __restrict__ ipos_a, const float4* __restrict__ ivel_a,
