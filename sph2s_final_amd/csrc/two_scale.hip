@@ -192,17 +192,19 @@ __device__ __forceinline__ bool other_nearby(const Nearby& nb, const GridView& g
 }
 
 // ---------------------------------------------------------------------------
-// The complementary sums by wave.  Few particles have a particle of the other
+// The complementary sums by lane group.  Few particles have a particle of the other
 // set within h0 (those near the interface), but a low particle there sees
 // hundreds of high ones (8 per low-resolution volume), and one lane walking
 // them alone holds its whole wave -- and the kernel -- for the length of that
 // serial walk.  So the walkers are listed first (k_cross_list, which also
-// finalises the others), and each listed particle gets a whole wave: the
-// lanes probe its stencil rows in parallel, the rows' runs are laid end to
-// end, lane l takes candidates l, l + 64, ... of that sequence, and the
-// lanes' partial sums are reduced in a fixed order (so the result does not
-// depend on which wave takes which particle or on the list's order).
-constexpr int kWalkMaxRows = 64;
+// finalises the others), and each listed particle gets a group of 16, 32 or
+// 64 lanes (as many as its stencil has rows, rounded up): the lanes probe its
+// stencil rows in parallel, the rows' runs are laid end to end, lane l takes
+// candidates l, l + kG, ... of that sequence, and the lanes' partial sums are
+// reduced in a fixed order (so the result does not depend on which wave or
+// group takes which particle or on the list's order).  A wave holds several
+// groups so that it has several walkers' loads in flight: the walks wait on
+// memory, not on issue.
 constexpr uint32_t kWalkBlocks = 4096;  // 16384 waves share a step's walk list
 // The walk list is kept in 64 segments, each with its own counter 64 bytes
 // from the next (appends from one block go to segment blockIdx % 64), so the
@@ -268,24 +270,31 @@ __device__ __forceinline__ void append_flagged(bool flag, uint32_t value,
     if (flag) list[base + __popcll(m & lanes_below(lane))] = value;
 }
 
-// Sum over the wave's 64 lanes in a fixed butterfly order (every lane active).
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+// Sum over each group of kG lanes (lanes q kG .. q kG + kG - 1) in a fixed
+// butterfly order (every lane of the wave active).
+template <int kG>
+__device__ __forceinline__ float group_sum(float v) {
+    for (int off = kG / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
 // Calls body(j) for the candidates j of one particle's stencil rows over grid
-// g, spread over the wave's lanes: row r is probed by lane r, its run
+// g, spread over a group of kG lanes.  The wave holds 64 / kG groups, each
+// with its own particle (a group with `active` false has no rows).  Row r of
+// a chunk of kG rows is probed by the group's lane r, its run
 // [rb[r], rb[r] + len_r) takes positions [incl[r] - len_r, incl[r]) of the
-// rows laid end to end, and lane l visits positions l, l + 64, ...  rb_s and
-// incl_s are the wave's own 64-entry LDS rows.  Every lane of the wave calls
-// it with the same particle.
-template <class Body>
-__device__ __forceinline__ void wave_walk(const GridView& g, const uint32_t* __restrict__ cs,
-                                          const uint32_t* __restrict__ ce, uint32_t n, float x,
-                                          float y, float z, uint32_t* rb_s, uint32_t* incl_s,
-                                          Body&& body) {
+// chunk's rows laid end to end, and the group's lane l visits positions
+// l, l + kG, ...  rb_s and incl_s are the wave's own 64-entry LDS rows
+// (group q uses entries q kG .. q kG + kG - 1).  Every lane of the wave calls
+// it.
+template <int kG, class Body>
+__device__ __forceinline__ void group_walk(const GridView& g, const uint32_t* __restrict__ cs,
+                                           const uint32_t* __restrict__ ce, uint32_t n,
+                                           bool active, float x, float y, float z,
+                                           uint32_t* rb_s, uint32_t* incl_s, Body&& body) {
+    static_assert(kG == 16 || kG == 32 || kG == 64, "walker group width");
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gl = lane & (kG - 1u), g0 = lane - gl;
     const int sx = cell_coord_off(x, -g.h, g.ox, g.spacing, g.dx);
     const int sy = cell_coord_off(y, -g.h, g.oy, g.spacing, g.dy);
     const int sz = cell_coord_off(z, -g.h, g.oz, g.spacing, g.dz);
@@ -296,11 +305,13 @@ __device__ __forceinline__ void wave_walk(const GridView& g, const uint32_t* __r
     const float fy = grid_offset(y, g.oy, g.spacing, g.dy);
     const float fz = grid_offset(z, g.oz, g.spacing, g.dz);
     const int ny = ey - sy + 1;
-    const int nrows = ny * (ez - sz + 1);
-    // Rows in chunks of 64 (a search radius of one cell pitch or two gives 9
-    // or 25 rows; the chunk loop only guards larger ratios).
-    for (int r0 = 0; r0 < nrows; r0 += kWalkMaxRows) {
-        const int r = r0 + static_cast<int>(lane);
+    const int nrows = active ? ny * (ez - sz + 1) : 0;
+    // Rows in chunks of kG (the launch picks kG so that one chunk holds a
+    // walker's rows); the wave runs as many chunks as its largest group needs.
+    const uint32_t chunks =
+        __ockl_wfred_max_u32(static_cast<uint32_t>((nrows + kG - 1) / kG));
+    for (uint32_t ch = 0; ch < chunks; ++ch) {
+        const int r = static_cast<int>(ch * kG + gl);
         uint32_t b = kEmptyCell, e = 0u;
         if (r < nrows) {
             const int cy = sy + r % ny, cz = sz + r / ny;
@@ -317,28 +328,42 @@ __device__ __forceinline__ void wave_walk(const GridView& g, const uint32_t* __r
             }
         }
         const uint32_t len = (b < e) ? e - b : 0u;
-        // Inclusive prefix of the run lengths over the lanes.
+        // Inclusive prefix of the run lengths over the group's lanes.
         uint32_t incl = len;
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off));
-            if (lane >= static_cast<uint32_t>(off)) incl += v;
+        for (int off = 1; off < kG; off <<= 1) {
+            const uint32_t v = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, kG));
+            if (gl >= static_cast<uint32_t>(off)) incl += v;
         }
-        const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63));
+        const uint32_t total =
+            static_cast<uint32_t>(__shfl(static_cast<int>(incl), static_cast<int>(g0 + kG - 1u)));
         rb_s[lane] = b;
         incl_s[lane] = incl;
         // (the wave's own LDS rows: no barrier beyond the wave is needed)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // incl_s[63] = total > f, so the row search stops within the 64 rows.
-        uint32_t row = 0u;
-        for (uint32_t f = lane; f < total; f += 64u) {
+        // incl_s[g0 + kG - 1] = total > f, so the row search stays in the group.
+        uint32_t row = g0;
+        for (uint32_t f = gl; f < total; f += kG) {
             while (incl_s[row] <= f) ++row;
-            const uint32_t begin = (row > 0u) ? incl_s[row - 1u] : 0u;
+            const uint32_t begin = (row > g0) ? incl_s[row - 1u] : 0u;
             body(rb_s[row] + (f - begin));
         }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Lanes per walker for walks over grid g: the smallest group of 16, 32 or 64
+// lanes that probes a walker's stencil rows in one chunk.  A search radius h
+// over cells of pitch `spacing` spans at most ceil(2 h / spacing) + 1 cells per
+// axis (3 for the low grid at h0, 5 for the high grid at h0 = 2 h1), so 9 or 25
+// rows.  The width fixes the order in which a walker's terms are added (lane
+// partial sums, then the group's butterfly), so a set's walks always use the
+// same one.
+inline int walk_lanes(const GridView& g) {
+    const int cells = static_cast<int>(std::ceil(2.0f * g.h * g.inv_spacing - 1e-3f)) + 1;
+    const int rows = cells * cells;
+    return rows <= 16 ? 16 : rows <= 32 ? 32 : 64;
 }
 
 // Lists set A's particles whose complementary walk is not empty and
@@ -369,9 +394,10 @@ __global__ __launch_bounds__(kBlock) void k_cross_list(
 }
 
 // complementary density of the listed particles: rho_i += m_B sum_{j in B,
-// r < h0} 1/2 (W_h0 + W_h1), then A_i, B_i.  One wave per listed particle.
+// r < h0} 1/2 (W_h0 + W_h1), then A_i, B_i.  One group of kG lanes per listed
+// particle, 64 / kG particles per wave.
 // kBlend: every term weighted by lambda_j (tbl_b[j].x; Solver.cu:252-290).
-template <bool kBlend>
+template <bool kBlend, int kG>
 __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
     const float4* __restrict__ tpos_a, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ list, uint32_t seg_cap, const float4* __restrict__ tpos_b,
@@ -381,15 +407,19 @@ __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
     __shared__ uint32_t rb_s[kBlock / 64][64];
     __shared__ uint32_t incl_s[kBlock / 64][64];
     __shared__ uint32_t seg_s[kBlock / 64][64];
+    constexpr uint32_t kPer = 64 / kG;  // walkers per wave
     const uint32_t nb = *n_b;
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t nw = walk_segments(counts, seg_s[wv]);
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    for (uint32_t w = blockIdx.x * (kBlock / 64) + wv; w < nw; w += waves) {
-        const uint32_t s = walk_entry(list, seg_cap, seg_s[wv], w);
-        const float4 xi = tpos_a[s];
+    const uint32_t stride = gridDim.x * (kBlock / 64) * kPer;
+    for (uint32_t w0 = (blockIdx.x * (kBlock / 64) + wv) * kPer; w0 < nw; w0 += stride) {
+        const uint32_t w = w0 + lane / kG;
+        const bool act = w < nw;
+        const uint32_t s = act ? walk_entry(list, seg_cap, seg_s[wv], w) : 0u;
+        const float4 xi = act ? tpos_a[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float sum = 0.0f;
-        wave_walk(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv], [&](uint32_t j) {
+        group_walk<kG>(gb, cs_b, ce_b, nb, act, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv],
+                       [&](uint32_t j) {
             const float4 xj = tpos_b[j];
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -404,8 +434,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
                     sum += 0.5f * (w0 + w1);
             }
         });
-        sum = wave_sum(sum);
-        if (lane == 0u) {
+        sum = group_sum<kG>(sum);
+        if (act && (lane & (kG - 1u)) == 0u) {
             const float r = rho_a[s] + c.m_other * sum;
             rho_a[s] = r;
             const float inv = __builtin_amdgcn_rcpf(r);
@@ -415,10 +445,10 @@ __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
 }
 
 // complementary force (TWO_SCALE/Solver.cu:394-499) of the listed particles.
-// One wave each.  Fixed coupling (lambda = 1): added to their own-set
+// One group of kG lanes each.  Fixed coupling (lambda = 1): added to their own-set
 // acceleration in acc_a.  kBlend: every term weighted by lambda_j (tbl_b[j].x)
 // and the sum written to acc_a, which the blended force kernel adds.
-template <bool kTension, bool kBlend>
+template <bool kTension, bool kBlend, int kG>
 __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
     const float4* __restrict__ tpos_a, const float4* __restrict__ tvel_a,
     const float2* __restrict__ aux_a, const uint32_t* __restrict__ counts,
@@ -430,18 +460,23 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
     __shared__ uint32_t rb_s[kBlock / 64][64];
     __shared__ uint32_t incl_s[kBlock / 64][64];
     __shared__ uint32_t seg_s[kBlock / 64][64];
+    constexpr uint32_t kPer = 64 / kG;  // walkers per wave
     const uint32_t nb = *n_b;
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t nw = walk_segments(counts, seg_s[wv]);
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    for (uint32_t w = blockIdx.x * (kBlock / 64) + wv; w < nw; w += waves) {
-        const uint32_t s = walk_entry(list, seg_cap, seg_s[wv], w);
-        const float4 xi = tpos_a[s];
-        const float4 vi = tvel_a[s];
-        const float2 ai = aux_a[s];
+    const uint32_t stride = gridDim.x * (kBlock / 64) * kPer;
+    for (uint32_t w0 = (blockIdx.x * (kBlock / 64) + wv) * kPer; w0 < nw; w0 += stride) {
+        const uint32_t w = w0 + lane / kG;
+        const bool act = w < nw;
+        const uint32_t s = act ? walk_entry(list, seg_cap, seg_s[wv], w) : 0u;
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 xi = act ? tpos_a[s] : zero;
+        const float4 vi = act ? tvel_a[s] : zero;
+        const float2 ai = act ? aux_a[s] : make_float2(0.0f, 0.0f);
         float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f,
               Tz = 0.f;
-        wave_walk(gb, cs_b, ce_b, nb, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv], [&](uint32_t j) {
+        group_walk<kG>(gb, cs_b, ce_b, nb, act, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv],
+                       [&](uint32_t j) {
             const float4 xj = tpos_b[j];
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -475,18 +510,18 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
                 }
             }
         });
-        Px = wave_sum(Px);
-        Py = wave_sum(Py);
-        Pz = wave_sum(Pz);
-        Vx = wave_sum(Vx);
-        Vy = wave_sum(Vy);
-        Vz = wave_sum(Vz);
+        Px = group_sum<kG>(Px);
+        Py = group_sum<kG>(Py);
+        Pz = group_sum<kG>(Pz);
+        Vx = group_sum<kG>(Vx);
+        Vy = group_sum<kG>(Vy);
+        Vz = group_sum<kG>(Vz);
         if (kTension) {
-            Tx = wave_sum(Tx);
-            Ty = wave_sum(Ty);
-            Tz = wave_sum(Tz);
+            Tx = group_sum<kG>(Tx);
+            Ty = group_sum<kG>(Ty);
+            Tz = group_sum<kG>(Tz);
         }
-        if (lane == 0u) {
+        if (act && (lane & (kG - 1u)) == 0u) {
             const float mv = c.m_other * c.visc * ai.y;
             float ax = c.m_other * Px + mv * Vx, ay = c.m_other * Py + mv * Vy,
                   az = c.m_other * Pz + mv * Vz;
@@ -504,6 +539,34 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
             }
         }
     }
+}
+
+// The walks' launches at the group width that walk_lanes picks for the grid
+// walked.
+template <bool kBlend, class... Args>
+void launch_density_walk(int lanes, hipStream_t s, Args... args) {
+    if (lanes == 16)
+        hipLaunchKernelGGL((k_cross_density_walk<kBlend, 16>), dim3(kWalkBlocks), dim3(kBlock), 0,
+                           s, args...);
+    else if (lanes == 32)
+        hipLaunchKernelGGL((k_cross_density_walk<kBlend, 32>), dim3(kWalkBlocks), dim3(kBlock), 0,
+                           s, args...);
+    else
+        hipLaunchKernelGGL((k_cross_density_walk<kBlend, 64>), dim3(kWalkBlocks), dim3(kBlock), 0,
+                           s, args...);
+}
+
+template <bool kTension, bool kBlend, class... Args>
+void launch_force_walk(int lanes, hipStream_t s, Args... args) {
+    if (lanes == 16)
+        hipLaunchKernelGGL((k_cross_force_walk<kTension, kBlend, 16>), dim3(kWalkBlocks),
+                           dim3(kBlock), 0, s, args...);
+    else if (lanes == 32)
+        hipLaunchKernelGGL((k_cross_force_walk<kTension, kBlend, 32>), dim3(kWalkBlocks),
+                           dim3(kBlock), 0, s, args...);
+    else
+        hipLaunchKernelGGL((k_cross_force_walk<kTension, kBlend, 64>), dim3(kWalkBlocks),
+                           dim3(kBlock), 0, s, args...);
 }
 
 // integrateD of one set (TWO_SCALE/Solver.cu:1163-1223 with lambda = 1):
@@ -1422,10 +1485,10 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         hipLaunchKernelGGL(k_cross_list, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], c, A.buf.rho, A.buf.aux,
                            A.accel, walk_counts(t, r), A.walk, walk_segment_cap(A.cap));
-        hipLaunchKernelGGL(k_cross_density_walk<true>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
-                           A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
-                           B.buf.tpos, B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                           X[1 - r], c, A.buf.rho, A.buf.aux);
+        launch_density_walk<true>(walk_lanes(X[1 - r]), s, A.buf.tpos, walk_counts(t, r), A.walk,
+                                  walk_segment_cap(A.cap), B.buf.tpos, B.tblend,
+                                  B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r], c,
+                                  A.buf.rho, A.buf.aux);
     }
     SPH_HIP(hipEventRecord(t->ev[3], sl));  // the low set's A, B
     SPH_HIP(hipEventRecord(t->ev[4], st));  // the high set's A, B
@@ -1437,17 +1500,17 @@ int launch_blended_passes(sph_two_scale* t, const StepParams* P, const GridView*
         const hipStream_t s = sst[r];
         CrossParams c = cross_params(t, r);
         if (c.tension != 0.0f)
-            hipLaunchKernelGGL((k_cross_force_walk<true, true>), dim3(kWalkBlocks), dim3(kBlock),
-                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
-                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
-                               B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
-                               c, A.accel);
+            launch_force_walk<true, true>(walk_lanes(X[1 - r]), s, A.buf.tpos, A.buf.tvel,
+                                          A.buf.aux, walk_counts(t, r), A.walk,
+                                          walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel,
+                                          B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end,
+                                          B.buf.n_dev, X[1 - r], c, A.accel);
         else
-            hipLaunchKernelGGL((k_cross_force_walk<false, true>), dim3(kWalkBlocks), dim3(kBlock),
-                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
-                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
-                               B.tblend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
-                               c, A.accel);
+            launch_force_walk<false, true>(walk_lanes(X[1 - r]), s, A.buf.tpos, A.buf.tvel,
+                                          A.buf.aux, walk_counts(t, r), A.walk,
+                                          walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel,
+                                          B.buf.aux, B.tblend, B.buf.cell_start, B.buf.cell_end,
+                                          B.buf.n_dev, X[1 - r], c, A.accel);
         c.vmax = velocity_limit(t, r, dt);
         float4* acc = accel_out ? A.buf.accel : nullptr;
         const float inc = t->opt.blend_increment;
@@ -1564,10 +1627,10 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
                            A.buf.n_dev, B.buf.n_dev, nearby(t, r), X[0], cross_params(t, r),
                            A.buf.rho, A.buf.aux, kNoXacc, walk_counts(t, r), A.walk,
                            walk_segment_cap(A.cap));
-        hipLaunchKernelGGL(k_cross_density_walk<false>, dim3(kWalkBlocks), dim3(kBlock), 0, s,
-                           A.buf.tpos, walk_counts(t, r), A.walk, walk_segment_cap(A.cap),
-                           B.buf.tpos, kNoBlend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev,
-                           X[1 - r], cross_params(t, r), A.buf.rho, A.buf.aux);
+        launch_density_walk<false>(walk_lanes(X[1 - r]), s, A.buf.tpos, walk_counts(t, r),
+                                   A.walk, walk_segment_cap(A.cap), B.buf.tpos, kNoBlend,
+                                   B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
+                                   cross_params(t, r), A.buf.rho, A.buf.aux);
         SPH_HIP(hipEventRecord(t->ev[4 - r], s));  // event 3: high A, B final; 4: low
         rc = launch_gather_aux(P[r], A.buf, s);
         if (rc == SPH_OK) rc = launch_force_own_x(P[r], G[r], A.buf, A.accel, s);
@@ -1580,17 +1643,17 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
         SPH_HIP(hipStreamWaitEvent(s, t->ev[3 + r], 0));  // the other set's final A, B
         CrossParams c = cross_params(t, r);
         if (c.tension != 0.0f)
-            hipLaunchKernelGGL((k_cross_force_walk<true, false>), dim3(kWalkBlocks), dim3(kBlock),
-                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
-                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
-                               kNoBlend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
-                               c, A.accel);
+            launch_force_walk<true, false>(walk_lanes(X[1 - r]), s, A.buf.tpos, A.buf.tvel,
+                                           A.buf.aux, walk_counts(t, r), A.walk,
+                                           walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel,
+                                           B.buf.aux, kNoBlend, B.buf.cell_start,
+                                           B.buf.cell_end, B.buf.n_dev, X[1 - r], c, A.accel);
         else
-            hipLaunchKernelGGL((k_cross_force_walk<false, false>), dim3(kWalkBlocks), dim3(kBlock),
-                               0, s, A.buf.tpos, A.buf.tvel, A.buf.aux, walk_counts(t, r), A.walk,
-                               walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel, B.buf.aux,
-                               kNoBlend, B.buf.cell_start, B.buf.cell_end, B.buf.n_dev, X[1 - r],
-                               c, A.accel);
+            launch_force_walk<false, false>(walk_lanes(X[1 - r]), s, A.buf.tpos, A.buf.tvel,
+                                           A.buf.aux, walk_counts(t, r), A.walk,
+                                           walk_segment_cap(A.cap), B.buf.tpos, B.buf.tvel,
+                                           B.buf.aux, kNoBlend, B.buf.cell_start,
+                                           B.buf.cell_end, B.buf.n_dev, X[1 - r], c, A.accel);
         c.vmax = velocity_limit(t, r, dt);
         hipLaunchKernelGGL(k_cross_integrate, blocks_for(A.cap), dim3(kBlock), 0, s, A.buf.tpos,
                            A.buf.tvel, A.accel, A.buf.n_dev, c, wall_acc(t, r), dt, A.buf.pos,
