@@ -87,28 +87,48 @@ __device__ __forceinline__ float child_offset(int k, int axis, float d) {
 }
 
 // ---------------------------------------------------------------------------
+// The walk list is kept in 64 segments, each with its own counter 64 bytes
+// from the next (appends from one block go to segment blockIdx % 64), so the
+// appends do not all contend for one address.
+constexpr uint32_t kWalkSegments = 64;
+constexpr uint32_t kWalkCounterStride = 16;  // words
+
 // Occupancy of the low-resolution grid by each set, so that a particle with no
 // particle of the other set within reach skips the walk over that set's grid
 // (most low particles lie away from the refinement region, and most high
-// particles deep inside it).  Per low-grid cell: occ_h = 1 when it holds a high
-// particle, occ_l = 1 when it holds a low particle; near = the OR of both over
-// the cell's 3 x 3 x 3 block (bit 0 high, bit 1 low).
-__global__ void k_clear_words(uint32_t* __restrict__ w, uint32_t count) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-        w[i] = 0u;
+// particles deep inside it).  Per low-grid cell: occ_h holds the step's mark
+// when the cell holds a high particle, occ_l when it holds a low particle;
+// near = both tests ORed over the cell's 3 x 3 x 3 block (bit 0 high, bit 1
+// low).  The mark is *stamp + 1, and *stamp counts the steps (k_commit_counts
+// advances it), so a new step's marks need no clearing: a cell marked in an
+// earlier step holds an older value.  After 2^32 steps an old value can come
+// round again; such a cell reads as occupied, which only lets a walk run that
+// finds no candidates (the test only skips empty sums).
+__device__ __forceinline__ uint32_t occupancy_mark(const uint32_t* __restrict__ stamp) {
+    return *stamp + 1u;
 }
 
+// Marks the cells of set S's particles; block 0 also zeroes the set's
+// walk-list counters for the step's k_cross_list (which runs later on the
+// same stream).
 __global__ void k_mark_occupancy(const float4* __restrict__ pos, const uint32_t* __restrict__ n_dev,
-                                 GridView g, uint32_t* __restrict__ occ) {
+                                 GridView g, const uint32_t* __restrict__ stamp,
+                                 uint32_t* __restrict__ occ, uint32_t* __restrict__ walk_counts) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < kWalkSegments; k += blockDim.x)
+            walk_counts[k * kWalkCounterStride] = 0u;
     if (i >= *n_dev) return;
     const float4 x = pos[i];
-    occ[cell_hash(x.x, x.y, x.z, g.ox, g.oy, g.oz, g.spacing, g.dx, g.dy, g.dz)] = 1u;
+    occ[cell_hash(x.x, x.y, x.z, g.ox, g.oy, g.oz, g.spacing, g.dx, g.dy, g.dz)] =
+        occupancy_mark(stamp);
 }
 
 __global__ void k_dilate_occupancy(const uint32_t* __restrict__ occ_h,
                                    const uint32_t* __restrict__ occ_l, int dx, int dy, int dz,
+                                   const uint32_t* __restrict__ stamp,
                                    uint32_t* __restrict__ near) {
+    const uint32_t mark = occupancy_mark(stamp);
     const uint32_t cells = static_cast<uint32_t>(dx) * static_cast<uint32_t>(dy) *
                            static_cast<uint32_t>(dz);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells;
@@ -116,13 +136,13 @@ __global__ void k_dilate_occupancy(const uint32_t* __restrict__ occ_h,
         const int cx = static_cast<int>(c % static_cast<uint32_t>(dx));
         const int cy = static_cast<int>((c / static_cast<uint32_t>(dx)) % static_cast<uint32_t>(dy));
         const int cz = static_cast<int>(c / (static_cast<uint32_t>(dx) * static_cast<uint32_t>(dy)));
-        uint32_t h = 0u, l = 0u;
+        bool h = false, l = false;
         for (int z = max(cz - 1, 0); z <= min(cz + 1, dz - 1); ++z)
             for (int y = max(cy - 1, 0); y <= min(cy + 1, dy - 1); ++y) {
                 const uint32_t row = static_cast<uint32_t>(dx) * (y + dy * z);
                 for (int x = max(cx - 1, 0); x <= min(cx + 1, dx - 1); ++x) {
-                    h |= occ_h[row + x];
-                    l |= occ_l[row + x];
+                    h |= occ_h[row + x] == mark;
+                    l |= occ_l[row + x] == mark;
                 }
             }
         near[c] = (h ? 1u : 0u) | (l ? 2u : 0u);
@@ -130,11 +150,12 @@ __global__ void k_dilate_occupancy(const uint32_t* __restrict__ occ_h,
 }
 
 // The skip test of one cross walk: occ marks the low-grid cells holding a
-// particle of the other set (NULL: no test, always walk), near the dilated
-// marks (bit `bit`).
+// particle of the other set (NULL: no test, always walk) with the step's mark
+// (occupancy_mark(stamp)), near the dilated marks (bit `bit`).
 struct Nearby {
     const uint32_t* occ;
     const uint32_t* near;
+    const uint32_t* stamp;
     uint32_t bit;
 };
 
@@ -182,13 +203,14 @@ __device__ __forceinline__ bool other_nearby(const Nearby& nb, const GridView& g
         ez <= cz + 1)
         return (nb.near[static_cast<uint32_t>(cx) +
                         static_cast<uint32_t>(ga.dx) * (cy + ga.dy * cz)] & nb.bit) != 0u;
-    uint32_t any = 0u;
+    const uint32_t mark = occupancy_mark(nb.stamp);
+    bool any = false;
     for (int cz2 = sz; cz2 <= ez; ++cz2)
         for (int cy2 = sy; cy2 <= ey; ++cy2) {
             const uint32_t row = static_cast<uint32_t>(ga.dx) * (cy2 + ga.dy * cz2);
-            for (int cx2 = sx; cx2 <= ex; ++cx2) any |= nb.occ[row + cx2];
+            for (int cx2 = sx; cx2 <= ex; ++cx2) any |= nb.occ[row + cx2] == mark;
         }
-    return any != 0u;
+    return any;
 }
 
 // ---------------------------------------------------------------------------
@@ -206,11 +228,6 @@ __device__ __forceinline__ bool other_nearby(const Nearby& nb, const GridView& g
 // groups so that it has several walkers' loads in flight: the walks wait on
 // memory, not on issue.
 constexpr uint32_t kWalkBlocks = 4096;  // 16384 waves share a step's walk list
-// The walk list is kept in 64 segments, each with its own counter 64 bytes
-// from the next (appends from one block go to segment blockIdx % 64), so the
-// appends do not all contend for one address.
-constexpr uint32_t kWalkSegments = 64;
-constexpr uint32_t kWalkCounterStride = 16;  // words
 
 // Null arguments of the fixed coupling's walks (no blend weights, no separate
 // complementary acceleration).
@@ -915,100 +932,110 @@ __global__ __launch_bounds__(kBlock) void k_merge_mark(
     }
 }
 
+// The block's counts over its particles' flags (every thread of the block
+// calls it; `live` false for slots past the set's count): counts[block] =
+// kept (bit 0 clear), counts[nblocks + block] = emitting (bit 1 set).
+__device__ __forceinline__ void block_flag_counts(bool live, uint32_t f,
+                                                  uint32_t* __restrict__ counts,
+                                                  uint32_t nblocks) {
+    __shared__ uint32_t wc[2][kBlock / 64];
+    const uint64_t b0 = __ballot(live && !(f & 1u)), b1 = __ballot(live && (f & 2u) != 0u);
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0u) {
+        wc[0][wave] = static_cast<uint32_t>(__popcll(b0));
+        wc[1][wave] = static_cast<uint32_t>(__popcll(b1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c0 = 0u, c1 = 0u;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            c0 += wc[0][w];
+            c1 += wc[1][w];
+        }
+        counts[blockIdx.x] = c0;
+        counts[nblocks + blockIdx.x] = c1;
+    }
+}
+
 // Low set.  Fixed: inside the region -> split and removed (3).  Blended:
 // removed (1); or split (2) when settled inside the region or requested
 // (state 3) and no child of its group is alive; every live particle marks
-// its id in grp_live bit 1.
+// its id in grp_live bit 1.  Then the block's counts (block_flag_counts).
 __global__ __launch_bounds__(kBlock) void k_split_mark(const float4* __restrict__ pos_l,
                                                        const float2* __restrict__ bl_l,
                                                        const uint32_t* __restrict__ n_l, Region rg,
                                                        uint32_t* __restrict__ grp_live,
                                                        uint32_t num_groups,
-                                                       uint32_t* __restrict__ flag_l) {
+                                                       uint32_t* __restrict__ flag_l,
+                                                       uint32_t* __restrict__ counts_l,
+                                                       uint32_t nbl) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= *n_l) return;
-    const float4 x = pos_l[i];
-    const bool in = inside(rg, x.x, x.y, x.z);
-    if (!bl_l) {
-        flag_l[i] = in ? 3u : 0u;
-        return;
+    const bool live = i < *n_l;
+    uint32_t f = 0u;
+    if (live) {
+        const float4 x = pos_l[i];
+        const bool in = inside(rg, x.x, x.y, x.z);
+        if (!bl_l) {
+            f = in ? 3u : 0u;
+        } else {
+            const float2 b = bl_l[i];
+            if (blend_removed(b)) {
+                f = 1u;
+            } else {
+                const uint32_t id = __float_as_uint(x.w);
+                uint32_t children = 0u;
+                if (id < num_groups) children = atomicOr(&grp_live[id], 2u) & 1u;
+                const uint32_t mode = __float_as_uint(b.y) & 3u;
+                const bool want = mode == 3u || (mode == 0u && in);
+                f = (want && !children) ? 2u : 0u;
+            }
+        }
+        flag_l[i] = f;
     }
-    const float2 b = bl_l[i];
-    if (blend_removed(b)) {
-        flag_l[i] = 1u;
-        return;
-    }
-    const uint32_t id = __float_as_uint(x.w);
-    uint32_t children = 0u;
-    if (id < num_groups) children = atomicOr(&grp_live[id], 2u) & 1u;
-    const uint32_t mode = __float_as_uint(b.y) & 3u;
-    const bool want = mode == 3u || (mode == 0u && in);
-    flag_l[i] = (want && !children) ? 2u : 0u;
+    block_flag_counts(live, f, counts_l, nbl);
 }
 
 // High set.  Fixed: a group with 8 children outside the expanded region
 // merges (1, leader 1 | 2).  Blended: removed (1); a group of 8 settled
 // children outside, with no live low particle of its id, starts fading out
-// (4, leader 4 | 2).
+// (4, leader 4 | 2).  Then the block's counts (block_flag_counts).
 __global__ __launch_bounds__(kBlock) void k_merge_flags(const float4* __restrict__ vel_h,
                                                         const float2* __restrict__ bl_h,
                                                         const uint32_t* __restrict__ n_h,
                                                         const uint32_t* __restrict__ grp_cnt,
                                                         const uint32_t* __restrict__ grp_live,
                                                         uint32_t num_groups,
-                                                        uint32_t* __restrict__ flag_h) {
+                                                        uint32_t* __restrict__ flag_h,
+                                                        uint32_t* __restrict__ counts_h,
+                                                        uint32_t nbh) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= *n_h) return;
-    const uint32_t tag = __float_as_uint(vel_h[i].w);
-    const uint32_t g = tag & kGroupMask, k = tag >> 29;
-    const uint32_t lead = (k == 0u) ? 2u : 0u;
-    if (!bl_h) {
-        const bool merge = g < num_groups && grp_cnt[g] == 8u;
-        flag_h[i] = merge ? (1u | lead) : 0u;
-        return;
+    const bool live = i < *n_h;
+    uint32_t f = 0u;
+    if (live) {
+        const uint32_t tag = __float_as_uint(vel_h[i].w);
+        const uint32_t g = tag & kGroupMask, k = tag >> 29;
+        const uint32_t lead = (k == 0u) ? 2u : 0u;
+        if (!bl_h) {
+            const bool merge = g < num_groups && grp_cnt[g] == 8u;
+            f = merge ? (1u | lead) : 0u;
+        } else if (blend_removed(bl_h[i])) {
+            f = 1u;
+        } else {
+            const bool merge = g < num_groups && grp_cnt[g] == 8u && !(grp_live[g] & 2u);
+            f = merge ? (4u | lead) : 0u;
+        }
+        flag_h[i] = f;
     }
-    if (blend_removed(bl_h[i])) {
-        flag_h[i] = 1u;
-        return;
-    }
-    const bool merge = g < num_groups && grp_cnt[g] == 8u && !(grp_live[g] & 2u);
-    flag_h[i] = merge ? (4u | lead) : 0u;
+    block_flag_counts(live, f, counts_h, nbh);
 }
 
-// Per-block counts over the flags of one set: c0 = kept (bit 0 clear),
-// c1 = emitting (bit 1 set).
-__global__ __launch_bounds__(kBlock) void k_block_counts(const uint32_t* __restrict__ flag,
-                                                         const uint32_t* __restrict__ n_dev,
-                                                         uint32_t* __restrict__ counts,
-                                                         uint32_t nblocks) {
-    __shared__ uint32_t c0s, c1s;
-    if (threadIdx.x == 0) c0s = c1s = 0u;
-    __syncthreads();
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t n = *n_dev;
-    bool c0 = false, c1 = false;
-    if (i < n) {
-        const uint32_t f = flag[i];
-        c0 = !(f & 1u);
-        c1 = (f & 2u) != 0u;
-    }
-    const uint64_t b0 = __ballot(c0), b1 = __ballot(c1);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&c0s, static_cast<uint32_t>(__popcll(b0)));
-        atomicAdd(&c1s, static_cast<uint32_t>(__popcll(b1)));
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        counts[blockIdx.x] = c0s;
-        counts[nblocks + blockIdx.x] = c1s;
-    }
-}
-
-// Exclusive scans of the four block-count sequences (one workgroup), new
-// totals with the capacity check.  tot[] = {keepL, splitL, keepH, leadH,
-// newL, newH}.
+// Exclusive scans of the four block-count sequences (one workgroup each),
+// over the blocks that hold live particles (the scatters read no other
+// block's offsets).  tot[] = {keepL, splitL, keepH, leadH, newL, newH}.
 __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cl, uint32_t nbl,
                                                       uint32_t* __restrict__ ch, uint32_t nbh,
+                                                      const uint32_t* __restrict__ n_l,
+                                                      const uint32_t* __restrict__ n_h,
                                                       uint32_t* __restrict__ tot) {
     // Block q scans count sequence q (kept low, split low, kept high, merge
     // leaders): a thread sums a contiguous chunk of at most a few counts, the
@@ -1017,7 +1044,8 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cl,
     __shared__ uint32_t wtot[16];
     const uint32_t q = blockIdx.x, t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     uint32_t* d = (q == 0) ? cl : (q == 1) ? cl + nbl : (q == 2) ? ch : ch + nbh;
-    const uint32_t len = (q < 2) ? nbl : nbh;
+    const uint32_t live_blocks = ((q < 2 ? *n_l : *n_h) + kBlock - 1u) / kBlock;
+    const uint32_t len = min((q < 2) ? nbl : nbh, live_blocks);
     const uint32_t chunk = (len + 1023) / 1024;
     const uint32_t b = min(t * chunk, len), e = min(b + chunk, len);
     uint32_t sum = 0;
@@ -1185,10 +1213,13 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
 }
 
 // tot[6], tot[7] keep the counts the step ran with (for inspection).
+// The step stamp advances here, so the next step's occupancy marks differ
+// from this step's.
 __global__ void k_commit_counts(uint32_t* __restrict__ tot, uint32_t* __restrict__ n_l,
                                 uint32_t* __restrict__ n_h, uint32_t cap_l, uint32_t cap_h,
-                                uint32_t* __restrict__ error_word) {
+                                uint32_t* __restrict__ error_word, uint32_t* __restrict__ stamp) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *stamp += 1u;
         // New counts from the totals (kept + merged low; kept + 8 x split high),
         // clamped to the capacities; a split or merge past them was dropped.
         const uint64_t new_l = static_cast<uint64_t>(tot[0]) + tot[3];
@@ -1267,8 +1298,9 @@ struct sph_two_scale {
     float4* grp_pos = nullptr;
     float4* grp_vel = nullptr;
     uint32_t* grp_cnt = nullptr;
-    uint32_t* tot = nullptr;     // 6 words
+    uint32_t* tot = nullptr;     // 8 words (k_scan_counts, k_commit_counts)
     uint32_t* error_word = nullptr;
+    uint32_t* stamp = nullptr;   // steps run (the occupancy marks' base)
     uint32_t* occ = nullptr;     // per low-grid cell: 1 when it holds a high particle
     uint32_t* grp_live = nullptr;  // per group: bit 0 live child, bit 1 live low particle (blended)
     Walls walls{};               // boundary particles, sorted on the low grid
@@ -1333,6 +1365,7 @@ Nearby nearby(const sph_two_scale* t, int target) {
     Nearby nb;
     nb.occ = (target == 0) ? t->occ : t->occ + C0;
     nb.near = t->occ + 2 * C0;
+    nb.stamp = t->stamp;
     nb.bit = (target == 0) ? 1u : 2u;
     return nb;
 }
@@ -1407,13 +1440,9 @@ int launch_split_merge_marks(sph_two_scale* t, hipStream_t hs, hipStream_t ls) {
                        H.buf.n_dev, t->region, sp.merge, t->grp_pos, t->grp_vel, t->grp_cnt,
                        live, t->num_groups);
     hipLaunchKernelGGL(k_split_mark, dim3(nbl), dim3(kBlock), 0, ls, L.buf.pos, bl_l, L.buf.n_dev,
-                       t->region, live, t->num_groups, L.flag);
+                       t->region, live, t->num_groups, L.flag, L.counts, nbl);
     hipLaunchKernelGGL(k_merge_flags, dim3(nbh), dim3(kBlock), 0, hs, H.buf.vel, bl_h,
-                       H.buf.n_dev, t->grp_cnt, live, t->num_groups, H.flag);
-    hipLaunchKernelGGL(k_block_counts, dim3(nbl), dim3(kBlock), 0, ls, L.flag, L.buf.n_dev,
-                       L.counts, nbl);
-    hipLaunchKernelGGL(k_block_counts, dim3(nbh), dim3(kBlock), 0, hs, H.flag, H.buf.n_dev,
-                       H.counts, nbh);
+                       H.buf.n_dev, t->grp_cnt, live, t->num_groups, H.flag, H.counts, nbh);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -1432,7 +1461,7 @@ int launch_split_merge_commit(sph_two_scale* t) {
     sp.ref_offsets = t->opt.split_offsets == SPH2_SPLIT_REFERENCE ? 1u : 0u;
     sp.m0 = L.cfg.mass;
     hipLaunchKernelGGL(k_scan_counts, dim3(4), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
-                       t->tot);
+                       L.buf.n_dev, H.buf.n_dev, t->tot);
     const bool col = (t->debug & SPH_DEBUG_COLOR) != 0u;
     hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel,
                        L.buf.rho, bl_l, L.flag, L.buf.n_dev, L.counts, nbl, t->tot, t->region, sp,
@@ -1445,7 +1474,7 @@ int launch_split_merge_commit(sph_two_scale* t) {
                        L.state_pos, L.state_vel, L.state_blend, L.color, H.state_pos, H.state_vel,
                        H.state_blend, H.color);
     hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(64), 0, st, t->tot, L.buf.n_dev,
-                       H.buf.n_dev, L.cap, H.cap, t->error_word);
+                       H.buf.n_dev, L.cap, H.cap, t->error_word, t->stamp);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -1587,10 +1616,8 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
             if (rc != SPH_OK) return rc;
         }
         uint32_t* occ = (r == 1) ? t->occ : t->occ + C0;
-        hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 1024u)),
-                           dim3(kBlock), 0, s, occ, C0);
         hipLaunchKernelGGL(k_mark_occupancy, blocks_for(S.cap), dim3(kBlock), 0, s, S.buf.tpos,
-                           S.buf.n_dev, G[0], occ);
+                           S.buf.n_dev, G[0], t->stamp, occ, walk_counts(t, r));
         if (t->debug & SPH_DEBUG_COLOR) {
             rc = launch_energy_color(S.buf.tpos, S.buf.tvel, S.buf.cell_start, S.buf.cell_end,
                                      S.buf.n_dev, S.cap, G[r], S.cfg.mass, S.tcolor, s);
@@ -1600,14 +1627,12 @@ int launch_two_scale_step(sph_two_scale* t, float dt, bool accel_out) {
             rc = launch_wall_accel(t->walls, S.buf.tpos, S.buf.n_dev, S.cap, t->bacc[r], s);
             if (rc != SPH_OK) return rc;
         }
-        hipLaunchKernelGGL(k_clear_words, dim3(4), dim3(kBlock), 0, s, walk_counts(t, r),
-                           kWalkSegments * kWalkCounterStride);
     }
     rc = wait_for(t, 1, sl, st);  // the low set's sorted state and marks
     if (rc != SPH_OK) return rc;
     hipLaunchKernelGGL(k_dilate_occupancy, dim3(std::min<uint32_t>((C0 + kBlock - 1) / kBlock, 2048u)),
                        dim3(kBlock), 0, st, t->occ, t->occ + C0, G[0].dx, G[0].dy, G[0].dz,
-                       t->occ + 2 * C0);
+                       t->stamp, t->occ + 2 * C0);
     // The blended coupling's passes read both sets throughout: they run on
     // t->stream after the join above.
     if (blended(t)) return launch_blended_passes(t, P, G, X, dt, accel_out);
@@ -1948,7 +1973,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
     add(4 * static_cast<size_t>(t->num_groups));    // grp_cnt
-    add(64);                                        // tot[8] + error word
+    add(64);                                        // tot[8], error word, step stamp
     add(12 * static_cast<size_t>(t->set[0].num_cells));  // low-grid occupancy: occ_h, occ_l, near
     add(4 * static_cast<size_t>(t->num_groups));    // grp_live
     add(4 * static_cast<size_t>(t->num_groups));    // surf
@@ -2017,6 +2042,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     t->grp_cnt = static_cast<uint32_t*>(take());
     t->tot = static_cast<uint32_t*>(take());
     t->error_word = t->tot + 8;
+    t->stamp = t->tot + 9;
     t->occ = static_cast<uint32_t*>(take());
     t->grp_live = static_cast<uint32_t*>(take());
     t->surf = static_cast<uint32_t*>(take());
