@@ -99,7 +99,7 @@ constexpr uint32_t kWalkCounterStride = 16;  // words
 // particles deep inside it).  Per low-grid cell: occ_h holds the step's mark
 // when the cell holds a high particle, occ_l when it holds a low particle;
 // near = both tests ORed over the cell's 3 x 3 x 3 block (bit 0 high, bit 1
-// low).  The mark is *stamp + 1, and *stamp counts the steps (k_commit_counts
+// low).  The mark is *stamp + 1, and *stamp counts the steps (k_scan_counts
 // advances it), so a new step's marks need no clearing: a cell marked in an
 // earlier step holds an older value.  After 2^32 steps an old value can come
 // round again; such a cell reads as occupied, which only lets a walk run that
@@ -1029,14 +1029,26 @@ __global__ __launch_bounds__(kBlock) void k_merge_flags(const float4* __restrict
     block_flag_counts(live, f, counts_h, nbh);
 }
 
+// Words of the split/merge totals block `tot`: 0-3 the four scan totals
+// (kept low, split low, kept high, merge leaders), 4-5 the new counts, 6-7 the
+// counts the step ran with, then the error word, the step stamp and the
+// scan's ticket.
+constexpr uint32_t kTotWords = 16;
+constexpr uint32_t kErrorWordIndex = 8;
+constexpr uint32_t kStampWord = 9;
+constexpr uint32_t kTicketWord = 10;
+
 // Exclusive scans of the four block-count sequences (one workgroup each),
 // over the blocks that hold live particles (the scatters read no other
 // block's offsets).  tot[] = {keepL, splitL, keepH, leadH, newL, newH}.
 __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cl, uint32_t nbl,
                                                       uint32_t* __restrict__ ch, uint32_t nbh,
-                                                      const uint32_t* __restrict__ n_l,
-                                                      const uint32_t* __restrict__ n_h,
-                                                      uint32_t* __restrict__ tot) {
+                                                      uint32_t* __restrict__ n_l,
+                                                      uint32_t* __restrict__ n_h,
+                                                      uint32_t cap_l, uint32_t cap_h,
+                                                      uint32_t* __restrict__ tot,
+                                                      uint32_t* __restrict__ error_word,
+                                                      uint32_t* __restrict__ stamp) {
     // Block q scans count sequence q (kept low, split low, kept high, merge
     // leaders): a thread sums a contiguous chunk of at most a few counts, the
     // chunk sums are scanned with wave prefix sums and the 16 wave totals,
@@ -1065,7 +1077,34 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* __restrict__ cl,
         d[i] = run;
         run += v;
     }
-    if (t == 1023) tot[q] = before + x;
+    if (t != 1023) return;
+    // Publish the total; the last of the four blocks to do so commits the
+    // step's counts (it reads the totals after the others' releases).
+    __hip_atomic_store(&tot[q], before + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t* ticket = tot + kTicketWord;
+    const uint32_t done =
+        __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (done != gridDim.x - 1u) return;
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t tq[4];
+    for (int k = 0; k < 4; ++k)
+        tq[k] = __hip_atomic_load(&tot[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // New counts from the totals (kept + merged low; kept + 8 x split high),
+    // clamped to the capacities; a split or merge past them was dropped.
+    // tot[6], tot[7] keep the counts the step ran with (the scatters' bound,
+    // and for inspection).  Every block read *n_l, *n_h before its ticket.
+    const uint64_t new_l = static_cast<uint64_t>(tq[0]) + tq[3];
+    const uint64_t new_h = static_cast<uint64_t>(tq[2]) + 8ull * tq[1];
+    if (new_l > cap_l || new_h > cap_h) atomicOr(error_word, 2u);
+    tot[4] = static_cast<uint32_t>(new_l < cap_l ? new_l : cap_l);
+    tot[5] = static_cast<uint32_t>(new_h < cap_h ? new_h : cap_h);
+    tot[6] = *n_l;
+    tot[7] = *n_h;
+    *n_l = tot[4];
+    *n_h = tot[5];
+    // The step stamp advances, so the next step's occupancy marks differ
+    // from this step's.
+    *stamp += 1u;
 }
 
 // In-block exclusive rank of a predicate (block of 256, in index order).
@@ -1089,19 +1128,19 @@ __device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t* wsum) {
 // offsets) r/sqrt(3) per axis with r = 1/2 (3 m0 / (4 pi rho))^(1/3)
 // (insertHighResParticles, TWO_SCALE/Solver.cu:541-586).  Blend state words
 // (bl_l != NULL): children start at lambda 0, state 6.
-__global__ __launch_bounds__(kBlock) void k_scatter_low(
-    const float4* __restrict__ pos_l, const float4* __restrict__ vel_l,
+__device__ __forceinline__ void scatter_low(
+    uint32_t blk, const float4* __restrict__ pos_l, const float4* __restrict__ vel_l,
     const float* __restrict__ rho_l, const float2* __restrict__ bl_l,
-    const uint32_t* __restrict__ flag_l, const uint32_t* __restrict__ n_l,
-    const uint32_t* __restrict__ cl, uint32_t nbl, const uint32_t* __restrict__ tot,
+    const uint32_t* __restrict__ flag_l, const uint32_t* __restrict__ cl, uint32_t nbl,
+    const uint32_t* __restrict__ tot,
     Region rg, SplitParams sp, uint32_t id_base, uint32_t cap_l, uint32_t cap_h,
     uint32_t* __restrict__ grp_live, uint32_t num_groups, const float* __restrict__ col_l,
     float4* __restrict__ new_pos_l, float4* __restrict__ new_vel_l, float2* __restrict__ new_bl_l,
     float* __restrict__ new_col_l, float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
     float2* __restrict__ new_bl_h, float* __restrict__ new_col_h) {
     __shared__ uint32_t wsum[kBlock / 64];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t n = *n_l;
+    const uint32_t i = blk * kBlock + threadIdx.x;
+    const uint32_t n = tot[6];  // the count the step ran with
     const bool live = i < n;
     const uint32_t f = live ? flag_l[i] : 1u;
     const uint32_t rk = block_rank(live && !(f & 1u), wsum);
@@ -1112,7 +1151,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_low(
     const uint32_t pid = __float_as_uint(x.w);
     if (grp_live && pid < num_groups) grp_live[pid] = 0u;
     if (!(f & 1u)) {
-        const uint32_t dst = cl[blockIdx.x] + rk;
+        const uint32_t dst = cl[blk] + rk;
         if (dst < cap_l) {
             new_pos_l[dst] = x;
             new_vel_l[dst] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -1124,7 +1163,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_low(
         }
     }
     if (f & 2u) {
-        const uint32_t base = tot[2] + 8u * (cl[nbl + blockIdx.x] + rs);
+        const uint32_t base = tot[2] + 8u * (cl[nbl + blk] + rs);
         float d = rg.child_d;
         if (sp.ref_offsets) {
             // 3 / (4 pi) and 1/sqrt(3) as the reference's float constants
@@ -1151,10 +1190,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter_low(
 // velocity, id = group; blended: lambda 0, state 2) to
 // new_l[keepL + lead rank].  Every high particle clears its group's counter
 // and live bits for the next step.
-__global__ __launch_bounds__(kBlock) void k_scatter_high(
-    const float4* __restrict__ pos_h, const float4* __restrict__ vel_h,
+__device__ __forceinline__ void scatter_high(
+    uint32_t blk, const float4* __restrict__ pos_h, const float4* __restrict__ vel_h,
     const float2* __restrict__ bl_h, const uint32_t* __restrict__ flag_h,
-    const uint32_t* __restrict__ n_h, const uint32_t* __restrict__ ch, uint32_t nbh,
+    const uint32_t* __restrict__ ch, uint32_t nbh,
     const uint32_t* __restrict__ tot, const float4* __restrict__ grp_pos,
     const float4* __restrict__ grp_vel, uint32_t* __restrict__ grp_cnt,
     uint32_t* __restrict__ grp_live, uint32_t num_groups, uint32_t cap_l, uint32_t cap_h,
@@ -1163,8 +1202,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
     float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
     float2* __restrict__ new_bl_h, float* __restrict__ new_col_h) {
     __shared__ uint32_t wsum[kBlock / 64];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t n = *n_h;
+    const uint32_t i = blk * kBlock + threadIdx.x;
+    const uint32_t n = tot[7];
     const bool live = i < n;
     const uint32_t f = live ? flag_h[i] : 1u;
     const uint32_t rk = block_rank(live && !(f & 1u), wsum);
@@ -1177,7 +1216,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
         if (grp_live) grp_live[g] = 0u;
     }
     if (!(f & 1u)) {
-        const uint32_t dst = ch[blockIdx.x] + rk;
+        const uint32_t dst = ch[blk] + rk;
         if (dst < cap_h) {
             new_pos_h[dst] = pos_h[i];
             new_vel_h[dst] = v;
@@ -1191,7 +1230,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
         }
     }
     if (f & 2u) {
-        const uint32_t dst = tot[0] + ch[nbh + blockIdx.x] + rl;
+        const uint32_t dst = tot[0] + ch[nbh + blk] + rl;
         if (dst < cap_l) {
             float sx = 0.f, sy = 0.f, sz = 0.f, ux = 0.f, uy = 0.f, uz = 0.f;
             for (int k = 0; k < 8; ++k) {
@@ -1212,26 +1251,33 @@ __global__ __launch_bounds__(kBlock) void k_scatter_high(
     }
 }
 
-// tot[6], tot[7] keep the counts the step ran with (for inspection).
-// The step stamp advances here, so the next step's occupancy marks differ
-// from this step's.
-__global__ void k_commit_counts(uint32_t* __restrict__ tot, uint32_t* __restrict__ n_l,
-                                uint32_t* __restrict__ n_h, uint32_t cap_l, uint32_t cap_h,
-                                uint32_t* __restrict__ error_word, uint32_t* __restrict__ stamp) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *stamp += 1u;
-        // New counts from the totals (kept + merged low; kept + 8 x split high),
-        // clamped to the capacities; a split or merge past them was dropped.
-        const uint64_t new_l = static_cast<uint64_t>(tot[0]) + tot[3];
-        const uint64_t new_h = static_cast<uint64_t>(tot[2]) + 8ull * tot[1];
-        if (new_l > cap_l || new_h > cap_h) atomicOr(error_word, 2u);
-        tot[4] = static_cast<uint32_t>(new_l < cap_l ? new_l : cap_l);
-        tot[5] = static_cast<uint32_t>(new_h < cap_h ? new_h : cap_h);
-        tot[6] = *n_l;
-        tot[7] = *n_h;
-        *n_l = tot[4];
-        *n_h = tot[5];
-    }
+// Both sets' scatters in one launch: blocks 0 .. nbl - 1 take the low set's
+// slots, the next nbh blocks the high set's.  Their writes are disjoint: the
+// low set's kept particles go to new_l[0, keepL) and its children to
+// new_h[keepH, ...), the high set's kept particles to new_h[0, keepH) and its
+// merged particles to new_l[keepL, ...); both only zero the group words.
+__global__ __launch_bounds__(kBlock) void k_scatter(
+    const float4* __restrict__ pos_l, const float4* __restrict__ vel_l,
+    const float* __restrict__ rho_l, const float2* __restrict__ bl_l,
+    const uint32_t* __restrict__ flag_l, const uint32_t* __restrict__ cl, uint32_t nbl,
+    const float* __restrict__ col_l, const float4* __restrict__ pos_h,
+    const float4* __restrict__ vel_h, const float2* __restrict__ bl_h,
+    const uint32_t* __restrict__ flag_h, const uint32_t* __restrict__ ch, uint32_t nbh,
+    const float* __restrict__ col_h, const uint32_t* __restrict__ tot, Region rg, SplitParams sp,
+    uint32_t id_base, uint32_t cap_l, uint32_t cap_h, const float4* __restrict__ grp_pos,
+    const float4* __restrict__ grp_vel, uint32_t* __restrict__ grp_cnt,
+    uint32_t* __restrict__ grp_live, uint32_t num_groups, float4* __restrict__ new_pos_l,
+    float4* __restrict__ new_vel_l, float2* __restrict__ new_bl_l, float* __restrict__ new_col_l,
+    float4* __restrict__ new_pos_h, float4* __restrict__ new_vel_h,
+    float2* __restrict__ new_bl_h, float* __restrict__ new_col_h) {
+    if (blockIdx.x < nbl)
+        scatter_low(blockIdx.x, pos_l, vel_l, rho_l, bl_l, flag_l, cl, nbl, tot, rg, sp, id_base,
+                    cap_l, cap_h, grp_live, num_groups, col_l, new_pos_l, new_vel_l, new_bl_l,
+                    new_col_l, new_pos_h, new_vel_h, new_bl_h, new_col_h);
+    else
+        scatter_high(blockIdx.x - nbl, pos_h, vel_h, bl_h, flag_h, ch, nbh, tot, grp_pos, grp_vel,
+                     grp_cnt, grp_live, num_groups, cap_l, cap_h, col_h, new_pos_l, new_vel_l,
+                     new_bl_l, new_col_l, new_pos_h, new_vel_h, new_bl_h, new_col_h);
 }
 
 __global__ void k_pack_two_scale(const float* __restrict__ pxyz, const float* __restrict__ vxyz,
@@ -1298,7 +1344,7 @@ struct sph_two_scale {
     float4* grp_pos = nullptr;
     float4* grp_vel = nullptr;
     uint32_t* grp_cnt = nullptr;
-    uint32_t* tot = nullptr;     // 8 words (k_scan_counts, k_commit_counts)
+    uint32_t* tot = nullptr;     // 16 words: kTotWords
     uint32_t* error_word = nullptr;
     uint32_t* stamp = nullptr;   // steps run (the occupancy marks' base)
     uint32_t* occ = nullptr;     // per low-grid cell: 1 when it holds a high particle
@@ -1461,20 +1507,14 @@ int launch_split_merge_commit(sph_two_scale* t) {
     sp.ref_offsets = t->opt.split_offsets == SPH2_SPLIT_REFERENCE ? 1u : 0u;
     sp.m0 = L.cfg.mass;
     hipLaunchKernelGGL(k_scan_counts, dim3(4), dim3(1024), 0, st, L.counts, nbl, H.counts, nbh,
-                       L.buf.n_dev, H.buf.n_dev, t->tot);
+                       L.buf.n_dev, H.buf.n_dev, L.cap, H.cap, t->tot, t->error_word, t->stamp);
     const bool col = (t->debug & SPH_DEBUG_COLOR) != 0u;
-    hipLaunchKernelGGL(k_scatter_low, dim3(nbl), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel,
-                       L.buf.rho, bl_l, L.flag, L.buf.n_dev, L.counts, nbl, t->tot, t->region, sp,
-                       t->id_base, L.cap, H.cap, live, t->num_groups, col ? L.tcolor : nullptr,
-                       L.state_pos, L.state_vel, L.state_blend, L.color, H.state_pos, H.state_vel,
-                       H.state_blend, H.color);
-    hipLaunchKernelGGL(k_scatter_high, dim3(nbh), dim3(kBlock), 0, st, H.buf.pos, H.buf.vel, bl_h,
-                       H.flag, H.buf.n_dev, H.counts, nbh, t->tot, t->grp_pos, t->grp_vel,
-                       t->grp_cnt, live, t->num_groups, L.cap, H.cap, col ? H.tcolor : nullptr,
-                       L.state_pos, L.state_vel, L.state_blend, L.color, H.state_pos, H.state_vel,
-                       H.state_blend, H.color);
-    hipLaunchKernelGGL(k_commit_counts, dim3(1), dim3(64), 0, st, t->tot, L.buf.n_dev,
-                       H.buf.n_dev, L.cap, H.cap, t->error_word, t->stamp);
+    hipLaunchKernelGGL(k_scatter, dim3(nbl + nbh), dim3(kBlock), 0, st, L.buf.pos, L.buf.vel,
+                       L.buf.rho, bl_l, L.flag, L.counts, nbl, col ? L.tcolor : nullptr, H.buf.pos,
+                       H.buf.vel, bl_h, H.flag, H.counts, nbh, col ? H.tcolor : nullptr, t->tot,
+                       t->region, sp, t->id_base, L.cap, H.cap, t->grp_pos, t->grp_vel,
+                       t->grp_cnt, live, t->num_groups, L.state_pos, L.state_vel, L.state_blend,
+                       L.color, H.state_pos, H.state_vel, H.state_blend, H.color);
     SPH_HIP(hipGetLastError());
     return SPH_OK;
 }
@@ -1973,7 +2013,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
     add(128 * static_cast<size_t>(t->num_groups));  // grp_vel
     add(4 * static_cast<size_t>(t->num_groups));    // grp_cnt
-    add(64);                                        // tot[8], error word, step stamp
+    add(4 * kTotWords);                             // tot: totals, error word, stamp, ticket
     add(12 * static_cast<size_t>(t->set[0].num_cells));  // low-grid occupancy: occ_h, occ_l, near
     add(4 * static_cast<size_t>(t->num_groups));    // grp_live
     add(4 * static_cast<size_t>(t->num_groups));    // surf
@@ -2041,8 +2081,8 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
     t->grp_vel = static_cast<float4*>(take());
     t->grp_cnt = static_cast<uint32_t*>(take());
     t->tot = static_cast<uint32_t*>(take());
-    t->error_word = t->tot + 8;
-    t->stamp = t->tot + 9;
+    t->error_word = t->tot + kErrorWordIndex;
+    t->stamp = t->tot + kStampWord;
     t->occ = static_cast<uint32_t*>(take());
     t->grp_live = static_cast<uint32_t*>(take());
     t->surf = static_cast<uint32_t*>(take());
