@@ -295,18 +295,23 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-// Calls body(j) for the candidates j of one particle's stencil rows over grid
-// g, spread over a group of kG lanes.  The wave holds 64 / kG groups, each
+// Calls body(j, pos_b[j]) for the candidates j of one particle's stencil rows
+// over grid g, spread over a group of kG lanes.  The wave holds 64 / kG groups, each
 // with its own particle (a group with `active` false has no rows).  Row r of
 // a chunk of kG rows is probed by the group's lane r, its run
 // [rb[r], rb[r] + len_r) takes positions [incl[r] - len_r, incl[r]) of the
 // chunk's rows laid end to end, and the group's lane l visits positions
 // l, l + kG, ...  rb_s and incl_s are the wave's own 64-entry LDS rows
-// (group q uses entries q kG .. q kG + kG - 1).  Every lane of the wave calls
-// it.
+// (group q uses entries q kG .. q kG + kG - 1).  A lane takes its positions
+// kUnroll at a time: it finds their candidates, loads their positions
+// together (several loads in flight: the walks wait on memory), then calls
+// body in position order, so its terms are added in the same order as one at
+// a time.  Every lane of the wave calls it.
+constexpr int kWalkUnroll = 4;
 template <int kG, class Body>
 __device__ __forceinline__ void group_walk(const GridView& g, const uint32_t* __restrict__ cs,
-                                           const uint32_t* __restrict__ ce, uint32_t n,
+                                           const uint32_t* __restrict__ ce,
+                                           const float4* __restrict__ pos_b, uint32_t n,
                                            bool active, float x, float y, float z,
                                            uint32_t* rb_s, uint32_t* incl_s, Body&& body) {
     static_assert(kG == 16 || kG == 32 || kG == 64, "walker group width");
@@ -361,10 +366,25 @@ __device__ __forceinline__ void group_walk(const GridView& g, const uint32_t* __
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // incl_s[g0 + kG - 1] = total > f, so the row search stays in the group.
         uint32_t row = g0;
-        for (uint32_t f = gl; f < total; f += kG) {
-            while (incl_s[row] <= f) ++row;
-            const uint32_t begin = (row > g0) ? incl_s[row - 1u] : 0u;
-            body(rb_s[row] + (f - begin));
+        for (uint32_t f = gl; f < total; f += kWalkUnroll * kG) {
+            uint32_t j[kWalkUnroll];
+            float4 xj[kWalkUnroll];
+#pragma unroll
+            for (int u = 0; u < kWalkUnroll; ++u) {
+                const uint32_t fu = f + static_cast<uint32_t>(u) * kG;
+                j[u] = 0u;
+                if (fu < total) {
+                    while (incl_s[row] <= fu) ++row;
+                    const uint32_t begin = (row > g0) ? incl_s[row - 1u] : 0u;
+                    j[u] = rb_s[row] + (fu - begin);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kWalkUnroll; ++u)
+                if (f + static_cast<uint32_t>(u) * kG < total) xj[u] = pos_b[j[u]];
+#pragma unroll
+            for (int u = 0; u < kWalkUnroll; ++u)
+                if (f + static_cast<uint32_t>(u) * kG < total) body(j[u], xj[u]);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -435,9 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_density_walk(
         const uint32_t s = act ? walk_entry(list, seg_cap, seg_s[wv], w) : 0u;
         const float4 xi = act ? tpos_a[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float sum = 0.0f;
-        group_walk<kG>(gb, cs_b, ce_b, nb, act, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv],
-                       [&](uint32_t j) {
-            const float4 xj = tpos_b[j];
+        group_walk<kG>(gb, cs_b, ce_b, tpos_b, nb, act, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv],
+                       [&](uint32_t j, const float4& xj) {
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < c.h02) {
@@ -492,9 +511,8 @@ __global__ __launch_bounds__(kBlock) void k_cross_force_walk(
         const float2 ai = act ? aux_a[s] : make_float2(0.0f, 0.0f);
         float Px = 0.f, Py = 0.f, Pz = 0.f, Vx = 0.f, Vy = 0.f, Vz = 0.f, Tx = 0.f, Ty = 0.f,
               Tz = 0.f;
-        group_walk<kG>(gb, cs_b, ce_b, nb, act, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv],
-                       [&](uint32_t j) {
-            const float4 xj = tpos_b[j];
+        group_walk<kG>(gb, cs_b, ce_b, tpos_b, nb, act, xi.x, xi.y, xi.z, rb_s[wv], incl_s[wv],
+                       [&](uint32_t j, const float4& xj) {
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
             if (r2 < c.h02 && r2 > 0.0f) {
