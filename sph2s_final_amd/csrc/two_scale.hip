@@ -225,8 +225,10 @@ __device__ __forceinline__ bool other_nearby(const Nearby& nb, const GridView& g
 // candidates l, l + kG, ... of that sequence, and the lanes' partial sums are
 // reduced in a fixed order (so the result does not depend on which wave or
 // group takes which particle or on the list's order).  A wave holds several
-// groups so that it has several walkers' loads in flight: the walks wait on
-// memory, not on issue.
+// groups: a walker's row probes, prefix sum and reductions then share their
+// instructions with the other groups' (a high walker has 9 rows and a few
+// hundred candidates, little for 64 lanes), and the wave has several walkers'
+// chains of dependent loads in flight.
 constexpr uint32_t kWalkBlocks = 4096;  // 16384 waves share a step's walk list
 
 // Null arguments of the fixed coupling's walks (no blend weights, no separate
@@ -303,10 +305,10 @@ __device__ __forceinline__ float group_sum(float v) {
 // chunk's rows laid end to end, and the group's lane l visits positions
 // l, l + kG, ...  rb_s and incl_s are the wave's own 64-entry LDS rows
 // (group q uses entries q kG .. q kG + kG - 1).  A lane takes its positions
-// kUnroll at a time: it finds their candidates, loads their positions
-// together (several loads in flight: the walks wait on memory), then calls
-// body in position order, so its terms are added in the same order as one at
-// a time.  Every lane of the wave calls it.
+// kWalkUnroll at a time: it finds their candidates, loads their positions
+// together (several loads in flight), then calls body in position order, so
+// its terms are added in the same order as one at a time.  Every lane of the
+// wave calls it.
 constexpr int kWalkUnroll = 4;
 template <int kG, class Body>
 __device__ __forceinline__ void group_walk(const GridView& g, const uint32_t* __restrict__ cs,
