@@ -427,14 +427,17 @@ __device__ __forceinline__ void density_block(uint32_t block0, uint32_t n, float
     }
 }
 
-// Occupancy targets of the single-scale neighbour passes (waves per SIMD):
-// the density pass fits 6 waves in 84 VGPRs, the force pass 5 in 102 with
-// the 640-candidate tile (LDS 28.8 KB per block); the hints make the
-// compiler allocate within those budgets.  Other instantiations (batched
-// scenes, tension, two-scale own-set passes) keep the compiler's choice.
-constexpr int density_waves(bool batched, int ppl = 1) { return batched ? 1 : (ppl == 1 ? 6 : 4); }
-constexpr int force_waves(bool batched, bool tension, bool integrate, int ppl = 1) {
-    return (!batched && !tension && integrate) ? (ppl == 1 ? 5 : 4) : 1;
+// Occupancy targets of the neighbour passes (waves per SIMD): the density
+// pass fits 6 waves in 84 VGPRs, the force pass 5 in 102 with the
+// 640-candidate tile (LDS 28.8 KB per block); the hints make the compiler
+// allocate within those budgets.  Batched scenes take the same targets (left
+// to itself the compiler gave their force pass 116 VGPRs, 4 waves; with the
+// hints the batch step went from 1.92-1.96 to 1.89 ms).  The tension and
+// two-scale own-set forces keep the compiler's choice (94 VGPRs for the
+// latter, 5 waves already).
+constexpr int density_waves(bool /*batched*/, int ppl = 1) { return ppl == 1 ? 6 : 4; }
+constexpr int force_waves(bool /*batched*/, bool tension, bool integrate, int ppl = 1) {
+    return (!tension && integrate) ? (ppl == 1 ? 5 : 4) : 1;
 }
 
 template <bool kKahan, bool kFinalize, bool kBatched, int kTile, bool kX, int kB, int kP>
