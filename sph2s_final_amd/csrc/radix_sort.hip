@@ -41,15 +41,11 @@ constexpr int kPassBlock = SPH_SORT_BLOCK;
 constexpr int kWaves = kPassBlock / 64;
 static_assert(kPassBlock >= 256 && kPassBlock <= 1024 && kPassBlock % 64 == 0,
               "digit-pass block shape");
-#ifndef SPH_HIST_ITEMS
-#define SPH_HIST_ITEMS 8
-#endif
 constexpr int kItems = SPH_SORT_ITEMS;
 constexpr int kWaveKeys = 64 * kItems;
 constexpr int kTile = kPassBlock * kItems;
-// The histogram kernel reads its keys in smaller tiles (more blocks in flight).
-constexpr int kHistItems = SPH_HIST_ITEMS;
-constexpr int kHistTile = kBlock * kHistItems;
+// The histogram kernel reads its keys in smaller tiles (more blocks in
+// flight): kHistItems keys per thread, set by each key source below.
 #ifndef SPH_SORT_DIGIT_BITS
 #define SPH_SORT_DIGIT_BITS 10
 #endif
@@ -117,13 +113,18 @@ __global__ __launch_bounds__(kBlock) void k_prepare(uint32_t* __restrict__ zero,
 
 // Keys read from an array.
 struct ArrayKeys {
+    static constexpr int kHistItems = 8;
     const uint32_t* keys;
     __device__ uint32_t operator()(uint32_t i) const { return keys[i]; }
 };
 
 // Keys computed as cell hashes of float4 positions (computeHashs,
 // SPH2S_FINAL_SINGLE_SCALE/Solver.cu:401-424), written out for the first pass.
+// kHistItems: 4 keys per thread (1,024-key blocks, 977 blocks for 1M keys)
+// took the hash + histogram pass of the 1M-particle step from 19.4 to 16.8 us
+// against 8; 2 was as fast, 1, 16 and 32 slower.
 struct HashKeys {
+    static constexpr int kHistItems = 4;
     const float4* pos;
     uint32_t* keys_out;
     float ox, oy, oz, spacing;
@@ -139,7 +140,10 @@ struct HashKeys {
 // Batched scenes: the scene id is in vel.w; the key is the scene's first global
 // cell plus the reference hash of the position in the scene's own grid, so a
 // stable sort keeps every scene contiguous and in its own hash order.
+// (kHistItems 8: with 4 the 4.1M-key batch of 64 scenes ran its step 1%
+// slower.)
 struct BatchHashKeys {
+    static constexpr int kHistItems = 8;
     const float4* pos;
     const float4* vel;
     const SceneParams* scenes;
@@ -163,6 +167,8 @@ template <class Src>
 __global__ __launch_bounds__(kBlock) void k_hist(Src src, const uint32_t* __restrict__ n_dev,
                                                  int passes, uint32_t key_bits,
                                                  uint32_t* __restrict__ ghist) {
+    constexpr int kHistItems = Src::kHistItems;
+    constexpr uint32_t kHistTile = kBlock * kHistItems;
     const uint32_t n = *n_dev;
     if (blockIdx.x * kHistTile >= n) return;  // block-uniform: the grid covers the capacity
     __shared__ uint32_t cnt[kMaxPasses][kRadix];
@@ -404,6 +410,7 @@ int prepare_hist(Src src, uint32_t cap, const uint32_t* n_src, uint32_t n_host, 
     const uint32_t prep_blocks = std::min<uint32_t>((prep_words + kBlock - 1) / kBlock, 2048u);
     hipLaunchKernelGGL(k_prepare, dim3(prep_blocks), dim3(kBlock), 0, stream, L.header, zero_words,
                        fill_a, fill_b, fill_words, n_src, n_host);
+    constexpr uint32_t kHistTile = kBlock * Src::kHistItems;
     const uint32_t hist_blocks = std::max(1u, (cap + kHistTile - 1) / kHistTile);
     hipLaunchKernelGGL(k_hist<Src>, dim3(hist_blocks), dim3(kBlock), 0, stream, src,
                        L.header + kNWord, passes, key_bits, L.header);
