@@ -650,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_integrate(
 // neighbour term carries the neighbour's blend weight lambda_j.  Each set's
 // own sums walk its own grid in the internal x-order with the block-staged
 // stencil walk (the kernels below); the complementary sums are the fixed
-// coupling's walks (one wave per listed particle) with lambda weights.  Blend
+// coupling's walks (one lane group per listed particle) with lambda weights.  Blend
 // state words: float2 (lambda, state bits).
 
 constexpr int kBlendDensityTile = 1024;
@@ -1550,7 +1550,7 @@ int launch_split_merge(sph_two_scale* t) {
 // split/merge stage.
 // The blended coupling's passes (Solver.cu:215-499, 732-1083) in the fixed
 // coupling's shape: each set's lambda-weighted own-set density; the
-// complementary sums of the particles with a non-empty walk, one wave each
+// complementary sums of the particles with a non-empty walk, one lane group each
 // (k_cross_list lists them and finalises the others), weighted by lambda_j;
 // then each set's own-set force with the integration and blend update, which
 // adds the walk's complementary acceleration (kept in the set's own-set
