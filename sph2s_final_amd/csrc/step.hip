@@ -138,18 +138,18 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
     ivel[q] = vs;
     i2r[q] = s;
 
-    // The cell's x-bin table: cumulative counts of its 8 x-bins (packed
-    // 16-bit).  Bins are non-decreasing along the x-order, so the particle of
-    // rank r holds the count r + 1 for every bin from its own up to (not
+    // The cell's x-bin table: cumulative counts of its kXBins x-bins (one
+    // byte each).  Bins are non-decreasing along the x-order, so the particle
+    // of rank r holds the count r + 1 for every bin from its own up to (not
     // including) the next particle's bin, or up to the last bin if it is the
     // cell's last particle; the first particle also writes 0 for the bins
     // below its own.  Every entry is written by exactly one particle.  Cells
-    // of more than 0xFFFE particles get the all-ones marker (no trimming).
-    uint16_t* t = reinterpret_cast<uint16_t*>(bins + key);
+    // of more than kXBinMax particles get the all-ones marker (no trimming).
+    uint8_t* t = reinterpret_cast<uint8_t*>(bins + key);
     const uint32_t cnt = e - b;
-    if (cnt > 0xFFFEu) {
+    if (cnt > kXBinMax) {
         if (rank == 0)
-            for (int k = 0; k < 8; ++k) t[k] = 0xFFFFu;
+            for (int k = 0; k < kXBins; ++k) t[k] = static_cast<uint8_t>(kXBinUnbinned);
         return;
     }
     float ox = p.ox, inv = 1.0f / p.spacing;
@@ -163,10 +163,10 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
     }
     const float cx = static_cast<float>(local % dx);
     const int kq = xbin_of((xs.x - ox) * inv - cx);
-    const int kn = (rank + 1 < cnt) ? max(kq, xbin_of((xg - ox) * inv - cx)) : 8;
+    const int kn = (rank + 1 < cnt) ? max(kq, xbin_of((xg - ox) * inv - cx)) : kXBins;
     if (rank == 0)
         for (int k = 0; k < kq; ++k) t[k] = 0u;
-    const uint16_t c = static_cast<uint16_t>(rank + 1);
+    const uint8_t c = static_cast<uint8_t>(rank + 1);
     for (int k = kq; k < kn; ++k) t[k] = c;
 }
 

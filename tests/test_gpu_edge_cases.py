@@ -157,8 +157,8 @@ def test_face_aligned_lattice_matches_oracle():
 
 
 def test_cell_too_full_for_the_x_bins():
-    # 66,000 particles in one cell: more than the 16-bit x-bin counts hold, so
-    # the cell is marked unbinned and its rows are walked whole.  Checked
+    # 66,000 particles in one cell: more than the x-bin counts hold (254 per
+    # cell), so the cell is marked unbinned and its rows are walked whole.  Checked
     # against a brute-force FP64 density of 256 sampled particles (every pair
     # is within h), and, at rest, the pressure pairs' momentum balance:
     # sum_i a_i = N g (the pair terms (A_i + A_j) F(r) x_ij are antisymmetric;
