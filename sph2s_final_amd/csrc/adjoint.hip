@@ -580,7 +580,8 @@ int sph_rollout_create(const sph_config* cfgs, uint32_t n_scenes, const uint32_t
     r->n_sets = (r->n_ckpt > 1) ? 2u : 1u;
     std::vector<size_t> sizes = {al(16 * N * n_stored), al(16 * N * n_stored)};
     for (uint32_t k = 0; k < K * r->n_sets; ++k)
-        for (size_t v : {16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 4 * N, 4 * N, 4 * C, 4 * C, 16 * C})
+        for (size_t v : {16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 4 * N, 4 * N, 4 * C, 4 * C,
+                         kXBinBytes * C})
             sizes.push_back(al(v));
     for (size_t v : {16 * N, 16 * N, 16 * N, 16 * N, 16 * N, 4 * N, 8 * N, 8 * N, 16 * S})
         sizes.push_back(al(v));

@@ -407,7 +407,7 @@ int sph_create(const sph_config* cfg, const float* pos, const float* vel, uint32
         align_up(8 * N),                                                         // aux
         align_up(4),                                                             // live count
         align_up(16 * N), align_up(16 * N), align_up(8 * N),                    // ipos ivel iaux
-        align_up(4 * N),  align_up(16 * C)};                                     // i2r xbins
+        align_up(4 * N),  align_up(kXBinBytes * C)};                             // i2r xbins
     size_t total = 0;
     for (size_t b : sizes) total += b;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);

@@ -119,7 +119,7 @@ int batch_alloc(const sph_config* cfgs, uint32_t n_scenes, const uint32_t* count
                             al(16 * N), al(4 * N),  al(4 * N),  al(4 * N),  al(4 * C),
                             al(4 * C),  al(sort_temp), al(8 * N), al(4),
                             al(sizeof(SceneParams) * n_scenes), al(4 * (n_scenes + 1)),
-                            al(16 * N), al(16 * N), al(8 * N), al(4 * N), al(16 * C)};
+                            al(16 * N), al(16 * N), al(8 * N), al(4 * N), al(kXBinBytes * C)};
     size_t total = 0;
     for (size_t v : sizes) total += v;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);

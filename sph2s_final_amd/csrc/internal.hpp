@@ -151,7 +151,7 @@ struct StepBuffers {
     float4* ivel;
     float2* iaux;       // aux in the internal order (density -> force)
     uint32_t* i2r;      // internal slot -> sorted slot
-    uint4* xbins;       // per cell: cumulative counts of its 16 x-bins (8 bits each)
+    uint4* xbins;       // per cell: cumulative counts of its kXBins x-bins (8 bits each)
     // Boundary-particle walls (NULL: none); bacc holds each sorted slot's
     // boundary acceleration, computed before the force pass adds it.
     const Walls* walls;

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_xorder(const uint32_t* __restrict__ 
     // cell's last particle; the first particle also writes 0 for the bins
     // below its own.  Every entry is written by exactly one particle.  Cells
     // of more than kXBinMax particles get the all-ones marker (no trimming).
-    uint8_t* t = reinterpret_cast<uint8_t*>(bins + key);
+    uint8_t* t = reinterpret_cast<uint8_t*>(bins + static_cast<size_t>(key) * kXBinQuads);
     const uint32_t cnt = e - b;
     if (cnt > kXBinMax) {
         if (rank == 0)

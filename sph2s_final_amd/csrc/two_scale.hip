@@ -2027,7 +2027,7 @@ int sph2_create(const sph_config* cfg, const float region_lo[3], const float reg
         add(16 * N);                               // ivel
         add(8 * N);                                // iaux
         add(4 * N);                                // i2r
-        add(16 * C);                               // xbins
+        add(kXBinBytes * C);                       // xbins
         add(8 * N);                                // iblend
     }
     add(128 * static_cast<size_t>(t->num_groups));  // grp_pos: 8 float4 per group
