@@ -3,9 +3,10 @@
 Empty, tiny, ragged, extremely dense and face-aligned inputs.  Each is checked
 against the FP64 oracle with the bars of test_gpu_step (hashes, permutation and
 cell ranges bit-exact; density 1e-5 relative; acceleration 1e-5 of max |a*|),
-and the smallest ones also against closed forms.  A cell holding more
-particles than the x-bin tables can count (65,535) would take the oracle
-minutes, so that case is checked against a brute-force density of sampled
+and the smallest ones also against closed forms.  Cells at the x-bin tables'
+limit (254 particles binned, 255 not) are checked against the oracle.  A cell
+of 66,000 particles would take the oracle minutes, so that case is checked
+against a brute-force density of sampled
 particles (to 1e-4: its rows are single FP32 runs of 66,000 terms) and the
 momentum balance of the pressure pairs.
 """
@@ -43,7 +44,7 @@ def _gpu_step(cfg, pos, vel, dt=scenes.DT, max_n=None):
                     vel=s.velocities())
 
 
-def _check(name, got, ref):
+def _check(name, got, ref, rho_tol=RHO_TOL, acc_tol=ACC_TOL, pos_tol=1e-6):
     assert np.array_equal(got["hash"], ref["hash"]), name
     assert np.array_equal(got["perm"], ref["perm"]), name
     assert np.array_equal(got["cell_start"], ref["cell_start"]), name
@@ -53,9 +54,9 @@ def _check(name, got, ref):
     pos_err = np.max(np.abs(got["pos"] - ref["pos"]))
     print(f"case {name}: n={len(got['rho'])} rho {rho_err:.3e} acc {acc_err:.3e} "
           f"pos {pos_err:.3e}", flush=True)
-    assert rho_err <= RHO_TOL, name
-    assert acc_err <= ACC_TOL, name
-    assert pos_err <= 1e-6, name
+    assert rho_err <= rho_tol, name
+    assert acc_err <= acc_tol, name
+    assert pos_err <= pos_tol, name
 
 
 def _oracle_step(cfg, pos, vel, dt=scenes.DT):
@@ -154,6 +155,35 @@ def test_face_aligned_lattice_matches_oracle():
     vel = np.zeros_like(pos)
     got = _gpu_step(cfg, pos, vel)
     _check("face-aligned lattice", got, _oracle_step(cfg, pos, vel))
+
+
+@pytest.mark.parametrize("count", [254, 255])
+def test_cells_at_the_x_bin_limit(count):
+    # One cell holding `count` particles spread through it, inside a lattice
+    # of spacing h/2 around it: 254 is the most the 8-bit x-bin counts take
+    # (the cell is binned and runs through it are trimmed at its bins), 255
+    # the fewest that leave it unbinned (walked whole).  Both against the FP64
+    # oracle: hashes, permutation and ranges bit-exact; a miscounted or
+    # wrapped bin would drop or double neighbours, an O(1) error.  Density and
+    # acceleration are held to the usual 1e-5 (measured 7-9e-7).
+    cfg = _config()
+    h = float(cfg.h)
+    rng = np.random.default_rng(100 + count)
+    lo = np.array([3.0, 3.0, 3.0]) * h
+    full = lo + rng.uniform(0.02 * h, 0.98 * h, size=(count, 3))
+    i = np.arange(8)
+    gx, gy, gz = np.meshgrid(i, i, i, indexing="ij")
+    lat = (np.stack([gx, gy, gz], axis=-1).reshape(-1, 3) + 0.5) * (h / 2) + 1.5 * h
+    inside = np.all((lat >= lo) & (lat < lo + h), axis=1)
+    pos = np.concatenate([full, lat[~inside]]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    print(f"case x-bin limit cell count={count} n={len(pos)}", flush=True)
+    got = _gpu_step(cfg, pos, vel)
+    ref = _oracle_step(cfg, pos, vel)
+    cs, ce = got["cell_start"], got["cell_end"]
+    occupied = cs != SPH_EMPTY_CELL
+    assert int(np.max(ce[occupied] - cs[occupied])) == count
+    _check(f"x-bin limit {count}", got, ref)
 
 
 def test_cell_too_full_for_the_x_bins():
